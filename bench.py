@@ -1,0 +1,111 @@
+"""Benchmark: cell-steps/s of the ghost-cell AB2/CN step on one MI355X.
+
+Workload (one step = one full AB2/CN fractional step of one case): the channel
+22D x 4.1D with a D=1 cylinder at (2D, 2D), h = D/60 -> 1320 x 246 pressure cells,
+parabolic inflow U_m = 1.5, Re = 100 (nu = 0.01), dt = 5e-4, FP64, forces every step,
+started from the inlet profile with a seeded 1e-3 Gaussian transverse v perturbation
+(seed 20261203).  The state stays resident in HBM for the whole timed region.
+
+Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+WORKLOAD = dict(nx=1320, ny=246, Lx=22., Ly=4.1, D=1., xc=2., yc=2., nu=0.01, rho=1., inflow='parabolic', U_m=1.5)
+DT = 5e-4
+SEED = 20261203
+HBM_PEAK = 8.0e12          # B/s
+FP64_PEAK = 78.6e12        # FLOP/s
+STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB2/CN step
+
+
+def cpu_baseline(seconds=20.0):
+    """The oracle (NumPy/SciPy restatement) on this host's cores, on a bounded sample of steps."""
+    from oracle.ghost_oracle import CylinderFlow
+    cores = len(os.sched_getaffinity(0))
+    s = CylinderFlow(initial='profile', v_perturbation=dict(seed=SEED), fft_workers=cores, **WORKLOAD)
+    s.step(DT)       # startup step builds the per-dt Helmholtz factors (setup, not timed)
+    s.step(DT)
+    times = []
+    start = time.perf_counter()
+    while time.perf_counter() - start < seconds and len(times) < 400:
+        t0 = time.perf_counter()
+        s.step(DT)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    cells = WORKLOAD['nx'] * WORKLOAD['ny']
+    return dict(value=cells / med, unit='cell-steps/s', cores=cores, kind='port',
+                sample=f'{len(times)} steps of the 1320x246 Re=100 case after 2 warm-up steps, median step time '
+                       f'{med * 1e3:.1f} ms (oracle/ghost_oracle.py, scipy.fft workers={cores}, single-threaded SpMV)')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2000)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py: this build runs one case on one GPU; --gpus {args.gpus} is not supported', file=sys.stderr)
+        sys.exit(2)
+    import cfd_amd_loader
+    pkg = cfd_amd_loader.load()
+    t0 = time.perf_counter()
+    flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='profile',
+                                v_perturbation=dict(seed=SEED), **WORKLOAD)
+    dev = flow.gg.device
+    flow._push()
+    dev.set_dt(DT)
+    setup_s = time.perf_counter() - t0
+    print(f'bench: setup {setup_s:.2f} s; warmup {args.warmup} steps', file=sys.stderr, flush=True)
+    if args.warmup:
+        dev.run(args.warmup, DT)
+    print(f'bench: timing {args.steps} steps', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    diag = dev.run(args.steps, DT)     # synchronises before returning
+    elapsed = time.perf_counter() - t0
+    cells = WORKLOAD['nx'] * WORKLOAD['ny']
+    value = cells * args.steps / elapsed
+    max_div = float(np.max(diag[:, 10]))
+    # dominant kernels timed with HIP events on the library's stream
+    prof = dev.profile(20, DT)
+    gemm_ms, pcr_ms, step_ms = prof[0], prof[1], prof[2]
+    yy, xx = WORKLOAD['ny'], WORKLOAD['nx']
+    gemm_flops = 2.0 * yy * yy * xx
+    gemm_achieved = gemm_flops / (gemm_ms * 1e-3)
+    out = dict(metric='cell-steps/s on 1 MI355X (plain vs batched+NN warm-start) at div<1e-12', value=value,
+               unit='cell-steps/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=elapsed / args.steps * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
+               dtype='f64', data='synthetic',
+               config=dict(workload='channel 22Dx4.1D, cylinder D=1 at (2D,2D), 1320x246 cells (h=D/60), '
+                                    'parabolic Um=1.5, Re=100, dt=5e-4, FP64, plain ghost AB2/CN step, forces every step',
+                           path='plain', cells=cells, max_divergence=max_div),
+               roofline=dict(bound='mfma', achieved=gemm_achieved / 1e12, peak=FP64_PEAK / 1e12, unit='TFLOP/s',
+                             frac=gemm_achieved / FP64_PEAK, traffic=None,
+                             kernel='k_gemm (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
+                             launch_ms=gemm_ms),
+               step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
+                                  achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                                  frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
+               kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms), setup_seconds=setup_s)
+    if not args.no_cpu_baseline:
+        print('bench: CPU baseline (oracle)', file=sys.stderr, flush=True)
+        out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)
+    else:
+        out['cpu_baseline'] = None
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

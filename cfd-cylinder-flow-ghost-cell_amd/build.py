@@ -1,0 +1,31 @@
+"""Build libghostflow.so in-tree for gfx950 with hipcc (no JIT, no cache outside the tree)."""
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+SOURCES = [HERE / 'csrc' / 'ghostflow.hip']
+OUTPUT = HERE / 'libghostflow.so'
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         f'-I{ROOT / "include"}']
+
+
+def build(verbose=False):
+    tmp = OUTPUT.with_suffix('.so.tmp')
+    cmd = [HIPCC, *FLAGS, '-o', str(tmp), *map(str, SOURCES)]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError('hipcc failed for libghostflow.so')
+    os.replace(tmp, OUTPUT)
+    return OUTPUT
+
+
+if __name__ == '__main__':
+    print(build(verbose=True))

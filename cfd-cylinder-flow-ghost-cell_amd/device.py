@@ -1,0 +1,296 @@
+"""Upload of one case's tables to the HIP library and thin wrappers of its entry points.
+
+The host tables come from ``tables.GhostOperators`` (assembled like the
+reference integrator, solver/optimized_ghost_cell.py:62-150).  The device
+keeps the reference's staggered arrays and applies every operator as a
+structured stencil plus "override" rows: the rows of the assembled sparse
+operator that differ from the cylinder-free stencil (those touching ghost
+nodes).  Those rows are passed verbatim, so the device operator is the
+reference's operator.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+from . import _native as nat
+from .tables import RectangleFactor, SchurFactor, boundary_rows
+
+
+def solid_intervals(body):
+    """Per-row [lo, hi] of the solid columns of a boolean array (hi < lo: none); rows must be convex."""
+    rows, cols = body.shape
+    lo = np.ones(rows, dtype=np.int32)
+    hi = np.zeros(rows, dtype=np.int32)
+    for j in range(rows):
+        idx = np.flatnonzero(body[j])
+        if len(idx):
+            if idx[-1] - idx[0] + 1 != len(idx):
+                raise ValueError('Solid region is not convex along a grid row')
+            lo[j], hi[j] = idx[0], idx[-1]
+    return lo, hi
+
+
+def csr_rows(matrix, rows):
+    """CSR (ptr, col, val) of the selected rows."""
+    sub = matrix.tocsr()[rows]
+    sub.sort_indices()
+    return nat.i32(sub.indptr), sub.indices, nat.f64(sub.data)
+
+
+def changed_rows(actual, regular, rel=1e-13):
+    """Rows where two sparse operators differ by more than rel * max|regular|."""
+    diff = (actual.tocsr() - regular.tocsr()).tocsr()
+    scale = max(abs(regular).max(), 1e-300)
+    diff.data[abs(diff.data) <= rel * scale] = 0.
+    diff.eliminate_zeros()
+    return np.flatnonzero(np.diff(diff.indptr))
+
+
+class GhostDevice:
+    """Device state and tables of one case (one ``mcf_handle``)."""
+
+    def __init__(self, ops, u_inlet):
+        self.ops = o = ops
+        self.h = nat.Handle(o.nx, o.ny, o.dx, o.dy, o.nu, o.rho)
+        self.set_inlet(u_inlet)
+        gc = o.gc
+        self.body = dict(u=gc.dist_u <= 1e-13, v=gc.dist_v <= 1e-13, p=gc.dist_p <= 1e-13)
+        for field, key in ((nat.FIELD_U, 'u'), (nat.FIELD_V, 'v'), (nat.FIELD_P, 'p')):
+            lo, hi = solid_intervals(self.body[key])
+            self.h.call('mcf_set_solid_intervals', field, nat.iptr(lo), nat.iptr(hi))
+        self._upload_ghosts()
+        self._upload_overrides()
+        self.pressure_factor = RectangleFactor('pressure', o.nx, o.ny, o.dx, o.dy, rho=o.rho)
+        self.pressure_schur = SchurFactor(o.A, o.pi, self.pressure_factor)
+        self._upload_rectangle(nat.PRESSURE, self.pressure_factor, self.pressure_schur)
+        self._upload_surface()
+        self.dt = None
+        self.helmholtz_factors = None
+
+    # ---------------------------------------------------------------- tables
+    def set_inlet(self, u_inlet):
+        self.u_inlet = nat.f64(u_inlet).copy()
+        self.h.call('mcf_set_inlet', nat.dptr(self.u_inlet))
+
+    def _upload_ghosts(self):
+        o = self.ops
+        for field, E, ids, info in ((nat.FIELD_U, o.Eu, o.ui, o.ghost_u), (nat.FIELD_V, o.Ev, o.vi, o.ghost_v),
+                                    (nat.FIELD_P, o.P, o.pi, o.ghost_p)):
+            nodes = np.asarray(info['ghost_nodes'])
+            ptr, col, val = csr_rows(E, nodes)
+            self.h.call('mcf_set_ghost', field, len(nodes), nat.iptr(nat.i32(nodes)), nat.iptr(ptr),
+                        nat.iptr(nat.i32(ids[col])), nat.dptr(val))
+
+    def _vel_index(self, qcols):
+        """Packed velocity index -> vel-layout index."""
+        o = self.ops
+        qcols = np.asarray(qcols)
+        return np.where(qcols < o.nu_act, o.ui[np.minimum(qcols, o.nu_act - 1)],
+                        o.usize + o.vi[np.clip(qcols - o.nu_act, 0, len(o.vi) - 1)])
+
+    def _set_override(self, op, out_index, matrix, rows, col_map):
+        ptr, col, val = csr_rows(matrix, rows)
+        self.h.call('mcf_set_override', op, len(rows), nat.iptr(nat.i32(out_index[rows])), nat.iptr(ptr),
+                    nat.iptr(nat.i32(col_map(col))), nat.dptr(val))
+        return len(rows)
+
+    def _upload_overrides(self):
+        o = self.ops
+        nu = o.nu_act
+        # gradient: regular stencil = raw_g / rho over the full p array
+        Gfull = sp.csr_matrix((o.G.data, o.pi[o.G.indices], o.G.indptr), shape=(o.qsize, o.nx * o.ny))
+        rows = changed_rows(Gfull, o.raw_g / o.rho)
+        ru, rv = rows[rows < nu], rows[rows >= nu]
+        self.n_override = {}
+        self.n_override['grad_u'] = self._set_override(nat.OP_GRAD_U, o.ui, Gfull[:nu], ru, lambda c: c)
+        self.n_override['grad_v'] = self._set_override(nat.OP_GRAD_V, o.vi, Gfull[nu:], rv - nu, lambda c: c)
+        # divergence: regular stencil = Dfull without the inlet column and the v wall rows
+        Dmapped = sp.csr_matrix((o.Dop.data, self._vel_index(o.Dop.indices), o.Dop.indptr), shape=o.Dfull.shape)
+        keep = np.ones(o.Dfull.shape[1], bool)
+        keep[np.arange(o.ny) * (o.nx + 1)] = False
+        keep[o.usize + np.arange(o.nx)] = False
+        keep[o.usize + o.ny * o.nx + np.arange(o.nx)] = False
+        Dreg = o.Dfull @ sp.diags(keep.astype(float))
+        rows = changed_rows(Dmapped, Dreg)
+        self.n_override['div'] = self._set_override(nat.OP_DIV, o.pi, Dmapped, rows, lambda c: c)
+        # pressure operator: rows where A differs from the rectangle (pressure_schur.py:78-92)
+        rect = self.pressure_factor_matrix()
+        brows, _, _ = boundary_rows(o.A, o.pi, rect)
+        pos = np.full(o.nx * o.ny, -1)
+        pos[o.pi] = np.arange(len(o.pi))
+        arows = pos[brows]
+        arows = arows[arows >= 0]
+        self.n_override['pressure'] = self._set_override(nat.OP_PRESSURE, o.pi, o.A.tocsr(), arows,
+                                                         lambda c: o.pi[c])
+
+    def pressure_factor_matrix(self):
+        from .tables import rectangle_matrix
+        o = self.ops
+        return rectangle_matrix('pressure', o.nx, o.ny, o.dx, o.dy, rho=o.rho)
+
+    def _upload_rectangle(self, kind, factor, schur):
+        yy, xx = factor.shape
+        self.h.call('mcf_set_rectangle', kind, yy, xx, float(factor.a), float(factor.b), nat.dptr(nat.f64(factor.V)),
+                    nat.dptr(nat.f64(factor.lam)), nat.dptr(nat.f64(factor.lower)), nat.dptr(nat.f64(factor.diag)),
+                    nat.dptr(nat.f64(factor.upper)))
+        C = schur.Cp.tocsr()
+        C.sort_indices()
+        self.h.call('mcf_set_schur', kind, schur.rank, nat.iptr(nat.i32(schur.rows)), len(schur.points),
+                    nat.iptr(nat.i32(schur.points)), nat.iptr(nat.i32(C.indptr)), nat.iptr(nat.i32(C.indices)),
+                    nat.dptr(nat.f64(C.data)), nat.dptr(nat.f64(schur.Sinv)))
+
+    def _upload_surface(self):
+        o = self.ops
+        mats = []
+        for M in (o.pressure_surface, o.du_dn, o.dv_dn):
+            M = M.tocsr(); M.sort_indices()
+            mats.append((nat.i32(M.indptr), nat.i32(M.indices), nat.f64(M.data)))
+        (pp, pc, pv), (up, uc, uv), (vp, vc, vv) = mats
+        self.uref = None
+        self._surface = mats
+        self._normals = nat.f64(o.normals)
+        self._weights = nat.f64(o.normal_weights)
+
+    def set_reference_velocity(self, u_ref):
+        (pp, pc, pv), (up, uc, uv), (vp, vc, vv) = self._surface
+        self.h.call('mcf_set_surface', len(self._normals), nat.iptr(pp), nat.iptr(pc), nat.dptr(pv), nat.iptr(up),
+                    nat.iptr(uc), nat.dptr(uv), nat.iptr(vp), nat.iptr(vc), nat.dptr(vv), nat.dptr(self._normals),
+                    nat.dptr(self._weights), float(self.ops.D), float(u_ref))
+        self.uref = u_ref
+
+    def set_dt(self, dt):
+        """Build and upload the u/v Helmholtz factors for dt (optimized_ghost_cell.py:243-255)."""
+        if self.dt == dt:
+            return
+        o = self.ops
+        infos = []
+        factors = []
+        for (kind, selected, actual), code, op in zip(o.helmholtz_actual(dt), (nat.U, nat.V),
+                                                      (nat.OP_HELM_U, nat.OP_HELM_V)):
+            factor = RectangleFactor(kind, o.nx, o.ny, o.dx, o.dy, nu=o.nu, dt=dt, rho=o.rho)
+            schur = SchurFactor(actual, selected, factor)
+            self._upload_rectangle(code, factor, schur)
+            pos = np.full(factor.shape[0] * factor.shape[1], -1)
+            pos[selected] = np.arange(len(selected))
+            arows = pos[schur.rows]
+            arows = arows[arows >= 0]
+            if kind == 'u':
+                out_index = o.ui
+                col_map = lambda c: o.ui[c]
+            else:
+                out_index = o.vi
+                col_map = lambda c: o.usize + o.vi[c]
+            self._set_override(op, out_index, actual.tocsr(), arows, col_map)
+            infos.append(dict(rank=schur.rank, dense_bytes=int(schur.S.nbytes),
+                              roundoff_drop_tolerance=schur.tolerance, override_rows=len(arows)))
+            factors.append((factor, schur))
+        self.helmholtz_factors = factors
+        self.helmholtz_inverse_info = infos
+        self.dt = dt
+
+    # ---------------------------------------------------------------- state
+    def vel_from(self, u, v):
+        return np.concatenate((np.ascontiguousarray(u, dtype=float).ravel(), np.ascontiguousarray(v, dtype=float).ravel()))
+
+    def split_vel(self, vel):
+        o = self.ops
+        return vel[:o.usize].reshape(o.ushape), vel[o.usize:].reshape(o.vshape)
+
+    def q_to_vel(self, q):
+        o = self.ops
+        vel = np.zeros(o.usize + o.vsize)
+        vel[o.ui] = q[:o.nu_act]
+        vel[o.usize + o.vi] = q[o.nu_act:]
+        return vel
+
+    def vel_to_q(self, vel):
+        o = self.ops
+        return np.concatenate((vel[o.ui], vel[o.usize + o.vi]))
+
+    def p_to_q(self, p):
+        return np.ascontiguousarray(p).ravel()[self.ops.pi]
+
+    def q_to_p(self, q):
+        o = self.ops
+        p = np.zeros(o.nx * o.ny)
+        p[o.pi] = q
+        return p
+
+    def set_state(self, u, v, p, previous_advection=None, half_pressure=None, previous_dt=None, time=0.):
+        vel = nat.f64(self.vel_from(u, v))
+        pp = nat.f64(np.ascontiguousarray(p, dtype=float).ravel())
+        prev = nat.f64(self.q_to_vel(previous_advection)) if previous_advection is not None else None
+        half = nat.f64(self.q_to_p(half_pressure)) if half_pressure is not None else None
+        self.h.call('mcf_set_state', nat.dptr(vel), nat.dptr(pp), nat.dptr(prev), nat.dptr(half),
+                    float(previous_dt or 0.), float(time))
+
+    def get_state(self):
+        o = self.ops
+        vel = np.empty(o.usize + o.vsize)
+        p = np.empty(o.nx * o.ny)
+        prev = np.zeros(o.usize + o.vsize)
+        half = np.zeros(o.nx * o.ny)
+        prev_dt = np.zeros(1); t = np.zeros(1); hist = np.zeros(1, dtype=np.int32)
+        self.h.call('mcf_get_state', nat.dptr(vel), nat.dptr(p), nat.dptr(prev), nat.dptr(half), nat.dptr(prev_dt),
+                    nat.dptr(t), nat.iptr(hist))
+        u, v = self.split_vel(vel)
+        has = bool(hist[0])
+        return dict(u=u.copy(), v=v.copy(), p=p.reshape(o.pshape),
+                    previous_advection=self.vel_to_q(prev) if has else None,
+                    half_pressure=self.p_to_q(half.reshape(o.pshape)) if has else None,
+                    previous_dt=float(prev_dt[0]) if has else None, time=float(t[0]))
+
+    # ---------------------------------------------------------------- operations
+    def run(self, n, dt):
+        self.set_dt(dt)
+        diag = np.zeros((n, nat.DIAG_LEN))
+        self.h.call('mcf_run', int(n), float(dt), nat.dptr(diag))
+        return diag
+
+    def project(self, q, dt):
+        """q (packed) -> (projected q, phi (packed)); optimized_ghost_cell.py:218-229."""
+        o = self.ops
+        vel = nat.f64(self.q_to_vel(q))
+        out = np.empty_like(vel)
+        phi = np.empty(o.nx * o.ny)
+        info = np.zeros(4)
+        self.h.call('mcf_project', nat.dptr(vel), float(dt), nat.dptr(out), nat.dptr(phi), nat.dptr(info))
+        return self.vel_to_q(out), self.p_to_q(phi), info
+
+    def extend(self, q):
+        vel = nat.f64(self.q_to_vel(q))
+        out = np.empty_like(vel)
+        self.h.call('mcf_extend', nat.dptr(vel), nat.dptr(out))
+        return self.split_vel(out)
+
+    def forces(self, u, v, p, mu):
+        vel = nat.f64(self.vel_from(u, v))
+        pp = nat.f64(np.ascontiguousarray(p, dtype=float).ravel())
+        out = np.zeros(6)
+        self.h.call('mcf_forces', nat.dptr(vel), nat.dptr(pp), float(mu), nat.dptr(out))
+        return tuple(out)
+
+    def rect_inverse(self, kind, rhs):
+        rhs = nat.f64(rhs)
+        out = np.empty_like(rhs)
+        self.h.call('mcf_rect_inverse', kind, nat.dptr(rhs), nat.dptr(out))
+        return out
+
+    def schur_solve(self, kind, rhs):
+        rhs = nat.f64(rhs)
+        out = np.empty_like(rhs)
+        info = np.zeros(4)
+        self.h.call('mcf_schur_solve', kind, nat.dptr(rhs), nat.dptr(out), nat.dptr(info))
+        return out, info
+
+    def divergence_stats(self, u, v, mask):
+        vel = nat.f64(self.vel_from(u, v))
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        out = np.zeros(5)
+        self.h.call('mcf_divergence_stats', nat.dptr(vel), m.ctypes.data_as(nat._U8), nat.dptr(out))
+        return out[0], out[1], int(round(out[2])), out[3], int(round(out[4]))
+
+    def profile(self, n, dt):
+        self.set_dt(dt)
+        out = np.zeros(5)
+        self.h.call('mcf_profile', int(n), float(dt), nat.dptr(out))
+        return out

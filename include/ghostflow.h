@@ -1,0 +1,132 @@
+/*
+ * ghostflow.h -- C ABI of the MI355X ghost-cell AB2/CN cylinder step (libghostflow.so).
+ *
+ * One handle owns the device state of one channel/cylinder case: the staggered
+ * fields, the AB2/CN history, the ghost and operator tables, the separable
+ * rectangle factors and the boundary-row (Schur) capacitance inverses.  All
+ * arguments are plain host pointers and sizes; arrays are FP64 (double) or
+ * int32, row-major, in the reference's staggered shapes:
+ *     u (ny, nx+1), v (ny+1, nx), p (ny, nx).
+ * Velocity arrays handed to the library are the concatenation [u.ravel(), v.ravel()]
+ * ("vel layout", length ny*(nx+1) + (ny+1)*nx).
+ *
+ * Status codes: 0 ok, 1 invalid argument (ValueError), 2 solve/gate failure
+ * (RuntimeError), 3 non-finite result (FloatingPointError), 4 HIP error.
+ * A handle is not thread-safe; calls on one handle are serialised on its stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   mcf_step            GhostCellIntegrator.advance            solver/optimized_ghost_cell.py:236-278
+ *                       + per-step body of MAC_CylinderFlow.solve  solver/main.py:271-434
+ *   mcf_run             the step loop of MAC_CylinderFlow.solve  solver/main.py:271-434
+ *   mcf_project         GhostCellIntegrator.project             solver/optimized_ghost_cell.py:218-229
+ *   mcf_forces          GhostCellIntegrator.surface_forces      solver/optimized_ghost_cell.py:280-286
+ *   mcf_rect_inverse    rectangle(...)[1] (the separable solve) solver/rectangular_inverse.py:48-53
+ *   mcf_schur_solve     boundary_row_inverse(...)[0] / monitored_pressure_solver(...)[0]
+ *                                                               solver/pressure_schur.py:16-72, 103-124
+ *   mcf_divergence_stats calculate_mac_divergence_cpp           solver/src/bindings.cpp:343-390
+ *   mcf_set_* / mcf_get_state   construction of GhostCellIntegrator and its history attributes
+ *                                                               solver/optimized_ghost_cell.py:62-150,
+ *                                                               solver/main.py:547-566 (checkpoint restore)
+ */
+#ifndef GHOSTFLOW_H
+#define GHOSTFLOW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mcf_handle mcf_handle;
+
+/* Rectangle / operator kinds. */
+enum { MCF_PRESSURE = 0, MCF_U = 1, MCF_V = 2 };
+/* Fields of the ghost tables. */
+enum { MCF_FIELD_U = 0, MCF_FIELD_V = 1, MCF_FIELD_P = 2 };
+/* Operators carrying override rows (rows that differ from the structured stencil). */
+enum { MCF_OP_GRAD_U = 0, MCF_OP_GRAD_V = 1, MCF_OP_DIV = 2, MCF_OP_HELM_U = 3, MCF_OP_HELM_V = 4,
+       MCF_OP_PRESSURE = 5, MCF_N_OPS = 6 };
+
+/* Per-step diagnostics written by mcf_step / mcf_run (MCF_DIAG_LEN doubles per step). */
+enum {
+  MCF_D_TIME = 0, MCF_D_DT, MCF_D_CD, MCF_D_CL, MCF_D_CD_P, MCF_D_CD_V, MCF_D_CL_P, MCF_D_CL_V,
+  MCF_D_FX, MCF_D_FY, MCF_D_MAXDIV, MCF_D_CFL, MCF_D_MOMENTUM_RES, MCF_D_COUPLED_RES,
+  MCF_D_P_RES0, MCF_D_P_RES1, MCF_D_P_REL0, MCF_D_P_REL1, MCF_D_P_REFINE, MCF_D_P_TARGET0, MCF_D_P_TARGET1,
+  MCF_D_P_ROUND0, MCF_D_P_ROUND1, MCF_D_P_BACKWARD0, MCF_D_P_BACKWARD1, MCF_D_H_REFINE,
+  MCF_D_STATUS, MCF_D_RHS_MAX, MCF_D_P_SOLVES,
+  MCF_DIAG_LEN = 32
+};
+
+/* Creation: grid, physics and the inlet profile (u at x=0, length ny). */
+int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_handle** out);
+void mcf_destroy(mcf_handle* h);
+const char* mcf_last_error(const mcf_handle* h);
+int mcf_device_name(char* buf, int len);
+
+int mcf_set_inlet(mcf_handle* h, const double* u_inlet);
+/* Rows of the solid interval per array row: lo[j] <= i <= hi[j] are solid (hi < lo: none).
+   u: ny rows over nx+1 columns; v: ny+1 rows over nx columns; p: ny rows over nx columns. */
+int mcf_set_solid_intervals(mcf_handle* h, int field, const int* lo, const int* hi);
+/* Ghost extension rows (optimized_ghost_cell.py:14-58): node[k] = flat node index of ghost k,
+   ptr[n+1], col[] = flat index of an active node of the same field, w[] = weight. */
+int mcf_set_ghost(mcf_handle* h, int field, int n, const int* node, const int* ptr, const int* col,
+                  const double* w);
+/* Override rows of an operator: row[k] = flat output index, CSR over the input array
+   (vel layout for DIV/HELM, p for GRAD/PRESSURE).  Box = bounding box of the rows. */
+int mcf_set_override(mcf_handle* h, int op, int n, const int* row, const int* ptr, const int* col,
+                     const double* val);
+/* Separable rectangle factor: operator = a I + b (lx + ly); V (yy*yy) orthonormal y-eigenbasis,
+   lam (yy) its eigenvalues, lx given by lower/diag/upper (xx-1, xx, xx-1). */
+int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double b, const double* V,
+                      const double* lam, const double* lower, const double* diag, const double* upper);
+/* Boundary-row capacitance: rows (r, rectangle flat indices), points (npts, rectangle flat
+   indices of C's columns), C as CSR over points (r rows), Sinv dense r*r. */
+int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, const int* points,
+                  const int* c_ptr, const int* c_col, const double* c_val, const double* Sinv);
+/* Surface quadrature: n points; pressure stencil CSR over p, du/dn over u, dv/dn over v;
+   normals (n*2: nx, ny) and weights (n*2). */
+int mcf_set_surface(mcf_handle* h, int n, const int* pp, const int* pc, const double* pv,
+                    const int* up, const int* uc, const double* uv, const int* vp, const int* vc,
+                    const double* vv, const double* normals, const double* normal_weights,
+                    double diameter, double u_ref);
+
+/* State: u/v in vel layout (extended arrays, ghost values included), p (ny*nx, full array),
+   history: prev_adv (vel layout q-array, zeros off the active faces; NULL = none),
+   half_p (ny*nx q-array; NULL = none), prev_dt. */
+int mcf_set_state(mcf_handle* h, const double* vel, const double* p, const double* prev_adv,
+                  const double* half_p, double prev_dt, double time);
+int mcf_get_state(mcf_handle* h, double* vel, double* p, double* prev_adv, double* half_p,
+                  double* prev_dt, double* time, int* has_history);
+
+/* One step; diag receives MCF_DIAG_LEN doubles (may be NULL). */
+int mcf_step(mcf_handle* h, double dt, double* diag);
+/* n steps at a fixed dt; diag receives n*MCF_DIAG_LEN doubles (may be NULL). */
+int mcf_run(mcf_handle* h, int n, double dt, double* diag);
+
+/* Projection of a vel-layout q-array (zeros off the active faces): out = q - dt G phi. */
+int mcf_project(mcf_handle* h, const double* q, double dt, double* out, double* phi, double* info4);
+/* Ghost-fill a vel-layout q-array in place (unpack: E q + offset). */
+int mcf_extend(mcf_handle* h, const double* q, double* out);
+/* Surface forces (Fx, Fy, Fx_p, Fy_p, Fx_v, Fy_v) of the given full arrays. */
+int mcf_forces(mcf_handle* h, const double* vel, const double* p, double mu, double* out6);
+/* Separable rectangle inverse of kind on a (yy, xx) right-hand side. */
+int mcf_rect_inverse(mcf_handle* h, int kind, const double* rhs, double* out);
+/* Boundary-row corrected solve of the embedded operator (values on the active nodes given as
+   a q-array in the kind's full array layout); info4 = residual_linf, refinements, roundoff, target. */
+int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, double* info4);
+/* Operators on packed-equivalent arrays (optimized_ghost_cell.py:186-216, 231-234):
+   ADVECTION -N(q), DIFFUSION L q + diff_offset (vel -> vel), GRADIENT G p (p -> vel),
+   DIVERGENCE D q + div_offset (vel -> p), HMATVEC q - dt/2 L q (vel -> vel). */
+enum { MCF_APPLY_ADVECTION = 0, MCF_APPLY_DIFFUSION = 1, MCF_APPLY_GRADIENT = 2, MCF_APPLY_DIVERGENCE = 3,
+       MCF_APPLY_HMATVEC = 4 };
+int mcf_apply_operator(mcf_handle* h, int op, const double* in, double* out, double dt);
+/* max|div|, sum div^2, count, core sum div^2, core count over cells with mask==0. */
+int mcf_divergence_stats(mcf_handle* h, const double* vel, const unsigned char* mask, double* out5);
+
+/* Time the dominant kernels: average ms per launch of the dense y transform, the x tridiagonal
+   sweep and the whole step over n steps at dt (state advances). out: [gemm_ms, pcr_ms, step_ms,
+   gemm_launches_per_step, pcr_launches_per_step]. */
+int mcf_profile(mcf_handle* h, int n, double dt, double* out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,125 @@
+"""GPU parity: libghostflow.so (through the C ABI) against the CPU oracle on the same inputs."""
+import numpy as np
+import pytest
+
+from oracle import ghost_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CH = dict(nx=88, ny=16, Lx=22., Ly=4.1, xc=2., yc=2.)
+CFG1 = dict(nx=440, ny=82, Lx=22., Ly=4.1, xc=2., yc=2., nu=0.05, inflow='parabolic', U_m=1.5)
+
+
+def rel(a, b):
+    return float(np.max(abs(np.asarray(a) - np.asarray(b))) / max(np.max(abs(np.asarray(b))), 1e-300))
+
+
+@pytest.fixture(scope='module')
+def pair88(pkg):
+    print('case 88x16 channel', flush=True)
+    ref = O.CylinderFlow(**CH)
+    gpu = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CH)
+    return ref, gpu
+
+
+def test_initial_projection(pair88):
+    ref, gpu = pair88
+    assert rel(gpu.u, ref.u) < 1e-12 and rel(gpu.v, ref.v) < 1e-12
+
+
+@pytest.mark.parametrize('kind', ['pressure', 'u', 'v'])
+def test_rectangle_inverse(pair88, kind):
+    ref, gpu = pair88
+    dev = gpu.gg.device
+    dev.set_dt(1e-3)
+    code = {'pressure': 0, 'u': 1, 'v': 2}[kind]
+    factor = dev.pressure_factor if kind == 'pressure' else dev.helmholtz_factors[code - 1][0]
+    b = np.random.default_rng(3).normal(size=factor.shape)
+    x = dev.rect_inverse(code, b)
+    _, inv, _ = O.rectangle(gpu.nx, gpu.ny, gpu.dx, gpu.dy, kind, gpu.nu, 1e-3, gpu.rho)
+    assert rel(x, inv(b)) < 1e-12
+    # residual self-check of rectangular_inverse.py:54-57
+    assert np.max(abs(factor.matrix @ x.ravel() - b.ravel())) < 1e-10
+
+
+def test_pressure_schur_solve(pair88):
+    ref, gpu = pair88
+    g = ref.gg
+    vals = np.random.default_rng(4).normal(size=len(g.pi))
+    xr = g.pressure_solve(vals)
+    rhs = np.zeros(gpu.nx * gpu.ny); rhs[g.pi] = vals
+    x, info = gpu.gg.device.schur_solve(0, rhs)
+    assert rel(x[g.pi], xr) < 1e-11
+    assert np.max(abs(g.A @ x[g.pi] - vals)) <= info[3]
+    covered = np.ones(gpu.nx * gpu.ny, bool); covered[g.pi] = False
+    assert np.all(x[covered] == 0.)
+
+
+def test_operators(pair88):
+    ref, gpu = pair88
+    g, d = ref.gg, gpu.gg
+    rng = np.random.default_rng(6)
+    q = rng.normal(size=g.qsize)
+    assert rel(d.advection(q), g.advection(q)) < 1e-12
+    assert rel(d.diffusion(q), g.diffusion(q)) < 1e-12
+    assert rel(d.divergence(q), g.divergence(q)) < 1e-12
+    assert rel(d.H_matvec(q, 1e-3), g.H_matvec(q, 1e-3)) < 1e-12
+    ph = rng.normal(size=len(g.pi))
+    assert rel(d.gradient(ph), g.gradient(ph)) < 1e-12
+    qp, phi = d.project(q, .003)
+    qr, phr = g.project(q, .003)
+    assert rel(qp, qr) < 1e-11 and rel(phi, phr) < 1e-10
+    assert np.max(abs(g.divergence(qp))) < 1e-10
+
+
+def test_steps_88x16(pair88):
+    """12 steps vs the oracle (the reference's compiled/pure parity uses 1e-10, test_numba_ghost_cell.py:72-84)."""
+    ref, gpu = pair88
+    ref = O.CylinderFlow(**CH)
+    t, cd, cl = gpu.solve(n_steps=12, dt=1e-3, perturb_transient=False, save_artifacts=False, verbose=False)
+    for _ in range(12):
+        ref.step(1e-3)
+    for key in ('u', 'v', 'p'):
+        assert rel(getattr(gpu, key), getattr(ref, key)) < 1e-11, key
+    dr = ref.last_step_diagnostics
+    assert abs(cd[-1] - dr['cd']) <= 1e-10 * abs(dr['cd'])
+    assert abs(cl[-1] - dr['cl']) <= 1e-10 * abs(dr['cl'])
+    assert gpu.last_step_diagnostics['max_divergence'] < 1e-12
+
+
+def test_surface_forces_closed_form(pkg):
+    """p = x gives F_x,p = -pi R^2 on the device too (tests/test_numba_ghost_cell.py:49-62)."""
+    s = pkg.MAC_CylinderFlow(nx=80, ny=60, Lx=4., Ly=3., xc=1.5, yc=1.5, boundary='ghost_cell',
+                             time_scheme='ab2_cn', verbose=False)
+    gc = s.gc
+    p = np.broadcast_to(gc.xp, s.p.shape).copy()
+    f = s.gg.surface_forces(np.zeros_like(s.u), np.zeros_like(s.v), p, s.nu * s.rho)
+    assert abs(f[2] + np.pi * (s.D / 2)**2) < 1e-10 and abs(f[3]) < 1e-10
+
+
+def test_config1_steps(pkg):
+    """440x82 Re=20 channel from the zero field: 20 steps, fields 1e-11 and forces 1e-10 relative."""
+    print('case 440x82 Re=20, 20 steps', flush=True)
+    ref = O.CylinderFlow(initial='zero', **CFG1)
+    gpu = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='zero', **CFG1)
+    t, cd, cl = gpu.solve(n_steps=20, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
+    cdr, clr = [], []
+    for _ in range(20):
+        ref.step(2e-3)
+        cdr.append(ref.last_step_diagnostics['cd']); clr.append(ref.last_step_diagnostics['cl'])
+    for key in ('u', 'v', 'p'):
+        assert rel(getattr(gpu, key), getattr(ref, key)) < 1e-11, key
+    assert np.max(abs(cd - cdr) / abs(np.array(cdr))) < 1e-10
+    assert np.max(abs(cl - clr)) < 1e-10 * np.max(abs(np.array(clr)))
+    assert gpu.last_step_diagnostics['max_divergence'] < 1e-12
+
+
+def test_divergence_stats(pkg, pair88):
+    ref, gpu = pair88
+    rng = np.random.default_rng(8)
+    u = rng.normal(size=gpu.u.shape); v = rng.normal(size=gpu.v.shape)
+    mine = gpu.gg.device.divergence_stats(u, v, gpu.mask)
+    want = O.divergence_stats(u, v, gpu.mask, gpu.dx, gpu.dy)
+    assert mine[2] == want[2] and mine[4] == want[4]
+    assert abs(mine[0] - want[0]) <= 1e-15 * want[0]
+    assert abs(mine[1] - want[1]) <= 1e-12 * want[1]

@@ -1,0 +1,80 @@
+"""The product's host tables (vectorised builder) against the oracle's loop restatement:
+integer index tables bit-exact, operator values bit-exact (same IEEE expressions), and the
+eigenbasis rectangle inverse / capacitance matrix within round-off of the SciPy DCT path."""
+import numpy as np
+import pytest
+
+from oracle import ghost_oracle as O
+
+CASES = [dict(nx=88, ny=16, Lx=22., Ly=4.1, D=1., xc=2., yc=2., nu=0.01),
+         dict(nx=40, ny=30, Lx=4., Ly=3., D=1., xc=1.5, yc=1.5, nu=0.01),
+         dict(nx=440, ny=82, Lx=22., Ly=4.1, D=1., xc=2., yc=2., nu=0.05, inflow='parabolic', U_m=1.5)]
+
+
+def build(pkg, case, interpolation='bilinear'):
+    ref = O.CylinderFlow(ghost_velocity_interpolation=interpolation, **case)
+    ops = pkg.GhostOperators(case['nx'], case['ny'], case['Lx'], case['Ly'], case['D'], case['xc'], case['yc'],
+                             case['nu'], 1., ref.u_inlet_base, interpolation=interpolation)
+    return ref, ops
+
+
+def same_csr(a, b):
+    a = a.tocsr(); b = b.tocsr()
+    a.sort_indices(); b.sort_indices()
+    return (a.shape == b.shape and np.array_equal(a.indptr, b.indptr) and np.array_equal(a.indices, b.indices)
+            and np.array_equal(a.data, b.data))
+
+
+@pytest.mark.parametrize('case', CASES, ids=['88x16', '40x30', '440x82'])
+def test_operator_tables_bit_exact(pkg, case):
+    ref, ops = build(pkg, case)
+    g = ref.gg
+    for name in ('ui', 'vi', 'pi', 'outlet_u'):
+        assert np.array_equal(getattr(ops, name), getattr(g, name)), name
+    pairs = dict(E=g.E, P=g.P, Dop=g.D, G=g.G, A=g.A, L=g.L, pressure_surface=g.pressure_surface,
+                 du_dn=g.du_dn, dv_dn=g.dv_dn)
+    for name, other in pairs.items():
+        assert same_csr(getattr(ops, name), other), name
+    assert np.array_equal(ops.div_offset, g.div_offset) and np.array_equal(ops.diff_offset, g.diff_offset)
+    assert np.array_equal(ops.normal_weights, g.normal_weights)
+
+
+def test_ghost_probe_records_match(pkg):
+    """Ghost nodes, probe growth s and tries of every ghost node equal the oracle's loop."""
+    case = CASES[2]
+    ref, ops = build(pkg, case)
+    gc = ref.gc
+    active_u = gc.dist_u > 1e-13
+    active_u[:, 0] = False
+    _, _, probes = O.extension(gc.xu, gc.yu, gc.sdf, active_u, return_probes=True)
+    info = ops.ghost_u
+    assert np.array_equal(info['ghost_nodes'], [p['node'] for p in probes])
+    assert np.array_equal(info['s'], [p['s'] for p in probes])
+    assert np.array_equal(info['tries'], [p['tries'] for p in probes])
+
+
+def test_biquadratic_tables_bit_exact(pkg):
+    ref, ops = build(pkg, CASES[1], interpolation='biquadratic')
+    assert same_csr(ops.E, ref.gg.E) and same_csr(ops.L, ref.gg.L)
+
+
+@pytest.mark.parametrize('kind', ['pressure', 'u', 'v'])
+def test_eigenbasis_rectangle_inverse(pkg, kind):
+    nx, ny, dx, dy = 440, 82, 22 / 440, 4.1 / 82
+    f = pkg.RectangleFactor(kind, nx, ny, dx, dy, nu=.05, dt=.002)
+    rect, inv, shape = O.rectangle(nx, ny, dx, dy, kind, .05, .002)
+    assert abs(rect - f.matrix).max() == 0
+    b = np.random.default_rng(4).normal(size=shape)
+    x = f.solve(b)
+    assert np.max(abs(x - inv(b))) < 1e-13 * np.max(abs(x))
+    V = f.V
+    assert np.max(abs(V.T @ V - np.eye(V.shape[0]))) < 1e-13
+
+
+def test_capacitance_rows_and_matrix(pkg):
+    ref, ops = build(pkg, CASES[2])
+    factor = pkg.RectangleFactor('pressure', ops.nx, ops.ny, ops.dx, ops.dy)
+    schur = pkg.SchurFactor(ops.A, ops.pi, factor)
+    info = ref.gg.pressure_inverse_info
+    assert schur.rank == info['rank'] and np.array_equal(schur.rows, info['rows'])
+    assert np.max(abs(schur.S - info['schur'])) < 1e-12 * np.max(abs(info['schur']))
