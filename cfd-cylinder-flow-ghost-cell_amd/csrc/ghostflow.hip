@@ -138,22 +138,25 @@ DEV bool pred_refine(const u64* sl, int slot0, int res_slot) {
   return res > woodbury_target(val) || res != res;
 }
 
-DEV double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+// max that propagates NaN (a NaN anywhere must fail the gates that read the maximum)
+DEV double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+DEV double wave_max_abs(double v) {
+  v = fabs(v);
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 DEV double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
-// Block-wide max of up to 4 values; thread 0 returns them. blockDim <= 1024.
+// Block-wide max |value| of up to 4 values, one atomic per block and value. blockDim <= 1024.
 template <int N>
 DEV void block_max_atomic(double (&v)[N], u64* const (&dst)[N]) {
   __shared__ double red[16][4];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   #pragma unroll
   for (int k = 0; k < N; ++k) {
-    double m = wave_max(v[k]);
+    double m = wave_max_abs(v[k]);
     if (lane == 0) red[w][k] = m;
   }
   __syncthreads();
@@ -162,7 +165,7 @@ DEV void block_max_atomic(double (&v)[N], u64* const (&dst)[N]) {
     for (int k = 0; k < N; ++k) {
       if (!dst[k]) continue;
       double m = 0.0;
-      for (int q = 0; q < nw; ++q) m = (red[q][k] > m || red[q][k] != red[q][k]) ? red[q][k] : m;
+      for (int q = 0; q < nw; ++q) m = nanmax(red[q][k], m);
       if (m != 0.0 || m != m) atomic_max_abs(dst[k], m);
     }
   }

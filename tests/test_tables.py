@@ -78,3 +78,24 @@ def test_capacitance_rows_and_matrix(pkg):
     info = ref.gg.pressure_inverse_info
     assert schur.rank == info['rank'] and np.array_equal(schur.rows, info['rows'])
     assert np.max(abs(schur.S - info['schur'])) < 1e-12 * np.max(abs(info['schur']))
+
+
+def test_force_floor_of_exact_inverses(pkg, monkeypatch):
+    """Two exact FP64 inverses of the same rectangle (SciPy DCT vs eigenbasis + banded) change the
+    impulsive-start C_L of the 440x82 case by ~1e-10 absolute while C_D ~ 36: the parity floor for
+    forces is relative to the force magnitude, not to |C_L|."""
+    cfg = dict(nx=440, ny=82, Lx=22., Ly=4.1, xc=2., yc=2., nu=0.05, inflow='parabolic', U_m=1.5)
+    a = O.CylinderFlow(initial='zero', **cfg)
+    orig = O.rectangle
+
+    def eigen_rectangle(nx, ny, dx, dy, kind, nu=0., dt=0., rho=1., workers=None):
+        m, _, shape = orig(nx, ny, dx, dy, kind, nu, dt, rho, workers)
+        return m, pkg.RectangleFactor(kind, nx, ny, dx, dy, nu, dt, rho).solve, shape
+    monkeypatch.setattr(O, 'rectangle', eigen_rectangle)
+    b = O.CylinderFlow(initial='zero', **cfg)
+    monkeypatch.setattr(O, 'rectangle', orig)
+    a.step(2e-3); b.step(2e-3)
+    da, db = a.last_step_diagnostics, b.last_step_diagnostics
+    dcl = abs(da['cl'] - db['cl'])
+    assert 1e-12 < dcl < 1e-10 * abs(da['cd'])
+    assert np.max(abs(a.u - b.u)) < 1e-12 * np.max(abs(a.u))
