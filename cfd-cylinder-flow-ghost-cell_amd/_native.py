@@ -44,7 +44,7 @@ _SIGNATURES = {
     'mcf_set_override': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, _I, _I, _D]),
     'mcf_set_rectangle': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, _D, _D, _D, _D, _D]),
-    'mcf_set_schur': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I, _I, _I, _D, _D]),
+    'mcf_set_schur': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I, _D]),
     'mcf_set_surface': (ctypes.c_int, [_H, ctypes.c_int, _I, _I, _D, _I, _I, _D, _I, _I, _D, _D, _D,
                                        ctypes.c_double, ctypes.c_double]),
     'mcf_set_state': (ctypes.c_int, [_H, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double]),

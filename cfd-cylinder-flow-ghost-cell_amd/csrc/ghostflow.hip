@@ -24,6 +24,7 @@
 #include <vector>
 #include <map>
 #include <algorithm>
+#include <cstdlib>
 #include "ghostflow.h"
 
 #define DEV __device__ __forceinline__
@@ -52,9 +53,13 @@ enum Slot {
   SL_UMAX = 0, SL_VMAX, SL_DIV0, SL_DIV1, SL_DIV2, SL_COUPLED, SL_NONFINITE, SL_SPARE,
   SL_HU = 8, SL_HV = 16,             // Helmholtz u / v solve blocks
   SL_P0 = 24, SL_P1 = 32, SL_P2 = 40, // pressure solve blocks (projection 1, 2, startup)
-  SL_FORCE = 48,                     // 6 doubles (written by one block)
+  SL_FORCE = 48,                     // 6 doubles (written by one block, stripe 0)
   NSLOT = 64
 };
+// Every slot is striped over NSTRIPE words so that the blocks of a reduction hit different
+// addresses (same-address L2 atomics serialise); readers reduce over the stripes.
+constexpr int NSTRIPE = 64;
+constexpr int NSLOTW = NSLOT * NSTRIPE;
 
 struct Override {
   int j0, i0, bh, bw;
@@ -65,11 +70,10 @@ struct Override {
 };
 
 struct GhostTab {
-  int n;
+  int n, width;       // rows, entries per row (padded with zero weights)
   const int* node;
-  const int* ptr;
-  const int* col;
-  const double* w;
+  const int* col;     // n*width
+  const double* w;    // n*width
 };
 
 struct CaseDev {
@@ -78,6 +82,7 @@ struct CaseDev {
   const double* inlet;
   const int* lo[3];
   const int* hi[3];
+  int sbox[3][4];   // bounding box (j0, j1, i0, i1) of the solid nodes of each field
   Override ov[MCF_N_OPS];
   GhostTab gt[3];
   double* buf[NBUF];
@@ -87,7 +92,10 @@ struct CaseDev {
   const int *pp, *pc, *up, *uc, *vp, *vc;
   const double *pv, *uv, *vv, *nrm, *nw;
   double diameter, uref;
+  double* fpart; // per-wave partial force sums (4 per wave)
   double* diag;  // diag ring
+  int* clock_step;     // row of the next diagnostics entry
+  double* clock_time;  // simulated time of the last completed step
 };
 
 // One separable solve problem (one rectangle of one case)
@@ -103,21 +111,21 @@ struct SolveDesc {
   const double* lower;  // xx-1
   const double* diag;   // xx
   const double* upper;  // xx-1
+  const double* luL;    // yy*xx: sub-diagonal multipliers of the LU of each mode's x system
+  const double* luI;    // yy*xx: inverse pivots
   double* F;            // yy*xx scratch
   // capacitance
-  int r, npts, ngroups;
-  const int* grp_ptr;   // ngroups+1 (points grouped by x)
-  const int* grp_x;     // ngroups
-  const int* pt_j;      // npts (group order)
+  int r, npts;
+  const int* pt_j;      // npts (ordered by x column, then y)
+  const int* pt_x;      // npts (group order)
   double* firstK;       // npts
-  const int* c_ptr;     // r+1, CSR of C over points (group order)
-  const int* c_col;
-  const double* c_val;
-  const double* Sinv;   // r*r
+  const double* SC;     // r*npts: S^{-1} C over the points (group order)
   double* y;            // r
-  const int* rc_ptr;    // xx+1: rows grouped by x
-  const int* rc_j;      // r
-  const int* rc_id;     // r
+  int rg_n;             // number of x columns holding capacitance rows
+  const int* rg_ptr;    // rg_n+1 into rc_j / rc_id
+  const int* rg_x;      // rg_n
+  const int* rc_j;      // r: y index of each row (grouped by x)
+  const int* rc_id;     // r: index into y
   const int* lo;        // solid intervals of the array rows
   const int* hi;
   double* const* bufs;  // the case's buffer table (device)
@@ -126,17 +134,13 @@ struct SolveDesc {
 };
 
 DEV double slot_d(const u64* s, int k) { return __longlong_as_double((long long)s[k]); }
+DEV int stripe_id() { return (blockIdx.x + 7 * blockIdx.y) & (NSTRIPE - 1); }
 DEV void atomic_max_abs(u64* slot, double v) {
   double a = fabs(v);
   u64 bits = (u64)__double_as_longlong(a);
   atomicMax(slot, bits);  // non-negative doubles order like their bit patterns; NaN sorts above inf
 }
 DEV double woodbury_target(double vmax) { return fmax(1e-10, 32.0 * kEps * fmax(1.0, vmax)); }
-DEV bool pred_refine(const u64* sl, int slot0, int res_slot) {
-  double res = slot_d(sl, slot0 + res_slot);
-  double val = slot_d(sl, slot0 + SV_VAL);
-  return res > woodbury_target(val) || res != res;
-}
 
 // max that propagates NaN (a NaN anywhere must fail the gates that read the maximum)
 DEV double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
@@ -148,6 +152,20 @@ DEV double wave_max_abs(double v) {
 DEV double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+// Slot value (max over stripes, or sum); must be called by all 64 lanes of a wave.
+DEV double slot_max(const u64* sl, int k) {
+  const int lane = threadIdx.x & 63;
+  return wave_max_abs(__longlong_as_double((long long)sl[k * NSTRIPE + lane]));
+}
+DEV double slot_sum(const u64* sl, int k) {
+  const int lane = threadIdx.x & 63;
+  return wave_sum(__longlong_as_double((long long)sl[k * NSTRIPE + lane]));
+}
+DEV bool pred_refine(const u64* sl, int slot0, int res_slot) {
+  const double res = slot_max(sl, slot0 + res_slot);
+  const double val = slot_max(sl, slot0 + SV_VAL);
+  return res > woodbury_target(val) || res != res;
 }
 // Block-wide max |value| of up to 4 values, one atomic per block and value. blockDim <= 1024.
 template <int N>
@@ -166,7 +184,7 @@ DEV void block_max_atomic(double (&v)[N], u64* const (&dst)[N]) {
       if (!dst[k]) continue;
       double m = 0.0;
       for (int q = 0; q < nw; ++q) m = nanmax(red[q][k], m);
-      if (m != 0.0 || m != m) atomic_max_abs(dst[k], m);
+      if (m != 0.0 || m != m) atomic_max_abs(dst[k] + stripe_id(), m);
     }
   }
 }
@@ -179,11 +197,16 @@ DEV void block_sum_atomic(double v, double* dst) {
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int q = 0; q < nw; ++q) s += reds[q];
-    if (s != 0.0) atomicAdd(dst, s);
+    if (s != 0.0) unsafeAtomicAdd(dst + stripe_id(), s);
   }
 }
 
-DEV bool solid(const CaseDev& c, int field, int j, int i) { return c.lo[field][j] <= i && i <= c.hi[field][j]; }
+// Solid test: only nodes inside the solid's bounding box read the per-row interval table
+DEV bool solid(const CaseDev& c, int field, int j, int i) {
+  const int* b = c.sbox[field];
+  if (j < b[0] || j > b[1] || i < b[2] || i > b[3]) return false;
+  return c.lo[field][j] <= i && i <= c.hi[field][j];
+}
 DEV bool u_act(const CaseDev& c, int j, int i) { return i >= 1 && !solid(c, 0, j, i); }
 DEV bool v_act(const CaseDev& c, int j, int i) { return j >= 1 && j <= c.ny - 1 && !solid(c, 1, j, i); }
 DEV bool p_act(const CaseDev& c, int j, int i) { return !solid(c, 2, j, i); }
@@ -210,9 +233,13 @@ DEV void ov_dot_abs(const Override& o, int r, const double* x, double& acc, doub
 }
 
 // ------------------------------------------------------------------------------------
+// Element-wise and stencil kernels run grid-stride over a bounded grid (GSB blocks per case):
+// fewer, fuller blocks keep the dispatch and per-block reduction cost small.
+// ------------------------------------------------------------------------------------
+constexpr int GSB = 4096;
+
 // Stage 1: advection (flux form), AB2, diffusion and the CN right-hand side
 // (numba_kernels.py:17-142, 596-616; time_integration.py:30-93)
-// ------------------------------------------------------------------------------------
 struct AdvArgs {
   int q_id, adv_id, p_id, prev_id, now_id, out_id;
   int mode;  // 0: AB2 rhs, 1: projected-midpoint, 2: rhs with explicit = adv(adv_id), 3: out = L q + offset
@@ -222,22 +249,22 @@ struct AdvArgs {
 __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const double* Q = c.buf[a.q_id];
+  const double* A = c.buf[a.adv_id];
+  const double* Pp = c.buf[a.p_id];
+  const double* prev = a.prev_id >= 0 ? c.buf[a.prev_id] : nullptr;
+  double* now = a.now_id >= 0 ? c.buf[a.now_id] : nullptr;
+  double* out = c.buf[a.out_id];
+  const double dx = c.dx, dy = c.dy, idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy);
+  const double* in = c.inlet;
+  const double* Av = A + c.usize;
+  const double* Qv = Q + c.usize;
   double mu = 0.0, mv = 0.0;
-  if (blockIdx.x * blockDim.x < c.velsize) {
-    const double* Q = c.buf[a.q_id];
-    const double* A = c.buf[a.adv_id];
-    const double* Pp = c.buf[a.p_id];
-    const double* prev = a.prev_id >= 0 ? c.buf[a.prev_id] : nullptr;
-    double* now = a.now_id >= 0 ? c.buf[a.now_id] : nullptr;
-    double* out = c.buf[a.out_id];
-    const double dx = c.dx, dy = c.dy, idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy);
-    const double* in = c.inlet;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
+    double res = 0.0, nw = 0.0;
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
-      double res = 0.0, nw = 0.0;
       if (u_act(c, j, i)) {
-        const double* Av = A + c.usize;
         auto AU = [&](int jj, int ii) { return ii == 0 ? in[jj] : A[jj * nx1 + ii]; };
         auto AV = [&](int jj, int ii) { return Av[jj * nx + ii]; };
         auto QU = [&](int jj, int ii) { return ii == 0 ? in[jj] : Q[jj * nx1 + ii]; };
@@ -281,16 +308,11 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
           res = q + a.dt * (adv + 0.5 * diff - grad) + 0.5 * a.dt * doff;
         }
       }
-      if (now) now[idx] = nw;
-      out[idx] = res;
-      mu = res;
-    } else if (idx < c.velsize) {
+      mu = nanmax(mu, fabs(res));
+    } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
-      double res = 0.0, nw = 0.0;
       if (v_act(c, j, i)) {
-        const double* Av = A + c.usize;
-        const double* Qv = Q + c.usize;
         auto AU = [&](int jj, int ii) { return ii == 0 ? in[jj] : A[jj * nx1 + ii]; };
         auto AV = [&](int jj, int ii) { return Av[jj * nx + ii]; };
         auto QV = [&](int jj, int ii) { return Qv[jj * nx + ii]; };
@@ -327,21 +349,19 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
           res = q + a.dt * (adv + 0.5 * diff - grad);
         }
       }
-      if (now) now[idx] = nw;
-      out[idx] = res;
-      mv = res;
+      mv = nanmax(mv, fabs(res));
     }
+    if (now) now[idx] = nw;
+    out[idx] = res;
   }
   double v2[2] = {mu, mv};
   const bool rhs_mode = a.mode == 0 || a.mode == 2;
-  u64* const d2[2] = {rhs_mode ? c.sl + SL_HU + SV_VAL : nullptr, rhs_mode ? c.sl + SL_HV + SV_VAL : nullptr};
+  u64* const d2[2] = {rhs_mode ? c.sl + (SL_HU + SV_VAL) * NSTRIPE : nullptr, rhs_mode ? c.sl + (SL_HV + SV_VAL) * NSTRIPE : nullptr};
   block_max_atomic<2>(v2, d2);
 }
 
-// ------------------------------------------------------------------------------------
 // Helmholtz residual / defect on vel q-arrays: H x = x - dt/2 L x
 // (time_integration.py:71-74, 97-119; numba_kernels.py:634-642)
-// ------------------------------------------------------------------------------------
 struct HelmArgs {
   int b_id, x_id, out_id, out2_id;
   int mode;   // 0: residual (out = b - Hx, slot RES_k), 1: final (max only), 2: defect
@@ -353,91 +373,88 @@ struct HelmArgs {
 __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  double mu = 0.0, mv = 0.0, xu = 0.0, xv = 0.0;
   const bool run_u = a.pred_slot < 0 || pred_refine(c.sl, SL_HU, a.pred_slot);
   const bool run_v = a.pred_slot < 0 || pred_refine(c.sl, SL_HV, a.pred_slot);
-  if (blockIdx.x * blockDim.x < c.velsize) {
+  double mu = 0.0, mv = 0.0, xu = 0.0, xv = 0.0;
+  if (run_u || run_v) {
     const double* B = c.buf[a.b_id];
     const double* X = c.buf[a.x_id];
+    const double* Xv = X + c.usize;
     double* O = a.out_id >= 0 ? c.buf[a.out_id] : nullptr;
     double* O2 = a.out2_id >= 0 ? c.buf[a.out2_id] : nullptr;
     const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
     const double h = -0.5 * a.dt;
-    if (idx < c.usize && run_u) {
-      const int j = idx / nx1, i = idx - j * nx1;
+    const int lo = run_u ? 0 : c.usize, hi = run_v ? c.velsize : c.usize;
+    for (int idx = lo + blockIdx.x * blockDim.x + threadIdx.x; idx < hi; idx += gridDim.x * blockDim.x) {
       double res = 0.0, r2 = 0.0;
-      if (u_act(c, j, i)) {
-        auto XU = [&](int jj, int ii) { return X[jj * nx1 + ii]; };
-        const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);
-        double hx;
-        if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_U], r, X);
-        else {
+      if (idx < c.usize) {
+        const int j = idx / nx1, i = idx - j * nx1;
+        if (u_act(c, j, i)) {
+          auto XU = [&](int jj, int ii) { return X[jj * nx1 + ii]; };
+          const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);
+          double hx;
           const double x = XU(j, i);
-          double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * x
-                       + ((i == nx) ? 2.0 : 1.0) * idx2 * XU(j, i - 1);
-          if (i < nx) lap += idx2 * XU(j, i + 1);
-          if (j < ny - 1) lap += idy2 * XU(j + 1, i);
-          if (j > 0) lap += idy2 * XU(j - 1, i);
-          hx = x + h * (c.nu * lap);
+          if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_U], r, X);
+          else {
+            double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * x
+                         + ((i == nx) ? 2.0 : 1.0) * idx2 * XU(j, i - 1);
+            if (i < nx) lap += idx2 * XU(j, i + 1);
+            if (j < ny - 1) lap += idy2 * XU(j + 1, i);
+            if (j > 0) lap += idy2 * XU(j - 1, i);
+            hx = x + h * (c.nu * lap);
+          }
+          res = B[idx] - hx;
+          if (a.mode == 2) { r2 = x + res; xu = nanmax(xu, fabs(x)); }
         }
-        const double b = B[idx];
-        res = b - hx;
-        if (a.mode == 2) { r2 = XU(j, i) + res; xu = XU(j, i); }
-      }
-      if (a.mode == 0 || a.mode == 2) O[idx] = res;
-      if (a.mode == 2) O2[idx] = r2;
-      mu = res;
-    } else if (idx >= c.usize && idx < c.velsize && run_v) {
-      const int t = idx - c.usize;
-      const int j = t / nx, i = t - j * nx;
-      double res = 0.0, r2 = 0.0;
-      if (v_act(c, j, i)) {
-        const double* Xv = X + c.usize;
-        auto XV = [&](int jj, int ii) { return Xv[jj * nx + ii]; };
-        const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
-        double hx;
-        if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_V], r, X);
-        else {
+        mu = nanmax(mu, fabs(res));
+      } else {
+        const int t = idx - c.usize;
+        const int j = t / nx, i = t - j * nx;
+        if (v_act(c, j, i)) {
+          auto XV = [&](int jj, int ii) { return Xv[jj * nx + ii]; };
+          const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
+          double hx;
           const double x = XV(j, i);
-          double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * x
-                       + idy2 * (XV(j + 1, i) + XV(j - 1, i));
-          if (i < nx - 1) lap += idx2 * XV(j, i + 1);
-          if (i > 0) lap += idx2 * XV(j, i - 1);
-          hx = x + h * (c.nu * lap);
+          if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_V], r, X);
+          else {
+            double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * x
+                         + idy2 * (XV(j + 1, i) + XV(j - 1, i));
+            if (i < nx - 1) lap += idx2 * XV(j, i + 1);
+            if (i > 0) lap += idx2 * XV(j, i - 1);
+            hx = x + h * (c.nu * lap);
+          }
+          res = B[idx] - hx;
+          if (a.mode == 2) { r2 = x + res; xv = nanmax(xv, fabs(x)); }
         }
-        const double b = B[idx];
-        res = b - hx;
-        if (a.mode == 2) { r2 = XV(j, i) + res; xv = XV(j, i); }
+        mv = nanmax(mv, fabs(res));
       }
       if (a.mode == 0 || a.mode == 2) O[idx] = res;
       if (a.mode == 2) O2[idx] = r2;
-      mv = res;
     }
   }
   if (a.mode == 2) {
     double v2[2] = {xu, xv};
-    u64* const d2[2] = {c.sl + SL_UMAX, c.sl + SL_VMAX};
+    u64* const d2[2] = {c.sl + SL_UMAX * NSTRIPE, c.sl + SL_VMAX * NSTRIPE};
     block_max_atomic<2>(v2, d2);
   } else {
     double v2[2] = {mu, mv};
-    u64* const d2[2] = {c.sl + SL_HU + a.res_slot, c.sl + SL_HV + a.res_slot};
+    u64* const d2[2] = {run_u ? c.sl + (SL_HU + a.res_slot) * NSTRIPE : nullptr,
+                        run_v ? c.sl + (SL_HV + a.res_slot) * NSTRIPE : nullptr};
     block_max_atomic<2>(v2, d2);
   }
 }
 
-// ------------------------------------------------------------------------------------
 // Divergence of a vel q-array: (D q + div_offset) * scale  (optimized_ghost_cell.py:212-216)
-// ------------------------------------------------------------------------------------
 struct DivArgs { int x_id, out_id; double scale; int slot; };
 
 __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const int nx = c.nx, nx1 = nx + 1;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const double* X = c.buf[a.x_id];
+  const double* Xv = X + c.usize;
+  double* O = a.out_id >= 0 ? c.buf[a.out_id] : nullptr;
   double m = 0.0;
-  if (idx < c.psize) {
-    const double* X = c.buf[a.x_id];
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double d = 0.0;
     if (p_act(c, j, i)) {
@@ -445,34 +462,32 @@ __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
       const double off = (i == 0) ? -c.inlet[j] / c.dx : 0.0;
       if (r >= 0) d = ov_dot(c.ov[MCF_OP_DIV], r, X) + off;
       else {
-        const double* Xv = X + c.usize;
         const double ul = (i == 0) ? c.inlet[j] : X[j * nx1 + i];
         d = (X[j * nx1 + i + 1] - ul) / c.dx + (Xv[(j + 1) * nx + i] - Xv[j * nx + i]) / c.dy;
       }
       d *= a.scale;
     }
-    if (a.out_id >= 0) c.buf[a.out_id][idx] = d;
-    m = d;
+    if (O) O[idx] = d;
+    m = nanmax(m, fabs(d));
   }
   double v1[1] = {m};
-  u64* const d1[1] = {c.sl + a.slot};
+  u64* const d1[1] = {c.sl + a.slot * NSTRIPE};
   block_max_atomic<1>(v1, d1);
 }
 
-// ------------------------------------------------------------------------------------
 // Projection update out = q - dt G phi (numba_kernels.py:619-623), with the optional
 // coupled momentum residual -defect + dt G phi (time_integration.py:137-142)
-// ------------------------------------------------------------------------------------
 struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dt; };
 
 __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a) {
   const CaseDev& c = cases[blockIdx.y];
-  const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = c.nx, nx1 = nx + 1;
+  const double* X = c.buf[a.x_id];
+  const double* Pp = c.buf[a.phi_id];
+  double* O = c.buf[a.out_id];
+  const double* Df = a.defect_id >= 0 ? c.buf[a.defect_id] : nullptr;
   double m = 0.0;
-  if (idx < c.velsize) {
-    const double* X = c.buf[a.x_id];
-    const double* Pp = c.buf[a.phi_id];
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
     double out = 0.0, g = 0.0;
     bool act;
     if (idx < c.usize) {
@@ -491,46 +506,44 @@ __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a)
         const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
         g = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp) : (Pp[j * nx + i] - Pp[(j - 1) * nx + i]) / c.dy / c.rho;
       }
-      (void)ny;
     }
     if (act) out = X[idx] - a.dt * g;
-    c.buf[a.out_id][idx] = out;
-    if (a.defect_id >= 0 && act) m = -c.buf[a.defect_id][idx] + a.dt * g;
+    O[idx] = out;
+    if (Df && act) m = nanmax(m, fabs(-Df[idx] + a.dt * g));
   }
-  if (a.defect_id >= 0) {
+  if (Df) {
     double v1[1] = {m};
-    u64* const d1[1] = {c.sl + SL_COUPLED};
+    u64* const d1[1] = {c.sl + SL_COUPLED * NSTRIPE};
     block_max_atomic<1>(v1, d1);
   }
 }
 
-// ------------------------------------------------------------------------------------
 // Pressure residual b - A phi and the monitored-solve statistics (pressure_schur.py:250-273)
-// ------------------------------------------------------------------------------------
 struct PresArgs { int b_id, x_id, out_id; int mode; int slot0; int res_slot; int pred_slot; };
 
 __global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) {
   const CaseDev& c = cases[blockIdx.y];
-  const int nx = c.nx, ny = c.ny;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (a.pred_slot >= 0 && !pred_refine(c.sl, a.slot0, a.pred_slot)) return;
-  double res = 0.0, rnd = 0.0, sc = 0.0, ssr = 0.0, ssv = 0.0;
-  if (idx < c.psize) {
-    const double* B = c.buf[a.b_id];
-    const double* X = c.buf[a.x_id];
+  const int nx = c.nx, ny = c.ny;
+  const double* B = c.buf[a.b_id];
+  const double* X = c.buf[a.x_id];
+  double* O = a.mode == 0 ? c.buf[a.out_id] : nullptr;
+  const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
+  const double ox = idx2 / c.rho, oy = idy2 / c.rho;
+  double mres = 0.0, mrnd = 0.0, msc = 0.0, ssr = 0.0, ssv = 0.0;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
+    double res = 0.0;
     if (p_act(c, j, i)) {
       const int r = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
       double ax, aax; int nnz;
       if (r >= 0) ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz);
       else {
-        const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
         const double x = X[idx];
         const double cx = (i == 0) ? -1.0 : ((i == nx - 1) ? -3.0 : -2.0);
         const double cy = (j == 0 || j == ny - 1) ? -1.0 : -2.0;
         const double dg = (cx * idx2 + cy * idy2) / c.rho;
         ax = dg * x; aax = fabs(dg) * fabs(x); nnz = 1;
-        const double ox = idx2 / c.rho, oy = idy2 / c.rho;
         if (i > 0) { double t = X[idx - 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
         if (i < nx - 1) { double t = X[idx + 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
         if (j > 0) { double t = X[idx - nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
@@ -538,26 +551,29 @@ __global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) 
       }
       const double b = B[idx];
       res = b - ax;
-      sc = aax + fabs(b);
-      rnd = (2.0 * nnz + 2.0) * kEps * sc;
-      ssr = res * res;
-      ssv = b * b;
+      const double sc = aax + fabs(b);
+      mres = nanmax(mres, fabs(res));
+      msc = nanmax(msc, sc);
+      mrnd = nanmax(mrnd, (2.0 * nnz + 2.0) * kEps * sc);
+      ssr += res * res;
+      ssv += b * b;
     }
-    if (a.mode == 0) c.buf[a.out_id][idx] = res;
+    if (O) O[idx] = res;
   }
-  u64* s = c.sl + a.slot0;
+  u64* s = c.sl;
+  const int b0 = a.slot0;
   if (a.mode == 0) {
-    double v1[1] = {res};
-    u64* const d1[1] = {s + a.res_slot};
+    double v1[1] = {mres};
+    u64* const d1[1] = {s + (b0 + a.res_slot) * NSTRIPE};
     block_max_atomic<1>(v1, d1);
   } else {
-    double v3[3] = {res, rnd, sc};
-    u64* const d3[3] = {s + SV_FINAL, s + SV_ROUND, s + SV_ASCALE};
+    double v3[3] = {mres, mrnd, msc};
+    u64* const d3[3] = {s + (b0 + SV_FINAL) * NSTRIPE, s + (b0 + SV_ROUND) * NSTRIPE, s + (b0 + SV_ASCALE) * NSTRIPE};
     block_max_atomic<3>(v3, d3);
     __syncthreads();
-    block_sum_atomic(ssr, reinterpret_cast<double*>(s + SV_SSQR));
+    block_sum_atomic(ssr, reinterpret_cast<double*>(s + (b0 + SV_SSQR) * NSTRIPE));
     __syncthreads();
-    block_sum_atomic(ssv, reinterpret_cast<double*>(s + SV_SSQV));
+    block_sum_atomic(ssv, reinterpret_cast<double*>(s + (b0 + SV_SSQV) * NSTRIPE));
   }
 }
 
@@ -575,199 +591,233 @@ struct GemmArgs {
 DEV const double* desc_in(const SolveDesc& d, int id) { return d.bufs[id] + d.off; }
 DEV double* desc_out(const SolveDesc& d, int id) { return d.bufs[id] + d.off; }
 
-constexpr int GT = 64, GK = 16;
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int GTM = 32, GTN = 64;   // block tile of the transform (4 waves of 16 x 32)
+constexpr int GKC = 64;             // K chunk staged in LDS (16 MFMA k-steps)
 
+// Dense FP64 y transform on the matrix cores: C (M x N) = A (M x K) * B (K x N), M = K = yy, N = xx.
+// A^T is read in row-major (k, m) order: forward (A = V^T) reads V, inverse (A = V) reads V^T.
+// Each block stages 64-deep K chunks of its A and B tiles in LDS; the next chunk is fetched into
+// registers while the current one feeds the MFMAs, so a transform costs ceil(K/64) memory round
+// trips.  Fragment layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l holds
+// A[l%16][l/16], B[l/16][l%16] and D[4*r + l/16][l%16] in accumulator element r.
 __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g) {
   const SolveDesc& d = descs[blockIdx.z];
   const int M = d.yy, N = d.xx, K = d.yy;
-  const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
+  const int row0 = blockIdx.y * GTM, col0 = blockIdx.x * GTN;
   if (row0 >= M || col0 >= N) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
-  const double* Am = g.forward ? d.VT : d.V;
+  const double* AT = g.forward ? d.V : d.VT;
   const double* Bm;
   int ldb;
   if (g.forward) { Bm = desc_in(d, g.in_id); ldb = d.ld; }
   else { Bm = d.F; ldb = d.xx; }
-  __shared__ double As[GK][GT + 1];
-  __shared__ double Bs[GK][GT + 1];
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  double acc[4][4];
-  #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  __shared__ double As[GKC][GTM];
+  __shared__ double Bs[GKC][GTN];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int kq = lane >> 4, l16 = lane & 15;
+  constexpr int NA = GKC * GTM / 256, NB = GKC * GTN / 256;
+  double ra[NA], rb[NB];
+  auto gload = [&](int k0) {
     #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[r][q] = 0.0;
-  for (int k0 = 0; k0 < K; k0 += GK) {
-    #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int e = tid + 256 * t;
-      const int r = e >> 4, kk = e & 15;
-      const int gr = row0 + r, gk = k0 + kk;
-      As[kk][r] = (gr < M && gk < K) ? Am[(size_t)gr * K + gk] : 0.0;
-      const int kb = e >> 6, cc = e & 63;
-      const int gkb = k0 + kb, gc = col0 + cc;
-      Bs[kb][cc] = (gkb < K && gc < N) ? Bm[(size_t)gkb * ldb + gc] : 0.0;
+    for (int t = 0; t < NA; ++t) {
+      const int e = tid + 256 * t, kk = k0 + (e >> 5), mm = row0 + (e & 31);
+      ra[t] = (kk < K && mm < M) ? AT[(size_t)kk * M + mm] : 0.0;
     }
-    __syncthreads();
     #pragma unroll
-    for (int kk = 0; kk < GK; ++kk) {
-      double av[4], bv[4];
-      #pragma unroll
-      for (int r = 0; r < 4; ++r) av[r] = As[kk][ty + 16 * r];
-      #pragma unroll
-      for (int q = 0; q < 4; ++q) bv[q] = Bs[kk][tx + 16 * q];
-      #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[r][q] = fma(av[r], bv[q], acc[r][q]);
+    for (int t = 0; t < NB; ++t) {
+      const int e = tid + 256 * t, kk = k0 + (e >> 6), nn = col0 + (e & 63);
+      rb[t] = (kk < K && nn < N) ? Bm[(size_t)kk * ldb + nn] : 0.0;
+    }
+  };
+  auto sstore = [&]() {
+    #pragma unroll
+    for (int t = 0; t < NA; ++t) { const int e = tid + 256 * t; As[e >> 5][e & 31] = ra[t]; }
+    #pragma unroll
+    for (int t = 0; t < NB; ++t) { const int e = tid + 256 * t; Bs[e >> 6][e & 63] = rb[t]; }
+  };
+  d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  gload(0);
+  for (int k0 = 0; k0 < K; k0 += GKC) {
+    sstore();
+    __syncthreads();
+    if (k0 + GKC < K) gload(k0 + GKC);
+    #pragma unroll
+    for (int t = 0; t < GKC / 4; ++t) {
+      const int kl = 4 * t + kq;
+      const double a = As[kl][16 * wm + l16];
+      const double b0 = Bs[kl][32 * wn + l16];
+      const double b1 = Bs[kl][32 * wn + 16 + l16];
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
     }
     __syncthreads();
   }
+  const int n0 = col0 + 32 * wn + l16, n1 = n0 + 16;
+  const bool n0ok = n0 < N, n1ok = n1 < N;
+  const int rbase = row0 + 16 * wm + kq;
   if (g.forward) {
     #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int gr = row0 + ty + 16 * r;
+      const int gr = rbase + 4 * r;
       if (gr >= M) continue;
-      #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gc = col0 + tx + 16 * q;
-        if (gc < N) d.F[(size_t)gr * N + gc] = acc[r][q];
-      }
+      if (n0ok) d.F[(size_t)gr * N + n0] = acc0[r];
+      if (n1ok) d.F[(size_t)gr * N + n1] = acc1[r];
     }
   } else {
     double* O = desc_out(d, g.out_id);
     #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int gr = row0 + ty + 16 * r;
+      const int gr = rbase + 4 * r;
       if (gr >= M) continue;
       const int aj = gr + d.joff;
       const int slo = d.lo ? d.lo[aj] : 1, shi = d.lo ? d.hi[aj] : 0;
       #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gc = col0 + tx + 16 * q;
+      for (int t = 0; t < 2; ++t) {
+        const int gc = t ? n1 : n0;
         if (gc >= N) continue;
+        const double v = t ? acc1[r] : acc0[r];
         const int ai = gc + d.ioff;
-        double* p = O + (size_t)gr * d.ld + gc;
-        if (g.epi == 0) *p = acc[r][q];
-        else {
-          const bool cov = (slo <= ai && ai <= shi);
-          if (cov) *p = 0.0;
-          else *p = (g.epi == 2) ? (*p + acc[r][q]) : acc[r][q];
-        }
+        double* pp = O + (size_t)gr * d.ld + gc;
+        if (g.epi == 0) *pp = v;
+        else if (slo <= ai && ai <= shi) *pp = 0.0;
+        else *pp = (g.epi == 2) ? (*pp + v) : v;
       }
     }
   }
 }
 
 struct PcrArgs { int mode; int pred_slot; int slot_shift; };
-constexpr int PCR_T = 512, PCR_EPT = 8;   // xx <= 4096
 
-__global__ __launch_bounds__(PCR_T) void k_pcr(const SolveDesc* descs, PcrArgs p) {
+// Block-wide exclusive scan of affine maps y -> A y + B (composition in thread order, or in
+// reverse order when `reverse`); returns the value reached from 0 before this thread's chunk.
+DEV double block_affine_scan(double A, double B, bool reverse) {
+  __shared__ double sa[4], sb[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int off = 1; off < 64; off <<= 1) {
+    const double Ap = reverse ? __shfl_down(A, off, 64) : __shfl_up(A, off, 64);
+    const double Bp = reverse ? __shfl_down(B, off, 64) : __shfl_up(B, off, 64);
+    const bool ok = reverse ? (lane + off < 64) : (lane >= off);
+    if (ok) { B = A * Bp + B; A = A * Ap; }
+  }
+  if (lane == (reverse ? 0 : 63)) { sa[w] = A; sb[w] = B; }
+  __syncthreads();
+  // value entering this wave: compose the totals of the waves before it (after it, reversed)
+  double vin = 0.0;
+  if (!reverse) { for (int q = 0; q < w; ++q) vin = sa[q] * vin + sb[q]; }
+  else { for (int q = 3; q > w; --q) vin = sa[q] * vin + sb[q]; }
+  // inclusive value of the neighbouring lane
+  const double Bn = reverse ? __shfl_down(A * vin + B, 1, 64) : __shfl_up(A * vin + B, 1, 64);
+  const bool edge = reverse ? (lane == 63) : (lane == 0);
+  __syncthreads();
+  return edge ? vin : Bn;
+}
+
+// Per-mode tridiagonal solve along x with the precomputed LU factors of (a + b lam_k) I + b lx
+// (constant super-diagonal b*upper): one 256-thread block per mode, each thread runs the
+// forward and backward recurrences on a contiguous chunk held in registers; chunks are joined
+// by block scans of the affine maps.  mode 0: F[k] <- T_k^{-1} F[k];  mode 1: F[k] <- F[k] -
+// T_k^{-1} z_k with z_k built from the capacitance correction y (rows grouped by x column).
+constexpr int TRI_T = 256, TRI_C = 16, TRI_MAXX = TRI_T * TRI_C;
+__global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p) {
   const SolveDesc& d = descs[blockIdx.y];
   const int k = blockIdx.x;
   if (k >= d.yy) return;
   if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
-  const int xx = d.xx, yy = d.yy;
-  extern __shared__ double lds[];
-  double* A = lds; double* Bd = A + xx; double* C = Bd + xx; double* R = C + xx;
-  const double shift = d.a + d.b * d.lam[k];
+  const int xx = d.xx, yy = d.yy, tid = threadIdx.x;
+  const int C = (xx + TRI_T - 1) / TRI_T;
+  const int i0 = min(tid * C, xx), n = min(i0 + C, xx) - i0;
   double* Frow = d.F + (size_t)k * xx;
-  for (int i = threadIdx.x; i < xx; i += PCR_T) {
-    A[i] = (i > 0) ? d.b * d.lower[i - 1] : 0.0;
-    C[i] = (i < xx - 1) ? d.b * d.upper[i] : 0.0;
-    Bd[i] = shift + d.b * d.diag[i];
-    if (p.mode == 0) R[i] = Frow[i];
-    else {
+  const double* Lrow = d.luL + (size_t)k * xx;
+  const double* Irow = d.luI + (size_t)k * xx;
+  const double cu = d.b * d.upper[0];
+  __shared__ double zl[TRI_MAXX];
+  if (p.mode == 1) {
+    // z_k = V^T (rows of the capacitance correction scattered by y), dense only on the
+    // x columns that hold a correction row
+    for (int i = tid; i < xx; i += TRI_T) zl[i] = 0.0;
+    __syncthreads();
+    for (int gq = tid; gq < d.rg_n; gq += TRI_T) {
       double s = 0.0;
-      for (int t = d.rc_ptr[i]; t < d.rc_ptr[i + 1]; ++t) s += d.V[(size_t)d.rc_j[t] * yy + k] * d.y[d.rc_id[t]];
-      R[i] = s;
-    }
-  }
-  __syncthreads();
-  for (int s = 1; s < xx; s <<= 1) {
-    double na[PCR_EPT], nb[PCR_EPT], nc[PCR_EPT], nr[PCR_EPT];
-    #pragma unroll
-    for (int e = 0; e < PCR_EPT; ++e) {
-      const int i = threadIdx.x + e * PCR_T;
-      if (i < xx) {
-        const int im = i - s, ip = i + s;
-        double a_ = 0.0, b_ = Bd[i], c_ = 0.0, r_ = R[i];
-        if (im >= 0) {
-          const double k1 = A[i] / Bd[im];
-          a_ = -A[im] * k1;
-          b_ -= C[im] * k1;
-          r_ -= R[im] * k1;
-        }
-        if (ip < xx) {
-          const double k2 = C[i] / Bd[ip];
-          c_ = -C[ip] * k2;
-          b_ -= A[ip] * k2;
-          r_ -= R[ip] * k2;
-        }
-        na[e] = a_; nb[e] = b_; nc[e] = c_; nr[e] = r_;
-      }
-    }
-    __syncthreads();
-    #pragma unroll
-    for (int e = 0; e < PCR_EPT; ++e) {
-      const int i = threadIdx.x + e * PCR_T;
-      if (i < xx) { A[i] = na[e]; Bd[i] = nb[e]; C[i] = nc[e]; R[i] = nr[e]; }
+      for (int t = d.rg_ptr[gq]; t < d.rg_ptr[gq + 1]; ++t) s += d.V[(size_t)d.rc_j[t] * yy + k] * d.y[d.rc_id[t]];
+      zl[d.rg_x[gq]] = s;
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < xx; i += PCR_T) {
-    const double x = R[i] / Bd[i];
-    if (p.mode == 0) Frow[i] = x;
-    else Frow[i] -= x;
+  double r[TRI_C], l[TRI_C], id[TRI_C];
+  #pragma unroll
+  for (int c = 0; c < TRI_C; ++c) {
+    const int i = i0 + c;
+    if (c < n) {
+      l[c] = Lrow[i];
+      id[c] = Irow[i];
+      r[c] = (p.mode == 0) ? Frow[i] : zl[i];
+    } else { l[c] = 0.0; id[c] = 0.0; r[c] = 0.0; }
   }
+  // forward: y_i = r_i - l_i y_{i-1}
+  double A = 1.0, B = 0.0;
+  #pragma unroll
+  for (int c = 0; c < TRI_C; ++c)
+    if (c < n) { const double al = -l[c]; A *= al; B = al * B + r[c]; }
+  double y = block_affine_scan(A, B, false);
+  #pragma unroll
+  for (int c = 0; c < TRI_C; ++c)
+    if (c < n) { y = r[c] - l[c] * y; r[c] = y; }
+  // backward: x_i = (y_i - cu x_{i+1}) / d_i
+  A = 1.0; B = 0.0;
+  #pragma unroll
+  for (int c = TRI_C - 1; c >= 0; --c)
+    if (c < n) {
+      const double al = (i0 + c < xx - 1) ? -cu * id[c] : 0.0;
+      A *= al; B = al * B + r[c] * id[c];
+    }
+  double x = block_affine_scan(A, B, true);
+  #pragma unroll
+  for (int c = TRI_C - 1; c >= 0; --c)
+    if (c < n) {
+      const double al = (i0 + c < xx - 1) ? -cu * id[c] : 0.0;
+      x = al * x + r[c] * id[c];
+      r[c] = x;
+    }
+  #pragma unroll
+  for (int c = 0; c < TRI_C; ++c)
+    if (c < n) {
+      const int i = i0 + c;
+      if (p.mode == 0) Frow[i] = r[c];
+      else Frow[i] -= r[c];
+    }
 }
 
-// first = R^{-1} b evaluated at the capacitance points, grouped by x column
+// first = R^{-1} b at the capacitance points: one wave per point, lanes over the y modes
 __global__ __launch_bounds__(256) void k_points(const SolveDesc* descs, PcrArgs p) {
   const SolveDesc& d = descs[blockIdx.y];
-  const int g = blockIdx.x;
-  if (g >= d.ngroups) return;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blockIdx.x * 4 >= d.npts) return;
   if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
-  const int yy = d.yy, xx = d.xx, x = d.grp_x[g];
-  extern __shared__ double col[];
-  for (int k = threadIdx.x; k < yy; k += blockDim.x) col[k] = d.F[(size_t)k * xx + x];
-  __syncthreads();
-  __shared__ double red[4];
-  for (int t = d.grp_ptr[g]; t < d.grp_ptr[g + 1]; ++t) {
-    const double* Vr = d.V + (size_t)d.pt_j[t] * yy;
-    double s = 0.0;
-    for (int k = threadIdx.x; k < yy; k += blockDim.x) s += Vr[k] * col[k];
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) d.firstK[t] = red[0] + red[1] + red[2] + red[3];
-    __syncthreads();
-  }
+  if (q >= d.npts) return;
+  const int yy = d.yy, xx = d.xx, lane = threadIdx.x & 63;
+  const int x = d.pt_x[q];
+  const double* Vr = d.V + (size_t)d.pt_j[q] * yy;
+  double s = 0.0;
+  for (int kk = lane; kk < yy; kk += 64) s += Vr[kk] * d.F[(size_t)kk * xx + x];
+  s = wave_sum(s);
+  if (lane == 0) d.firstK[q] = s;
 }
 
-// y = S^{-1} (C first)
+// y = (S^{-1} C) first: one wave per row of the precomputed r x npts matrix S^{-1} C
 __global__ __launch_bounds__(256) void k_capacitance(const SolveDesc* descs, PcrArgs p) {
   const SolveDesc& d = descs[blockIdx.y];
-  const int r = d.r;
-  const int row0 = blockIdx.x * 64;
-  if (row0 >= r) return;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blockIdx.x * 4 >= d.r) return;
   if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
-  extern __shared__ double cv[];
-  for (int m = threadIdx.x; m < r; m += blockDim.x) {
-    double s = 0.0;
-    for (int t = d.c_ptr[m]; t < d.c_ptr[m + 1]; ++t) s += d.c_val[t] * d.firstK[d.c_col[t]];
-    cv[m] = s;
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int rr = w; rr < 64; rr += 4) {
-    const int row = row0 + rr;
-    if (row >= r) break;
-    const double* Sr = d.Sinv + (size_t)row * r;
-    double s = 0.0;
-    for (int m = lane; m < r; m += 64) s += Sr[m] * cv[m];
-    s = wave_sum(s);
-    if (lane == 0) d.y[row] = s;
-  }
+  if (row >= d.r) return;
+  const int lane = threadIdx.x & 63, np = d.npts;
+  const double* Sr = d.SC + (size_t)row * np;
+  double s = 0.0;
+  for (int q = lane; q < np; q += 64) s += Sr[q] * d.firstK[q];
+  s = wave_sum(s);
+  if (lane == 0) d.y[row] = s;
 }
 
 // ------------------------------------------------------------------------------------
@@ -780,16 +830,26 @@ __global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) 
   double* X = c.buf[a.x_id];
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   double mu = 0.0, mv = 0.0;
+  auto row = [&](const GhostTab& g, int k, const double* base) {
+    double s = 0.0;
+    const int* cl = g.col + (size_t)k * g.width;
+    const double* wt = g.w + (size_t)k * g.width;
+    if (g.width == 8) {
+      #pragma unroll
+      for (int q = 0; q < 8; ++q) s += wt[q] * base[cl[q]];
+    } else {
+      for (int q = 0; q < g.width; ++q) s += wt[q] * base[cl[q]];
+    }
+    return s;
+  };
   if (a.vel) {
     const GhostTab& gu = c.gt[0];
     const GhostTab& gv = c.gt[1];
     if (t < gu.n) {
-      double s = 0.0;
-      for (int k = gu.ptr[t]; k < gu.ptr[t + 1]; ++k) s += gu.w[k] * X[gu.col[k]];
+      const double s = row(gu, t, X);
       X[gu.node[t]] = s; mu = s;
     } else if ((t -= gu.n) < gv.n) {
-      double s = 0.0;
-      for (int k = gv.ptr[t]; k < gv.ptr[t + 1]; ++k) s += gv.w[k] * X[c.usize + gv.col[k]];
+      const double s = row(gv, t, X + c.usize);
       X[c.usize + gv.node[t]] = s; mv = s;
     } else if ((t -= gv.n) < c.ny) {
       const double v = a.inlet ? c.inlet[t] : 0.0;
@@ -797,15 +857,11 @@ __global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) 
     }
   } else {
     const GhostTab& gp = c.gt[2];
-    if (t < gp.n) {
-      double s = 0.0;
-      for (int k = gp.ptr[t]; k < gp.ptr[t + 1]; ++k) s += gp.w[k] * X[gp.col[k]];
-      X[gp.node[t]] = s;
-    }
+    if (t < gp.n) X[gp.node[t]] = row(gp, t, X);
   }
   if (a.vel && a.track) {
     double v2[2] = {mu, mv};
-    u64* const d2[2] = {c.sl + SL_UMAX, c.sl + SL_VMAX};
+    u64* const d2[2] = {c.sl + SL_UMAX * NSTRIPE, c.sl + SL_VMAX * NSTRIPE};
     block_max_atomic<2>(v2, d2);
   }
 }
@@ -814,123 +870,164 @@ __global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) 
 struct PressArgs { int p_id, inc_id, half_id, out_id; double factor; };
 __global__ __launch_bounds__(256) void k_pressures(const CaseDev* cases, PressArgs a) {
   const CaseDev& c = cases[blockIdx.y];
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= c.psize) return;
-  const double inc = c.buf[a.inc_id][idx];
-  const double half = c.buf[a.p_id][idx] + inc;
-  c.buf[a.half_id][idx] = half;
-  c.buf[a.out_id][idx] = half + a.factor * inc;
+  const double* P = c.buf[a.p_id];
+  const double* I = c.buf[a.inc_id];
+  double* H = c.buf[a.half_id];
+  double* O = c.buf[a.out_id];
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
+    const double inc = I[idx];
+    const double half = P[idx] + inc;
+    H[idx] = half;
+    O[idx] = half + a.factor * inc;
+  }
 }
 
-// Surface forces from the pressure q-array and the extended velocity (one block per case)
+// Surface forces: one lane per quadrature point, one partial sum per wave (summed in a fixed
+// order by k_force_total, so the forces are bit-reproducible)
+constexpr int FORCE_MAXB = 64;   // up to 64 * 64 surface points
 struct ForceArgs { int vel_id, p_id; double mu; };
-__global__ __launch_bounds__(256) void k_forces(const CaseDev* cases, ForceArgs a) {
+__global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const double* Vel = c.buf[a.vel_id];
   const double* Pp = c.buf[a.p_id];
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (blockIdx.x * 64 >= c.nsurf) return;
   double s[4] = {0.0, 0.0, 0.0, 0.0};  // pressure x, y; traction x, y
-  for (int k = threadIdx.x; k < c.nsurf; k += blockDim.x) {
+  if (k < c.nsurf) {
     double ps = 0.0, du = 0.0, dv = 0.0;
     for (int t = c.pp[k]; t < c.pp[k + 1]; ++t) ps += c.pv[t] * Pp[c.pc[t]];
     for (int t = c.up[k]; t < c.up[k + 1]; ++t) du += c.uv[t] * Vel[c.uc[t]];
     for (int t = c.vp[k]; t < c.vp[k + 1]; ++t) dv += c.vv[t] * Vel[c.usize + c.vc[t]];
     const double nxk = c.nrm[2 * k], nyk = c.nrm[2 * k + 1];
     const double dot = du * nxk + dv * nyk;
-    s[0] += ps * c.nw[2 * k];
-    s[1] += ps * c.nw[2 * k + 1];
-    s[2] += a.mu * (du + nxk * dot);
-    s[3] += a.mu * (dv + nyk * dot);
+    s[0] = ps * c.nw[2 * k];
+    s[1] = ps * c.nw[2 * k + 1];
+    s[2] = a.mu * (du + nxk * dot);
+    s[3] = a.mu * (dv + nyk * dot);
   }
-  __shared__ double red[4][4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    double v = wave_sum(s[q]);
-    if (lane == 0) red[w][q] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[4];
-    for (int q = 0; q < 4; ++q) t[q] = red[0][q] + red[1][q] + red[2][q] + red[3][q];
-    const double scale = M_PI * c.diameter / c.nsurf;
-    double* f = reinterpret_cast<double*>(c.sl + SL_FORCE);
-    const double fxp = -t[0], fyp = -t[1], fxv = t[2] * scale, fyv = t[3] * scale;
-    f[0] = fxp + fxv; f[1] = fyp + fyv; f[2] = fxp; f[3] = fyp; f[4] = fxv; f[5] = fyv;
+    const double v = wave_sum(s[q]);
+    if (threadIdx.x == 0) c.fpart[blockIdx.x * 4 + q] = v;
   }
 }
 
-// Gather the step's scalars into the diagnostics ring and clear the slots
-struct FinArgs { int row; double time, dt, rho; int first; int nproj; };
-__global__ void k_finalize(const CaseDev* cases, FinArgs a) {
+__global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
   const CaseDev& c = cases[blockIdx.y];
   if (threadIdx.x != 0) return;
+  const int nb = (c.nsurf + 63) / 64;
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < nb; ++b)
+    for (int q = 0; q < 4; ++q) t[q] += c.fpart[b * 4 + q];
+  const double scale = M_PI * c.diameter / c.nsurf;
+  double* f = reinterpret_cast<double*>(c.sl + SL_FORCE * NSTRIPE);
+  const double fxp = -t[0], fyp = -t[1], fxv = t[2] * scale, fyv = t[3] * scale;
+  const double vals[6] = {fxp + fxv, fyp + fyv, fxp, fyp, fxv, fyv};
+  for (int q = 0; q < 6; ++q) f[q * NSTRIPE] = vals[q];
+}
+
+// Gather the step's scalars into the diagnostics ring and clear the slots (one wave per case)
+struct FinArgs { double dt, rho; int first; int nproj; };
+constexpr int FIN_N = 34;
+__global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a) {
+  const CaseDev& c = cases[blockIdx.y];
   const u64* s = c.sl;
-  double* d = c.diag + (size_t)a.row * MCF_DIAG_LEN;
-  const double* f = reinterpret_cast<const double*>(s + SL_FORCE);
-  const double denom = 0.5 * a.rho * c.uref * c.uref * c.diameter;
-  for (int k = 0; k < MCF_DIAG_LEN; ++k) d[k] = 0.0;
-  d[MCF_D_TIME] = a.time; d[MCF_D_DT] = a.dt;
-  d[MCF_D_FX] = f[0]; d[MCF_D_FY] = f[1];
-  d[MCF_D_CD] = f[0] / denom; d[MCF_D_CL] = f[1] / denom;
-  d[MCF_D_CD_P] = f[2] / denom; d[MCF_D_CL_P] = f[3] / denom;
-  d[MCF_D_CD_V] = f[4] / denom; d[MCF_D_CL_V] = f[5] / denom;
-  d[MCF_D_MAXDIV] = slot_d(s, SL_DIV0);
-  d[MCF_D_CFL] = a.dt * (slot_d(s, SL_UMAX) / c.dx + slot_d(s, SL_VMAX) / c.dy);
-  const double rhs_max = fmax(slot_d(s, SL_HU + SV_VAL), slot_d(s, SL_HV + SV_VAL));
-  const double mres = fmax(slot_d(s, SL_HU + SV_FINAL), slot_d(s, SL_HV + SV_FINAL));
-  d[MCF_D_MOMENTUM_RES] = mres;
-  d[MCF_D_RHS_MAX] = rhs_max;
-  d[MCF_D_COUPLED_RES] = slot_d(s, SL_COUPLED);
-  int status = 0;
-  if (!(mres <= 1e-10 * fmax(1.0, rhs_max))) status |= 1;
-  int refine = 0, hrefine = 0;
-  const int blocks[3] = {SL_P0, SL_P1, SL_P2};
-  const int nsolve = a.nproj;
-  double worst = -1.0;
-  for (int q = 0; q < nsolve; ++q) {
-    const int b = blocks[q];
-    const double val = slot_d(s, b + SV_VAL);
-    const double t1 = woodbury_target(val);
-    const double r1 = slot_d(s, b + SV_RES1), r2 = slot_d(s, b + SV_RES2);
-    refine += (r1 > t1 ? 1 : 0) + (r1 > t1 && r2 > t1 ? 1 : 0);
-    const double fin = slot_d(s, b + SV_FINAL);
-    double target = fmax(1e-10, 64.0 * kEps * fmax(1.0, val));
-    const double rnd = slot_d(s, b + SV_ROUND);
-    target = fmax(target, rnd);
-    if (!(fin <= target)) status |= 2;
-    const double ssr = slot_d(s, b + SV_SSQR), ssv = slot_d(s, b + SV_SSQV);
-    const double rel = sqrt(ssr) / fmax(sqrt(ssv), kTiny);
-    d[MCF_D_P_REL0] = fmax(d[MCF_D_P_REL0], rel);
-    if (q == 0) d[MCF_D_P_RES1] = fin;
-    if (fin > worst || fin != fin) {
-      worst = fin;
-      d[MCF_D_P_RES0] = fin;
-      d[MCF_D_P_TARGET0] = target;
-      d[MCF_D_P_ROUND0] = rnd;
-      d[MCF_D_P_BACKWARD0] = fin / fmax(slot_d(s, b + SV_ASCALE), kTiny);
+  const int lane = threadIdx.x;
+  // slots read by this kernel; all loads are issued before any reduction
+  const int ids[FIN_N] = {SL_UMAX, SL_VMAX, SL_HU + SV_VAL, SL_HV + SV_VAL, SL_HU + SV_FINAL, SL_HV + SV_FINAL,
+                          SL_HU + SV_RES1, SL_HU + SV_RES2, SL_HV + SV_RES1, SL_HV + SV_RES2, SL_COUPLED,
+                          SL_DIV0, SL_DIV1, SL_DIV2,
+                          SL_P0 + SV_VAL, SL_P0 + SV_RES1, SL_P0 + SV_RES2, SL_P0 + SV_FINAL, SL_P0 + SV_ROUND, SL_P0 + SV_ASCALE,
+                          SL_P1 + SV_VAL, SL_P1 + SV_RES1, SL_P1 + SV_RES2, SL_P1 + SV_FINAL, SL_P1 + SV_ROUND, SL_P1 + SV_ASCALE,
+                          SL_P2 + SV_VAL, SL_P2 + SV_RES1, SL_P2 + SV_RES2, SL_P2 + SV_FINAL, SL_P2 + SV_ROUND, SL_P2 + SV_ASCALE,
+                          SL_P0 + SV_SSQR, SL_P0 + SV_SSQV};
+  double raw[FIN_N], sq[4];
+  #pragma unroll
+  for (int q = 0; q < FIN_N; ++q) raw[q] = __longlong_as_double((long long)s[ids[q] * NSTRIPE + lane]);
+  const int sids[4] = {SL_P1 + SV_SSQR, SL_P1 + SV_SSQV, SL_P2 + SV_SSQR, SL_P2 + SV_SSQV};
+  #pragma unroll
+  for (int q = 0; q < 4; ++q) sq[q] = __longlong_as_double((long long)s[sids[q] * NSTRIPE + lane]);
+  double v[FIN_N];
+  #pragma unroll
+  for (int q = 0; q < FIN_N; ++q) v[q] = (q >= FIN_N - 2) ? wave_sum(raw[q]) : wave_max_abs(raw[q]);
+  double vs[4];
+  #pragma unroll
+  for (int q = 0; q < 4; ++q) vs[q] = wave_sum(sq[q]);
+  double f[6];
+  for (int q = 0; q < 6; ++q) f[q] = slot_d(s, (SL_FORCE + q) * NSTRIPE);
+  if (lane == 0) {
+    const int row = *c.clock_step;
+    const double time = *c.clock_time + a.dt;
+    *c.clock_step = row + 1;
+    *c.clock_time = time;
+    const double umax = v[0], vmax = v[1], hu_val = v[2], hv_val = v[3], hu_fin = v[4], hv_fin = v[5];
+    const double hu_r1 = v[6], hu_r2 = v[7], hv_r1 = v[8], hv_r2 = v[9], coupled = v[10];
+    const double div[3] = {v[11], v[12], v[13]};
+    double pval[3], pr1[3], pr2[3], pfin[3], prnd[3], pasc[3];
+    for (int q = 0; q < 3; ++q) {
+      pval[q] = v[14 + 6 * q]; pr1[q] = v[15 + 6 * q]; pr2[q] = v[16 + 6 * q];
+      pfin[q] = v[17 + 6 * q]; prnd[q] = v[18 + 6 * q]; pasc[q] = v[19 + 6 * q];
     }
+    const double pssr[3] = {v[32], vs[0], vs[2]}, pssv[3] = {v[33], vs[1], vs[3]};
+    double* d = c.diag + (size_t)row * MCF_DIAG_LEN;
+    const double denom = 0.5 * a.rho * c.uref * c.uref * c.diameter;
+    for (int k = 0; k < MCF_DIAG_LEN; ++k) d[k] = 0.0;
+    d[MCF_D_TIME] = time; d[MCF_D_DT] = a.dt;
+    d[MCF_D_FX] = f[0]; d[MCF_D_FY] = f[1];
+    d[MCF_D_CD] = f[0] / denom; d[MCF_D_CL] = f[1] / denom;
+    d[MCF_D_CD_P] = f[2] / denom; d[MCF_D_CL_P] = f[3] / denom;
+    d[MCF_D_CD_V] = f[4] / denom; d[MCF_D_CL_V] = f[5] / denom;
+    d[MCF_D_MAXDIV] = div[0];
+    d[MCF_D_CFL] = a.dt * (umax / c.dx + vmax / c.dy);
+    const double rhs_max = fmax(hu_val, hv_val);
+    const double mres = nanmax(hu_fin, hv_fin);
+    d[MCF_D_MOMENTUM_RES] = mres;
+    d[MCF_D_RHS_MAX] = rhs_max;
+    d[MCF_D_COUPLED_RES] = coupled;
+    int status = 0;
+    if (!(mres <= 1e-10 * fmax(1.0, rhs_max))) status |= 1;
+    int refine = 0;
+    double worst = -1.0;
+    for (int b = 0; b < a.nproj; ++b) {
+      const double t1 = woodbury_target(pval[b]);
+      refine += (pr1[b] > t1 ? 1 : 0) + (pr1[b] > t1 && pr2[b] > t1 ? 1 : 0);
+      double target = fmax(1e-10, 64.0 * kEps * fmax(1.0, pval[b]));
+      target = fmax(target, prnd[b]);
+      if (!(pfin[b] <= target)) status |= 2;
+      d[MCF_D_P_REL0] = fmax(d[MCF_D_P_REL0], sqrt(pssr[b]) / fmax(sqrt(pssv[b]), kTiny));
+      if (b == 0) d[MCF_D_P_RES1] = pfin[b];
+      if (pfin[b] > worst || pfin[b] != pfin[b]) {
+        worst = pfin[b];
+        d[MCF_D_P_RES0] = pfin[b];
+        d[MCF_D_P_TARGET0] = target;
+        d[MCF_D_P_ROUND0] = prnd[b];
+        d[MCF_D_P_BACKWARD0] = pfin[b] / fmax(pasc[b], kTiny);
+      }
+    }
+    const double tu = woodbury_target(hu_val), tv = woodbury_target(hv_val);
+    const int hrefine = (hu_r1 > tu) + (hu_r1 > tu && hu_r2 > tu) + (hv_r1 > tv) + (hv_r1 > tv && hv_r2 > tv);
+    d[MCF_D_P_REFINE] = refine;
+    d[MCF_D_H_REFINE] = hrefine;
+    d[MCF_D_P_SOLVES] = a.nproj;
+    double divg = 0.0;
+    for (int q = 0; q < 3; ++q) divg = nanmax(divg, div[q]);
+    if (!(divg <= 1e-10)) status |= 4;
+    if (!(d[MCF_D_MAXDIV] < 1e-10) || !(d[MCF_D_CFL] < 0.5)) status |= 8;
+    if (!isfinite(d[MCF_D_CD]) || !isfinite(d[MCF_D_CL])) status |= 16;
+    d[MCF_D_STATUS] = status;
   }
-  for (int hb = SL_HU; hb <= SL_HV; hb += 8) {
-    const double t1 = woodbury_target(slot_d(s, hb + SV_VAL));
-    const double r1 = slot_d(s, hb + SV_RES1), r2 = slot_d(s, hb + SV_RES2);
-    hrefine += (r1 > t1 ? 1 : 0) + (r1 > t1 && r2 > t1 ? 1 : 0);
-  }
-  d[MCF_D_P_REFINE] = refine;
-  d[MCF_D_H_REFINE] = hrefine;
-  d[MCF_D_P_SOLVES] = nsolve;
-  double divg = 0.0;
-  for (int q = 0; q < 3; ++q) divg = fmax(divg, slot_d(s, SL_DIV0 + q));
-  if (!(divg <= 1e-10)) status |= 4;
-  if (!(d[MCF_D_MAXDIV] < 1e-10) || !(d[MCF_D_CFL] < 0.5)) status |= 8;
-  if (!isfinite(d[MCF_D_CD]) || !isfinite(d[MCF_D_CL])) status |= 16;
-  d[MCF_D_STATUS] = status;
-  u64* w = c.sl;
-  for (int k = 0; k < NSLOT; ++k) w[k] = 0ull;
+  u64* wsl = c.sl;
+  for (int k = lane; k < NSLOTW; k += 64) wsl[k] = 0ull;
+}
+
+__global__ void k_set_clock(const CaseDev* cases, double t0) {
+  const CaseDev& c = cases[blockIdx.y];
+  if (threadIdx.x == 0) { *c.clock_step = 0; *c.clock_time = t0; }
 }
 
 __global__ void k_clear_slots(const CaseDev* cases) {
   const CaseDev& c = cases[blockIdx.y];
-  for (int k = threadIdx.x; k < NSLOT; k += blockDim.x) c.sl[k] = 0ull;
+  for (int k = threadIdx.x; k < NSLOTW; k += blockDim.x) c.sl[k] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void k_copy(const CaseDev* cases, int src, int dst, int n) {
@@ -960,10 +1057,10 @@ __global__ __launch_bounds__(256) void k_divstats(const double* u, const double*
   double v1[1] = {m};
   u64* const d1[1] = {reinterpret_cast<u64*>(out)};
   block_max_atomic<1>(v1, d1);
-  __syncthreads(); block_sum_atomic(s, out + 1);
-  __syncthreads(); block_sum_atomic(cnt, out + 2);
-  __syncthreads(); block_sum_atomic(cs, out + 3);
-  __syncthreads(); block_sum_atomic(cc, out + 4);
+  __syncthreads(); block_sum_atomic(s, out + 1 * NSTRIPE);
+  __syncthreads(); block_sum_atomic(cnt, out + 2 * NSTRIPE);
+  __syncthreads(); block_sum_atomic(cs, out + 3 * NSTRIPE);
+  __syncthreads(); block_sum_atomic(cc, out + 4 * NSTRIPE);
 }
 
 }  // namespace
@@ -996,6 +1093,12 @@ struct mcf_handle {
   int parity = 0;      // state in S[parity], history in H[parity]
   double* tmp_rect = nullptr;   // scratch for the rectangle-inverse entry point
   size_t tmp_rect_n = 0;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double graph_dt = -1.0;
+  bool use_graphs = true;
+  int* clock_step = nullptr;
+  double* clock_time = nullptr;
 };
 
 namespace {
@@ -1026,13 +1129,17 @@ int upload(mcf_handle* h, const std::string& key, const T* src, size_t n, T** ds
   return 0;
 }
 
+void drop_graphs(mcf_handle* h);
 int sync_descs(mcf_handle* h) {
+  drop_graphs(h);
   HIPCHK(h, hipMemcpy(h->dc, &h->hc, sizeof(CaseDev), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(h->ds, h->hs, sizeof(h->hs), hipMemcpyHostToDevice));
   return 0;
 }
 
 inline dim3 grid1(int n, int nb = 256) { return dim3((n + nb - 1) / nb, 1, 1); }
+// grid of the grid-stride kernels
+inline dim3 gridS(int n) { return dim3(std::min((n + 255) / 256, GSB), 1, 1); }
 
 int launch_check(mcf_handle* h, const char* what) {
   hipError_t e = hipGetLastError();
@@ -1049,22 +1156,22 @@ int sv_block(int kind, int psolve) {
 // One Woodbury apply (in -> out): epi 1 store masked, 2 accumulate masked.
 // Problems [first, first+count) of the SolveDesc table.
 int enqueue_apply(mcf_handle* h, int first, int count, int in_id, int out_id, int epi, int pred_slot, int slot_shift) {
-  int maxyy = 0, maxxx = 0, maxg = 0, maxr = 0;
+  int maxyy = 0, maxxx = 0, maxp = 0, maxr = 0;
   for (int q = first; q < first + count; ++q) {
     maxyy = std::max(maxyy, h->hs[q].yy); maxxx = std::max(maxxx, h->hs[q].xx);
-    maxg = std::max(maxg, h->hs[q].ngroups); maxr = std::max(maxr, h->hs[q].r);
+    maxp = std::max(maxp, h->hs[q].npts); maxr = std::max(maxr, h->hs[q].r);
   }
   const SolveDesc* D = h->ds + first;
   GemmArgs g{1, in_id, out_id, 0, pred_slot, slot_shift};
-  dim3 gg((maxxx + GT - 1) / GT, (maxyy + GT - 1) / GT, count);
+  dim3 gg((maxxx + GTN - 1) / GTN, (maxyy + GTM - 1) / GTM, count);
   hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, D, g);
   PcrArgs p0{0, pred_slot, slot_shift};
-  hipLaunchKernelGGL(k_pcr, dim3(maxyy, count), dim3(PCR_T), 4 * maxxx * sizeof(double), h->stream, D, p0);
+  hipLaunchKernelGGL(k_tri, dim3(maxyy, count), dim3(TRI_T), 0, h->stream, D, p0);
   if (maxr > 0) {
-    hipLaunchKernelGGL(k_points, dim3(maxg, count), dim3(256), maxyy * sizeof(double), h->stream, D, p0);
-    hipLaunchKernelGGL(k_capacitance, dim3((maxr + 63) / 64, count), dim3(256), maxr * sizeof(double), h->stream, D, p0);
+    hipLaunchKernelGGL(k_points, dim3((maxp + 3) / 4, count), dim3(256), 0, h->stream, D, p0);
+    hipLaunchKernelGGL(k_capacitance, dim3((maxr + 3) / 4, count), dim3(256), 0, h->stream, D, p0);
     PcrArgs p1{1, pred_slot, slot_shift};
-    hipLaunchKernelGGL(k_pcr, dim3(maxyy, count), dim3(PCR_T), 4 * maxxx * sizeof(double), h->stream, D, p1);
+    hipLaunchKernelGGL(k_tri, dim3(maxyy, count), dim3(TRI_T), 0, h->stream, D, p1);
   }
   GemmArgs gi{0, in_id, out_id, epi, pred_slot, slot_shift};
   hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, D, gi);
@@ -1085,20 +1192,20 @@ int enqueue_solve(mcf_handle* h, int kind, int in_id, int out_id, int psolve, do
     const int pred = it == 0 ? -1 : SV_RES1;
     if (pres) {
       PresArgs a{in_id, out_id, B_RESP, 0, slot0, res_slot, pred};
-      hipLaunchKernelGGL(k_pres, grid1(h->psize), dim3(256), 0, h->stream, h->dc, a);
+      hipLaunchKernelGGL(k_pres, gridS(h->psize), dim3(256), 0, h->stream, h->dc, a);
       if ((rc = enqueue_apply(h, first, count, B_RESP, out_id, 2, res_slot, shift))) return rc;
     } else {
       HelmArgs a{in_id, out_id, B_RES, -1, 0, res_slot, pred, dt};
-      hipLaunchKernelGGL(k_helm, grid1(h->velsize), dim3(256), 0, h->stream, h->dc, a);
+      hipLaunchKernelGGL(k_helm, gridS(h->velsize), dim3(256), 0, h->stream, h->dc, a);
       if ((rc = enqueue_apply(h, first, count, B_RES, out_id, 2, res_slot, shift))) return rc;
     }
   }
   if (pres) {
     PresArgs a{in_id, out_id, -1, 1, slot0, SV_FINAL, -1};
-    hipLaunchKernelGGL(k_pres, grid1(h->psize), dim3(256), 0, h->stream, h->dc, a);
+    hipLaunchKernelGGL(k_pres, gridS(h->psize), dim3(256), 0, h->stream, h->dc, a);
   } else {
     HelmArgs a{in_id, out_id, -1, -1, 1, SV_FINAL, -1, dt};
-    hipLaunchKernelGGL(k_helm, grid1(h->velsize), dim3(256), 0, h->stream, h->dc, a);
+    hipLaunchKernelGGL(k_helm, gridS(h->velsize), dim3(256), 0, h->stream, h->dc, a);
   }
   return launch_check(h, "woodbury solve");
 }
@@ -1107,12 +1214,12 @@ int enqueue_solve(mcf_handle* h, int kind, int in_id, int out_id, int psolve, do
 int enqueue_project(mcf_handle* h, int x_id, int out_id, int phi_id, double dt, int psolve, int defect_id) {
   int rc;
   DivArgs d{x_id, B_DIVP, 1.0 / dt, SL_P0 + 8 * psolve + SV_VAL};
-  hipLaunchKernelGGL(k_div, grid1(h->psize), dim3(256), 0, h->stream, h->dc, d);
+  hipLaunchKernelGGL(k_div, gridS(h->psize), dim3(256), 0, h->stream, h->dc, d);
   if ((rc = enqueue_solve(h, MCF_PRESSURE, B_DIVP, phi_id, psolve, dt))) return rc;
   UpdArgs u{x_id, phi_id, out_id, defect_id, dt};
-  hipLaunchKernelGGL(k_update, grid1(h->velsize), dim3(256), 0, h->stream, h->dc, u);
+  hipLaunchKernelGGL(k_update, gridS(h->velsize), dim3(256), 0, h->stream, h->dc, u);
   DivArgs g{out_id, -1, 1.0, SL_DIV0 + psolve};
-  hipLaunchKernelGGL(k_div, grid1(h->psize), dim3(256), 0, h->stream, h->dc, g);
+  hipLaunchKernelGGL(k_div, gridS(h->psize), dim3(256), 0, h->stream, h->dc, g);
   return launch_check(h, "project");
 }
 
@@ -1130,15 +1237,15 @@ int check_ready(mcf_handle* h, double dt) {
   return 0;
 }
 
-// One AB2/CN step (time_integration.py:7-145 + optimized_ghost_cell.py:236-286 + main.py:383-415)
-int enqueue_step(mcf_handle* h, double dt, int row) {
+// One AB2/CN step (time_integration.py:7-145 + optimized_ghost_cell.py:236-286 + main.py:383-415).
+// Launches only; the host-side state (parity, history, dt) is advanced by the caller.
+int enqueue_step_kernels(mcf_handle* h, double dt, int parity, bool first, double prev_dt) {
   int rc;
-  const int S = B_S0 + h->parity, Snew = B_S0 + 1 - h->parity;
-  const int Hprev = B_H0 + 1 - h->parity, Hnow = B_H0 + h->parity;
-  const bool first = !h->has_history;
-  const double alpha = first ? 0.0 : dt / (2.0 * h->prev_dt);
-  const double factor = first ? 1.0 : dt / (dt + h->prev_dt);
-  const dim3 gv = grid1(h->velsize), gp = grid1(h->psize);
+  const int S = B_S0 + parity, Snew = B_S0 + 1 - parity;
+  const int Hprev = B_H0 + 1 - parity, Hnow = B_H0 + parity;
+  const double alpha = first ? 0.0 : dt / (2.0 * prev_dt);
+  const double factor = first ? 1.0 : dt / (dt + prev_dt);
+  const dim3 gv = gridS(h->velsize), gp = gridS(h->psize);
   int nproj = 2;
   if (first) {
     // projected midpoint startup (time_integration.py:33-47)
@@ -1166,14 +1273,61 @@ int enqueue_step(mcf_handle* h, double dt, int row) {
   const int nf = h->hc.gt[0].n + h->hc.gt[1].n + h->ny;
   hipLaunchKernelGGL(k_fill, grid1(nf), dim3(256), 0, h->stream, h->dc, f);
   ForceArgs fa{Snew, B_POUT, h->nu * h->rho};
-  hipLaunchKernelGGL(k_forces, dim3(1, 1), dim3(256), 0, h->stream, h->dc, fa);
-  FinArgs fin{row, h->time + dt, dt, h->rho, first ? 1 : 0, nproj};
+  hipLaunchKernelGGL(k_forces, dim3((h->hc.nsurf + 63) / 64, 1), dim3(64), 0, h->stream, h->dc, fa);
+  hipLaunchKernelGGL(k_force_total, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
+  FinArgs fin{dt, h->rho, first ? 1 : 0, nproj};
   hipLaunchKernelGGL(k_finalize, dim3(1, 1), dim3(64), 0, h->stream, h->dc, fin);
-  if ((rc = launch_check(h, "step"))) return rc;
+  return launch_check(h, "step");
+}
+
+void advance_host_state(mcf_handle* h, double dt) {
   h->parity = 1 - h->parity;
   h->prev_dt = dt;
   h->has_history = true;
   h->time += dt;
+}
+
+void drop_graphs(mcf_handle* h) {
+  for (int p = 0; p < 2; ++p) {
+    if (h->gexec[p]) (void)hipGraphExecDestroy(h->gexec[p]);
+    if (h->graph[p]) (void)hipGraphDestroy(h->graph[p]);
+    h->gexec[p] = nullptr; h->graph[p] = nullptr;
+  }
+  h->graph_dt = -1.0;
+}
+
+// Capture the steady AB2 step (history present, previous dt == dt) once per parity.
+int build_graphs(mcf_handle* h, double dt) {
+  drop_graphs(h);
+  for (int p = 0; p < 2; ++p) {
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step_kernels(h, dt, p, false, dt);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(h, 4, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    h->graph[p] = g;
+    HIPCHK(h, hipGraphInstantiate(&h->gexec[p], g, nullptr, nullptr, 0));
+  }
+  h->graph_dt = dt;
+  return 0;
+}
+
+// n steps: steps without matching history run eagerly, the rest replay the captured graphs.
+int enqueue_steps(mcf_handle* h, int n, double dt) {
+  hipLaunchKernelGGL(k_set_clock, dim3(1, 1), dim3(64), 0, h->stream, h->dc, h->time);
+  for (int s = 0; s < n; ++s) {
+    const bool steady = h->has_history && h->prev_dt == dt;
+    if (!steady || !h->use_graphs) {
+      if (int rc = enqueue_step_kernels(h, dt, h->parity, !h->has_history, h->prev_dt)) return rc;
+    } else {
+      if (h->graph_dt != dt || !h->gexec[0]) {
+        if (int rc = build_graphs(h, dt)) return rc;
+      }
+      HIPCHK(h, hipGraphLaunch(h->gexec[h->parity], h->stream));
+    }
+    advance_host_state(h, dt);
+  }
   return 0;
 }
 
@@ -1201,6 +1355,27 @@ int ensure_diag(mcf_handle* h, int rows) {
   h->diag_cap = cap;
   h->hc.diag = h->diag;
   return sync_descs(h);
+}
+
+// Host view of the slots: max |.| over stripes, sums for the sum slots, stripe 0 for the forces.
+int read_slots(mcf_handle* h, std::vector<double>& out) {
+  std::vector<double> raw(NSLOTW);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(raw.data(), h->slots, NSLOTW * sizeof(double), hipMemcpyDeviceToHost));
+  out.assign(NSLOT, 0.0);
+  for (int k = 0; k < NSLOT; ++k) {
+    const bool force = k >= SL_FORCE && k < SL_FORCE + 6;
+    const bool sum = k >= SL_HU && k < SL_FORCE && ((k - SL_HU) % 8 == SV_SSQR || (k - SL_HU) % 8 == SV_SSQV);
+    double v = 0.0;
+    for (int t = 0; t < NSTRIPE; ++t) {
+      const double x = raw[(size_t)k * NSTRIPE + t];
+      if (force) { v = raw[(size_t)k * NSTRIPE]; break; }
+      if (sum) v += x;
+      else v = (std::fabs(x) > v || x != x) ? std::fabs(x) : v;
+    }
+    out[k] = v;
+  }
+  return 0;
 }
 
 int status_error(mcf_handle* h, const double* d, int step) {
@@ -1241,10 +1416,14 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
     h->bufs.push_back(p);
     h->hc.buf[b] = static_cast<double*>(p);
   }
-  HIPCHK(h, hipMalloc(&h->slots, NSLOT * sizeof(u64)));
-  HIPCHK(h, hipMemset(h->slots, 0, NSLOT * sizeof(u64)));
+  HIPCHK(h, hipMalloc(&h->slots, NSLOTW * sizeof(u64)));
+  HIPCHK(h, hipMemset(h->slots, 0, NSLOTW * sizeof(u64)));
   HIPCHK(h, hipMalloc(&h->inlet, ny * sizeof(double)));
   HIPCHK(h, hipMemset(h->inlet, 0, ny * sizeof(double)));
+  HIPCHK(h, hipMalloc(&h->clock_step, sizeof(int)));
+  HIPCHK(h, hipMalloc(&h->clock_time, sizeof(double)));
+  HIPCHK(h, hipMemset(h->clock_step, 0, sizeof(int)));
+  HIPCHK(h, hipMemset(h->clock_time, 0, sizeof(double)));
   HIPCHK(h, hipMalloc(&h->dc, sizeof(CaseDev)));
   HIPCHK(h, hipMalloc(&h->ds, sizeof(h->hs)));
   CaseDev& c = h->hc;
@@ -1252,6 +1431,9 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   c.dx = dx; c.dy = dy; c.nu = nu; c.rho = rho;
   c.inlet = h->inlet;
   c.sl = h->slots;
+  c.clock_step = h->clock_step;
+  c.clock_time = h->clock_time;
+  if (const char* e = std::getenv("GHOSTFLOW_NO_GRAPH")) h->use_graphs = (e[0] == '0');
   for (int k = 0; k < 3; ++k) {
     SolveDesc& s = h->hs[k];
     s.kind = k;
@@ -1266,6 +1448,8 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
 void mcf_destroy(mcf_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graphs(h);
+  (void)hipFree(h->clock_step); (void)hipFree(h->clock_time);
   for (auto& kv : h->owned) (void)hipFree(kv.second);
   for (void* p : h->bufs) (void)hipFree(p);
   (void)hipFree(h->slots); (void)hipFree(h->inlet); (void)hipFree(h->dc); (void)hipFree(h->ds);
@@ -1301,6 +1485,10 @@ int mcf_set_solid_intervals(mcf_handle* h, int field, const int* lo, const int* 
   if (int rc = upload(h, k + "lo", lo, rows, &dlo)) return rc;
   if (int rc = upload(h, k + "hi", hi, rows, &dhi)) return rc;
   h->hc.lo[field] = dlo; h->hc.hi[field] = dhi;
+  int j0 = rows, j1 = -1, i0 = cols, i1 = -1;
+  for (int j = 0; j < rows; ++j)
+    if (lo[j] <= hi[j]) { j0 = std::min(j0, j); j1 = std::max(j1, j); i0 = std::min(i0, lo[j]); i1 = std::max(i1, hi[j]); }
+  h->hc.sbox[field][0] = j0; h->hc.sbox[field][1] = j1; h->hc.sbox[field][2] = i0; h->hc.sbox[field][3] = i1;
   g_host[h].lo[field].assign(lo, lo + rows);
   g_host[h].hi[field].assign(hi, hi + rows);
   for (int q = 0; q < 3; ++q) {
@@ -1320,13 +1508,22 @@ int mcf_set_ghost(mcf_handle* h, int field, int n, const int* node, const int* p
   for (int k = 0; k < nnz; ++k) if (col[k] < 0 || col[k] >= size) return fail(h, 1, "ghost column out of range");
   GhostTab& g = h->hc.gt[field];
   std::string key = "ghost" + std::to_string(field);
-  int *dn, *dp, *dcol; double* dw;
-  std::vector<int> p0(ptr, ptr + (n ? n + 1 : 0));
+  int width = 1;
+  for (int k = 0; k < n; ++k) width = std::max(width, ptr[k + 1] - ptr[k]);
+  std::vector<int> pc((size_t)n * width);
+  std::vector<double> pw((size_t)n * width, 0.0);
+  for (int k = 0; k < n; ++k)
+    for (int q = 0; q < width; ++q) {
+      const int t = ptr[k] + q;
+      const bool real = t < ptr[k + 1];
+      pc[(size_t)k * width + q] = real ? col[t] : col[ptr[k]];
+      pw[(size_t)k * width + q] = real ? w[t] : 0.0;
+    }
+  int *dn, *dcol; double* dw;
   if (int rc = upload(h, key + "n", node, n, &dn)) return rc;
-  if (int rc = upload(h, key + "p", n ? ptr : nullptr, n ? n + 1 : 0, &dp)) return rc;
-  if (int rc = upload(h, key + "c", col, nnz, &dcol)) return rc;
-  if (int rc = upload(h, key + "w", w, nnz, &dw)) return rc;
-  g.n = n; g.node = dn; g.ptr = dp; g.col = dcol; g.w = dw;
+  if (int rc = upload(h, key + "c", pc.data(), pc.size(), &dcol)) return rc;
+  if (int rc = upload(h, key + "w", pw.data(), pw.size(), &dw)) return rc;
+  g.n = n; g.width = width; g.node = dn; g.col = dcol; g.w = dw;
   h->has_ghost[field] = true;
   return sync_descs(h);
 }
@@ -1375,7 +1572,9 @@ int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double 
   if (!h || kind < 0 || kind > 2 || !V || !lam || !lower || !diag || !upper) return fail(h, 1, "invalid rectangle");
   const int wy = kind == MCF_V ? h->ny - 1 : h->ny;
   if (yy != wy || xx != h->nx) return fail(h, 1, "rectangle shape does not match the grid");
-  if (xx > PCR_T * PCR_EPT) return fail(h, 1, "rectangle wider than the tridiagonal kernel supports");
+  if (xx > TRI_MAXX) return fail(h, 1, "rectangle wider than the tridiagonal kernel supports");
+  for (int i = 1; i < xx - 1; ++i)
+    if (upper[i] != upper[0]) return fail(h, 1, "the x super-diagonal must be constant");
   SolveDesc& s = h->hs[kind];
   std::string key = "rect" + std::to_string(kind);
   std::vector<double> VT((size_t)yy * yy);
@@ -1389,6 +1588,25 @@ int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double 
   if (int rc = upload(h, key + "dg", diag, xx, &ddg)) return rc;
   if (int rc = upload(h, key + "up", upper, xx - 1, &dup)) return rc;
   if (int rc = upload<double>(h, key + "F", nullptr, (size_t)yy * xx, &dF)) return rc;
+  // LU (no pivoting; the x systems are diagonally dominant) of (a + b lam_k) I + b lx, per mode
+  std::vector<double> Lm((size_t)yy * xx), Im((size_t)yy * xx);
+  for (int k = 0; k < yy; ++k) {
+    const double shift = a + b * lam[k];
+    double dprev = shift + b * diag[0];
+    Lm[(size_t)k * xx] = 0.0;
+    Im[(size_t)k * xx] = 1.0 / dprev;
+    for (int i = 1; i < xx; ++i) {
+      const double l = (b * lower[i - 1]) / dprev;
+      const double di = shift + b * diag[i] - l * (b * upper[i - 1]);
+      Lm[(size_t)k * xx + i] = l;
+      Im[(size_t)k * xx + i] = 1.0 / di;
+      dprev = di;
+    }
+  }
+  double *dL, *dI;
+  if (int rc = upload(h, key + "luL", Lm.data(), Lm.size(), &dL)) return rc;
+  if (int rc = upload(h, key + "luI", Im.data(), Im.size(), &dI)) return rc;
+  s.luL = dL; s.luI = dI;
   s.yy = yy; s.xx = xx; s.a = a; s.b = b;
   s.V = dV; s.VT = dVT; s.lam = dl; s.lower = dlo; s.diag = ddg; s.upper = dup; s.F = dF;
   if (kind == MCF_PRESSURE) { s.off = 0; s.ld = h->nx; s.joff = 0; s.ioff = 0; s.field = MCF_FIELD_P; }
@@ -1400,63 +1618,59 @@ int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double 
 }
 
 int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, const int* points,
-                  const int* c_ptr, const int* c_col, const double* c_val, const double* Sinv) {
-  if (!h || kind < 0 || kind > 2 || r < 0 || npts < 0) return fail(h, 1, "invalid capacitance data");
+                  const double* SC) {
+  if (!h || kind < 0 || kind > 2 || r < 0 || npts < 0 || (r && (!rows || !SC)) || (npts && !points))
+    return fail(h, 1, "invalid capacitance data");
   SolveDesc& s = h->hs[kind];
   if (!h->has_rect[kind]) return fail(h, 1, "set the rectangle before its capacitance data");
   if (r > 2048) return fail(h, 1, "Ghost inverse exceeds the bounded boundary-row rank");
   const int yy = s.yy, xx = s.xx, n = yy * xx;
   for (int k = 0; k < r; ++k) if (rows[k] < 0 || rows[k] >= n) return fail(h, 1, "capacitance row out of range");
   for (int k = 0; k < npts; ++k) if (points[k] < 0 || points[k] >= n) return fail(h, 1, "capacitance point out of range");
-  if (r && (c_ptr[0] != 0 || c_ptr[r] < 0)) return fail(h, 1, "invalid capacitance CSR");
-  for (int k = 0; r && k < c_ptr[r]; ++k) if (c_col[k] < 0 || c_col[k] >= npts) return fail(h, 1, "capacitance column out of range");
-  // group the points by x column
+  // points ordered by (x column, y)
   std::vector<int> order(npts);
   for (int k = 0; k < npts; ++k) order[k] = k;
   std::sort(order.begin(), order.end(), [&](int a, int b) {
     int xa = points[a] % xx, xb = points[b] % xx;
     return xa != xb ? xa < xb : points[a] < points[b];
   });
-  std::vector<int> pos(npts), pt_j(npts), grp_ptr, grp_x;
+  std::vector<int> pos(npts), pt_j(npts), pt_x(npts);
   for (int t = 0; t < npts; ++t) {
     const int k = order[t];
     pos[k] = t;
     pt_j[t] = points[k] / xx;
-    const int x = points[k] % xx;
-    if (grp_x.empty() || grp_x.back() != x) { grp_x.push_back(x); grp_ptr.push_back(t); }
+    pt_x[t] = points[k] % xx;
   }
-  grp_ptr.push_back(npts);
-  const int nnz = r ? c_ptr[r] : 0;
-  std::vector<int> ccol(nnz);
-  for (int k = 0; k < nnz; ++k) ccol[k] = pos[c_col[k]];
-  // rows grouped by x
-  std::vector<int> rc_ptr(xx + 1, 0), rc_j(r), rc_id(r);
-  for (int k = 0; k < r; ++k) rc_ptr[rows[k] % xx + 1]++;
-  for (int x = 0; x < xx; ++x) rc_ptr[x + 1] += rc_ptr[x];
-  std::vector<int> fillp(rc_ptr.begin(), rc_ptr.end() - 1);
+  std::vector<double> scg((size_t)r * npts);
+  for (int m = 0; m < r; ++m)
+    for (int k = 0; k < npts; ++k) scg[(size_t)m * npts + pos[k]] = SC[(size_t)m * npts + k];
+  // capacitance rows grouped by x column
+  std::vector<int> cnt(xx, 0);
+  for (int k = 0; k < r; ++k) cnt[rows[k] % xx]++;
+  std::vector<int> rg_ptr, rg_x, start(xx, -1);
+  int acc = 0;
+  for (int x = 0; x < xx; ++x)
+    if (cnt[x]) { rg_x.push_back(x); rg_ptr.push_back(acc); start[x] = acc; acc += cnt[x]; }
+  rg_ptr.push_back(acc);
+  std::vector<int> rc_j(r), rc_id(r);
   for (int k = 0; k < r; ++k) {
-    const int x = rows[k] % xx, t = fillp[x]++;
+    const int x = rows[k] % xx, t = start[x]++;
     rc_j[t] = rows[k] / xx; rc_id[t] = k;
   }
   std::string key = "schur" + std::to_string(kind);
-  int *dgp, *dgx, *dpj, *dcp, *dcc, *drp, *drj, *dri; double *dcv, *dS, *dfk, *dy;
-  const int ng = (int)grp_x.size();
-  if (int rc = upload(h, key + "gp", grp_ptr.data(), grp_ptr.size(), &dgp)) return rc;
-  if (int rc = upload(h, key + "gx", grp_x.data(), grp_x.size(), &dgx)) return rc;
+  int *dpj, *dpx, *drp, *drx, *drj, *dri; double *dS, *dfk, *dy;
   if (int rc = upload(h, key + "pj", pt_j.data(), pt_j.size(), &dpj)) return rc;
-  if (int rc = upload(h, key + "cp", c_ptr, r ? r + 1 : 0, &dcp)) return rc;
-  if (int rc = upload(h, key + "cc", ccol.data(), ccol.size(), &dcc)) return rc;
-  if (int rc = upload(h, key + "cv", c_val, nnz, &dcv)) return rc;
-  if (int rc = upload(h, key + "S", Sinv, (size_t)r * r, &dS)) return rc;
-  if (int rc = upload(h, key + "rp", rc_ptr.data(), rc_ptr.size(), &drp)) return rc;
+  if (int rc = upload(h, key + "px", pt_x.data(), pt_x.size(), &dpx)) return rc;
+  if (int rc = upload(h, key + "SC", scg.data(), scg.size(), &dS)) return rc;
+  if (int rc = upload(h, key + "rp", rg_ptr.data(), rg_ptr.size(), &drp)) return rc;
+  if (int rc = upload(h, key + "rx", rg_x.data(), rg_x.size(), &drx)) return rc;
   if (int rc = upload(h, key + "rj", rc_j.data(), rc_j.size(), &drj)) return rc;
   if (int rc = upload(h, key + "ri", rc_id.data(), rc_id.size(), &dri)) return rc;
   if (int rc = upload<double>(h, key + "fk", nullptr, std::max(npts, 1), &dfk)) return rc;
   if (int rc = upload<double>(h, key + "y", nullptr, std::max(r, 1), &dy)) return rc;
-  s.r = r; s.npts = npts; s.ngroups = ng;
-  s.grp_ptr = dgp; s.grp_x = dgx; s.pt_j = dpj; s.firstK = dfk;
-  s.c_ptr = dcp; s.c_col = dcc; s.c_val = dcv; s.Sinv = dS; s.y = dy;
-  s.rc_ptr = drp; s.rc_j = drj; s.rc_id = dri;
+  s.r = r; s.npts = npts;
+  s.pt_j = dpj; s.pt_x = dpx; s.firstK = dfk; s.SC = dS; s.y = dy;
+  s.rg_n = (int)rg_x.size(); s.rg_ptr = drp; s.rg_x = drx; s.rc_j = drj; s.rc_id = dri;
   h->has_schur[kind] = true;
   return sync_descs(h);
 }
@@ -1485,6 +1699,10 @@ int mcf_set_surface(mcf_handle* h, int n, const int* pp, const int* pc, const do
   if (int rc = upload(h, "snw", normal_weights, 2 * n, &nw)) return rc;
   c.nsurf = n; c.pp = a; c.pc = b; c.pv = x; c.up = d; c.uc = e; c.uv = y; c.vp = f; c.vc = g; c.vv = z;
   c.nrm = nr; c.nw = nw; c.diameter = diameter; c.uref = u_ref;
+  if (n > 64 * FORCE_MAXB) return fail(h, 1, "too many surface points");
+  double* fp;
+  if (int rc = upload<double>(h, "sfp", nullptr, (size_t)4 * ((n + 63) / 64), &fp)) return rc;
+  c.fpart = fp;
   h->has_surface = true;
   return sync_descs(h);
 }
@@ -1545,8 +1763,7 @@ int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   if (int rc = check_ready(h, dt)) return rc;
   if (int rc = ensure_diag(h, n)) return rc;
   h->owned.erase("p_raw_valid");
-  for (int s = 0; s < n; ++s)
-    if (int rc = enqueue_step(h, dt, s)) return rc;
+  if (int rc = enqueue_steps(h, n, dt)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
   HIPCHK(h, hipMemcpy(d.data(), h->diag, d.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1567,9 +1784,9 @@ int mcf_project(mcf_handle* h, const double* q, double dt, double* out, double* 
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (out) HIPCHK(h, hipMemcpy(out, h->hc.buf[B_TV0], h->velsize * sizeof(double), hipMemcpyDeviceToHost));
   if (phi) HIPCHK(h, hipMemcpy(phi, h->hc.buf[B_TP0], h->psize * sizeof(double), hipMemcpyDeviceToHost));
-  std::vector<u64> s(NSLOT);
-  HIPCHK(h, hipMemcpy(s.data(), h->slots, NSLOT * sizeof(u64), hipMemcpyDeviceToHost));
-  auto sd = [&](int k) { double v; std::memcpy(&v, &s[k], 8); return v; };
+  std::vector<double> sv;
+  if (int rc = read_slots(h, sv)) return rc;
+  auto sd = [&](int k) { return sv[k]; };
   const double val = sd(SL_P0 + SV_VAL);
   const double t1 = std::max(1e-10, 32 * kEps * std::max(1.0, val));
   const double r1 = sd(SL_P0 + SV_RES1), r2 = sd(SL_P0 + SV_RES2);
@@ -1600,9 +1817,12 @@ int mcf_forces(mcf_handle* h, const double* vel, const double* p, double mu, dou
   HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TV0], vel, h->velsize * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TP0], p, h->psize * sizeof(double), hipMemcpyHostToDevice, h->stream));
   ForceArgs fa{B_TV0, B_TP0, mu};
-  hipLaunchKernelGGL(k_forces, dim3(1, 1), dim3(256), 0, h->stream, h->dc, fa);
+  hipLaunchKernelGGL(k_forces, dim3((h->hc.nsurf + 63) / 64, 1), dim3(64), 0, h->stream, h->dc, fa);
+  hipLaunchKernelGGL(k_force_total, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(out6, h->slots + SL_FORCE, 6 * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> sv;
+  if (int rc = read_slots(h, sv)) return rc;
+  for (int q = 0; q < 6; ++q) out6[q] = sv[SL_FORCE + q];
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -1626,11 +1846,11 @@ int mcf_rect_inverse(mcf_handle* h, int kind, const double* rhs, double* out) {
   t.bufs = tb; t.off = 0; t.ld = t.xx; t.lo = nullptr; t.hi = nullptr;
   HIPCHK(h, hipMemcpy(td, &t, sizeof(SolveDesc), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(h->tmp_rect, rhs, n * sizeof(double), hipMemcpyHostToDevice));
-  dim3 gg((t.xx + GT - 1) / GT, (t.yy + GT - 1) / GT, 1);
+  dim3 gg((t.xx + GTN - 1) / GTN, (t.yy + GTM - 1) / GTM, 1);
   GemmArgs g{1, 0, 0, 0, -1, 0};
   hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, td, g);
   PcrArgs p0{0, -1, 0};
-  hipLaunchKernelGGL(k_pcr, dim3(t.yy, 1), dim3(PCR_T), 4 * t.xx * sizeof(double), h->stream, td, p0);
+  hipLaunchKernelGGL(k_tri, dim3(t.yy, 1), dim3(TRI_T), 0, h->stream, td, p0);
   GemmArgs gi{0, 0, 0, 0, -1, 0};
   hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, td, gi);
   int rc = launch_check(h, "rect inverse");
@@ -1657,16 +1877,17 @@ int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, dou
     const int comp = pres ? 0 : (k < h->usize ? 0 : 1);
     vm[comp] = std::max(vm[comp], std::fabs(hv[k]));
   }
-  std::vector<u64> s(NSLOT, 0ull);
-  if (pres) std::memcpy(&s[SL_P0 + SV_VAL], &vm[0], 8);
-  else { std::memcpy(&s[SL_HU + SV_VAL], &vm[0], 8); std::memcpy(&s[SL_HV + SV_VAL], &vm[1], 8); }
-  HIPCHK(h, hipMemcpyAsync(h->slots, s.data(), NSLOT * sizeof(u64), hipMemcpyHostToDevice, h->stream));
+  std::vector<double> s0(NSLOTW, 0.0);
+  if (pres) s0[(size_t)(SL_P0 + SV_VAL) * NSTRIPE] = vm[0];
+  else { s0[(size_t)(SL_HU + SV_VAL) * NSTRIPE] = vm[0]; s0[(size_t)(SL_HV + SV_VAL) * NSTRIPE] = vm[1]; }
+  HIPCHK(h, hipMemcpyAsync(h->slots, s0.data(), NSLOTW * sizeof(double), hipMemcpyHostToDevice, h->stream));
   const double dt = pres ? 1.0 : -2.0 * h->hs[1].b / h->nu;
   if (int rc = enqueue_solve(h, pres ? MCF_PRESSURE : MCF_U, in_id, out_id, 0, dt)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(out, h->hc.buf[out_id], n * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHK(h, hipMemcpy(s.data(), h->slots, NSLOT * sizeof(u64), hipMemcpyDeviceToHost));
-  auto sd = [&](int k) { double v; std::memcpy(&v, &s[k], 8); return v; };
+  std::vector<double> sv;
+  if (int rc = read_slots(h, sv)) return rc;
+  auto sd = [&](int k) { return sv[k]; };
   if (info4) {
     if (pres) {
       const double val = sd(SL_P0 + SV_VAL), t1 = std::max(1e-10, 32 * kEps * std::max(1.0, val));
@@ -1692,7 +1913,7 @@ int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, dou
 int mcf_apply_operator(mcf_handle* h, int op, const double* in, double* out, double dt) {
   if (!h || !in || !out || op < 0 || op > 4) return fail(h, 1, "invalid operator arguments");
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
-  const dim3 gv = grid1(h->velsize), gp = grid1(h->psize);
+  const dim3 gv = gridS(h->velsize), gp = gridS(h->psize);
   if (op == MCF_APPLY_GRADIENT) {
     HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TP0], in, h->psize * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemsetAsync(h->hc.buf[B_TV0], 0, h->velsize * sizeof(double), h->stream));
@@ -1733,14 +1954,20 @@ int mcf_divergence_stats(mcf_handle* h, const double* vel, const unsigned char* 
   unsigned char* dm = nullptr;
   double* acc = nullptr;
   HIPCHK(h, hipMalloc(&dm, h->psize));
-  HIPCHK(h, hipMalloc(&acc, 5 * sizeof(double)));
+  HIPCHK(h, hipMalloc(&acc, 5 * NSTRIPE * sizeof(double)));
   HIPCHK(h, hipMemcpy(dm, mask, h->psize, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemset(acc, 0, 5 * sizeof(double)));
+  HIPCHK(h, hipMemset(acc, 0, 5 * NSTRIPE * sizeof(double)));
   HIPCHK(h, hipMemcpy(h->hc.buf[B_TV0], vel, h->velsize * sizeof(double), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_divstats, grid1(h->psize), dim3(256), 0, h->stream, h->hc.buf[B_TV0],
                      h->hc.buf[B_TV0] + h->usize, dm, h->nx, h->ny, h->dx, h->dy, acc);
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(out5, acc, 5 * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> raw(5 * NSTRIPE);
+  HIPCHK(h, hipMemcpy(raw.data(), acc, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int q = 0; q < 5; ++q) {
+    double v = 0.0;
+    for (int t = 0; t < NSTRIPE; ++t) v = q == 0 ? std::max(v, raw[q * NSTRIPE + t]) : v + raw[q * NSTRIPE + t];
+    out5[q] = v;
+  }
   (void)hipFree(dm); (void)hipFree(acc);
   return 0;
 }
@@ -1752,9 +1979,10 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   hipEvent_t e0, e1;
   HIPCHK(h, hipEventCreate(&e0)); HIPCHK(h, hipEventCreate(&e1));
   h->owned.erase("p_raw_valid");
+  // one untimed step settles the history so that the timed steps replay the captured graph
+  if (int rc = enqueue_steps(h, 1, dt)) return rc;
   HIPCHK(h, hipEventRecord(e0, h->stream));
-  for (int s = 0; s < n; ++s)
-    if (int rc = enqueue_step(h, dt, s)) return rc;
+  if (int rc = enqueue_steps(h, n, dt)) return rc;
   HIPCHK(h, hipEventRecord(e1, h->stream));
   HIPCHK(h, hipEventSynchronize(e1));
   float ms = 0;
@@ -1762,7 +1990,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   out5[2] = ms / n;
   // dominant kernels, timed back to back on the same stream over the pressure problem
   const SolveDesc& p = h->hs[0];
-  dim3 gg((p.xx + GT - 1) / GT, (p.yy + GT - 1) / GT, 1);
+  dim3 gg((p.xx + GTN - 1) / GTN, (p.yy + GTM - 1) / GTM, 1);
   const int reps = 50;
   GemmArgs g{1, B_DIVP, B_DIVP, 0, -1, 0};
   HIPCHK(h, hipEventRecord(e0, h->stream));
@@ -1774,7 +2002,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   PcrArgs p0{0, -1, 0};
   HIPCHK(h, hipEventRecord(e0, h->stream));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(k_pcr, dim3(p.yy, 1), dim3(PCR_T), 4 * p.xx * sizeof(double), h->stream, h->ds, p0);
+    hipLaunchKernelGGL(k_tri, dim3(p.yy, 1), dim3(TRI_T), 0, h->stream, h->ds, p0);
   HIPCHK(h, hipEventRecord(e1, h->stream));
   HIPCHK(h, hipEventSynchronize(e1));
   HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));

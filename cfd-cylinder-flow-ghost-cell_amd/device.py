@@ -132,11 +132,8 @@ class GhostDevice:
         self.h.call('mcf_set_rectangle', kind, yy, xx, float(factor.a), float(factor.b), nat.dptr(nat.f64(factor.V)),
                     nat.dptr(nat.f64(factor.lam)), nat.dptr(nat.f64(factor.lower)), nat.dptr(nat.f64(factor.diag)),
                     nat.dptr(nat.f64(factor.upper)))
-        C = schur.Cp.tocsr()
-        C.sort_indices()
         self.h.call('mcf_set_schur', kind, schur.rank, nat.iptr(nat.i32(schur.rows)), len(schur.points),
-                    nat.iptr(nat.i32(schur.points)), nat.iptr(nat.i32(C.indptr)), nat.iptr(nat.i32(C.indices)),
-                    nat.dptr(nat.f64(C.data)), nat.dptr(nat.f64(schur.Sinv)))
+                    nat.iptr(nat.i32(schur.points)), nat.dptr(nat.f64(schur.SC)))
 
     def _upload_surface(self):
         o = self.ops

@@ -447,3 +447,4 @@ class SchurFactor:
         self.S = np.eye(self.rank) + Cp @ W
         self.Sinv = np.linalg.inv(self.S)
         self.Cp = Cp                                              # C restricted to the points
+        self.SC = np.ascontiguousarray((Cp.T @ self.Sinv.T).T)   # S^{-1} C, dense r x npts

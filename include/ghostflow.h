@@ -77,10 +77,12 @@ int mcf_set_override(mcf_handle* h, int op, int n, const int* row, const int* pt
    lam (yy) its eigenvalues, lx given by lower/diag/upper (xx-1, xx, xx-1). */
 int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double b, const double* V,
                       const double* lam, const double* lower, const double* diag, const double* upper);
-/* Boundary-row capacitance: rows (r, rectangle flat indices), points (npts, rectangle flat
-   indices of C's columns), C as CSR over points (r rows), Sinv dense r*r. */
+/* Boundary-row capacitance (pressure_schur.py:85-101): rows (r, rectangle flat indices of the
+   rows where the embedded operator differs from the rectangle), points (npts, rectangle flat
+   indices of the columns those rows touch) and SC = S^{-1} C restricted to the points, dense
+   r*npts row-major, with S = I + C R^{-1} E_rows the capacitance matrix. */
 int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, const int* points,
-                  const int* c_ptr, const int* c_col, const double* c_val, const double* Sinv);
+                  const double* SC);
 /* Surface quadrature: n points; pressure stencil CSR over p, du/dn over u, dv/dn over v;
    normals (n*2: nx, ny) and weights (n*2). */
 int mcf_set_surface(mcf_handle* h, int n, const int* pp, const int* pc, const double* pv,
