@@ -59,6 +59,10 @@ _SIGNATURES = {
     'mcf_apply_operator': (ctypes.c_int, [_H, ctypes.c_int, _D, _D, ctypes.c_double]),
     'mcf_divergence_stats': (ctypes.c_int, [_H, _D, _U8, _D]),
     'mcf_profile': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_double, _D]),
+    'mcf_batch_create': (ctypes.c_int, [ctypes.POINTER(_H), _D, ctypes.c_int, ctypes.POINTER(_H)]),
+    'mcf_batch_destroy': (None, [_H]),
+    'mcf_batch_last_error': (ctypes.c_char_p, [_H]),
+    'mcf_batch_run': (ctypes.c_int, [_H, ctypes.c_int, _D]),
 }
 
 _lib = None
@@ -97,6 +101,35 @@ def iptr(a):
     return a.ctypes.data_as(_I)
 
 
+def as_arg(a):
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous:
+            raise ValueError('array arguments must be C-contiguous')
+        if a.dtype == np.float64:
+            return a.ctypes.data_as(_D)
+        if a.dtype == np.int32:
+            return a.ctypes.data_as(_I)
+        if a.dtype == np.uint8:
+            return a.ctypes.data_as(_U8)
+        raise TypeError(f'unsupported array dtype {a.dtype}')
+    return a
+
+
+class Recorder:
+    """Stands in for a Handle while a case's tables are prepared away from the device: records the
+    table uploads (entry point name and array arguments) for ``replay`` on a real handle."""
+
+    def __init__(self):
+        self.calls = []
+
+    def call(self, name, *args):
+        self.calls.append((name, tuple(np.array(a, copy=True) if isinstance(a, np.ndarray) else a for a in args)))
+
+    def replay(self, handle):
+        for name, args in self.calls:
+            handle.call(name, *args)
+
+
 def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
@@ -124,7 +157,8 @@ class Handle:
             raise STATUS_EXCEPTIONS.get(rc, RuntimeError)(self.lib.mcf_last_error(self.h).decode())
 
     def call(self, name, *args):
-        self.check(getattr(self.lib, name)(self.h, *args))
+        """Call an entry point; float64 / int32 / uint8 numpy arrays are passed as pointers."""
+        self.check(getattr(self.lib, name)(self.h, *[as_arg(a) for a in args]))
 
     def close(self):
         if getattr(self, 'h', None) is not None and self.h.value:

@@ -96,6 +96,7 @@ struct CaseDev {
   double* diag;  // diag ring
   int* clock_step;     // row of the next diagnostics entry
   double* clock_time;  // simulated time of the last completed step
+  double* prm;         // per-case step parameters: dt, AB2 alpha = dt/(2 dt_prev), factor = dt/(dt+dt_prev)
 };
 
 // One separable solve problem (one rectangle of one case)
@@ -243,7 +244,6 @@ constexpr int GSB = 4096;
 struct AdvArgs {
   int q_id, adv_id, p_id, prev_id, now_id, out_id;
   int mode;  // 0: AB2 rhs, 1: projected-midpoint, 2: rhs with explicit = adv(adv_id), 3: out = L q + offset
-  double dt, alpha;
 };
 
 __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a) {
@@ -259,6 +259,7 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
   const double* in = c.inlet;
   const double* Av = A + c.usize;
   const double* Qv = Q + c.usize;
+  const double dt = c.prm[0], alpha = c.prm[1];
   double mu = 0.0, mv = 0.0;
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
     double res = 0.0, nw = 0.0;
@@ -296,16 +297,16 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
         const double doff = (i == 1) ? c.nu * in[j] * idx2 : 0.0;
         if (a.mode == 0) {
           nw = adv;
-          const double ex = (1.0 + a.alpha) * adv - a.alpha * prev[idx];
-          res = q + a.dt * (ex + 0.5 * diff - grad) + 0.5 * a.dt * doff;
+          const double ex = (1.0 + alpha) * adv - alpha * prev[idx];
+          res = q + dt * (ex + 0.5 * diff - grad) + 0.5 * dt * doff;
         } else if (a.mode == 1) {
           nw = adv;
-          res = ((adv + diff) - grad) * (0.5 * a.dt) + q;
+          res = ((adv + diff) - grad) * (0.5 * dt) + q;
         } else if (a.mode == 3) {
           nw = adv;
           res = diff;
         } else {
-          res = q + a.dt * (adv + 0.5 * diff - grad) + 0.5 * a.dt * doff;
+          res = q + dt * (adv + 0.5 * diff - grad) + 0.5 * dt * doff;
         }
       }
       mu = nanmax(mu, fabs(res));
@@ -337,16 +338,16 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
                                    : (Pp[j * nx + i] - Pp[(j - 1) * nx + i]) / dy / c.rho;
         if (a.mode == 0) {
           nw = adv;
-          const double ex = (1.0 + a.alpha) * adv - a.alpha * prev[idx];
-          res = q + a.dt * (ex + 0.5 * diff - grad);
+          const double ex = (1.0 + alpha) * adv - alpha * prev[idx];
+          res = q + dt * (ex + 0.5 * diff - grad);
         } else if (a.mode == 1) {
           nw = adv;
-          res = ((adv + diff) - grad) * (0.5 * a.dt) + q;
+          res = ((adv + diff) - grad) * (0.5 * dt) + q;
         } else if (a.mode == 3) {
           nw = adv;
           res = diff;
         } else {
-          res = q + a.dt * (adv + 0.5 * diff - grad);
+          res = q + dt * (adv + 0.5 * diff - grad);
         }
       }
       mv = nanmax(mv, fabs(res));
@@ -367,7 +368,6 @@ struct HelmArgs {
   int mode;   // 0: residual (out = b - Hx, slot RES_k), 1: final (max only), 2: defect
   int res_slot;  // SV_RES1/SV_RES2/SV_FINAL
   int pred_slot; // -1 always, else run only where that refinement happened
-  double dt;
 };
 
 __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) {
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) 
     double* O = a.out_id >= 0 ? c.buf[a.out_id] : nullptr;
     double* O2 = a.out2_id >= 0 ? c.buf[a.out2_id] : nullptr;
     const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
-    const double h = -0.5 * a.dt;
+    const double h = -0.5 * c.prm[0];
     const int lo = run_u ? 0 : c.usize, hi = run_v ? c.velsize : c.usize;
     for (int idx = lo + blockIdx.x * blockDim.x + threadIdx.x; idx < hi; idx += gridDim.x * blockDim.x) {
       double res = 0.0, r2 = 0.0;
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) 
 }
 
 // Divergence of a vel q-array: (D q + div_offset) * scale  (optimized_ghost_cell.py:212-216)
-struct DivArgs { int x_id, out_id; double scale; int slot; };
+struct DivArgs { int x_id, out_id; double dtmul; int slot; };   // dtmul 0: no scaling, else / (dtmul dt)
 
 __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
   const CaseDev& c = cases[blockIdx.y];
@@ -453,6 +453,7 @@ __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
   const double* X = c.buf[a.x_id];
   const double* Xv = X + c.usize;
   double* O = a.out_id >= 0 ? c.buf[a.out_id] : nullptr;
+  const double scale = a.dtmul == 0.0 ? 1.0 : 1.0 / (a.dtmul * c.prm[0]);
   double m = 0.0;
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
         const double ul = (i == 0) ? c.inlet[j] : X[j * nx1 + i];
         d = (X[j * nx1 + i + 1] - ul) / c.dx + (Xv[(j + 1) * nx + i] - Xv[j * nx + i]) / c.dy;
       }
-      d *= a.scale;
+      d *= scale;
     }
     if (O) O[idx] = d;
     m = nanmax(m, fabs(d));
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
 
 // Projection update out = q - dt G phi (numba_kernels.py:619-623), with the optional
 // coupled momentum residual -defect + dt G phi (time_integration.py:137-142)
-struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dt; };
+struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dtmul; };   // dt = dtmul * case dt
 
 __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a) {
   const CaseDev& c = cases[blockIdx.y];
@@ -486,6 +487,7 @@ __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a)
   const double* Pp = c.buf[a.phi_id];
   double* O = c.buf[a.out_id];
   const double* Df = a.defect_id >= 0 ? c.buf[a.defect_id] : nullptr;
+  const double dt = a.dtmul * c.prm[0];
   double m = 0.0;
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
     double out = 0.0, g = 0.0;
@@ -507,9 +509,9 @@ __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a)
         g = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp) : (Pp[j * nx + i] - Pp[(j - 1) * nx + i]) / c.dy / c.rho;
       }
     }
-    if (act) out = X[idx] - a.dt * g;
+    if (act) out = X[idx] - dt * g;
     O[idx] = out;
-    if (Df && act) m = nanmax(m, fabs(-Df[idx] + a.dt * g));
+    if (Df && act) m = nanmax(m, fabs(-Df[idx] + dt * g));
   }
   if (Df) {
     double v1[1] = {m};
@@ -867,18 +869,19 @@ __global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) 
 }
 
 // half_pressure = pressure + increment; output = half + factor * increment
-struct PressArgs { int p_id, inc_id, half_id, out_id; double factor; };
+struct PressArgs { int p_id, inc_id, half_id, out_id; };
 __global__ __launch_bounds__(256) void k_pressures(const CaseDev* cases, PressArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const double* P = c.buf[a.p_id];
   const double* I = c.buf[a.inc_id];
   double* H = c.buf[a.half_id];
   double* O = c.buf[a.out_id];
+  const double factor = c.prm[2];
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
     const double inc = I[idx];
     const double half = P[idx] + inc;
     H[idx] = half;
-    O[idx] = half + a.factor * inc;
+    O[idx] = half + factor * inc;
   }
 }
 
@@ -902,8 +905,9 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a
     const double dot = du * nxk + dv * nyk;
     s[0] = ps * c.nw[2 * k];
     s[1] = ps * c.nw[2 * k + 1];
-    s[2] = a.mu * (du + nxk * dot);
-    s[3] = a.mu * (dv + nyk * dot);
+    const double mu = a.mu > 0.0 ? a.mu : c.nu * c.rho;
+    s[2] = mu * (du + nxk * dot);
+    s[3] = mu * (dv + nyk * dot);
   }
   #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -927,7 +931,7 @@ __global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
 }
 
 // Gather the step's scalars into the diagnostics ring and clear the slots (one wave per case)
-struct FinArgs { double dt, rho; int first; int nproj; };
+struct FinArgs { int nproj; };
 constexpr int FIN_N = 34;
 __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a) {
   const CaseDev& c = cases[blockIdx.y];
@@ -956,8 +960,9 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
   double f[6];
   for (int q = 0; q < 6; ++q) f[q] = slot_d(s, (SL_FORCE + q) * NSTRIPE);
   if (lane == 0) {
+    const double dt = c.prm[0];
     const int row = *c.clock_step;
-    const double time = *c.clock_time + a.dt;
+    const double time = *c.clock_time + dt;
     *c.clock_step = row + 1;
     *c.clock_time = time;
     const double umax = v[0], vmax = v[1], hu_val = v[2], hv_val = v[3], hu_fin = v[4], hv_fin = v[5];
@@ -970,15 +975,15 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
     }
     const double pssr[3] = {v[32], vs[0], vs[2]}, pssv[3] = {v[33], vs[1], vs[3]};
     double* d = c.diag + (size_t)row * MCF_DIAG_LEN;
-    const double denom = 0.5 * a.rho * c.uref * c.uref * c.diameter;
+    const double denom = 0.5 * c.rho * c.uref * c.uref * c.diameter;
     for (int k = 0; k < MCF_DIAG_LEN; ++k) d[k] = 0.0;
-    d[MCF_D_TIME] = time; d[MCF_D_DT] = a.dt;
+    d[MCF_D_TIME] = time; d[MCF_D_DT] = dt;
     d[MCF_D_FX] = f[0]; d[MCF_D_FY] = f[1];
     d[MCF_D_CD] = f[0] / denom; d[MCF_D_CL] = f[1] / denom;
     d[MCF_D_CD_P] = f[2] / denom; d[MCF_D_CL_P] = f[3] / denom;
     d[MCF_D_CD_V] = f[4] / denom; d[MCF_D_CL_V] = f[5] / denom;
     d[MCF_D_MAXDIV] = div[0];
-    d[MCF_D_CFL] = a.dt * (umax / c.dx + vmax / c.dy);
+    d[MCF_D_CFL] = dt * (umax / c.dx + vmax / c.dy);
     const double rhs_max = fmax(hu_val, hv_val);
     const double mres = nanmax(hu_fin, hv_fin);
     d[MCF_D_MOMENTUM_RES] = mres;
@@ -1018,6 +1023,16 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
   }
   u64* wsl = c.sl;
   for (int k = lane; k < NSLOTW; k += 64) wsl[k] = 0ull;
+}
+
+// Step parameters of every case of the launch; dt <= 0 keeps each case's own dt
+__global__ void k_set_prm(const CaseDev* cases, double dt, double alpha, double factor) {
+  const CaseDev& c = cases[blockIdx.y];
+  if (threadIdx.x == 0) {
+    if (dt > 0.0) c.prm[0] = dt;
+    c.prm[1] = alpha;
+    c.prm[2] = factor;
+  }
 }
 
 __global__ void k_set_clock(const CaseDev* cases, double t0) {
@@ -1068,6 +1083,7 @@ __global__ __launch_bounds__(256) void k_divstats(const double* u, const double*
 // ======================================================================================
 // Host side
 // ======================================================================================
+struct StepGraphs;
 struct mcf_handle {
   int nx, ny, usize, vsize, psize, velsize;
   double dx, dy, nu, rho;
@@ -1093,12 +1109,14 @@ struct mcf_handle {
   int parity = 0;      // state in S[parity], history in H[parity]
   double* tmp_rect = nullptr;   // scratch for the rectangle-inverse entry point
   size_t tmp_rect_n = 0;
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   double graph_dt = -1.0;
   bool use_graphs = true;
   int* clock_step = nullptr;
   double* clock_time = nullptr;
+  double* prm = nullptr;
+  bool prm_valid = false;
+  double prm_dt = 0.0, prm_alpha = 0.0, prm_factor = 0.0;
+  StepGraphs* graphs = nullptr;
 };
 
 namespace {
@@ -1153,74 +1171,104 @@ int sv_block(int kind, int psolve) {
   return SL_P0 + 8 * psolve;
 }
 
-// One Woodbury apply (in -> out): epi 1 store masked, 2 accumulate masked.
-// Problems [first, first+count) of the SolveDesc table.
-int enqueue_apply(mcf_handle* h, int first, int count, int in_id, int out_id, int epi, int pred_slot, int slot_shift) {
-  int maxyy = 0, maxxx = 0, maxp = 0, maxr = 0;
-  for (int q = first; q < first + count; ++q) {
-    maxyy = std::max(maxyy, h->hs[q].yy); maxxx = std::max(maxxx, h->hs[q].xx);
-    maxp = std::max(maxp, h->hs[q].npts); maxr = std::max(maxr, h->hs[q].r);
+// Everything a launch needs: the case table, the solve problems and the launch shapes (maxima
+// over the cases).  One case and a batch of uneven cases use the same code.
+struct Ctx {
+  mcf_handle* err;       // receives error messages
+  hipStream_t stream;
+  const CaseDev* dc;     // n cases
+  const SolveDesc* dsp;  // n pressure problems
+  const SolveDesc* dsh;  // 2n Helmholtz problems (u, v per case)
+  int n;
+  int maxvel, maxp, maxfill, maxsurf;
+  int p_yy, p_xx, p_np, p_r, h_yy, h_xx, h_np, h_r;
+};
+
+void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
+  x.maxvel = std::max(x.maxvel, c.velsize);
+  x.maxp = std::max(x.maxp, c.psize);
+  x.maxfill = std::max(x.maxfill, c.gt[0].n + c.gt[1].n + c.ny);
+  x.maxsurf = std::max(x.maxsurf, c.nsurf);
+  x.p_yy = std::max(x.p_yy, hs3[0].yy); x.p_xx = std::max(x.p_xx, hs3[0].xx);
+  x.p_np = std::max(x.p_np, hs3[0].npts); x.p_r = std::max(x.p_r, hs3[0].r);
+  for (int q = 1; q < 3; ++q) {
+    x.h_yy = std::max(x.h_yy, hs3[q].yy); x.h_xx = std::max(x.h_xx, hs3[q].xx);
+    x.h_np = std::max(x.h_np, hs3[q].npts); x.h_r = std::max(x.h_r, hs3[q].r);
   }
-  const SolveDesc* D = h->ds + first;
+}
+
+Ctx ctx_of(mcf_handle* h) {
+  Ctx x{};
+  x.err = h; x.stream = h->stream; x.dc = h->dc; x.dsp = h->ds; x.dsh = h->ds + 1; x.n = 1;
+  ctx_shapes(x, h->hc, h->hs);
+  return x;
+}
+
+inline dim3 gridSn(int n, int cases) { dim3 g = gridS(n); g.y = cases; return g; }
+inline dim3 grid1n(int n, int cases) { dim3 g = grid1(std::max(n, 1)); g.y = cases; return g; }
+
+// One Woodbury apply (in -> out): epi 1 store masked, 2 accumulate masked.
+int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int pred_slot, int slot_shift) {
+  const SolveDesc* D = pres ? x.dsp : x.dsh;
+  const int count = pres ? x.n : 2 * x.n;
+  const int yy = pres ? x.p_yy : x.h_yy, xx = pres ? x.p_xx : x.h_xx;
+  const int np = pres ? x.p_np : x.h_np, r = pres ? x.p_r : x.h_r;
   GemmArgs g{1, in_id, out_id, 0, pred_slot, slot_shift};
-  dim3 gg((maxxx + GTN - 1) / GTN, (maxyy + GTM - 1) / GTM, count);
-  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, D, g);
+  dim3 gg((xx + GTN - 1) / GTN, (yy + GTM - 1) / GTM, count);
+  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, x.stream, D, g);
   PcrArgs p0{0, pred_slot, slot_shift};
-  hipLaunchKernelGGL(k_tri, dim3(maxyy, count), dim3(TRI_T), 0, h->stream, D, p0);
-  if (maxr > 0) {
-    hipLaunchKernelGGL(k_points, dim3((maxp + 3) / 4, count), dim3(256), 0, h->stream, D, p0);
-    hipLaunchKernelGGL(k_capacitance, dim3((maxr + 3) / 4, count), dim3(256), 0, h->stream, D, p0);
+  hipLaunchKernelGGL(k_tri, dim3(yy, count), dim3(TRI_T), 0, x.stream, D, p0);
+  if (r > 0) {
+    hipLaunchKernelGGL(k_points, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+    hipLaunchKernelGGL(k_capacitance, dim3((r + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
     PcrArgs p1{1, pred_slot, slot_shift};
-    hipLaunchKernelGGL(k_tri, dim3(maxyy, count), dim3(TRI_T), 0, h->stream, D, p1);
+    hipLaunchKernelGGL(k_tri, dim3(yy, count), dim3(TRI_T), 0, x.stream, D, p1);
   }
   GemmArgs gi{0, in_id, out_id, epi, pred_slot, slot_shift};
-  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, D, gi);
-  return launch_check(h, "woodbury apply");
+  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, x.stream, D, gi);
+  return launch_check(x.err, "woodbury apply");
 }
 
 // Woodbury solve with <= 2 refinements (pressure_schur.py:116-124) and the final residual.
-// kind: MCF_PRESSURE (problem 0, p-buffers) or MCF_U (problems 1,2: u and v together).
-int enqueue_solve(mcf_handle* h, int kind, int in_id, int out_id, int psolve, double dt) {
-  const bool pres = kind == MCF_PRESSURE;
-  const int first = pres ? 0 : 1, count = pres ? 1 : 2;
+int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve) {
   const int shift = pres ? 8 * psolve : 0;
   const int slot0 = pres ? SL_P0 + 8 * psolve : SL_HU;
   int rc;
-  if ((rc = enqueue_apply(h, first, count, in_id, out_id, 1, -1, shift))) return rc;
+  if ((rc = enqueue_apply(x, pres, in_id, out_id, 1, -1, shift))) return rc;
   for (int it = 0; it < 2; ++it) {
     const int res_slot = it == 0 ? SV_RES1 : SV_RES2;
     const int pred = it == 0 ? -1 : SV_RES1;
     if (pres) {
       PresArgs a{in_id, out_id, B_RESP, 0, slot0, res_slot, pred};
-      hipLaunchKernelGGL(k_pres, gridS(h->psize), dim3(256), 0, h->stream, h->dc, a);
-      if ((rc = enqueue_apply(h, first, count, B_RESP, out_id, 2, res_slot, shift))) return rc;
+      hipLaunchKernelGGL(k_pres, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, a);
+      if ((rc = enqueue_apply(x, pres, B_RESP, out_id, 2, res_slot, shift))) return rc;
     } else {
-      HelmArgs a{in_id, out_id, B_RES, -1, 0, res_slot, pred, dt};
-      hipLaunchKernelGGL(k_helm, gridS(h->velsize), dim3(256), 0, h->stream, h->dc, a);
-      if ((rc = enqueue_apply(h, first, count, B_RES, out_id, 2, res_slot, shift))) return rc;
+      HelmArgs a{in_id, out_id, B_RES, -1, 0, res_slot, pred};
+      hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
+      if ((rc = enqueue_apply(x, pres, B_RES, out_id, 2, res_slot, shift))) return rc;
     }
   }
   if (pres) {
     PresArgs a{in_id, out_id, -1, 1, slot0, SV_FINAL, -1};
-    hipLaunchKernelGGL(k_pres, gridS(h->psize), dim3(256), 0, h->stream, h->dc, a);
+    hipLaunchKernelGGL(k_pres, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, a);
   } else {
-    HelmArgs a{in_id, out_id, -1, -1, 1, SV_FINAL, -1, dt};
-    hipLaunchKernelGGL(k_helm, gridS(h->velsize), dim3(256), 0, h->stream, h->dc, a);
+    HelmArgs a{in_id, out_id, -1, -1, 1, SV_FINAL, -1};
+    hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
   }
-  return launch_check(h, "woodbury solve");
+  return launch_check(x.err, "woodbury solve");
 }
 
-// project(x, dt): div -> pressure solve -> update -> divergence gate (optimized_ghost_cell.py:218-229)
-int enqueue_project(mcf_handle* h, int x_id, int out_id, int phi_id, double dt, int psolve, int defect_id) {
+// project(x, dtmul * dt): div -> pressure solve -> update -> divergence gate (optimized_ghost_cell.py:218-229)
+int enqueue_project(const Ctx& x, int x_id, int out_id, int phi_id, double dtmul, int psolve, int defect_id) {
   int rc;
-  DivArgs d{x_id, B_DIVP, 1.0 / dt, SL_P0 + 8 * psolve + SV_VAL};
-  hipLaunchKernelGGL(k_div, gridS(h->psize), dim3(256), 0, h->stream, h->dc, d);
-  if ((rc = enqueue_solve(h, MCF_PRESSURE, B_DIVP, phi_id, psolve, dt))) return rc;
-  UpdArgs u{x_id, phi_id, out_id, defect_id, dt};
-  hipLaunchKernelGGL(k_update, gridS(h->velsize), dim3(256), 0, h->stream, h->dc, u);
-  DivArgs g{out_id, -1, 1.0, SL_DIV0 + psolve};
-  hipLaunchKernelGGL(k_div, gridS(h->psize), dim3(256), 0, h->stream, h->dc, g);
-  return launch_check(h, "project");
+  DivArgs d{x_id, B_DIVP, dtmul, SL_P0 + 8 * psolve + SV_VAL};
+  hipLaunchKernelGGL(k_div, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, d);
+  if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve))) return rc;
+  UpdArgs u{x_id, phi_id, out_id, defect_id, dtmul};
+  hipLaunchKernelGGL(k_update, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, u);
+  DivArgs g{out_id, -1, 0.0, SL_DIV0 + psolve};
+  hipLaunchKernelGGL(k_div, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, g);
+  return launch_check(x.err, "project");
 }
 
 int check_ready(mcf_handle* h, double dt) {
@@ -1237,48 +1285,79 @@ int check_ready(mcf_handle* h, double dt) {
   return 0;
 }
 
-// One AB2/CN step (time_integration.py:7-145 + optimized_ghost_cell.py:236-286 + main.py:383-415).
-// Launches only; the host-side state (parity, history, dt) is advanced by the caller.
-int enqueue_step_kernels(mcf_handle* h, double dt, int parity, bool first, double prev_dt) {
+// One AB2/CN step of every case of the context (time_integration.py:7-145,
+// optimized_ghost_cell.py:236-286, main.py:383-415).  Launches only: dt, alpha and the pressure
+// factor are read from each case's parameters (k_set_prm); the caller advances the host state.
+int enqueue_step_kernels(const Ctx& x, int parity, bool first) {
   int rc;
   const int S = B_S0 + parity, Snew = B_S0 + 1 - parity;
   const int Hprev = B_H0 + 1 - parity, Hnow = B_H0 + parity;
-  const double alpha = first ? 0.0 : dt / (2.0 * prev_dt);
-  const double factor = first ? 1.0 : dt / (dt + prev_dt);
-  const dim3 gv = gridS(h->velsize), gp = gridS(h->psize);
+  const dim3 gv = gridSn(x.maxvel, x.n), gp = gridSn(x.maxp, x.n);
   int nproj = 2;
   if (first) {
     // projected midpoint startup (time_integration.py:33-47)
-    AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1, dt, 0.0};
-    hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, h->stream, h->dc, m);
-    if ((rc = enqueue_project(h, B_TV0, B_HALF, B_TP0, 0.5 * dt, 2, -1))) return rc;
+    AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1};
+    hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, x.stream, x.dc, m);
+    if ((rc = enqueue_project(x, B_TV0, B_HALF, B_TP0, 0.5, 2, -1))) return rc;
     FillArgs f{B_HALF, 1, 1, 0};
-    const int nf = h->hc.gt[0].n + h->hc.gt[1].n + h->ny;
-    hipLaunchKernelGGL(k_fill, grid1(nf), dim3(256), 0, h->stream, h->dc, f);
-    AdvArgs r{S, B_HALF, B_PH, -1, -1, B_RHS, 2, dt, 0.0};
-    hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, h->stream, h->dc, r);
+    hipLaunchKernelGGL(k_fill, grid1n(x.maxfill, x.n), dim3(256), 0, x.stream, x.dc, f);
+    AdvArgs r{S, B_HALF, B_PH, -1, -1, B_RHS, 2};
+    hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, x.stream, x.dc, r);
     nproj = 3;
   } else {
-    AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0, dt, alpha};
-    hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, h->stream, h->dc, r);
+    AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0};
+    hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, x.stream, x.dc, r);
   }
-  if ((rc = enqueue_solve(h, MCF_U, B_RHS, B_STAR, 0, dt))) return rc;
-  if ((rc = enqueue_project(h, B_STAR, Snew, B_PHI, dt, 0, -1))) return rc;
-  HelmArgs dfa{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1, dt};
-  hipLaunchKernelGGL(k_helm, gv, dim3(256), 0, h->stream, h->dc, dfa);
-  if ((rc = enqueue_project(h, B_PROJ, B_PROJ, B_PHI2, dt, 1, B_DEFECT))) return rc;
-  PressArgs pa{B_PH, B_PHI2, B_PH, B_POUT, factor};
-  hipLaunchKernelGGL(k_pressures, gp, dim3(256), 0, h->stream, h->dc, pa);
+  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0))) return rc;
+  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1))) return rc;
+  HelmArgs dfa{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1};
+  hipLaunchKernelGGL(k_helm, gv, dim3(256), 0, x.stream, x.dc, dfa);
+  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT))) return rc;
+  PressArgs pa{B_PH, B_PHI2, B_PH, B_POUT};
+  hipLaunchKernelGGL(k_pressures, gp, dim3(256), 0, x.stream, x.dc, pa);
   FillArgs f{Snew, 1, 1, 1};
-  const int nf = h->hc.gt[0].n + h->hc.gt[1].n + h->ny;
-  hipLaunchKernelGGL(k_fill, grid1(nf), dim3(256), 0, h->stream, h->dc, f);
-  ForceArgs fa{Snew, B_POUT, h->nu * h->rho};
-  hipLaunchKernelGGL(k_forces, dim3((h->hc.nsurf + 63) / 64, 1), dim3(64), 0, h->stream, h->dc, fa);
-  hipLaunchKernelGGL(k_force_total, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
-  FinArgs fin{dt, h->rho, first ? 1 : 0, nproj};
-  hipLaunchKernelGGL(k_finalize, dim3(1, 1), dim3(64), 0, h->stream, h->dc, fin);
-  return launch_check(h, "step");
+  hipLaunchKernelGGL(k_fill, grid1n(x.maxfill, x.n), dim3(256), 0, x.stream, x.dc, f);
+  ForceArgs fa{Snew, B_POUT, -1.0};
+  hipLaunchKernelGGL(k_forces, dim3((x.maxsurf + 63) / 64, x.n), dim3(64), 0, x.stream, x.dc, fa);
+  hipLaunchKernelGGL(k_force_total, dim3(1, x.n), dim3(64), 0, x.stream, x.dc);
+  FinArgs fin{nproj};
+  hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, fin);
+  return launch_check(x.err, "step");
 }
+
+}  // namespace
+
+// Step graphs: one per parity, for steady AB2 steps (history present, unchanged dt).
+struct StepGraphs {
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  bool valid = false;
+  void drop() {
+    for (int p = 0; p < 2; ++p) {
+      if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+      if (graph[p]) (void)hipGraphDestroy(graph[p]);
+      exec[p] = nullptr; graph[p] = nullptr;
+    }
+    valid = false;
+  }
+  int build(const Ctx& x) {
+    drop();
+    for (int p = 0; p < 2; ++p) {
+      HIPCHK(x.err, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_step_kernels(x, p, false);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(x.stream, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return fail(x.err, 4, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      graph[p] = g;
+      HIPCHK(x.err, hipGraphInstantiate(&exec[p], g, nullptr, nullptr, 0));
+    }
+    valid = true;
+    return 0;
+  }
+};
+
+namespace {
 
 void advance_host_state(mcf_handle* h, double dt) {
   h->parity = 1 - h->parity;
@@ -1288,47 +1367,41 @@ void advance_host_state(mcf_handle* h, double dt) {
 }
 
 void drop_graphs(mcf_handle* h) {
-  for (int p = 0; p < 2; ++p) {
-    if (h->gexec[p]) (void)hipGraphExecDestroy(h->gexec[p]);
-    if (h->graph[p]) (void)hipGraphDestroy(h->graph[p]);
-    h->gexec[p] = nullptr; h->graph[p] = nullptr;
-  }
+  if (h->graphs) h->graphs->drop();
   h->graph_dt = -1.0;
-}
-
-// Capture the steady AB2 step (history present, previous dt == dt) once per parity.
-int build_graphs(mcf_handle* h, double dt) {
-  drop_graphs(h);
-  for (int p = 0; p < 2; ++p) {
-    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_step_kernels(h, dt, p, false, dt);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(h->stream, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) return fail(h, 4, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    h->graph[p] = g;
-    HIPCHK(h, hipGraphInstantiate(&h->gexec[p], g, nullptr, nullptr, 0));
-  }
-  h->graph_dt = dt;
-  return 0;
+  h->prm_valid = false;
 }
 
 // n steps: steps without matching history run eagerly, the rest replay the captured graphs.
 int enqueue_steps(mcf_handle* h, int n, double dt) {
+  const Ctx x = ctx_of(h);
   hipLaunchKernelGGL(k_set_clock, dim3(1, 1), dim3(64), 0, h->stream, h->dc, h->time);
   for (int s = 0; s < n; ++s) {
     const bool steady = h->has_history && h->prev_dt == dt;
+    const double alpha = !h->has_history ? 0.0 : dt / (2.0 * h->prev_dt);
+    const double factor = !h->has_history ? 1.0 : dt / (dt + h->prev_dt);
+    if (!(h->prm_valid && h->prm_dt == dt && h->prm_alpha == alpha && h->prm_factor == factor)) {
+      hipLaunchKernelGGL(k_set_prm, dim3(1, 1), dim3(64), 0, h->stream, h->dc, dt, alpha, factor);
+      h->prm_valid = true; h->prm_dt = dt; h->prm_alpha = alpha; h->prm_factor = factor;
+    }
     if (!steady || !h->use_graphs) {
-      if (int rc = enqueue_step_kernels(h, dt, h->parity, !h->has_history, h->prev_dt)) return rc;
+      if (int rc = enqueue_step_kernels(x, h->parity, !h->has_history)) return rc;
     } else {
-      if (h->graph_dt != dt || !h->gexec[0]) {
-        if (int rc = build_graphs(h, dt)) return rc;
+      if (h->graph_dt != dt || !h->graphs->valid) {
+        if (int rc = h->graphs->build(x)) return rc;
+        h->graph_dt = dt;
       }
-      HIPCHK(h, hipGraphLaunch(h->gexec[h->parity], h->stream));
+      HIPCHK(h, hipGraphLaunch(h->graphs->exec[h->parity], h->stream));
     }
     advance_host_state(h, dt);
   }
   return 0;
+}
+
+// Step parameters for a one-off operation at an explicit dt (projection, operator, solve entry points)
+void set_prm_once(mcf_handle* h, double dt) {
+  hipLaunchKernelGGL(k_set_prm, dim3(1, 1), dim3(64), 0, h->stream, h->dc, dt, 0.0, 1.0);
+  h->prm_valid = false;
 }
 
 }  // namespace
@@ -1424,6 +1497,9 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   HIPCHK(h, hipMalloc(&h->clock_time, sizeof(double)));
   HIPCHK(h, hipMemset(h->clock_step, 0, sizeof(int)));
   HIPCHK(h, hipMemset(h->clock_time, 0, sizeof(double)));
+  HIPCHK(h, hipMalloc(&h->prm, 4 * sizeof(double)));
+  HIPCHK(h, hipMemset(h->prm, 0, 4 * sizeof(double)));
+  h->graphs = new StepGraphs();
   HIPCHK(h, hipMalloc(&h->dc, sizeof(CaseDev)));
   HIPCHK(h, hipMalloc(&h->ds, sizeof(h->hs)));
   CaseDev& c = h->hc;
@@ -1433,6 +1509,7 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   c.sl = h->slots;
   c.clock_step = h->clock_step;
   c.clock_time = h->clock_time;
+  c.prm = h->prm;
   if (const char* e = std::getenv("GHOSTFLOW_NO_GRAPH")) h->use_graphs = (e[0] == '0');
   for (int k = 0; k < 3; ++k) {
     SolveDesc& s = h->hs[k];
@@ -1449,7 +1526,9 @@ void mcf_destroy(mcf_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graphs(h);
-  (void)hipFree(h->clock_step); (void)hipFree(h->clock_time);
+  delete h->graphs;
+  h->graphs = nullptr;
+  (void)hipFree(h->clock_step); (void)hipFree(h->clock_time); (void)hipFree(h->prm);
   for (auto& kv : h->owned) (void)hipFree(kv.second);
   for (void* p : h->bufs) (void)hipFree(p);
   (void)hipFree(h->slots); (void)hipFree(h->inlet); (void)hipFree(h->dc); (void)hipFree(h->ds);
@@ -1780,7 +1859,8 @@ int mcf_project(mcf_handle* h, const double* q, double dt, double* out, double* 
   if (!h->has_rect[0] || !h->has_schur[0]) return fail(h, 1, "pressure factor missing");
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TV1], q, h->velsize * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  if (int rc = enqueue_project(h, B_TV1, B_TV0, B_TP0, dt, 0, -1)) return rc;
+  set_prm_once(h, dt);
+  if (int rc = enqueue_project(ctx_of(h), B_TV1, B_TV0, B_TP0, 1.0, 0, -1)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (out) HIPCHK(h, hipMemcpy(out, h->hc.buf[B_TV0], h->velsize * sizeof(double), hipMemcpyDeviceToHost));
   if (phi) HIPCHK(h, hipMemcpy(phi, h->hc.buf[B_TP0], h->psize * sizeof(double), hipMemcpyDeviceToHost));
@@ -1882,7 +1962,8 @@ int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, dou
   else { s0[(size_t)(SL_HU + SV_VAL) * NSTRIPE] = vm[0]; s0[(size_t)(SL_HV + SV_VAL) * NSTRIPE] = vm[1]; }
   HIPCHK(h, hipMemcpyAsync(h->slots, s0.data(), NSLOTW * sizeof(double), hipMemcpyHostToDevice, h->stream));
   const double dt = pres ? 1.0 : -2.0 * h->hs[1].b / h->nu;
-  if (int rc = enqueue_solve(h, pres ? MCF_PRESSURE : MCF_U, in_id, out_id, 0, dt)) return rc;
+  set_prm_once(h, dt);
+  if (int rc = enqueue_solve(ctx_of(h), pres, in_id, out_id, 0)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(out, h->hc.buf[out_id], n * sizeof(double), hipMemcpyDeviceToHost));
   std::vector<double> sv;
@@ -1914,6 +1995,7 @@ int mcf_apply_operator(mcf_handle* h, int op, const double* in, double* out, dou
   if (!h || !in || !out || op < 0 || op > 4) return fail(h, 1, "invalid operator arguments");
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   const dim3 gv = gridS(h->velsize), gp = gridS(h->psize);
+  set_prm_once(h, dt > 0 ? dt : 1.0);
   if (op == MCF_APPLY_GRADIENT) {
     HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TP0], in, h->psize * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemsetAsync(h->hc.buf[B_TV0], 0, h->velsize * sizeof(double), h->stream));
@@ -1922,18 +2004,18 @@ int mcf_apply_operator(mcf_handle* h, int op, const double* in, double* out, dou
   } else {
     HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TV0], in, h->velsize * sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (op == MCF_APPLY_DIVERGENCE) {
-      DivArgs d{B_TV0, B_TP0, 1.0, SL_SPARE};
+      DivArgs d{B_TV0, B_TP0, 0.0, SL_SPARE};
       hipLaunchKernelGGL(k_div, gp, dim3(256), 0, h->stream, h->dc, d);
     } else if (op == MCF_APPLY_HMATVEC) {
       HIPCHK(h, hipMemsetAsync(h->hc.buf[B_TV1], 0, h->velsize * sizeof(double), h->stream));
-      HelmArgs a{B_TV1, B_TV0, B_RES, -1, 0, SV_SSQR, -1, dt};
+      HelmArgs a{B_TV1, B_TV0, B_RES, -1, 0, SV_SSQR, -1};
       hipLaunchKernelGGL(k_helm, gv, dim3(256), 0, h->stream, h->dc, a);
     } else {
       FillArgs f{B_TV0, 1, 1, 0};
       const int nf = h->hc.gt[0].n + h->hc.gt[1].n + h->ny;
       hipLaunchKernelGGL(k_fill, grid1(nf), dim3(256), 0, h->stream, h->dc, f);
       HIPCHK(h, hipMemsetAsync(h->hc.buf[B_TP0], 0, h->psize * sizeof(double), h->stream));
-      AdvArgs m{B_TV0, B_TV0, B_TP0, -1, B_TV1, B_RES, 3, 1.0, 0.0};
+      AdvArgs m{B_TV0, B_TV0, B_TP0, -1, B_TV1, B_RES, 3};
       hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, h->stream, h->dc, m);
     }
   }
@@ -2014,6 +2096,138 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   for (int s = 0; s < n; ++s)
     if (int rc = status_error(h, d.data() + (size_t)s * MCF_DIAG_LEN, s)) return rc;
   return launch_check(h, "profile");
+}
+
+}  // extern "C"
+
+// ======================================================================================
+// Batched ensemble: many cases of uneven size advance together, one launch per stage
+// ======================================================================================
+struct mcf_batch {
+  std::vector<mcf_handle*> cases;
+  std::vector<double> dt;       // each case's time step (its Helmholtz factors were built for it)
+  CaseDev* dc = nullptr;
+  SolveDesc* ds = nullptr;      // n pressure problems, then 2n Helmholtz problems
+  hipStream_t stream = nullptr;
+  StepGraphs graphs;
+  Ctx ctx{};
+  bool built = false;
+};
+
+namespace {
+
+int batch_build(mcf_batch* b) {
+  mcf_handle* e = b->cases[0];
+  const int n = (int)b->cases.size();
+  std::vector<CaseDev> hc(n);
+  std::vector<SolveDesc> hs(3 * n);
+  Ctx x{};
+  x.err = e; x.stream = b->stream; x.n = n;
+  for (int c = 0; c < n; ++c) {
+    mcf_handle* h = b->cases[c];
+    hc[c] = h->hc;
+    hs[c] = h->hs[0];
+    hs[n + 2 * c] = h->hs[1];
+    hs[n + 2 * c + 1] = h->hs[2];
+    ctx_shapes(x, h->hc, h->hs);
+  }
+  if (!b->dc) HIPCHK(e, hipMalloc(&b->dc, n * sizeof(CaseDev)));
+  if (!b->ds) HIPCHK(e, hipMalloc(&b->ds, 3 * n * sizeof(SolveDesc)));
+  HIPCHK(e, hipMemcpy(b->dc, hc.data(), n * sizeof(CaseDev), hipMemcpyHostToDevice));
+  HIPCHK(e, hipMemcpy(b->ds, hs.data(), 3 * n * sizeof(SolveDesc), hipMemcpyHostToDevice));
+  x.dc = b->dc; x.dsp = b->ds; x.dsh = b->ds + n;
+  b->ctx = x;
+  b->graphs.drop();
+  b->built = true;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcf_batch_create(mcf_handle* const* cases, const double* dts, int n, mcf_batch** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  if (!cases || !dts || n <= 0) return 1;
+  mcf_handle* e = cases[0];
+  for (int c = 0; c < n; ++c) {
+    mcf_handle* h = cases[c];
+    if (!h) return 1;
+    const double dt = dts[c];
+    if (!(dt > 0)) return fail(e, 1, "case " + std::to_string(c) + ": dt must be positive");
+    if (int rc = check_ready(h, dt)) { e->err = "case " + std::to_string(c) + ": " + h->err; return rc; }
+    if (h->parity != cases[0]->parity || h->has_history != cases[0]->has_history)
+      return fail(e, 1, "batched cases must share the step parity and history state");
+    if (h->has_history && h->prev_dt != dt) return fail(e, 1, "batched cases must continue at their previous dt");
+  }
+  mcf_batch* b = new mcf_batch();
+  b->cases.assign(cases, cases + n);
+  b->dt.assign(dts, dts + n);
+  if (hipStreamCreate(&b->stream) != hipSuccess) { delete b; return fail(e, 4, "hipStreamCreate failed"); }
+  *out = b;
+  return 0;
+}
+
+void mcf_batch_destroy(mcf_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->graphs.drop();
+  (void)hipFree(b->dc); (void)hipFree(b->ds);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* mcf_batch_last_error(const mcf_batch* b) { return b && !b->cases.empty() ? b->cases[0]->err.c_str() : "null batch"; }
+
+int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
+  if (!b || nsteps <= 0) return 1;
+  mcf_handle* e = b->cases[0];
+  const int n = (int)b->cases.size();
+  bool changed = !b->built;
+  for (mcf_handle* h : b->cases) {
+    (void)hipStreamSynchronize(h->stream);
+    const int cap = h->diag_cap;
+    if (int rc = ensure_diag(h, nsteps)) return rc;
+    changed = changed || cap != h->diag_cap;
+    h->owned.erase("p_raw_valid");
+  }
+  if (changed) if (int rc = batch_build(b)) return rc;
+  const Ctx& x = b->ctx;
+  for (int c = 0; c < n; ++c) {
+    mcf_handle* h = b->cases[c];
+    const int zero = 0;
+    HIPCHK(e, hipMemcpy(h->clock_step, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(h->clock_time, &h->time, sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(h->prm, &b->dt[c], sizeof(double), hipMemcpyHostToDevice));
+    h->prm_valid = false;
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    mcf_handle* h0 = b->cases[0];
+    const bool first = !h0->has_history;
+    hipLaunchKernelGGL(k_set_prm, dim3(1, n), dim3(64), 0, b->stream, b->dc, -1.0, first ? 0.0 : 0.5, first ? 1.0 : 0.5);
+    if (first || !e->use_graphs) {
+      if (int rc = enqueue_step_kernels(x, h0->parity, first)) return rc;
+    } else {
+      if (!b->graphs.valid) if (int rc = b->graphs.build(x)) return rc;
+      HIPCHK(e, hipGraphLaunch(b->graphs.exec[h0->parity], b->stream));
+    }
+    for (int c = 0; c < n; ++c) advance_host_state(b->cases[c], b->dt[c]);
+  }
+  HIPCHK(e, hipStreamSynchronize(b->stream));
+  std::vector<double> d((size_t)nsteps * MCF_DIAG_LEN);
+  int first_error = 0;
+  for (int c = 0; c < n; ++c) {
+    mcf_handle* h = b->cases[c];
+    HIPCHK(e, hipMemcpy(d.data(), h->diag, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (diag) std::memcpy(diag + (size_t)c * d.size(), d.data(), d.size() * sizeof(double));
+    for (int s = 0; s < nsteps && !first_error; ++s)
+      if (int rc = status_error(h, d.data() + (size_t)s * MCF_DIAG_LEN, s)) {
+        first_error = rc;
+        e->err = "case " + std::to_string(c) + ": " + h->err;
+      }
+  }
+  return first_error;
 }
 
 }  // extern "C"

@@ -45,31 +45,88 @@ def changed_rows(actual, regular, rel=1e-13):
     return np.flatnonzero(np.diff(diff.indptr))
 
 
+def helmholtz_pieces(ops, dt):
+    """Per-dt u/v Helmholtz factors, capacitance data and override rows (optimized_ghost_cell.py:243-255)."""
+    pieces = []
+    for kind, selected, actual in ops.helmholtz_actual(dt):
+        factor = RectangleFactor(kind, ops.nx, ops.ny, ops.dx, ops.dy, nu=ops.nu, dt=dt, rho=ops.rho)
+        schur = SchurFactor(actual, selected, factor)
+        pieces.append((kind, selected, actual, factor, schur))
+    return pieces
+
+
+def pressure_pieces(ops):
+    factor = RectangleFactor('pressure', ops.nx, ops.ny, ops.dx, ops.dy, rho=ops.rho)
+    return factor, SchurFactor(ops.A, ops.pi, factor)
+
+
+class PreparedCase:
+    """A case's recorded table uploads, prepared on the host without a device (picklable, compact):
+    the entry point calls with their arrays, the index layout and the solver info."""
+
+    def __init__(self, layout, calls, body, u_inlet, u_ref, dt, pressure_info, helmholtz_info):
+        self.layout, self.calls, self.body = layout, calls, body
+        self.u_inlet, self.u_ref, self.dt = u_inlet, u_ref, dt
+        self.pressure_info, self.helmholtz_info = pressure_info, helmholtz_info
+
+
+def prepare_case(ops, u_inlet, u_ref, dt, pressure=None):
+    """Build every table upload of one case for time step dt against a ``Recorder``."""
+    rec = nat.Recorder()
+    dev = GhostDevice(ops, u_inlet, pressure=pressure, handle=rec)
+    dev.set_reference_velocity(u_ref)
+    dev.set_dt(dt)
+    return PreparedCase(ops.layout(), rec.calls, dev.body, dev.u_inlet, u_ref, dt, dev.pressure_info,
+                        dev.helmholtz_inverse_info)
+
+
 class GhostDevice:
     """Device state and tables of one case (one ``mcf_handle``)."""
 
-    def __init__(self, ops, u_inlet):
+    def __init__(self, ops, u_inlet, pressure=None, handle=None):
         self.ops = o = ops
-        self.h = nat.Handle(o.nx, o.ny, o.dx, o.dy, o.nu, o.rho)
+        self.h = handle if handle is not None else nat.Handle(o.nx, o.ny, o.dx, o.dy, o.nu, o.rho)
+        self.prepared = False
+        self.uref = None
         self.set_inlet(u_inlet)
         gc = o.gc
         self.body = dict(u=gc.dist_u <= 1e-13, v=gc.dist_v <= 1e-13, p=gc.dist_p <= 1e-13)
         for field, key in ((nat.FIELD_U, 'u'), (nat.FIELD_V, 'v'), (nat.FIELD_P, 'p')):
             lo, hi = solid_intervals(self.body[key])
-            self.h.call('mcf_set_solid_intervals', field, nat.iptr(lo), nat.iptr(hi))
+            self.h.call('mcf_set_solid_intervals', field, lo, hi)
         self._upload_ghosts()
         self._upload_overrides()
-        self.pressure_factor = RectangleFactor('pressure', o.nx, o.ny, o.dx, o.dy, rho=o.rho)
-        self.pressure_schur = SchurFactor(o.A, o.pi, self.pressure_factor)
+        self.pressure_factor, self.pressure_schur = pressure if pressure is not None else pressure_pieces(o)
         self._upload_rectangle(nat.PRESSURE, self.pressure_factor, self.pressure_schur)
+        ps = self.pressure_schur
+        self.pressure_info = dict(backend='fft_boundary_schur', rank=ps.rank, dense_bytes=int(ps.S.nbytes),
+                                  roundoff_drop_tolerance=ps.tolerance)
         self._upload_surface()
         self.dt = None
         self.helmholtz_factors = None
 
+    @classmethod
+    def from_prepared(cls, prep):
+        """A device case from recorded uploads; it keeps the operators' index layout only."""
+        self = cls.__new__(cls)
+        self.ops = o = prep.layout
+        self.h = nat.Handle(o.nx, o.ny, o.dx, o.dy, o.nu, o.rho)
+        for name, args in prep.calls:
+            self.h.call(name, *args)
+        self.prepared = True
+        self.body = prep.body
+        self.u_inlet = prep.u_inlet
+        self.uref = prep.u_ref
+        self.dt = prep.dt
+        self.pressure_info = prep.pressure_info
+        self.helmholtz_inverse_info = prep.helmholtz_info
+        self.pressure_factor = self.pressure_schur = self.helmholtz_factors = None
+        return self
+
     # ---------------------------------------------------------------- tables
     def set_inlet(self, u_inlet):
         self.u_inlet = nat.f64(u_inlet).copy()
-        self.h.call('mcf_set_inlet', nat.dptr(self.u_inlet))
+        self.h.call('mcf_set_inlet', self.u_inlet)
 
     def _upload_ghosts(self):
         o = self.ops
@@ -77,8 +134,7 @@ class GhostDevice:
                                     (nat.FIELD_P, o.P, o.pi, o.ghost_p)):
             nodes = np.asarray(info['ghost_nodes'])
             ptr, col, val = csr_rows(E, nodes)
-            self.h.call('mcf_set_ghost', field, len(nodes), nat.iptr(nat.i32(nodes)), nat.iptr(ptr),
-                        nat.iptr(nat.i32(ids[col])), nat.dptr(val))
+            self.h.call('mcf_set_ghost', field, len(nodes), nat.i32(nodes), ptr, nat.i32(ids[col]), val)
 
     def _vel_index(self, qcols):
         """Packed velocity index -> vel-layout index."""
@@ -89,8 +145,7 @@ class GhostDevice:
 
     def _set_override(self, op, out_index, matrix, rows, col_map):
         ptr, col, val = csr_rows(matrix, rows)
-        self.h.call('mcf_set_override', op, len(rows), nat.iptr(nat.i32(out_index[rows])), nat.iptr(ptr),
-                    nat.iptr(nat.i32(col_map(col))), nat.dptr(val))
+        self.h.call('mcf_set_override', op, len(rows), nat.i32(out_index[rows]), ptr, nat.i32(col_map(col)), val)
         return len(rows)
 
     def _upload_overrides(self):
@@ -129,11 +184,10 @@ class GhostDevice:
 
     def _upload_rectangle(self, kind, factor, schur):
         yy, xx = factor.shape
-        self.h.call('mcf_set_rectangle', kind, yy, xx, float(factor.a), float(factor.b), nat.dptr(nat.f64(factor.V)),
-                    nat.dptr(nat.f64(factor.lam)), nat.dptr(nat.f64(factor.lower)), nat.dptr(nat.f64(factor.diag)),
-                    nat.dptr(nat.f64(factor.upper)))
-        self.h.call('mcf_set_schur', kind, schur.rank, nat.iptr(nat.i32(schur.rows)), len(schur.points),
-                    nat.iptr(nat.i32(schur.points)), nat.dptr(nat.f64(schur.SC)))
+        self.h.call('mcf_set_rectangle', kind, yy, xx, float(factor.a), float(factor.b), nat.f64(factor.V),
+                    nat.f64(factor.lam), nat.f64(factor.lower), nat.f64(factor.diag), nat.f64(factor.upper))
+        self.h.call('mcf_set_schur', kind, schur.rank, nat.i32(schur.rows), len(schur.points), nat.i32(schur.points),
+                    nat.f64(schur.SC))
 
     def _upload_surface(self):
         o = self.ops
@@ -142,40 +196,40 @@ class GhostDevice:
             M = M.tocsr(); M.sort_indices()
             mats.append((nat.i32(M.indptr), nat.i32(M.indices), nat.f64(M.data)))
         (pp, pc, pv), (up, uc, uv), (vp, vc, vv) = mats
-        self.uref = None
         self._surface = mats
         self._normals = nat.f64(o.normals)
         self._weights = nat.f64(o.normal_weights)
 
     def set_reference_velocity(self, u_ref):
+        if self.uref == u_ref:
+            return
+        if self.prepared:
+            raise ValueError('a prepared case keeps the reference velocity it was prepared with')
         (pp, pc, pv), (up, uc, uv), (vp, vc, vv) = self._surface
-        self.h.call('mcf_set_surface', len(self._normals), nat.iptr(pp), nat.iptr(pc), nat.dptr(pv), nat.iptr(up),
-                    nat.iptr(uc), nat.dptr(uv), nat.iptr(vp), nat.iptr(vc), nat.dptr(vv), nat.dptr(self._normals),
-                    nat.dptr(self._weights), float(self.ops.D), float(u_ref))
+        self.h.call('mcf_set_surface', len(self._normals), pp, pc, pv, up, uc, uv, vp, vc, vv, self._normals,
+                    self._weights, float(self.ops.D), float(u_ref))
         self.uref = u_ref
 
-    def set_dt(self, dt):
-        """Build and upload the u/v Helmholtz factors for dt (optimized_ghost_cell.py:243-255)."""
+    def set_dt(self, dt, pieces=None):
+        """Build (or take prebuilt) and upload the u/v Helmholtz factors for dt (optimized_ghost_cell.py:243-255)."""
         if self.dt == dt:
             return
+        if self.prepared:
+            raise ValueError(f'a prepared case keeps the time step it was prepared for ({self.dt!r}, asked {dt!r})')
         o = self.ops
-        infos = []
-        factors = []
-        for (kind, selected, actual), code, op in zip(o.helmholtz_actual(dt), (nat.U, nat.V),
-                                                      (nat.OP_HELM_U, nat.OP_HELM_V)):
-            factor = RectangleFactor(kind, o.nx, o.ny, o.dx, o.dy, nu=o.nu, dt=dt, rho=o.rho)
-            schur = SchurFactor(actual, selected, factor)
+        if pieces is None:
+            pieces = helmholtz_pieces(o, dt)
+        infos, factors = [], []
+        for (kind, selected, actual, factor, schur), code, op in zip(pieces, (nat.U, nat.V), (nat.OP_HELM_U, nat.OP_HELM_V)):
             self._upload_rectangle(code, factor, schur)
             pos = np.full(factor.shape[0] * factor.shape[1], -1)
             pos[selected] = np.arange(len(selected))
             arows = pos[schur.rows]
             arows = arows[arows >= 0]
             if kind == 'u':
-                out_index = o.ui
-                col_map = lambda c: o.ui[c]
+                out_index, col_map = o.ui, (lambda c: o.ui[c])
             else:
-                out_index = o.vi
-                col_map = lambda c: o.usize + o.vi[c]
+                out_index, col_map = o.vi, (lambda c: o.usize + o.vi[c])
             self._set_override(op, out_index, actual.tocsr(), arows, col_map)
             infos.append(dict(rank=schur.rank, dense_bytes=int(schur.S.nbytes),
                               roundoff_drop_tolerance=schur.tolerance, override_rows=len(arows)))

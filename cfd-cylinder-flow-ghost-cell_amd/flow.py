@@ -19,7 +19,7 @@ import numpy as np
 import scipy
 
 from . import _native as nat
-from .device import GhostDevice
+from .device import GhostDevice, PreparedCase
 from .tables import GhostOperators
 
 SOURCE_FILES = ('flow.py', 'device.py', 'tables.py', '_native.py', 'csrc/ghostflow.hip')
@@ -41,20 +41,30 @@ def gaussian_v_perturbation(xv, yv, xc, yc, R, D, seed, amplitude=1e-3, width=0.
 class GhostCellIntegrator:
     """The reference's ``GhostCellIntegrator`` surface (optimized_ghost_cell.py:61-286) over the device."""
 
-    def __init__(self, solver):
+    def __init__(self, solver, prebuilt=None):
         s = self.solver = solver
-        ops = self.ops = GhostOperators(s.nx, s.ny, s.Lx, s.Ly, s.D, s.xc, s.yc, s.nu, s.rho, s.u_inlet,
-                                        interpolation=s.ghost_velocity_interpolation)
+        if isinstance(prebuilt, PreparedCase):
+            # recorded uploads: the operators themselves stay on the host that prepared them
+            if not np.array_equal(prebuilt.u_inlet, s.u_inlet) or prebuilt.u_ref != s.U_ref:
+                raise ValueError('prepared case does not match this configuration')
+            self.device = GhostDevice.from_prepared(prebuilt)
+            ops = self.device.ops
+        else:
+            if prebuilt is not None:
+                ops, pressure = prebuilt
+            else:
+                ops = GhostOperators(s.nx, s.ny, s.Lx, s.Ly, s.D, s.xc, s.yc, s.nu, s.rho, s.u_inlet,
+                                     interpolation=s.ghost_velocity_interpolation)
+                pressure = None
+            self.device = GhostDevice(ops, s.u_inlet, pressure=pressure)
+            self.device.set_reference_velocity(s.U_ref)
+        self.ops = ops
         for name in ('ui', 'vi', 'pi', 'qsize', 'E', 'P', 'G', 'A', 'L', 'Dfull', 'offset', 'div_offset',
                      'diff_offset', 'outlet_u', 'du_dn', 'dv_dn', 'pressure_surface', 'normals',
                      'normal_weights', 'surface_points', 'usize'):
             setattr(self, name, getattr(ops, name))
         self.D = ops.Dop
-        self.device = GhostDevice(ops, s.u_inlet)
-        self.device.set_reference_velocity(s.U_ref)
-        self.pressure_inverse_info = dict(backend='fft_boundary_schur', rank=self.device.pressure_schur.rank,
-                                          dense_bytes=int(self.device.pressure_schur.S.nbytes),
-                                          roundoff_drop_tolerance=self.device.pressure_schur.tolerance)
+        self.pressure_inverse_info = dict(self.device.pressure_info)
         self.helmholtz_inverse_info = []
         self.previous_advection = self.previous_dt = self.half_pressure = None
         self.last_diagnostics = {}
@@ -154,7 +164,7 @@ class MAC_CylinderFlow:
                  inflow='uniform', U_m=None, slip_walls=False, verbose=True, boundary='immersed',
                  transport='centered', pressure_solver='direct', marker_spacing_factor=1.5, time_scheme='legacy',
                  ghost_pressure_extension='neumann', ghost_velocity_interpolation='bilinear', initial='profile',
-                 v_perturbation=None):
+                 v_perturbation=None, prebuilt=None):
         # argument checks (main.py:45-80)
         if boundary not in ('immersed', 'staircase', 'ghost_cell', 'cut_cell') or transport not in ('centered', 'upwind'):
             raise ValueError('Choose boundary=immersed/staircase/ghost_cell/cut_cell and transport=centered/upwind')
@@ -233,7 +243,7 @@ class MAC_CylinderFlow:
         else:
             raise ValueError("initial must be 'profile' or 'zero'")
         t0 = time.time()
-        self.gg = GhostCellIntegrator(self)
+        self.gg = GhostCellIntegrator(self, prebuilt=prebuilt)
         self.gc = self.gg.ops.gc
         if v_perturbation is not None:
             self.v += gaussian_v_perturbation(self.gc.xv, self.gc.yv, xc, yc, D / 2, D, **v_perturbation)

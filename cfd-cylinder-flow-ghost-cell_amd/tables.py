@@ -289,6 +289,18 @@ class GhostOperators:
             out.append((kind, selected, actual))
         return out
 
+    # assembled operators and offsets: needed to build the device tables, not to step or index states
+    OPERATOR_ATTRIBUTES = ('raw_l', 'L', 'A', 'G', 'raw_g', 'Dfull', 'Dop', 'E', 'Eu', 'Ev', 'P', 'offset',
+                           'diff_offset', 'div_offset', 'du_dn', 'dv_dn', 'pressure_surface')
+
+    def layout(self):
+        """A copy holding the geometry and index maps only (the assembled operators are None)."""
+        lay = object.__new__(GhostOperators)
+        lay.__dict__.update(self.__dict__)
+        for name in self.OPERATOR_ATTRIBUTES:
+            lay.__dict__[name] = None
+        return lay
+
 
 # ------------------------------------------------------------------------------------------
 # Rectangle operators and their y-eigenbasis (rectangular_inverse.py:8-46)

@@ -128,6 +128,17 @@ int mcf_divergence_stats(mcf_handle* h, const double* vel, const unsigned char* 
    gemm_launches_per_step, pcr_launches_per_step]. */
 int mcf_profile(mcf_handle* h, int n, double dt, double* out5);
 
+/* Batched ensemble (no reference counterpart; the reference runs one case per process):
+   cases are fully configured handles (tables, state, Helmholtz factors for dts[c]); all share the
+   step parity and history state.  Every stage of the step is one launch over all cases and
+   steady steps replay one HIP graph.  A handle in a live batch must not be stepped on its own
+   concurrently.  diag receives n_cases * nsteps * MCF_DIAG_LEN doubles (case-major). */
+typedef struct mcf_batch mcf_batch;
+int mcf_batch_create(mcf_handle* const* cases, const double* dts, int n, mcf_batch** out);
+void mcf_batch_destroy(mcf_batch* b);
+const char* mcf_batch_last_error(const mcf_batch* b);
+int mcf_batch_run(mcf_batch* b, int nsteps, double* diag);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,81 @@
+"""GPU: the batched ensemble (mcf_batch_*) against the same cases stepped one by one, and the oracle.
+
+Batched and single-case runs execute the same kernels on the same per-case data, so their fields
+must agree bit for bit; one case is also checked against the CPU oracle.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from oracle import ghost_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# three uneven small channels: (m, D/H, offset, Re)
+SMALL = [(4, 0.25, 0.0, 100.), (5, 0.2, 0.06, 60.), (6, 0.28, -0.08, 150.)]
+
+
+@pytest.fixture(scope='module')
+def ens(pkg):
+    return importlib.import_module(pkg.__name__ + '.ensemble')
+
+
+def small_cases(ens):
+    return [ens.channel_case(m, dh, dl, re, nx_range=(40, 400), ny_range=(12, 80)) for m, dh, dl, re in SMALL]
+
+
+def test_batched_equals_single(ens):
+    cases = small_cases(ens)
+    print('ensemble: shapes', [(c['nx'], c['ny']) for c in cases], flush=True)
+    batch = ens.Ensemble(cases, workers=1)
+    singles = ens.build_flows(cases, workers=1)
+    nsteps = 7                        # eager startup step, then graph replays of both parities
+    d_b = batch.run(3)
+    d_b = np.concatenate([d_b, batch.run(nsteps - 3)], axis=1)
+    batch.pull()
+    for c, (case, f) in enumerate(zip(cases, singles)):
+        d_s = f.gg.device.run(nsteps, case['dt'])
+        f._pull()
+        b = batch.flows[c]
+        for key in ('u', 'v', 'p'):
+            assert np.array_equal(getattr(b, key), getattr(f, key)), (c, key)
+        assert np.array_equal(b.gg.half_pressure, f.gg.half_pressure), c
+        for col in (0, 2, 3, 10):     # time, C_D, C_L, max divergence
+            assert np.array_equal(d_b[c, :, col], d_s[:, col]), (c, col)
+        assert np.all(d_b[c, :, 10] < 1e-12)
+        assert b.time == pytest.approx(nsteps * case['dt'], rel=1e-14)
+    batch.close()
+
+
+def test_batched_case_matches_oracle(ens):
+    case = small_cases(ens)[1]
+    batch = ens.Ensemble([case], workers=1)
+    nsteps = 5
+    d = batch.run(nsteps)
+    batch.pull()
+    ref = O.CylinderFlow(initial='profile', **ens.solver_kwargs(case))
+    for _ in range(nsteps):
+        ref.step(case['dt'])
+    f = batch.flows[0]
+    for key in ('u', 'v', 'p'):
+        a, r = getattr(f, key), getattr(ref, key)
+        assert np.max(abs(a - r)) <= 1e-11 * np.max(abs(r)), key
+    dr = ref.last_step_diagnostics
+    scale = max(abs(dr['cd']), abs(dr['cl']))
+    assert abs(d[0, -1, 2] - dr['cd']) <= 1e-10 * scale
+    assert abs(d[0, -1, 3] - dr['cl']) <= 1e-10 * scale
+    batch.close()
+
+
+def test_seeded_ensemble_divergence(ens):
+    """The first eight cases of the seeded 64-case ensemble at their real sizes: every step of every
+    case keeps max|div| < 1e-12 and finite forces."""
+    cases = [ens.ensemble_case(k) for k in range(8)]
+    print('ensemble: shapes', [(c['nx'], c['ny']) for c in cases], flush=True)
+    batch = ens.Ensemble(cases)
+    d = batch.run(12)
+    assert np.all(d[:, :, 10] < 1e-12), d[:, :, 10].max(axis=1)
+    assert np.all(np.isfinite(d[:, :, 2])) and np.all(np.isfinite(d[:, :, 3]))
+    assert np.all(d[:, :, 26] == 0)   # status word
+    batch.close()
