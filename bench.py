@@ -48,6 +48,42 @@ def cpu_baseline(seconds=20.0):
                        f'{med * 1e3:.1f} ms (oracle/ghost_oracle.py, scipy.fft workers={cores}, single-threaded SpMV)')
 
 
+def ensemble_leg(pkg, steps, plain_steps):
+    """The seeded 64-case ensemble (uneven 330x64 .. 1320x246 channels, Re 40..200, zero start,
+    dt = 0.05 h/U_m): aggregate cell-steps/s of the batched path (one launch per stage over all
+    cases, graph-captured steps) against the same cases stepped one after another on the plain
+    path.  Members that fail a gate of the reference step are dropped and listed."""
+    import importlib
+    ens = importlib.import_module(pkg.__name__ + '.ensemble')
+    cases = [ens.ensemble_case(k) for k in range(64)]
+    t0 = time.perf_counter()
+    batch = ens.Ensemble(cases)
+    setup_s = time.perf_counter() - t0
+    batch.run(3)
+    print(f'bench: ensemble of 64 cases ready in {setup_s:.1f} s; timing {steps} batched steps', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    d = batch.run(steps)
+    tb = time.perf_counter() - t0
+    batch.close()
+    # throughput over the members that ran all timed steps (a dropped member's cells are not counted)
+    ran = [k for k in range(64) if k not in batch.failures]
+    cells = np.array([cases[k]['nx'] * cases[k]['ny'] for k in ran], dtype=float)
+    tp = 0.
+    for k in ran:
+        f, c = batch.flows[k], cases[k]
+        f.gg.device.run(2, c['dt'])
+        t0 = time.perf_counter()
+        dp = f.gg.device.run(plain_steps, c['dt'])
+        tp += time.perf_counter() - t0
+        assert np.max(dp[:, 10]) < 1e-12
+    batched = cells.sum() * steps / tb
+    plain = cells.sum() * plain_steps / tp
+    return dict(cases=64, cases_run=len(ran), failures={str(k): v['message'] for k, v in batch.failures.items()},
+                cells=int(cells.sum()), grids='330x64 .. 1320x246, seeds 4000..4063', batched_steps=steps,
+                batched_value=batched, plain_steps=plain_steps, plain_value=plain, ratio=batched / plain,
+                max_divergence=float(np.nanmax(d[ran][:, :, 10])), setup_seconds=setup_s, unit='cell-steps/s')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -55,6 +91,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    ap.add_argument('--no-ensemble', action='store_true')
+    ap.add_argument('--ensemble-steps', type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build runs one case on one GPU; --gpus {args.gpus} is not supported', file=sys.stderr)
@@ -99,6 +137,7 @@ def main():
                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
                kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms), setup_seconds=setup_s)
+    out['ensemble'] = None if args.no_ensemble else ensemble_leg(pkg, args.ensemble_steps, max(args.ensemble_steps // 4, 10))
     if not args.no_cpu_baseline:
         print('bench: CPU baseline (oracle)', file=sys.stderr, flush=True)
         out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)

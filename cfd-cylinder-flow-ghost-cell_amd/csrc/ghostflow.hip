@@ -1877,7 +1877,11 @@ int mcf_project(mcf_handle* h, const double* q, double dt, double* out, double* 
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (!(fin <= target)) return fail(h, 2, "Pressure solve failed: residual " + std::to_string(fin));
-  if (!(div <= 1e-10)) return fail(h, 2, "Ghost projection failed: divergence=" + std::to_string(div));
+  if (!(div <= 1e-10)) {
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "Ghost projection failed: divergence=%.6g", div);
+    return fail(h, 2, buf);
+  }
   return 0;
 }
 

@@ -25,7 +25,10 @@ def channel_case(m, d_over_h, delta, re, nx_range=(330, 1320), ny_range=(62, 246
     parabolic inflow U_m = 1.5 (mean 1.0, so nu = 1/Re), dt = courant h / U_m; the grid is clamped
     to nx_range x ny_range.  courant = 0.05 is the dt U_m / h of the plain channel cases (0.06 at
     h = D/20, 0.045 at h = D/60) and keeps the step's CFL number (dt (max|u|/dx + max|v|/dy), gate
-    0.5 in main.py) near 0.3 from the projected start."""
+    0.5 in main.py) below 0.3.  The cases start from the zero field (inlet column at the profile,
+    then projected): the impulsive start from the profile copied across the channel drives the
+    centred AB2 advection unstable on the coarse, high-Re members (at a fixed physical time,
+    whatever dt)."""
     D = 1.0
     h = D / m
     nx = int(np.clip(round(22 * D / h), *nx_range))
@@ -66,9 +69,11 @@ def solver_kwargs(case):
     return {k: case[k] for k in ('nx', 'ny', 'Lx', 'Ly', 'D', 'xc', 'yc', 'nu', 'rho', 'inflow', 'U_m')}
 
 
-def build_flows(cases, workers=None, initial='profile'):
+def build_flows(cases, workers=None, initial='zero', failures=None):
     """One device-resident ``MAC_CylinderFlow`` per case, its Helmholtz factors built for its dt;
-    the host tables are prepared in a process pool."""
+    the host tables are prepared in a process pool.  With a ``failures`` dict, a case whose start
+    fails the reference's projection gate is recorded there (index -> message) and given None;
+    without one the error propagates."""
     cases = list(cases)
     workers = workers or min(16, len(os.sched_getaffinity(0)), len(cases))
     if workers > 1:
@@ -77,50 +82,100 @@ def build_flows(cases, workers=None, initial='profile'):
     else:
         prepared = [prepare(c) for c in cases]
     flows = []
-    for case, prep in zip(cases, prepared):
-        f = MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial=initial,
-                             prebuilt=prep, **solver_kwargs(case))
+    for k, (case, prep) in enumerate(zip(cases, prepared)):
+        try:
+            f = MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial=initial,
+                                 prebuilt=prep, **solver_kwargs(case))
+        except RuntimeError as e:
+            if failures is None:
+                raise
+            failures[k] = dict(step=0, message=f'start: {e}')
+            flows.append(None)
+            continue
         f._push()
         flows.append(f)
     return flows
 
 
-class Ensemble:
-    """A list of cases advanced together by ``mcf_batch_run``."""
+def _status_message(row, step):
+    st = int(row[26])
+    what = ('non-finite force' if st & 16 else 'CN momentum solve' if st & 1 else 'pressure solve' if st & 2
+            else 'projection' if st & 4 else 'divergence/CFL gate')
+    return f'{what} failed at step {step}: div={row[10]:.3g} cfl={row[11]:.3g}'
 
-    def __init__(self, cases, workers=None, initial='profile', flows=None):
+
+class Ensemble:
+    """Cases advanced together by ``mcf_batch_run`` (one launch per stage over all cases).
+
+    A case that fails a gate of the reference step (main.py:414-426) is dropped, with its step and
+    reason in ``failures``, and the others continue; ``drop_failed=False`` raises instead."""
+
+    def __init__(self, cases, workers=None, initial='zero', flows=None, drop_failed=True):
         self.cases = list(cases)
-        self.flows = flows if flows is not None else build_flows(self.cases, workers, initial)
+        self.drop_failed = drop_failed
+        self.failures = {}
+        if flows is None:
+            flows = build_flows(self.cases, workers, initial, failures=self.failures if drop_failed else None)
+        self.flows = flows
         self.cells = np.array([c['nx'] * c['ny'] for c in self.cases])
-        lib = self._lib = nat.library()
-        arr = (ctypes.c_void_p * len(self.flows))(*[f.gg.device.h.h.value for f in self.flows])
-        self.dts = np.array([c['dt'] for c in self.cases], dtype=np.float64)
+        self.steps_done = 0
+        self._lib = nat.library()
+        self.b = None
+        self._rebuild()
+
+    @property
+    def active(self):
+        return [k for k, f in enumerate(self.flows) if f is not None and k not in self.failures]
+
+    def _rebuild(self):
+        self.close()
+        idx = self.active
+        if not idx:
+            raise RuntimeError('no ensemble case left to run: ' + '; '.join(
+                f'case {k}: {v["message"]}' for k, v in sorted(self.failures.items())))
+        arr = (ctypes.c_void_p * len(idx))(*[self.flows[k].gg.device.h.h.value for k in idx])
+        self.dts = np.array([self.cases[k]['dt'] for k in idx], dtype=np.float64)
         b = ctypes.c_void_p()
-        rc = lib.mcf_batch_create(arr, nat.dptr(self.dts), len(self.flows), ctypes.byref(b))
+        rc = self._lib.mcf_batch_create(arr, nat.dptr(self.dts), len(idx), ctypes.byref(b))
         if rc:
             # a failed create returns no batch; the message is on the first case's handle
-            raise nat.STATUS_EXCEPTIONS.get(rc, RuntimeError)(lib.mcf_last_error(self.flows[0].gg.device.h.h).decode())
-        self.b = b
+            raise nat.STATUS_EXCEPTIONS.get(rc, RuntimeError)(
+                self._lib.mcf_last_error(self.flows[idx[0]].gg.device.h.h).decode())
+        self.b, self._idx = b, idx
 
     def run(self, nsteps):
-        """Advance every case by nsteps at its own dt; returns diagnostics (cases, nsteps, 32)."""
-        diag = np.zeros((len(self.flows), nsteps, nat.DIAG_LEN))
+        """Advance every active case by nsteps at its own dt.  Returns diagnostics
+        (n_cases, nsteps, 32) indexed by case; rows of inactive cases are NaN."""
+        idx = self._idx
+        diag = np.zeros((len(idx), nsteps, nat.DIAG_LEN))
         rc = self._lib.mcf_batch_run(self.b, int(nsteps), nat.dptr(diag))
+        out = np.full((len(self.cases), nsteps, nat.DIAG_LEN), np.nan)
+        out[idx] = diag
         if rc:
-            raise nat.STATUS_EXCEPTIONS.get(rc, RuntimeError)(self._lib.mcf_batch_last_error(self.b).decode())
-        for f, d in zip(self.flows, diag):
-            f.time = float(d[-1, 0])
-            f.last_step_diagnostics = diagnostics_from_row(d[-1])
-        return diag
+            msg = self._lib.mcf_batch_last_error(self.b).decode()
+            bad = [i for i in range(len(idx)) if np.any(diag[i, :, 26] != 0)]
+            if not self.drop_failed or not bad:
+                raise nat.STATUS_EXCEPTIONS.get(rc, RuntimeError)(msg)
+            for i in bad:
+                s = int(np.flatnonzero(diag[i, :, 26] != 0)[0])
+                self.failures[idx[i]] = dict(step=self.steps_done + s, message=_status_message(diag[i, s], self.steps_done + s))
+                out[idx[i], s + 1:] = np.nan
+            self._rebuild()
+        self.steps_done += nsteps
+        for k in idx:
+            if k not in self.failures:
+                self.flows[k].time = float(out[k, -1, 0])
+                self.flows[k].last_step_diagnostics = diagnostics_from_row(out[k, -1])
+        return out
 
     def pull(self):
-        for f in self.flows:
-            f._pull()
+        for k in self.active:
+            self.flows[k]._pull()
 
     def close(self):
         if getattr(self, 'b', None) is not None and self.b.value:
             self._lib.mcf_batch_destroy(self.b)
-            self.b = None
+        self.b = None
 
     def __del__(self):
         try:

@@ -54,7 +54,7 @@ def test_batched_case_matches_oracle(ens):
     nsteps = 5
     d = batch.run(nsteps)
     batch.pull()
-    ref = O.CylinderFlow(initial='profile', **ens.solver_kwargs(case))
+    ref = O.CylinderFlow(initial='zero', **ens.solver_kwargs(case))
     for _ in range(nsteps):
         ref.step(case['dt'])
     f = batch.flows[0]
@@ -75,7 +75,31 @@ def test_seeded_ensemble_divergence(ens):
     print('ensemble: shapes', [(c['nx'], c['ny']) for c in cases], flush=True)
     batch = ens.Ensemble(cases)
     d = batch.run(12)
+    assert not batch.failures, batch.failures
     assert np.all(d[:, :, 10] < 1e-12), d[:, :, 10].max(axis=1)
     assert np.all(np.isfinite(d[:, :, 2])) and np.all(np.isfinite(d[:, :, 3]))
     assert np.all(d[:, :, 26] == 0)   # status word
+    batch.close()
+
+
+def test_failed_case_is_dropped(ens):
+    """A member that fails the reference's CFL gate (dt five times too large) is dropped at its first
+    step with the reason recorded; the other members continue bit-identical to single-case runs."""
+    good = small_cases(ens)
+    bad = ens.channel_case(5, 0.2, 0.06, 60., nx_range=(40, 400), ny_range=(12, 80), courant=0.25)
+    cases = [good[0], bad, good[2]]
+    batch = ens.Ensemble(cases, workers=1)
+    d = batch.run(3)
+    assert list(batch.failures) == [1] and batch.failures[1]['step'] == 0
+    assert 'CFL' in batch.failures[1]['message'] and d[1, 0, 11] >= 0.5
+    assert np.all(np.isnan(d[1, 1:, 0]))
+    d2 = batch.run(3)
+    assert np.all(np.isnan(d2[1]))
+    batch.pull()
+    singles = ens.build_flows([good[0], good[2]], workers=1)
+    for k, f in zip((0, 2), singles):
+        f.gg.device.run(6, cases[k]['dt'])
+        f._pull()
+        for key in ('u', 'v', 'p'):
+            assert np.array_equal(getattr(batch.flows[k], key), getattr(f, key)), (k, key)
     batch.close()
