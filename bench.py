@@ -84,6 +84,37 @@ def ensemble_leg(pkg, steps, plain_steps):
                 max_divergence=float(np.nanmax(d[ran][:, :, 10])), setup_seconds=setup_s, unit='cell-steps/s')
 
 
+def ensemble_long(pkg, total, chunk=500):
+    """The 64-case ensemble for `total` steps in chunks, one progress line per chunk (stderr) and
+    one JSON summary line (stdout)."""
+    import importlib
+    ens = importlib.import_module(pkg.__name__ + '.ensemble')
+    cases = [ens.ensemble_case(k) for k in range(64)]
+    t0 = time.perf_counter()
+    b = ens.Ensemble(cases)
+    setup_s = time.perf_counter() - t0
+    done, cell_steps, wall, chunks = 0, 0., 0., []
+    while done < total:
+        act = b.active
+        t = time.perf_counter()
+        d = b.run(min(chunk, total - done))
+        dt = time.perf_counter() - t
+        n = d.shape[1]
+        done += n
+        survivors = b.active
+        cells = sum(cases[k]['nx'] * cases[k]['ny'] for k in survivors)
+        cell_steps += cells * n
+        wall += dt
+        row = dict(steps=done, active=len(survivors), rate=cells * n / dt, max_div=float(np.nanmax(d[survivors][:, :, 10])),
+                   max_cfl=float(np.nanmax(d[survivors][:, :, 11])), dropped=sorted(set(act) - set(survivors)))
+        chunks.append(row)
+        print('bench: ' + json.dumps(row), file=sys.stderr, flush=True)
+    out = dict(workload='64-case seeded ensemble, zero start, dt = 0.05 h/U_m', steps=total, setup_seconds=setup_s,
+               wall_seconds=wall, cell_steps_per_s=cell_steps / wall, cases_completed=len(b.active),
+               failures={str(k): v for k, v in sorted(b.failures.items())}, chunks=chunks)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -93,12 +124,18 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
     ap.add_argument('--no-ensemble', action='store_true')
     ap.add_argument('--ensemble-steps', type=int, default=200)
+    ap.add_argument('--no-warm-start', action='store_true')
+    ap.add_argument('--ensemble-long', type=int, default=0,
+                    help='only run the 64-case ensemble for this many steps and print its summary')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build runs one case on one GPU; --gpus {args.gpus} is not supported', file=sys.stderr)
         sys.exit(2)
     import cfd_amd_loader
     pkg = cfd_amd_loader.load()
+    if args.ensemble_long:
+        ensemble_long(pkg, args.ensemble_long)
+        return
     t0 = time.perf_counter()
     flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='profile',
                                 v_perturbation=dict(seed=SEED), **WORKLOAD)
@@ -138,6 +175,25 @@ def main():
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
                kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms), setup_seconds=setup_s)
     out['ensemble'] = None if args.no_ensemble else ensemble_leg(pkg, args.ensemble_steps, max(args.ensemble_steps // 4, 10))
+    if not args.no_warm_start:
+        import importlib
+        ws = importlib.import_module(pkg.__name__ + '.warmstart')
+        print('bench: warm-start study (pair dump, device training, GMRES on held-out channels)', file=sys.stderr, flush=True)
+        res = ws.study(log=lambda m: print('bench: ' + m, file=sys.stderr, flush=True))
+        conv_flops = 2.0 * 64 * 340 * 64 * 64 * 5
+        out['warm_start'] = dict(
+            training_pairs=res['training_pairs'], training_steps=res['training_steps'],
+            training_seconds=res['training_seconds'], loss_first=res['loss_first'], loss_last=res['loss_last'],
+            gmres_rtol=1e-12, heldout_cases=[dict(group=h['group'], case=h['case'], rank=h['rank']) for h in res['heldout']],
+            heldout_failed=res['failed'],
+            guesses={grp: {g: dict(mean_arnoldi_steps=v['mean_iterations'], median_initial_residual=v['median_initial_residual'],
+                                   max_relres=v['max_relres'], max_error_vs_direct=v['max_error_vs_direct'], systems=v['systems'])
+                           for g, v in gs.items()} for grp, gs in res['summary'].items()},
+            conv_roofline=dict(bound='mfma', kernel='k_conv (BF16 16x16x16 MFMA, 64 -> 64 channels, 5 taps, 21760 positions)',
+                               launch_ms=res['conv_ms_per_launch'],
+                               achieved=conv_flops / (res['conv_ms_per_launch'] * 1e-3) / 1e12, peak=2500.0,
+                               unit='TFLOP/s', frac=conv_flops / (res['conv_ms_per_launch'] * 1e-3) / 2.5e15),
+            seconds=res['seconds'])
     if not args.no_cpu_baseline:
         print('bench: CPU baseline (oracle)', file=sys.stderr, flush=True)
         out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)

@@ -28,6 +28,7 @@ DIAG_NAMES = ['time', 'dt', 'cd', 'cl', 'cd_pressure', 'cd_viscous', 'cl_pressur
 DIAG_LEN = 32
 
 _D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.POINTER(ctypes.c_int)
 _U8 = ctypes.POINTER(ctypes.c_ubyte)
 _H = ctypes.c_void_p
@@ -59,6 +60,24 @@ _SIGNATURES = {
     'mcf_apply_operator': (ctypes.c_int, [_H, ctypes.c_int, _D, _D, ctypes.c_double]),
     'mcf_divergence_stats': (ctypes.c_int, [_H, _D, _U8, _D]),
     'mcf_profile': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_double, _D]),
+    'mcf_set_schur_system': (ctypes.c_int, [_H, ctypes.c_int, _D, _I, _I, _D, _I]),
+    'mcf_set_dump': (ctypes.c_int, [_H, ctypes.c_int]),
+    'mcf_get_dump': (ctypes.c_int, [_H, ctypes.c_int, _D, _D, _I, _I]),
+    'mcf_schur_gmres': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int, _D, _I, _D]),
+    'mcf_net_create': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_ulonglong,
+                                      ctypes.POINTER(_H)]),
+    'mcf_net_destroy': (None, [_H]),
+    'mcf_net_last_error': (ctypes.c_char_p, [_H]),
+    'mcf_net_num_params': (ctypes.c_int, [_H]),
+    'mcf_net_get_params': (ctypes.c_int, [_H, _F]),
+    'mcf_net_set_params': (ctypes.c_int, [_H, _F]),
+    'mcf_net_clear_groups': (ctypes.c_int, [_H]),
+    'mcf_net_add_group': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _F, _F, _F, _F]),
+    'mcf_net_train': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong, _F]),
+    'mcf_net_predict': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _F, _F]),
+    'mcf_net_loss_grad': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _F]),
+    'mcf_net_profile': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, _F]),
     'mcf_batch_create': (ctypes.c_int, [ctypes.POINTER(_H), _D, ctypes.c_int, ctypes.POINTER(_H)]),
     'mcf_batch_destroy': (None, [_H]),
     'mcf_batch_last_error': (ctypes.c_char_p, [_H]),
@@ -107,6 +126,8 @@ def as_arg(a):
             raise ValueError('array arguments must be C-contiguous')
         if a.dtype == np.float64:
             return a.ctypes.data_as(_D)
+        if a.dtype == np.float32:
+            return a.ctypes.data_as(_F)
         if a.dtype == np.int32:
             return a.ctypes.data_as(_I)
         if a.dtype == np.uint8:
@@ -136,6 +157,10 @@ def i32(a):
 
 def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 class Handle:

@@ -6,7 +6,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
-SOURCES = [HERE / 'csrc' / 'ghostflow.hip']
+SOURCES = [HERE / 'csrc' / 'ghostflow.hip', HERE / 'csrc' / 'ghostnet.hip']
 OUTPUT = HERE / 'libghostflow.so'
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',

@@ -132,6 +132,18 @@ struct SolveDesc {
   double* const* bufs;  // the case's buffer table (device)
   u64* sl;              // the case's slots
   int slot0;            // first slot of this solve's block
+  // the capacitance (Schur) system itself, for the iterative solve and the pair dump
+  const double* Sth;    // r*r: S = I + C R^{-1} E with rows and columns in boundary-angle order
+  const int* Cptr;      // r+1: C over the capacitance points, CSR in row order
+  const int* Ccol;      // point index in group order (as firstK)
+  const double* Cval;
+  const int* apos;      // r: angle position of each capacitance row
+  double* dump_c;       // dump_cap*r: recorded right-hand sides C R^{-1} b (angle order)
+  double* dump_y;       // dump_cap*r: recorded multipliers S^{-1} C R^{-1} b (angle order)
+  int* dump_tag;        // dump_cap: 4*step + projection
+  int* dump_count;
+  int dump_cap;
+  const int* clock;     // the case's device step counter
 };
 
 DEV double slot_d(const u64* s, int k) { return __longlong_as_double((long long)s[k]); }
@@ -822,6 +834,30 @@ __global__ __launch_bounds__(256) void k_capacitance(const SolveDesc* descs, Pcr
   if (lane == 0) d.y[row] = s;
 }
 
+// Records the pair of a first pressure Schur solve: rhs = C first (first = R^{-1} b at the
+// capacitance points) and the multiplier y = S^{-1} rhs, both in boundary-angle order, tagged with
+// the step and the projection.  One block per problem; a full ring stops recording.
+__global__ __launch_bounds__(256) void k_dump(const SolveDesc* descs, int proj) {
+  const SolveDesc& d = descs[blockIdx.y];
+  if (!d.dump_y) return;
+  __shared__ int slot;
+  if (threadIdx.x == 0) slot = *d.dump_count;
+  __syncthreads();
+  if (slot >= d.dump_cap) return;
+  double* c = d.dump_c + (size_t)slot * d.r;
+  double* y = d.dump_y + (size_t)slot * d.r;
+  for (int m = threadIdx.x; m < d.r; m += blockDim.x) {
+    double s = 0.0;
+    for (int q = d.Cptr[m]; q < d.Cptr[m + 1]; ++q) s += d.Cval[q] * d.firstK[d.Ccol[q]];
+    c[d.apos[m]] = s;
+    y[d.apos[m]] = d.y[m];
+  }
+  if (threadIdx.x == 0) {
+    d.dump_tag[slot] = 4 * (*d.clock) + proj;
+    *d.dump_count = slot + 1;
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // Ghost extension fill (unpack: E q + offset), in place, plus max |value| of written nodes
 // ------------------------------------------------------------------------------------
@@ -1100,6 +1136,7 @@ struct mcf_handle {
   bool has_surface = false;
   std::map<std::string, void*> owned;
   std::vector<void*> bufs;
+  std::vector<int> pt_pos[3];   // capacitance point (caller order) -> group order
   double* inlet = nullptr;
   u64* slots = nullptr;
   double* diag = nullptr;
@@ -1174,6 +1211,7 @@ int sv_block(int kind, int psolve) {
 // Everything a launch needs: the case table, the solve problems and the launch shapes (maxima
 // over the cases).  One case and a batch of uneven cases use the same code.
 struct Ctx {
+  bool dump;             // record the (rhs, multiplier) pairs of first pressure solves
   mcf_handle* err;       // receives error messages
   hipStream_t stream;
   const CaseDev* dc;     // n cases
@@ -1200,6 +1238,7 @@ void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
 Ctx ctx_of(mcf_handle* h) {
   Ctx x{};
   x.err = h; x.stream = h->stream; x.dc = h->dc; x.dsp = h->ds; x.dsh = h->ds + 1; x.n = 1;
+  x.dump = h->hs[0].dump_cap > 0;
   ctx_shapes(x, h->hc, h->hs);
   return x;
 }
@@ -1221,6 +1260,8 @@ int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int p
   if (r > 0) {
     hipLaunchKernelGGL(k_points, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
     hipLaunchKernelGGL(k_capacitance, dim3((r + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+    if (x.dump && pres && pred_slot < 0)
+      hipLaunchKernelGGL(k_dump, dim3(1, count), dim3(256), 0, x.stream, D, slot_shift / 8);
     PcrArgs p1{1, pred_slot, slot_shift};
     hipLaunchKernelGGL(k_tri, dim3(yy, count), dim3(TRI_T), 0, x.stream, D, p1);
   }
@@ -1750,6 +1791,10 @@ int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, con
   s.r = r; s.npts = npts;
   s.pt_j = dpj; s.pt_x = dpx; s.firstK = dfk; s.SC = dS; s.y = dy;
   s.rg_n = (int)rg_x.size(); s.rg_ptr = drp; s.rg_x = drx; s.rc_j = drj; s.rc_id = dri;
+  // a new capacitance set invalidates the Schur system and any pair dump built on the old one
+  s.Sth = nullptr; s.Cptr = nullptr; s.Ccol = nullptr; s.Cval = nullptr; s.apos = nullptr;
+  s.dump_c = s.dump_y = nullptr; s.dump_tag = s.dump_count = nullptr; s.dump_cap = 0;
+  h->pt_pos[kind] = pos;
   h->has_schur[kind] = true;
   return sync_descs(h);
 }
@@ -2232,6 +2277,314 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       }
   }
   return first_error;
+}
+
+}  // extern "C"
+
+// ======================================================================================
+// The capacitance (Schur) system S y = C R^{-1} b: pair dump and a restarted GMRES on the device
+// ======================================================================================
+namespace {
+
+constexpr int GM_T = 1024, GM_W = GM_T / 64, GM_MAXM = 160, GM_MAXR = 2048;
+
+struct GmresArgs {
+  const double* S;      // r*r row-major
+  int r, m, max_iter;
+  double rtol;
+  const double* b;      // n*r
+  const double* x0;     // n*r or null
+  double* x;            // n*r
+  int* iters;           // n
+  double* relres;       // n: final true residual |b - S x| / |b| (2-norm)
+  double* V;            // n*(m+1)*r Krylov basis
+  double* R;            // n*m*m upper triangle, column-major (column i at R + i*m)
+};
+
+// sum over the block (16 waves); every thread returns the same value, summed in a fixed order
+DEV double gm_block_sum(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < GM_W; ++k) s += red[k];
+  return s;
+}
+
+// out = S v (v, out in LDS): one wave per row, lanes over the columns
+DEV void gm_matvec(const double* S, int r, const double* v, double* out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = w; i < r; i += GM_W) {
+    const double* Si = S + (size_t)i * r;
+    double s = 0.0;
+    for (int k = lane; k < r; k += 64) s += Si[k] * v[k];
+    s = wave_sum(s);
+    if (lane == 0) out[i] = s;
+  }
+  __syncthreads();
+}
+
+// out = b - S x in compensated arithmetic (Dot2 of Ogita, Rump & Oishi 2005: exact products by FMA,
+// error-free sums), so the restart residual is accurate to about the unit round-off of the
+// residual itself rather than of |S| |x|
+DEV void gm_two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+DEV void gm_residual(const double* S, int r, const double* x, const double* b, double* out) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = w; i < r; i += GM_W) {
+    const double* Si = S + (size_t)i * r;
+    double s = lane == 0 ? b[i] : 0.0, c = 0.0;
+    for (int k = lane; k < r; k += 64) {
+      const double p = -Si[k] * x[k];
+      const double pe = fma(-Si[k], x[k], -p);
+      double t, e;
+      gm_two_sum(s, p, t, e);
+      s = t;
+      c += e + pe;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double so = __shfl_xor(s, off), co = __shfl_xor(c, off);
+      double t, e;
+      gm_two_sum(s, so, t, e);
+      s = t;
+      c = c + co + e;
+    }
+    if (lane == 0) out[i] = s + c;
+  }
+  __syncthreads();
+}
+
+// GMRES(m) with classical Gram-Schmidt applied twice and Givens rotations (Saad & Schultz 1986),
+// one workgroup per system; stops when the true residual |b - S x| <= rtol |b|, after max_iter
+// Arnoldi steps, or when a restart no longer reduces the true residual by 10%.
+__global__ __launch_bounds__(GM_T) void k_gmres(GmresArgs a) {
+  extern __shared__ double lds[];
+  const int sys = blockIdx.x, r = a.r, m = a.m, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double* xs = lds;            // r
+  double* ws = xs + r;         // r
+  double* vs = ws + r;         // r
+  double* hc = vs + r;         // m+2: current Hessenberg column
+  double* tp = hc + m + 2;     // m+2: Gram-Schmidt coefficients of one pass
+  double* cs = tp + m + 2;     // m
+  double* sn = cs + m;         // m
+  double* g = sn + m;          // m+2
+  double* red = g + m + 2;     // GM_W
+  const double* b = a.b + (size_t)sys * r;
+  double* V = a.V + (size_t)sys * (m + 1) * r;
+  double* R = a.R + (size_t)sys * m * m;
+  for (int i = tid; i < r; i += GM_T) xs[i] = a.x0 ? a.x0[(size_t)sys * r + i] : 0.0;
+  double part = 0.0;
+  for (int i = tid; i < r; i += GM_T) part += b[i] * b[i];
+  const double bnorm = sqrt(gm_block_sum(part, red));
+  const double target = a.rtol * bnorm;
+  int total = 0, cycles = 0;
+  double beta = 0.0, beta_prev = 0.0;
+  while (true) {
+    __syncthreads();
+    gm_residual(a.S, r, xs, b, ws);
+    part = 0.0;
+    for (int i = tid; i < r; i += GM_T) part += ws[i] * ws[i];
+    beta = sqrt(gm_block_sum(part, red));
+    if (!(beta > target) || total >= a.max_iter) break;
+    // a restart that no longer reduces the true residual has reached the attainable accuracy
+    if (cycles > 0 && !(beta < 0.9 * beta_prev)) break;
+    beta_prev = beta;
+    ++cycles;
+    for (int i = tid; i < r; i += GM_T) V[i] = ws[i] / beta;
+    if (tid == 0) g[0] = beta;
+    int jdone = 0;
+    for (int j = 0; j < m; ++j) {
+      __syncthreads();
+      for (int i = tid; i < r; i += GM_T) vs[i] = V[(size_t)j * r + i];
+      __syncthreads();
+      gm_matvec(a.S, r, vs, ws);
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int i = w; i <= j; i += GM_W) {
+          const double* Vi = V + (size_t)i * r;
+          double d = 0.0;
+          for (int k = lane; k < r; k += 64) d += Vi[k] * ws[k];
+          d = wave_sum(d);
+          if (lane == 0) { tp[i] = d; hc[i] = pass ? hc[i] + d : d; }
+        }
+        __syncthreads();
+        for (int k = tid; k < r; k += GM_T) {
+          double acc = ws[k];
+          for (int i = 0; i <= j; ++i) acc -= tp[i] * V[(size_t)i * r + k];
+          ws[k] = acc;
+        }
+        __syncthreads();
+      }
+      part = 0.0;
+      for (int i = tid; i < r; i += GM_T) part += ws[i] * ws[i];
+      const double hn = sqrt(gm_block_sum(part, red));
+      if (hn > 0.0)
+        for (int i = tid; i < r; i += GM_T) V[(size_t)(j + 1) * r + i] = ws[i] / hn;
+      if (tid == 0) {
+        hc[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+          const double t0 = cs[i] * hc[i] + sn[i] * hc[i + 1];
+          hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+          hc[i] = t0;
+        }
+        const double rho = hypot(hc[j], hc[j + 1]);
+        cs[j] = rho > 0.0 ? hc[j] / rho : 1.0;
+        sn[j] = rho > 0.0 ? hc[j + 1] / rho : 0.0;
+        hc[j] = rho;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+      }
+      __syncthreads();
+      for (int i = tid; i <= j; i += GM_T) R[(size_t)j * m + i] = hc[i];
+      ++total;
+      jdone = j + 1;
+      if (!(fabs(g[j + 1]) > target) || total >= a.max_iter || !(hn > 0.0)) break;
+    }
+    __syncthreads();
+    // back substitution R z = g, column by column (z overwrites g)
+    for (int i = jdone - 1; i >= 0; --i) {
+      if (tid == 0) g[i] = g[i] / R[(size_t)i * m + i];
+      __syncthreads();
+      const double zi = g[i];
+      for (int k = tid; k < i; k += GM_T) g[k] -= R[(size_t)i * m + k] * zi;
+      __syncthreads();
+    }
+    for (int k = tid; k < r; k += GM_T) {
+      double acc = xs[k];
+      for (int i = 0; i < jdone; ++i) acc += g[i] * V[(size_t)i * r + k];
+      xs[k] = acc;
+    }
+  }
+  for (int i = tid; i < r; i += GM_T) a.x[(size_t)sys * r + i] = xs[i];
+  if (tid == 0) {
+    a.iters[sys] = total;
+    a.relres[sys] = bnorm > 0.0 ? beta / bnorm : 0.0;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcf_set_schur_system(mcf_handle* h, int kind, const double* S, const int* Cptr, const int* Ccol,
+                         const double* Cval, const int* angle_order) {
+  if (!h || kind < 0 || kind > 2) return fail(h, 1, "invalid Schur system");
+  SolveDesc& s = h->hs[kind];
+  if (!h->has_schur[kind]) return fail(h, 1, "set the capacitance data before the Schur system");
+  const int r = s.r, npts = s.npts;
+  if (r == 0) return 0;
+  if (!S || !Cptr || !Ccol || !Cval || !angle_order) return fail(h, 1, "invalid Schur system");
+  if (Cptr[0] != 0) return fail(h, 1, "C row pointer must start at 0");
+  for (int m = 0; m < r; ++m) if (Cptr[m + 1] < Cptr[m]) return fail(h, 1, "C row pointer not monotone");
+  const int nnz = Cptr[r];
+  std::vector<int> col(nnz);
+  for (int q = 0; q < nnz; ++q) {
+    if (Ccol[q] < 0 || Ccol[q] >= npts) return fail(h, 1, "C column out of range");
+    col[q] = h->pt_pos[kind][Ccol[q]];
+  }
+  std::vector<int> apos(r, -1);
+  for (int t = 0; t < r; ++t) {
+    const int m = angle_order[t];
+    if (m < 0 || m >= r || apos[m] >= 0) return fail(h, 1, "angle order is not a permutation of the rows");
+    apos[m] = t;
+  }
+  std::vector<double> sth((size_t)r * r);
+  for (int t = 0; t < r; ++t)
+    for (int u = 0; u < r; ++u) sth[(size_t)t * r + u] = S[(size_t)angle_order[t] * r + angle_order[u]];
+  const std::string key = "ssys" + std::to_string(kind);
+  double *dS, *dv; int *dp, *dc, *da;
+  if (int rc = upload(h, key + "S", sth.data(), sth.size(), &dS)) return rc;
+  if (int rc = upload(h, key + "p", Cptr, (size_t)r + 1, &dp)) return rc;
+  if (int rc = upload(h, key + "c", col.data(), std::max<size_t>(col.size(), 1), &dc)) return rc;
+  if (int rc = upload(h, key + "v", Cval, std::max<size_t>(nnz, 1), &dv)) return rc;
+  if (int rc = upload(h, key + "a", apos.data(), apos.size(), &da)) return rc;
+  s.Sth = dS; s.Cptr = dp; s.Ccol = dc; s.Cval = dv; s.apos = da; s.clock = h->clock_step;
+  return sync_descs(h);
+}
+
+int mcf_set_dump(mcf_handle* h, int capacity) {
+  if (!h || capacity < 0) return fail(h, 1, "invalid dump capacity");
+  SolveDesc& s = h->hs[0];
+  if (capacity > 0 && !s.Sth) return fail(h, 1, "set the pressure Schur system before dumping pairs");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const size_t n = capacity > 0 ? (size_t)capacity * std::max(s.r, 1) : 0;
+  if (int rc = upload<double>(h, "dumpc", nullptr, n, &s.dump_c)) return rc;
+  if (int rc = upload<double>(h, "dumpy", nullptr, n, &s.dump_y)) return rc;
+  if (int rc = upload<int>(h, "dumpt", nullptr, (size_t)capacity, &s.dump_tag)) return rc;
+  if (int rc = upload<int>(h, "dumpn", nullptr, capacity > 0 ? 1 : 0, &s.dump_count)) return rc;
+  s.dump_cap = capacity > 0 && s.r > 0 ? capacity : 0;
+  return sync_descs(h);
+}
+
+int mcf_get_dump(mcf_handle* h, int max_pairs, double* c, double* y, int* tags, int* count) {
+  if (!h || !count || max_pairs < 0) return fail(h, 1, "invalid dump request");
+  const SolveDesc& s = h->hs[0];
+  *count = 0;
+  if (!s.dump_cap) return 0;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int n = 0;
+  HIPCHK(h, hipMemcpy(&n, s.dump_count, sizeof(int), hipMemcpyDeviceToHost));
+  n = std::min(n, max_pairs);
+  if (n > 0) {
+    if (!c || !y || !tags) return fail(h, 1, "invalid dump buffers");
+    HIPCHK(h, hipMemcpy(c, s.dump_c, (size_t)n * s.r * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(y, s.dump_y, (size_t)n * s.r * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(tags, s.dump_tag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  const int zero = 0;
+  HIPCHK(h, hipMemcpy(s.dump_count, &zero, sizeof(int), hipMemcpyHostToDevice));
+  *count = n;
+  return 0;
+}
+
+int mcf_schur_gmres(mcf_handle* h, int kind, int n, const double* rhs, const double* x0, double rtol,
+                    int max_iter, int restart, double* y, int* iters, double* relres) {
+  if (!h || kind < 0 || kind > 2 || n < 0 || !(rtol > 0) || max_iter < 0) return fail(h, 1, "invalid GMRES request");
+  const SolveDesc& s = h->hs[kind];
+  if (!s.Sth) return fail(h, 1, "Schur system not set");
+  const int r = s.r;
+  if (r > GM_MAXR) return fail(h, 1, "Schur system too large for the device GMRES");
+  if (n == 0) return 0;
+  if (!rhs || !y || !iters || !relres) return fail(h, 1, "invalid GMRES buffers");
+  const int m = std::max(1, std::min(restart > 0 ? restart : GM_MAXM, std::min(GM_MAXM, r)));
+  const int chunk = 256;
+  const size_t lds = sizeof(double) * (3 * (size_t)r + 4 * ((size_t)m + 2) + 2 * (size_t)m + GM_W);
+  double *db, *dx0 = nullptr, *dx, *dV, *dR, *drr; int* dit;
+  const int nc = std::min(n, chunk);
+  HIPCHK(h, hipMalloc(&db, (size_t)nc * r * sizeof(double)));
+  if (x0) HIPCHK(h, hipMalloc(&dx0, (size_t)nc * r * sizeof(double)));
+  HIPCHK(h, hipMalloc(&dx, (size_t)nc * r * sizeof(double)));
+  HIPCHK(h, hipMalloc(&dV, (size_t)nc * (m + 1) * r * sizeof(double)));
+  HIPCHK(h, hipMalloc(&dR, (size_t)nc * m * m * sizeof(double)));
+  HIPCHK(h, hipMalloc(&drr, (size_t)nc * sizeof(double)));
+  HIPCHK(h, hipMalloc(&dit, (size_t)nc * sizeof(int)));
+  int rc = 0;
+  for (int c0 = 0; c0 < n && !rc; c0 += chunk) {
+    const int k = std::min(chunk, n - c0);
+    const size_t off = (size_t)c0 * r;
+    if (hipMemcpy(db, rhs + off, (size_t)k * r * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        (x0 && hipMemcpy(dx0, x0 + off, (size_t)k * r * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)) {
+      rc = fail(h, 4, "GMRES upload failed");
+      break;
+    }
+    GmresArgs a{s.Sth, r, m, max_iter, rtol, db, dx0, dx, dit, drr, dV, dR};
+    hipLaunchKernelGGL(k_gmres, dim3(k), dim3(GM_T), lds, h->stream, a);
+    if ((rc = launch_check(h, "gmres"))) break;
+    if (hipStreamSynchronize(h->stream) != hipSuccess ||
+        hipMemcpy(y + off, dx, (size_t)k * r * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(iters + c0, dit, (size_t)k * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(relres + c0, drr, (size_t)k * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(h, 4, "GMRES failed on the device");
+  }
+  (void)hipFree(db); (void)hipFree(dx0); (void)hipFree(dx); (void)hipFree(dV); (void)hipFree(dR);
+  (void)hipFree(drr); (void)hipFree(dit);
+  return rc;
 }
 
 }  // extern "C"

@@ -64,10 +64,25 @@ class PreparedCase:
     """A case's recorded table uploads, prepared on the host without a device (picklable, compact):
     the entry point calls with their arrays, the index layout and the solver info."""
 
-    def __init__(self, layout, calls, body, u_inlet, u_ref, dt, pressure_info, helmholtz_info):
+    def __init__(self, layout, calls, body, u_inlet, u_ref, dt, pressure_info, helmholtz_info, geometry):
         self.layout, self.calls, self.body = layout, calls, body
+        self.schur_geometry = geometry
         self.u_inlet, self.u_ref, self.dt = u_inlet, u_ref, dt
         self.pressure_info, self.helmholtz_info = pressure_info, helmholtz_info
+
+
+def schur_geometry(ops, rows):
+    """Boundary-angle order of the capacitance rows (pressure cells, row-major rectangle index) and
+    per-row features for the warm-start network: signed distance of the cell centre to the
+    cylinder surface in cells, cos and sin of its angle about the centre."""
+    j, i = np.divmod(np.asarray(rows), ops.nx)
+    x, y = (i + 0.5) * ops.dx, (j + 0.5) * ops.dy
+    ang = np.arctan2(y - ops.yc, x - ops.xc)
+    dist = np.hypot(x - ops.xc, y - ops.yc)
+    order = np.lexsort((dist, ang)).astype(np.int32)
+    h = np.sqrt(ops.dx * ops.dy)
+    feats = np.stack(((dist - ops.R) / h, np.cos(ang), np.sin(ang)), axis=1)[order]
+    return order, np.ascontiguousarray(feats)
 
 
 def prepare_case(ops, u_inlet, u_ref, dt, pressure=None):
@@ -77,7 +92,7 @@ def prepare_case(ops, u_inlet, u_ref, dt, pressure=None):
     dev.set_reference_velocity(u_ref)
     dev.set_dt(dt)
     return PreparedCase(ops.layout(), rec.calls, dev.body, dev.u_inlet, u_ref, dt, dev.pressure_info,
-                        dev.helmholtz_inverse_info)
+                        dev.helmholtz_inverse_info, dev.schur_geometry)
 
 
 class GhostDevice:
@@ -99,6 +114,12 @@ class GhostDevice:
         self.pressure_factor, self.pressure_schur = pressure if pressure is not None else pressure_pieces(o)
         self._upload_rectangle(nat.PRESSURE, self.pressure_factor, self.pressure_schur)
         ps = self.pressure_schur
+        order, feats = schur_geometry(o, ps.rows)
+        Cp = ps.Cp.tocsr()
+        Cp.sort_indices()
+        self.h.call('mcf_set_schur_system', nat.PRESSURE, nat.f64(ps.S), nat.i32(Cp.indptr), nat.i32(Cp.indices),
+                    nat.f64(Cp.data), order)
+        self.schur_geometry = dict(order=order, features=feats)
         self.pressure_info = dict(backend='fft_boundary_schur', rank=ps.rank, dense_bytes=int(ps.S.nbytes),
                                   roundoff_drop_tolerance=ps.tolerance)
         self._upload_surface()
@@ -120,6 +141,7 @@ class GhostDevice:
         self.dt = prep.dt
         self.pressure_info = prep.pressure_info
         self.helmholtz_inverse_info = prep.helmholtz_info
+        self.schur_geometry = prep.schur_geometry
         self.pressure_factor = self.pressure_schur = self.helmholtz_factors = None
         return self
 
@@ -345,3 +367,29 @@ class GhostDevice:
         out = np.zeros(5)
         self.h.call('mcf_profile', int(n), float(dt), nat.dptr(out))
         return out
+
+    # ---------------------------------------------------------------- Schur system
+    def set_dump(self, capacity):
+        """Record the (c, y) pair of each projection's first pressure solve (angle order)."""
+        self.dump_capacity = int(capacity)
+        self.h.call('mcf_set_dump', self.dump_capacity)
+
+    def get_dump(self):
+        r = len(self.schur_geometry['order'])
+        cap = getattr(self, 'dump_capacity', 0)
+        c = np.empty((max(cap, 1), r)); y = np.empty_like(c)
+        tags = np.empty(max(cap, 1), dtype=np.int32); n = np.zeros(1, dtype=np.int32)
+        self.h.call('mcf_get_dump', cap, c, y, tags, n)
+        n = int(n[0])
+        return c[:n].copy(), y[:n].copy(), tags[:n].copy()
+
+    def schur_gmres(self, rhs, x0=None, rtol=1e-12, max_iter=400, restart=0):
+        """GMRES on the pressure Schur system for the rows of rhs (angle order)."""
+        rhs = nat.f64(np.atleast_2d(rhs))
+        n = rhs.shape[0]
+        y = np.empty_like(rhs)
+        its = np.empty(n, dtype=np.int32)
+        res = np.empty(n)
+        self.h.call('mcf_schur_gmres', nat.PRESSURE, n, rhs, None if x0 is None else nat.f64(np.atleast_2d(x0)),
+                    float(rtol), int(max_iter), int(restart), y, its, res)
+        return y, its, res

@@ -128,6 +128,52 @@ int mcf_divergence_stats(mcf_handle* h, const double* vel, const unsigned char* 
    gemm_launches_per_step, pcr_launches_per_step]. */
 int mcf_profile(mcf_handle* h, int n, double dt, double* out5);
 
+/* The capacitance (Schur) system S y = c of the boundary-row correction (pressure_schur.py:92-111:
+   S = I + C R^{-1} E_rows, c = C R^{-1} b) in boundary-angle order, for the pair dump and the
+   iterative solve.  S: r x r in the row order of mcf_set_schur; C: CSR over the capacitance points
+   (columns index the points array of mcf_set_schur); angle_order[t] = row at angle position t. */
+int mcf_set_schur_system(mcf_handle* h, int kind, const double* S, const int* Cptr, const int* Ccol,
+                         const double* Cval, const int* angle_order);
+/* Record the (c, y) pair of the first pressure solve of every projection of the steps that follow
+   (a ring of `capacity` pairs; 0 disables), in angle order, tagged 4*step + projection where step
+   counts from the start of each mcf_run / mcf_step call.  mcf_get_dump copies up to max_pairs
+   recorded pairs and empties the ring.  Build-only: the reference solves S directly. */
+int mcf_set_dump(mcf_handle* h, int capacity);
+int mcf_get_dump(mcf_handle* h, int max_pairs, double* c, double* y, int* tags, int* count);
+/* GMRES(restart) on S (angle order) for n right-hand sides (n x r), from x0 (null: zero) until
+   the true residual (compensated) |c - S y| <= rtol |c|, max_iter Arnoldi steps, or a restart that
+   no longer lowers it by 10%; returns y, the Arnoldi steps taken and the final true relative
+   residual per system.  The iterative counterpart of the direct S^{-1} apply
+   (pressure_schur.py:20-57 runs GMRES on the full pressure system instead). */
+int mcf_schur_gmres(mcf_handle* h, int kind, int n, const double* rhs, const double* x0, double rtol,
+                    int max_iter, int restart, double* y, int* iters, double* relres);
+
+/* Warm-start network for S y = c (no reference counterpart; the reference solves S directly,
+   pressure_schur.py:92-111).  A periodic 1D convolution network over the boundary angle: `layers`
+   convolutions of 5 taps with the given dilations, channels cin -> channels -> ... -> 1, ReLU
+   between layers; input per angle position: c / |c| and cin-1 geometry features; output y / |c|.
+   Forward and backward run on the matrix cores in BF16 with FP32 accumulation; weights, gradients
+   and Adam moments are FP32.  Parameters are laid out per layer as W[out][in][tap] then bias[out].
+   Training groups hold n samples of one boundary (length L): c, y (n x L, already scaled) and the
+   features (L x (cin-1)).  x for predict / loss_grad: n x L x cin.  Loss: mean squared error over
+   all positions of the output against y, or, when the group (or loss_grad) is given S (L x L,
+   angle order, FP32), of the residual S yhat - c against the input c (y is then not used). */
+typedef struct mcf_net mcf_net;
+int mcf_net_create(int layers, int cin, int channels, const int* dilations, unsigned long long seed, mcf_net** out);
+void mcf_net_destroy(mcf_net* n);
+const char* mcf_net_last_error(const mcf_net* n);
+int mcf_net_num_params(const mcf_net* n);
+int mcf_net_get_params(mcf_net* n, float* out);
+int mcf_net_set_params(mcf_net* n, const float* in);
+int mcf_net_clear_groups(mcf_net* n);
+int mcf_net_add_group(mcf_net* n, int count, int L, const float* c, const float* y, const float* features,
+                      const float* S);
+int mcf_net_train(mcf_net* n, int steps, int positions, float lr, unsigned long long seed, float* losses);
+int mcf_net_predict(mcf_net* n, int nsamp, int L, const float* x, float* y);
+int mcf_net_loss_grad(mcf_net* n, int nsamp, int L, const float* x, const float* y, const float* S, float* loss,
+                      float* grad);
+int mcf_net_profile(mcf_net* n, int nsamp, int L, int reps, float* ms_out);
+
 /* Batched ensemble (no reference counterpart; the reference runs one case per process):
    cases are fully configured handles (tables, state, Helmholtz factors for dts[c]); all share the
    step parity and history state.  Every stage of the step is one launch over all cases and
