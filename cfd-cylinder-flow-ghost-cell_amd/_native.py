@@ -78,6 +78,10 @@ _SIGNATURES = {
     'mcf_net_predict': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _F, _F]),
     'mcf_net_loss_grad': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, _F]),
     'mcf_net_profile': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, _F]),
+    'mcf_set_launch': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
+    'mcf_get_launch': (ctypes.c_int, [_H, _I]),
+    'mcf_batch_set_launch': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
+    'mcf_batch_get_launch': (ctypes.c_int, [_H, _I]),
     'mcf_batch_create': (ctypes.c_int, [ctypes.POINTER(_H), _D, ctypes.c_int, ctypes.POINTER(_H)]),
     'mcf_batch_destroy': (None, [_H]),
     'mcf_batch_last_error': (ctypes.c_char_p, [_H]),

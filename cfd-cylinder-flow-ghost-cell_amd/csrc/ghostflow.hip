@@ -606,19 +606,22 @@ DEV const double* desc_in(const SolveDesc& d, int id) { return d.bufs[id] + d.of
 DEV double* desc_out(const SolveDesc& d, int id) { return d.bufs[id] + d.off; }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-constexpr int GTM = 32, GTN = 64;   // block tile of the transform (4 waves of 16 x 32)
-constexpr int GKC = 64;             // K chunk staged in LDS (16 MFMA k-steps)
 
 // Dense FP64 y transform on the matrix cores: C (M x N) = A (M x K) * B (K x N), M = K = yy, N = xx.
 // A^T is read in row-major (k, m) order: forward (A = V^T) reads V, inverse (A = V) reads V^T.
-// Each block stages 64-deep K chunks of its A and B tiles in LDS; the next chunk is fetched into
-// registers while the current one feeds the MFMAs, so a transform costs ceil(K/64) memory round
+// Each block stages KC-deep K chunks of its A and B tiles in LDS; the next chunk is fetched into
+// registers while the current one feeds the MFMAs, so a transform costs ceil(K/KC) memory round
 // trips.  Fragment layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l holds
 // A[l%16][l/16], B[l/16][l%16] and D[4*r + l/16][l%16] in accumulator element r.
+// Block tile TM x TN (4 waves in 2 x 2, each (TM/2) x (TN/2) of 16 x 16 MFMA tiles) with a K chunk
+// of KC staged in LDS; the autotuner picks the shape per context.  Every shape accumulates each
+// output element over k in the same order, so all shapes give identical results.
+template <int TM, int TN, int KC>
 __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g) {
+  constexpr int MT = TM / 32, NT = TN / 32;
   const SolveDesc& d = descs[blockIdx.z];
   const int M = d.yy, N = d.xx, K = d.yy;
-  const int row0 = blockIdx.y * GTM, col0 = blockIdx.x * GTN;
+  const int row0 = blockIdx.y * TM, col0 = blockIdx.x * TN;
   if (row0 >= M || col0 >= N) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
   const double* AT = g.forward ? d.V : d.VT;
@@ -626,77 +629,84 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g
   int ldb;
   if (g.forward) { Bm = desc_in(d, g.in_id); ldb = d.ld; }
   else { Bm = d.F; ldb = d.xx; }
-  __shared__ double As[GKC][GTM];
-  __shared__ double Bs[GKC][GTN];
+  __shared__ double As[KC][TM];
+  __shared__ double Bs[KC][TN];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int kq = lane >> 4, l16 = lane & 15;
-  constexpr int NA = GKC * GTM / 256, NB = GKC * GTN / 256;
+  constexpr int NA = KC * TM / 256, NB = KC * TN / 256;
   double ra[NA], rb[NB];
   auto gload = [&](int k0) {
     #pragma unroll
     for (int t = 0; t < NA; ++t) {
-      const int e = tid + 256 * t, kk = k0 + (e >> 5), mm = row0 + (e & 31);
+      const int e = tid + 256 * t, kk = k0 + e / TM, mm = row0 + e % TM;
       ra[t] = (kk < K && mm < M) ? AT[(size_t)kk * M + mm] : 0.0;
     }
     #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      const int e = tid + 256 * t, kk = k0 + (e >> 6), nn = col0 + (e & 63);
+      const int e = tid + 256 * t, kk = k0 + e / TN, nn = col0 + e % TN;
       rb[t] = (kk < K && nn < N) ? Bm[(size_t)kk * ldb + nn] : 0.0;
     }
   };
   auto sstore = [&]() {
     #pragma unroll
-    for (int t = 0; t < NA; ++t) { const int e = tid + 256 * t; As[e >> 5][e & 31] = ra[t]; }
+    for (int t = 0; t < NA; ++t) { const int e = tid + 256 * t; As[e / TM][e % TM] = ra[t]; }
     #pragma unroll
-    for (int t = 0; t < NB; ++t) { const int e = tid + 256 * t; Bs[e >> 6][e & 63] = rb[t]; }
+    for (int t = 0; t < NB; ++t) { const int e = tid + 256 * t; Bs[e / TN][e % TN] = rb[t]; }
   };
-  d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  d4 acc[MT][NT];
+  #pragma unroll
+  for (int mi = 0; mi < MT; ++mi)
+    #pragma unroll
+    for (int ni = 0; ni < NT; ++ni) acc[mi][ni] = d4{0.0, 0.0, 0.0, 0.0};
   gload(0);
-  for (int k0 = 0; k0 < K; k0 += GKC) {
+  for (int k0 = 0; k0 < K; k0 += KC) {
     sstore();
     __syncthreads();
-    if (k0 + GKC < K) gload(k0 + GKC);
+    if (k0 + KC < K) gload(k0 + KC);
     #pragma unroll
-    for (int t = 0; t < GKC / 4; ++t) {
+    for (int t = 0; t < KC / 4; ++t) {
       const int kl = 4 * t + kq;
-      const double a = As[kl][16 * wm + l16];
-      const double b0 = Bs[kl][32 * wn + l16];
-      const double b1 = Bs[kl][32 * wn + 16 + l16];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+      double a[MT];
+      #pragma unroll
+      for (int mi = 0; mi < MT; ++mi) a[mi] = As[kl][(TM / 2) * wm + 16 * mi + l16];
+      #pragma unroll
+      for (int ni = 0; ni < NT; ++ni) {
+        const double b = Bs[kl][(TN / 2) * wn + 16 * ni + l16];
+        #pragma unroll
+        for (int mi = 0; mi < MT; ++mi) acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b, acc[mi][ni], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
-  const int n0 = col0 + 32 * wn + l16, n1 = n0 + 16;
-  const bool n0ok = n0 < N, n1ok = n1 < N;
-  const int rbase = row0 + 16 * wm + kq;
-  if (g.forward) {
+  #pragma unroll
+  for (int mi = 0; mi < MT; ++mi) {
+    const int rbase = row0 + (TM / 2) * wm + 16 * mi + kq;
     #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int gr = rbase + 4 * r;
-      if (gr >= M) continue;
-      if (n0ok) d.F[(size_t)gr * N + n0] = acc0[r];
-      if (n1ok) d.F[(size_t)gr * N + n1] = acc1[r];
-    }
-  } else {
-    double* O = desc_out(d, g.out_id);
-    #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int gr = rbase + 4 * r;
-      if (gr >= M) continue;
-      const int aj = gr + d.joff;
-      const int slo = d.lo ? d.lo[aj] : 1, shi = d.lo ? d.hi[aj] : 0;
-      #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int gc = t ? n1 : n0;
-        if (gc >= N) continue;
-        const double v = t ? acc1[r] : acc0[r];
+    for (int ni = 0; ni < NT; ++ni) {
+      const int gc = col0 + (TN / 2) * wn + 16 * ni + l16;
+      if (gc >= N) continue;
+      if (g.forward) {
+        #pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gr = rbase + 4 * r;
+          if (gr < M) d.F[(size_t)gr * N + gc] = acc[mi][ni][r];
+        }
+      } else {
+        double* O = desc_out(d, g.out_id);
         const int ai = gc + d.ioff;
-        double* pp = O + (size_t)gr * d.ld + gc;
-        if (g.epi == 0) *pp = v;
-        else if (slo <= ai && ai <= shi) *pp = 0.0;
-        else *pp = (g.epi == 2) ? (*pp + v) : v;
+        #pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gr = rbase + 4 * r;
+          if (gr >= M) continue;
+          const int aj = gr + d.joff;
+          const int slo = d.lo ? d.lo[aj] : 1, shi = d.lo ? d.hi[aj] : 0;
+          const double v = acc[mi][ni][r];
+          double* pp = O + (size_t)gr * d.ld + gc;
+          if (g.epi == 0) *pp = v;
+          else if (slo <= ai && ai <= shi) *pp = 0.0;
+          else *pp = (g.epi == 2) ? (*pp + v) : v;
+        }
       }
     }
   }
@@ -706,8 +716,9 @@ struct PcrArgs { int mode; int pred_slot; int slot_shift; };
 
 // Block-wide exclusive scan of affine maps y -> A y + B (composition in thread order, or in
 // reverse order when `reverse`); returns the value reached from 0 before this thread's chunk.
+template <int W>
 DEV double block_affine_scan(double A, double B, bool reverse) {
-  __shared__ double sa[4], sb[4];
+  __shared__ double sa[W], sb[W];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int off = 1; off < 64; off <<= 1) {
     const double Ap = reverse ? __shfl_down(A, off, 64) : __shfl_up(A, off, 64);
@@ -720,7 +731,7 @@ DEV double block_affine_scan(double A, double B, bool reverse) {
   // value entering this wave: compose the totals of the waves before it (after it, reversed)
   double vin = 0.0;
   if (!reverse) { for (int q = 0; q < w; ++q) vin = sa[q] * vin + sb[q]; }
-  else { for (int q = 3; q > w; --q) vin = sa[q] * vin + sb[q]; }
+  else { for (int q = W - 1; q > w; --q) vin = sa[q] * vin + sb[q]; }
   // inclusive value of the neighbouring lane
   const double Bn = reverse ? __shfl_down(A * vin + B, 1, 64) : __shfl_up(A * vin + B, 1, 64);
   const bool edge = reverse ? (lane == 63) : (lane == 0);
@@ -729,12 +740,16 @@ DEV double block_affine_scan(double A, double B, bool reverse) {
 }
 
 // Per-mode tridiagonal solve along x with the precomputed LU factors of (a + b lam_k) I + b lx
-// (constant super-diagonal b*upper): one 256-thread block per mode, each thread runs the
+// (constant super-diagonal b*upper): one block per mode, each thread runs the
 // forward and backward recurrences on a contiguous chunk held in registers; chunks are joined
 // by block scans of the affine maps.  mode 0: F[k] <- T_k^{-1} F[k];  mode 1: F[k] <- F[k] -
 // T_k^{-1} z_k with z_k built from the capacitance correction y (rows grouped by x column).
-constexpr int TRI_T = 256, TRI_C = 16, TRI_MAXX = TRI_T * TRI_C;
+// Launch shapes (chosen per context by the autotuner): TRI_T threads per mode, up to TRI_C
+// points per thread; every shape covers rows up to TRI_MAXX = 4096 points.
+constexpr int TRI_MAXX = 4096;
+template <int TRI_T>
 __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p) {
+  constexpr int TRI_C = TRI_MAXX / TRI_T;
   const SolveDesc& d = descs[blockIdx.y];
   const int k = blockIdx.x;
   if (k >= d.yy) return;
@@ -774,7 +789,7 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p
   #pragma unroll
   for (int c = 0; c < TRI_C; ++c)
     if (c < n) { const double al = -l[c]; A *= al; B = al * B + r[c]; }
-  double y = block_affine_scan(A, B, false);
+  double y = block_affine_scan<TRI_T / 64>(A, B, false);
   #pragma unroll
   for (int c = 0; c < TRI_C; ++c)
     if (c < n) { y = r[c] - l[c] * y; r[c] = y; }
@@ -786,7 +801,7 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p
       const double al = (i0 + c < xx - 1) ? -cu * id[c] : 0.0;
       A *= al; B = al * B + r[c] * id[c];
     }
-  double x = block_affine_scan(A, B, true);
+  double x = block_affine_scan<TRI_T / 64>(A, B, true);
   #pragma unroll
   for (int c = TRI_C - 1; c >= 0; --c)
     if (c < n) {
@@ -1119,6 +1134,32 @@ __global__ __launch_bounds__(256) void k_divstats(const double* u, const double*
 // ======================================================================================
 // Host side
 // ======================================================================================
+// Launch shapes of the transform (tile TM x TN, K chunk) and of the tridiagonal sweep (threads per
+// mode), per problem class; picked by the autotuner or pinned by the caller.
+constexpr int N_GEMM_V = 3, N_TRI_V = 3;
+constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64}, GEMM_TN[N_GEMM_V] = {64, 32, 64};
+constexpr int TRI_TH[N_TRI_V] = {128, 256, 512};
+struct Launch { int gemm_p = 0, gemm_h = 0, tri_p = 1, tri_h = 1; };
+
+inline dim3 gemm_grid(int v, int yy, int xx, int count) {
+  return dim3((xx + GEMM_TN[v] - 1) / GEMM_TN[v], (yy + GEMM_TM[v] - 1) / GEMM_TM[v], count);
+}
+void launch_gemm(int v, int yy, int xx, int count, hipStream_t st, const SolveDesc* D, GemmArgs g) {
+  const dim3 gg = gemm_grid(v, yy, xx, count);
+  switch (v) {
+    case 1: hipLaunchKernelGGL((k_gemm<32, 32, 64>), gg, dim3(256), 0, st, D, g); break;
+    case 2: hipLaunchKernelGGL((k_gemm<64, 64, 32>), gg, dim3(256), 0, st, D, g); break;
+    default: hipLaunchKernelGGL((k_gemm<32, 64, 64>), gg, dim3(256), 0, st, D, g); break;
+  }
+}
+void launch_tri(int v, int yy, int count, hipStream_t st, const SolveDesc* D, PcrArgs p) {
+  switch (v) {
+    case 0: hipLaunchKernelGGL(k_tri<128>, dim3(yy, count), dim3(128), 0, st, D, p); break;
+    case 2: hipLaunchKernelGGL(k_tri<512>, dim3(yy, count), dim3(512), 0, st, D, p); break;
+    default: hipLaunchKernelGGL(k_tri<256>, dim3(yy, count), dim3(256), 0, st, D, p); break;
+  }
+}
+
 struct StepGraphs;
 struct mcf_handle {
   int nx, ny, usize, vsize, psize, velsize;
@@ -1154,6 +1195,8 @@ struct mcf_handle {
   bool prm_valid = false;
   double prm_dt = 0.0, prm_alpha = 0.0, prm_factor = 0.0;
   StepGraphs* graphs = nullptr;
+  Launch ln;                 // launch shapes of this handle's solves
+  bool tuned = false;        // ln chosen (by the autotuner or the caller)
 };
 
 namespace {
@@ -1211,6 +1254,7 @@ int sv_block(int kind, int psolve) {
 // Everything a launch needs: the case table, the solve problems and the launch shapes (maxima
 // over the cases).  One case and a batch of uneven cases use the same code.
 struct Ctx {
+  Launch ln;             // launch shapes
   bool dump;             // record the (rhs, multiplier) pairs of first pressure solves
   mcf_handle* err;       // receives error messages
   hipStream_t stream;
@@ -1238,6 +1282,7 @@ void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
 Ctx ctx_of(mcf_handle* h) {
   Ctx x{};
   x.err = h; x.stream = h->stream; x.dc = h->dc; x.dsp = h->ds; x.dsh = h->ds + 1; x.n = 1;
+  x.ln = h->ln;
   x.dump = h->hs[0].dump_cap > 0;
   ctx_shapes(x, h->hc, h->hs);
   return x;
@@ -1252,21 +1297,21 @@ int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int p
   const int count = pres ? x.n : 2 * x.n;
   const int yy = pres ? x.p_yy : x.h_yy, xx = pres ? x.p_xx : x.h_xx;
   const int np = pres ? x.p_np : x.h_np, r = pres ? x.p_r : x.h_r;
+  const int gv = pres ? x.ln.gemm_p : x.ln.gemm_h, tv = pres ? x.ln.tri_p : x.ln.tri_h;
   GemmArgs g{1, in_id, out_id, 0, pred_slot, slot_shift};
-  dim3 gg((xx + GTN - 1) / GTN, (yy + GTM - 1) / GTM, count);
-  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, x.stream, D, g);
+  launch_gemm(gv, yy, xx, count, x.stream, D, g);
   PcrArgs p0{0, pred_slot, slot_shift};
-  hipLaunchKernelGGL(k_tri, dim3(yy, count), dim3(TRI_T), 0, x.stream, D, p0);
+  launch_tri(tv, yy, count, x.stream, D, p0);
   if (r > 0) {
     hipLaunchKernelGGL(k_points, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
     hipLaunchKernelGGL(k_capacitance, dim3((r + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
     if (x.dump && pres && pred_slot < 0)
       hipLaunchKernelGGL(k_dump, dim3(1, count), dim3(256), 0, x.stream, D, slot_shift / 8);
     PcrArgs p1{1, pred_slot, slot_shift};
-    hipLaunchKernelGGL(k_tri, dim3(yy, count), dim3(TRI_T), 0, x.stream, D, p1);
+    launch_tri(tv, yy, count, x.stream, D, p1);
   }
   GemmArgs gi{0, in_id, out_id, epi, pred_slot, slot_shift};
-  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, x.stream, D, gi);
+  launch_gemm(gv, yy, xx, count, x.stream, D, gi);
   return launch_check(x.err, "woodbury apply");
 }
 
@@ -1414,7 +1459,56 @@ void drop_graphs(mcf_handle* h) {
 }
 
 // n steps: steps without matching history run eagerly, the rest replay the captured graphs.
+// The autotuner: times every launch shape of the transform and of the tridiagonal sweep on the
+// context's own problems (forward transforms into the scratch F and in-place sweeps of F, which no
+// later kernel reads before rewriting; no live buffer is touched) and keeps the fastest per
+// problem class.  Transform shapes give identical results; sweep shapes differ in round-off.
+int autotune(const Ctx& x, Launch& out) {
+  hipEvent_t e0, e1;
+  HIPCHK(x.err, hipEventCreate(&e0));
+  HIPCHK(x.err, hipEventCreate(&e1));
+  const int reps = 10;
+  int rc = 0;
+  for (int cls = 0; cls < 2 && !rc; ++cls) {
+    const bool pres = cls == 0;
+    const SolveDesc* D = pres ? x.dsp : x.dsh;
+    const int count = pres ? x.n : 2 * x.n;
+    const int yy = pres ? x.p_yy : x.h_yy, xx = pres ? x.p_xx : x.h_xx;
+    const GemmArgs g{1, pres ? B_DIVP : B_RHS, 0, 0, -1, 0};
+    const PcrArgs p0{0, -1, 0};
+    float best = 1e30f;
+    int bg = 0, bt = 1;
+    for (int v = 0; v < N_GEMM_V + N_TRI_V && !rc; ++v) {
+      const bool gemm = v < N_GEMM_V;
+      const int k = gemm ? v : v - N_GEMM_V;
+      if (v == N_GEMM_V) best = 1e30f;
+      for (int r = 0; r < 2; ++r) gemm ? launch_gemm(k, yy, xx, count, x.stream, D, g) : launch_tri(k, yy, count, x.stream, D, p0);
+      (void)hipEventRecord(e0, x.stream);
+      for (int r = 0; r < reps; ++r) gemm ? launch_gemm(k, yy, xx, count, x.stream, D, g) : launch_tri(k, yy, count, x.stream, D, p0);
+      (void)hipEventRecord(e1, x.stream);
+      if ((rc = launch_check(x.err, "autotune"))) break;
+      float ms = 0.f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(x.err, 4, "autotune timing failed");
+        break;
+      }
+      if (ms < best) { best = ms; if (gemm) bg = k; else bt = k; }
+    }
+    if (pres) { out.gemm_p = bg; out.tri_p = bt; } else { out.gemm_h = bg; out.tri_h = bt; }
+  }
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int ensure_tuned(mcf_handle* h) {
+  if (h->tuned) return 0;
+  if (int rc = autotune(ctx_of(h), h->ln)) return rc;
+  h->tuned = true;
+  return 0;
+}
+
 int enqueue_steps(mcf_handle* h, int n, double dt) {
+  if (int rc = ensure_tuned(h)) return rc;
   const Ctx x = ctx_of(h);
   hipLaunchKernelGGL(k_set_clock, dim3(1, 1), dim3(64), 0, h->stream, h->dc, h->time);
   for (int s = 0; s < n; ++s) {
@@ -1975,13 +2069,13 @@ int mcf_rect_inverse(mcf_handle* h, int kind, const double* rhs, double* out) {
   t.bufs = tb; t.off = 0; t.ld = t.xx; t.lo = nullptr; t.hi = nullptr;
   HIPCHK(h, hipMemcpy(td, &t, sizeof(SolveDesc), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(h->tmp_rect, rhs, n * sizeof(double), hipMemcpyHostToDevice));
-  dim3 gg((t.xx + GTN - 1) / GTN, (t.yy + GTM - 1) / GTM, 1);
+  const int gv = kind == MCF_PRESSURE ? h->ln.gemm_p : h->ln.gemm_h, tv = kind == MCF_PRESSURE ? h->ln.tri_p : h->ln.tri_h;
   GemmArgs g{1, 0, 0, 0, -1, 0};
-  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, td, g);
+  launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, g);
   PcrArgs p0{0, -1, 0};
-  hipLaunchKernelGGL(k_tri, dim3(t.yy, 1), dim3(TRI_T), 0, h->stream, td, p0);
+  launch_tri(tv, t.yy, 1, h->stream, td, p0);
   GemmArgs gi{0, 0, 0, 0, -1, 0};
-  hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, td, gi);
+  launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, gi);
   int rc = launch_check(h, "rect inverse");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(out, h->tmp_rect, n * sizeof(double), hipMemcpyDeviceToHost));
@@ -2121,11 +2215,10 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   out5[2] = ms / n;
   // dominant kernels, timed back to back on the same stream over the pressure problem
   const SolveDesc& p = h->hs[0];
-  dim3 gg((p.xx + GTN - 1) / GTN, (p.yy + GTM - 1) / GTM, 1);
   const int reps = 50;
   GemmArgs g{1, B_DIVP, B_DIVP, 0, -1, 0};
   HIPCHK(h, hipEventRecord(e0, h->stream));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_gemm, gg, dim3(256), 0, h->stream, h->ds, g);
+  for (int r = 0; r < reps; ++r) launch_gemm(h->ln.gemm_p, p.yy, p.xx, 1, h->stream, h->ds, g);
   HIPCHK(h, hipEventRecord(e1, h->stream));
   HIPCHK(h, hipEventSynchronize(e1));
   HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
@@ -2133,7 +2226,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   PcrArgs p0{0, -1, 0};
   HIPCHK(h, hipEventRecord(e0, h->stream));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(k_tri, dim3(p.yy, 1), dim3(TRI_T), 0, h->stream, h->ds, p0);
+    launch_tri(h->ln.tri_p, p.yy, 1, h->stream, h->ds, p0);
   HIPCHK(h, hipEventRecord(e1, h->stream));
   HIPCHK(h, hipEventSynchronize(e1));
   HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
@@ -2145,6 +2238,28 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   for (int s = 0; s < n; ++s)
     if (int rc = status_error(h, d.data() + (size_t)s * MCF_DIAG_LEN, s)) return rc;
   return launch_check(h, "profile");
+}
+
+int mcf_set_launch(mcf_handle* h, int gemm, int tri) {
+  if (!h || gemm < -1 || gemm >= N_GEMM_V || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
+    return fail(h, 1, "invalid launch shape");
+  if (gemm < 0) {
+    h->ln = Launch();
+    h->tuned = false;
+  } else {
+    h->ln.gemm_p = h->ln.gemm_h = gemm;
+    h->ln.tri_p = h->ln.tri_h = tri;
+    h->tuned = true;
+  }
+  drop_graphs(h);
+  return 0;
+}
+
+int mcf_get_launch(mcf_handle* h, int* out5) {
+  if (!h || !out5) return 1;
+  out5[0] = h->ln.gemm_p; out5[1] = h->ln.gemm_h; out5[2] = h->ln.tri_p; out5[3] = h->ln.tri_h;
+  out5[4] = h->tuned ? 1 : 0;
+  return 0;
 }
 
 }  // extern "C"
@@ -2161,6 +2276,8 @@ struct mcf_batch {
   StepGraphs graphs;
   Ctx ctx{};
   bool built = false;
+  Launch ln;                    // launch shapes for the whole batch
+  bool tuned = false;
 };
 
 namespace {
@@ -2185,6 +2302,7 @@ int batch_build(mcf_batch* b) {
   HIPCHK(e, hipMemcpy(b->dc, hc.data(), n * sizeof(CaseDev), hipMemcpyHostToDevice));
   HIPCHK(e, hipMemcpy(b->ds, hs.data(), 3 * n * sizeof(SolveDesc), hipMemcpyHostToDevice));
   x.dc = b->dc; x.dsp = b->ds; x.dsh = b->ds + n;
+  x.ln = b->ln;
   b->ctx = x;
   b->graphs.drop();
   b->built = true;
@@ -2242,6 +2360,12 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
     h->owned.erase("p_raw_valid");
   }
   if (changed) if (int rc = batch_build(b)) return rc;
+  if (!b->tuned) {
+    if (int rc = autotune(b->ctx, b->ln)) return rc;
+    b->ctx.ln = b->ln;
+    b->tuned = true;
+    b->graphs.drop();
+  }
   const Ctx& x = b->ctx;
   for (int c = 0; c < n; ++c) {
     mcf_handle* h = b->cases[c];
@@ -2277,6 +2401,31 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       }
   }
   return first_error;
+}
+
+int mcf_batch_set_launch(mcf_batch* b, int gemm, int tri) {
+  if (!b) return 1;
+  mcf_handle* e = b->cases[0];
+  if (gemm < -1 || gemm >= N_GEMM_V || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
+    return fail(e, 1, "invalid launch shape");
+  if (gemm < 0) {
+    b->ln = Launch();
+    b->tuned = false;
+  } else {
+    b->ln.gemm_p = b->ln.gemm_h = gemm;
+    b->ln.tri_p = b->ln.tri_h = tri;
+    b->tuned = true;
+  }
+  b->ctx.ln = b->ln;
+  b->graphs.drop();
+  return 0;
+}
+
+int mcf_batch_get_launch(mcf_batch* b, int* out5) {
+  if (!b || !out5) return 1;
+  out5[0] = b->ln.gemm_p; out5[1] = b->ln.gemm_h; out5[2] = b->ln.tri_p; out5[3] = b->ln.tri_h;
+  out5[4] = b->tuned ? 1 : 0;
+  return 0;
 }
 
 }  // extern "C"

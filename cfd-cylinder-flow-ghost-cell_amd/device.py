@@ -368,6 +368,17 @@ class GhostDevice:
         self.h.call('mcf_profile', int(n), float(dt), nat.dptr(out))
         return out
 
+    # ---------------------------------------------------------------- launch shapes
+    def set_launch(self, gemm=-1, tri=-1):
+        """Pin the transform tile / sweep width variants (-1, -1: autotune at the next run)."""
+        self.h.call('mcf_set_launch', int(gemm), int(tri))
+
+    def get_launch(self):
+        out = np.zeros(5, dtype=np.int32)
+        self.h.call('mcf_get_launch', out)
+        return dict(gemm_pressure=int(out[0]), gemm_velocity=int(out[1]), tri_pressure=int(out[2]),
+                    tri_velocity=int(out[3]), chosen=bool(out[4]))
+
     # ---------------------------------------------------------------- Schur system
     def set_dump(self, capacity):
         """Record the (c, y) pair of each projection's first pressure solve (angle order)."""

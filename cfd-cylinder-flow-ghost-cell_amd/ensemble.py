@@ -121,6 +121,7 @@ class Ensemble:
         self.steps_done = 0
         self._lib = nat.library()
         self.b = None
+        self._launch = None
         self._rebuild()
 
     @property
@@ -142,6 +143,24 @@ class Ensemble:
             raise nat.STATUS_EXCEPTIONS.get(rc, RuntimeError)(
                 self._lib.mcf_last_error(self.flows[idx[0]].gg.device.h.h).decode())
         self.b, self._idx = b, idx
+        if self._launch is not None:
+            self._call('mcf_batch_set_launch', *self._launch)
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, name)(self.b, *[nat.as_arg(a) for a in args])
+        if rc:
+            raise nat.STATUS_EXCEPTIONS.get(rc, RuntimeError)(self._lib.mcf_batch_last_error(self.b).decode())
+
+    def set_launch(self, gemm=-1, tri=-1):
+        """Pin the batch's transform tile / sweep width variants (-1, -1: autotune at the next run)."""
+        self._launch = None if gemm < 0 else (int(gemm), int(tri))
+        self._call('mcf_batch_set_launch', int(gemm), int(tri))
+
+    def get_launch(self):
+        out = np.zeros(5, dtype=np.int32)
+        self._call('mcf_batch_get_launch', out)
+        return dict(gemm_pressure=int(out[0]), gemm_velocity=int(out[1]), tri_pressure=int(out[2]),
+                    tri_velocity=int(out[3]), chosen=bool(out[4]))
 
     def run(self, nsteps):
         """Advance every active case by nsteps at its own dt.  Returns diagnostics

@@ -174,6 +174,16 @@ int mcf_net_loss_grad(mcf_net* n, int nsamp, int L, const float* x, const float*
                       float* grad);
 int mcf_net_profile(mcf_net* n, int nsamp, int L, int reps, float* ms_out);
 
+/* Launch shapes of the rectangle inverse (no reference counterpart): the FP64 transform tile
+   (0: 32x64, 1: 32x32, 2: 64x64) and the tridiagonal sweep width (0: 128, 1: 256, 2: 512 threads
+   per mode).  By default the first mcf_run / mcf_step of a handle times every shape on the
+   handle's own problems and keeps the fastest per problem class (pressure, velocity); transform
+   shapes give identical results, sweep shapes differ in round-off.  mcf_set_launch(h, g, t) pins
+   both classes to (g, t); (-1, -1) returns to autotuning.  mcf_get_launch: [transform pressure,
+   transform velocity, sweep pressure, sweep velocity, chosen (1) or default (0)]. */
+int mcf_set_launch(mcf_handle* h, int gemm, int tri);
+int mcf_get_launch(mcf_handle* h, int* out5);
+
 /* Batched ensemble (no reference counterpart; the reference runs one case per process):
    cases are fully configured handles (tables, state, Helmholtz factors for dts[c]); all share the
    step parity and history state.  Every stage of the step is one launch over all cases and
@@ -184,6 +194,9 @@ int mcf_batch_create(mcf_handle* const* cases, const double* dts, int n, mcf_bat
 void mcf_batch_destroy(mcf_batch* b);
 const char* mcf_batch_last_error(const mcf_batch* b);
 int mcf_batch_run(mcf_batch* b, int nsteps, double* diag);
+/* Launch shapes of the batch (as mcf_set_launch / mcf_get_launch; autotuned at the first run). */
+int mcf_batch_set_launch(mcf_batch* b, int gemm, int tri);
+int mcf_batch_get_launch(mcf_batch* b, int* out5);
 
 #ifdef __cplusplus
 }

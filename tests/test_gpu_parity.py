@@ -143,3 +143,26 @@ def test_graph_replays_match_eager(pkg, monkeypatch):
         for key in ('u', 'v', 'p'):
             assert np.array_equal(getattr(eager, key), getattr(graph, key)), key
         assert np.array_equal(cde, cdg) and np.array_equal(cle, clg) and np.array_equal(te, tg)
+
+
+def test_launch_shapes_agree(pkg):
+    """Every transform tile gives bit-identical steps; every sweep width agrees to round-off."""
+    runs = {}
+    for g, t in [(0, 1), (1, 1), (2, 1), (0, 0), (0, 2)]:
+        f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
+        f.gg.device.set_launch(g, t)
+        f.solve(n_steps=6, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
+        runs[(g, t)] = (f.u.copy(), f.v.copy(), f.p.copy())
+        assert f.gg.device.get_launch()['tri_pressure'] == t
+    ref = runs[(0, 1)]
+    for key in [(1, 1), (2, 1)]:
+        for a, b in zip(runs[key], ref):
+            assert np.array_equal(a, b), key
+    for key in [(0, 0), (0, 2)]:
+        for a, b in zip(runs[key], ref):
+            assert np.max(abs(a - b)) <= 1e-12 * np.max(abs(b)), key
+    f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
+    f.solve(n_steps=2, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
+    ln = f.gg.device.get_launch()
+    print('autotuned launch shapes (440x82):', ln)
+    assert ln['chosen'] and all(0 <= ln[k] < 3 for k in ('gemm_pressure', 'gemm_velocity', 'tri_pressure', 'tri_velocity'))
