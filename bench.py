@@ -26,6 +26,9 @@ SEED = 20261203
 HBM_PEAK = 8.0e12          # B/s
 FP64_PEAK = 78.6e12        # FLOP/s
 STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB2/CN step
+# HBM bytes per forward launch of the pressure transform by tile variant (FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000 from rocprofv3 --pmc, scaled by a 256 MiB copy; profiles/k_gemm_1320x246_traffic.md)
+GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6}
 
 
 def cpu_baseline(seconds=20.0):
@@ -115,6 +118,27 @@ def ensemble_long(pkg, total, chunk=500):
     print(json.dumps(out))
 
 
+COPY_BYTES = 256 << 20
+
+
+def traffic_probe(pkg, steps, launch=None):
+    """Counter calibration and the measured kernels in one process: a plain device copy of
+    COPY_BYTES (reads and writes COPY_BYTES each), then `steps` steps of the 1320x246 workload
+    (with the transform tile / sweep width pinned to `launch` when given)."""
+    import torch
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='profile',
+                                v_perturbation=dict(seed=SEED), **WORKLOAD)
+    if launch is not None:
+        flow.gg.device.set_launch(*launch)
+    flow._push()
+    flow.gg.device.run(steps, DT)
+    print(f'bench: traffic probe ran a {COPY_BYTES} B copy and {steps} steps', file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -125,6 +149,13 @@ def main():
     ap.add_argument('--no-ensemble', action='store_true')
     ap.add_argument('--ensemble-steps', type=int, default=200)
     ap.add_argument('--no-warm-start', action='store_true')
+    ap.add_argument('--no-kernel-timing', action='store_true',
+                    help='skip the HIP-event timing of the dominant kernels (roofline becomes null)')
+    ap.add_argument('--traffic-probe', action='store_true',
+                    help='for rocprofv3 --pmc runs: one 256 MiB device copy (the counter calibration), then '
+                         '--steps steps of the workload; prints nothing else')
+    ap.add_argument('--launch', type=int, nargs=2, default=None, metavar=('TILE', 'WIDTH'),
+                    help='pin the transform tile and sweep width variants (traffic probe)')
     ap.add_argument('--ensemble-long', type=int, default=0,
                     help='only run the 64-case ensemble for this many steps and print its summary')
     args = ap.parse_args()
@@ -135,6 +166,9 @@ def main():
     pkg = cfd_amd_loader.load()
     if args.ensemble_long:
         ensemble_long(pkg, args.ensemble_long)
+        return
+    if args.traffic_probe:
+        traffic_probe(pkg, args.steps, args.launch)
         return
     t0 = time.perf_counter()
     flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='profile',
@@ -154,11 +188,15 @@ def main():
     value = cells * args.steps / elapsed
     max_div = float(np.max(diag[:, 10]))
     # dominant kernels timed with HIP events on the library's stream
-    prof = dev.profile(20, DT)
-    gemm_ms, pcr_ms, step_ms = prof[0], prof[1], prof[2]
+    tile = dev.get_launch()['gemm_pressure']
+    if args.no_kernel_timing:
+        gemm_ms = pcr_ms = step_ms = None
+    else:
+        prof = dev.profile(20, DT)
+        gemm_ms, pcr_ms, step_ms = prof[0], prof[1], prof[2]
     yy, xx = WORKLOAD['ny'], WORKLOAD['nx']
     gemm_flops = 2.0 * yy * yy * xx
-    gemm_achieved = gemm_flops / (gemm_ms * 1e-3)
+    gemm_achieved = gemm_flops / (gemm_ms * 1e-3) if gemm_ms else None
     out = dict(metric='cell-steps/s on 1 MI355X (plain vs batched+NN warm-start) at div<1e-12', value=value,
                unit='cell-steps/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=elapsed / args.steps * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
@@ -166,10 +204,11 @@ def main():
                config=dict(workload='channel 22Dx4.1D, cylinder D=1 at (2D,2D), 1320x246 cells (h=D/60), '
                                     'parabolic Um=1.5, Re=100, dt=5e-4, FP64, plain ghost AB2/CN step, forces every step',
                            path='plain', cells=cells, max_divergence=max_div),
-               roofline=dict(bound='mfma', achieved=gemm_achieved / 1e12, peak=FP64_PEAK / 1e12, unit='TFLOP/s',
-                             frac=gemm_achieved / FP64_PEAK, traffic=None,
-                             kernel='k_gemm (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
-                             launch_ms=gemm_ms),
+               roofline=None if gemm_achieved is None else dict(
+                   bound='mfma', achieved=gemm_achieved / 1e12, peak=FP64_PEAK / 1e12, unit='TFLOP/s',
+                   frac=gemm_achieved / FP64_PEAK, traffic=GEMM_TRAFFIC.get(tile),
+                   traffic_unit='bytes per launch', algorithmic_bytes=8.0 * (yy * yy + 2 * yy * xx), tile=tile,
+                   kernel='k_gemm (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)', launch_ms=gemm_ms),
                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),

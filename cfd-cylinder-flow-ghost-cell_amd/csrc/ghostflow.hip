@@ -619,9 +619,19 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 template <int TM, int TN, int KC>
 __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g) {
   constexpr int MT = TM / 32, NT = TN / 32;
-  const SolveDesc& d = descs[blockIdx.z];
+  // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by linear id, so each XCD is
+  // given a contiguous run of tiles (problem slowest, then column panel, row-block fastest) and
+  // the A and B panels its tiles share stay in that XCD's L2.  A bijection of [0, T).
+  const int gx = gridDim.x, gy = gridDim.y, T = gx * gy * gridDim.z;
+  const int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int xcd = id & 7;
+  int t = id >> 3;
+  for (int q = 0; q < xcd; ++q) t += (T - q + 7) >> 3;
+  const int z = t / (gx * gy), rem = t - z * gx * gy;
+  const int cx = rem / gy, ry = rem - cx * gy;
+  const SolveDesc& d = descs[z];
   const int M = d.yy, N = d.xx, K = d.yy;
-  const int row0 = blockIdx.y * TM, col0 = blockIdx.x * TN;
+  const int row0 = ry * TM, col0 = cx * TN;
   if (row0 >= M || col0 >= N) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
   const double* AT = g.forward ? d.V : d.VT;
