@@ -51,40 +51,47 @@ def cpu_baseline(seconds=20.0):
                        f'{med * 1e3:.1f} ms (oracle/ghost_oracle.py, scipy.fft workers={cores}, single-threaded SpMV)')
 
 
-def ensemble_leg(pkg, steps, plain_steps):
+def ensemble_leg(pkg, steps):
     """The seeded 64-case ensemble (uneven 330x64 .. 1320x246 channels, Re 40..200, zero start,
     dt = 0.05 h/U_m): aggregate cell-steps/s of the batched path (one launch per stage over all
     cases, graph-captured steps) against the same cases stepped one after another on the plain
-    path.  Members that fail a gate of the reference step are dropped and listed."""
+    path.  Both legs start from fresh copies of the cases and time the same steps (3 .. 3+steps);
+    members that fail a gate of the reference step are dropped and listed, and only members that
+    ran every timed step are counted in either leg."""
     import importlib
     ens = importlib.import_module(pkg.__name__ + '.ensemble')
     cases = [ens.ensemble_case(k) for k in range(64)]
     t0 = time.perf_counter()
     batch = ens.Ensemble(cases)
     setup_s = time.perf_counter() - t0
+    plain_flows = ens.build_flows(cases, failures={})
     batch.run(3)
     print(f'bench: ensemble of 64 cases ready in {setup_s:.1f} s; timing {steps} batched steps', file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     d = batch.run(steps)
     tb = time.perf_counter() - t0
+    spec = batch.get_speculation()
     batch.close()
-    # throughput over the members that ran all timed steps (a dropped member's cells are not counted)
-    ran = [k for k in range(64) if k not in batch.failures]
-    cells = np.array([cases[k]['nx'] * cases[k]['ny'] for k in ran], dtype=float)
+    ran = [k for k in range(64) if k not in batch.failures and plain_flows[k] is not None]
+    print(f'bench: timing {steps} plain steps of each of {len(ran)} cases', file=sys.stderr, flush=True)
     tp = 0.
     for k in ran:
-        f, c = batch.flows[k], cases[k]
-        f.gg.device.run(2, c['dt'])
+        f, c = plain_flows[k], cases[k]
+        f.gg.device.run(3, c['dt'])
         t0 = time.perf_counter()
-        dp = f.gg.device.run(plain_steps, c['dt'])
+        dp = f.gg.device.run(steps, c['dt'])
         tp += time.perf_counter() - t0
         assert np.max(dp[:, 10]) < 1e-12
+        f.gg.device.h.close()
+    cells = np.array([cases[k]['nx'] * cases[k]['ny'] for k in ran], dtype=float)
+    # the batched time also covers the dropped members' steps up to their failure: it is not rescaled
     batched = cells.sum() * steps / tb
-    plain = cells.sum() * plain_steps / tp
+    plain = cells.sum() * steps / tp
     return dict(cases=64, cases_run=len(ran), failures={str(k): v['message'] for k, v in batch.failures.items()},
-                cells=int(cells.sum()), grids='330x64 .. 1320x246, seeds 4000..4063', batched_steps=steps,
-                batched_value=batched, plain_steps=plain_steps, plain_value=plain, ratio=batched / plain,
-                max_divergence=float(np.nanmax(d[ran][:, :, 10])), setup_seconds=setup_s, unit='cell-steps/s')
+                cells=int(cells.sum()), grids='330x64 .. 1320x246, seeds 4000..4063', steps=steps,
+                window='steps 3 .. 3+steps of both legs, fresh copies', batched_value=batched, plain_value=plain,
+                ratio=batched / plain, max_divergence=float(np.nanmax(d[ran][:, :, 10])), setup_seconds=setup_s,
+                unit='cell-steps/s', batched_speculation=spec)
 
 
 def ensemble_long(pkg, total, chunk=500):
@@ -189,11 +196,13 @@ def main():
     max_div = float(np.max(diag[:, 10]))
     # dominant kernels timed with HIP events on the library's stream
     tile = dev.get_launch()['gemm_pressure']
+    lean_misses = None
     if args.no_kernel_timing:
         gemm_ms = pcr_ms = step_ms = None
     else:
         prof = dev.profile(20, DT)
         gemm_ms, pcr_ms, step_ms = prof[0], prof[1], prof[2]
+        lean_misses = int(prof[4])
     yy, xx = WORKLOAD['ny'], WORKLOAD['nx']
     gemm_flops = 2.0 * yy * yy * xx
     gemm_achieved = gemm_flops / (gemm_ms * 1e-3) if gemm_ms else None
@@ -212,8 +221,9 @@ def main():
                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
-               kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms), setup_seconds=setup_s)
-    out['ensemble'] = None if args.no_ensemble else ensemble_leg(pkg, args.ensemble_steps, max(args.ensemble_steps // 4, 10))
+               kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms, lean_misses=lean_misses),
+               speculation=dev.get_speculation(), setup_seconds=setup_s)
+    out['ensemble'] = None if args.no_ensemble else ensemble_leg(pkg, args.ensemble_steps)
     if not args.no_warm_start:
         import importlib
         ws = importlib.import_module(pkg.__name__ + '.warmstart')

@@ -80,6 +80,10 @@ _SIGNATURES = {
     'mcf_net_profile': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, _F]),
     'mcf_set_launch': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
     'mcf_get_launch': (ctypes.c_int, [_H, _I]),
+    'mcf_set_speculation': (ctypes.c_int, [_H, ctypes.c_int]),
+    'mcf_get_speculation': (ctypes.c_int, [_H, _I]),
+    'mcf_batch_set_speculation': (ctypes.c_int, [_H, ctypes.c_int]),
+    'mcf_batch_get_speculation': (ctypes.c_int, [_H, _I]),
     'mcf_batch_set_launch': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
     'mcf_batch_get_launch': (ctypes.c_int, [_H, _I]),
     'mcf_batch_create': (ctypes.c_int, [ctypes.POINTER(_H), _D, ctypes.c_int, ctypes.POINTER(_H)]),

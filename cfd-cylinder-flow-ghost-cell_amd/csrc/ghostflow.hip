@@ -41,8 +41,10 @@ constexpr double kTiny = 2.2250738585072014e-308;
 enum BufId {
   // vel-sized
   B_S0 = 0, B_S1, B_H0, B_H1, B_RHS, B_STAR, B_RES, B_DEFECT, B_PROJ, B_HALF, B_TV0, B_TV1,
+  B_CKS, B_CKH,        // checkpoint of the state and the advection history (speculative chunks)
   // p-sized
   B_PH, B_POUT, B_DIVP, B_PHI, B_RESP, B_PHI2, B_TP0, B_TP1,
+  B_CKP,               // checkpoint of the half pressure
   NBUF
 };
 constexpr int kFirstPBuf = B_PH;
@@ -449,10 +451,13 @@ __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) 
     u64* const d2[2] = {c.sl + SL_UMAX * NSTRIPE, c.sl + SL_VMAX * NSTRIPE};
     block_max_atomic<2>(v2, d2);
   } else {
-    double v2[2] = {mu, mv};
-    u64* const d2[2] = {run_u ? c.sl + (SL_HU + a.res_slot) * NSTRIPE : nullptr,
-                        run_v ? c.sl + (SL_HV + a.res_slot) * NSTRIPE : nullptr};
-    block_max_atomic<2>(v2, d2);
+    // mode 3 (lean solve, no refinement): the same residual is the first and the final one
+    double v4[4] = {mu, mv, mu, mv};
+    u64* const d4s[4] = {run_u ? c.sl + (SL_HU + a.res_slot) * NSTRIPE : nullptr,
+                         run_v ? c.sl + (SL_HV + a.res_slot) * NSTRIPE : nullptr,
+                         a.mode == 3 ? c.sl + (SL_HU + SV_FINAL) * NSTRIPE : nullptr,
+                         a.mode == 3 ? c.sl + (SL_HV + SV_FINAL) * NSTRIPE : nullptr};
+    block_max_atomic<4>(v4, d4s);
   }
 }
 
@@ -581,9 +586,12 @@ __global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) 
     u64* const d1[1] = {s + (b0 + a.res_slot) * NSTRIPE};
     block_max_atomic<1>(v1, d1);
   } else {
-    double v3[3] = {mres, mrnd, msc};
-    u64* const d3[3] = {s + (b0 + SV_FINAL) * NSTRIPE, s + (b0 + SV_ROUND) * NSTRIPE, s + (b0 + SV_ASCALE) * NSTRIPE};
-    block_max_atomic<3>(v3, d3);
+    // mode 1: final statistics; mode 3 (lean solve, no refinement): first residual and final
+    // statistics of the same x in one pass
+    double v4[4] = {mres, mrnd, msc, mres};
+    u64* const d4s[4] = {s + (b0 + SV_FINAL) * NSTRIPE, s + (b0 + SV_ROUND) * NSTRIPE, s + (b0 + SV_ASCALE) * NSTRIPE,
+                         a.mode == 3 ? s + (b0 + a.res_slot) * NSTRIPE : nullptr};
+    block_max_atomic<4>(v4, d4s);
     __syncthreads();
     block_sum_atomic(ssr, reinterpret_cast<double*>(s + (b0 + SV_SSQR) * NSTRIPE));
     __syncthreads();
@@ -992,7 +1000,7 @@ __global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
 }
 
 // Gather the step's scalars into the diagnostics ring and clear the slots (one wave per case)
-struct FinArgs { int nproj; };
+struct FinArgs { int nproj; int lean; int force_miss; };
 constexpr int FIN_N = 34;
 __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a) {
   const CaseDev& c = cases[blockIdx.y];
@@ -1080,6 +1088,9 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
     if (!(divg <= 1e-10)) status |= 4;
     if (!(d[MCF_D_MAXDIV] < 1e-10) || !(d[MCF_D_CFL] < 0.5)) status |= 8;
     if (!isfinite(d[MCF_D_CD]) || !isfinite(d[MCF_D_CL])) status |= 16;
+    // a lean step (refinement launches left out) whose first residual called for a refinement:
+    // the host replays the chunk with the refinements in place
+    if (a.lean && (refine > 0 || hrefine > 0 || a.force_miss)) status |= 32;
     d[MCF_D_STATUS] = status;
   }
   u64* wsl = c.sl;
@@ -1104,6 +1115,26 @@ __global__ void k_set_clock(const CaseDev* cases, double t0) {
 __global__ void k_clear_slots(const CaseDev* cases) {
   const CaseDev& c = cases[blockIdx.y];
   for (int k = threadIdx.x; k < NSLOTW; k += blockDim.x) c.sl[k] = 0ull;
+}
+
+// Checkpoint of what a steady step reads from earlier steps: the state S[parity], the advection
+// history H[1 - parity] and the half pressure (save = 1: copy out, 0: copy back)
+__global__ __launch_bounds__(256) void k_ckpt(const CaseDev* cases, int parity, int save) {
+  const CaseDev& c = cases[blockIdx.y];
+  double* S = c.buf[B_S0 + parity];
+  double* H = c.buf[B_H0 + 1 - parity];
+  double* P = c.buf[B_PH];
+  double* cs = c.buf[B_CKS];
+  double* ch = c.buf[B_CKH];
+  double* cp = c.buf[B_CKP];
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
+    if (save) { cs[idx] = S[idx]; ch[idx] = H[idx]; }
+    else { S[idx] = cs[idx]; H[idx] = ch[idx]; }
+    if (idx < c.psize) {
+      if (save) cp[idx] = P[idx];
+      else P[idx] = cp[idx];
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_copy(const CaseDev* cases, int src, int dst, int n) {
@@ -1207,6 +1238,12 @@ struct mcf_handle {
   StepGraphs* graphs = nullptr;
   Launch ln;                 // launch shapes of this handle's solves
   bool tuned = false;        // ln chosen (by the autotuner or the caller)
+  StepGraphs* lean_graphs = nullptr;
+  int spec = 1;              // speculative lean steps: 0 off, 1 on, 2 on with every lean step flagged
+  int replays = 0;           // chunks replayed with the refinements in place
+  int spec_chunk = 8;        // steps per speculative chunk: doubles after a clean chunk, back to 8 on a miss
+  int spec_full = 0;         // steps to run in full before speculating again (set after a miss)
+  int spec_backoff = 16;     // next full stretch: doubles while misses persist, back to 16 after a clean chunk
 };
 
 namespace {
@@ -1325,12 +1362,24 @@ int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int p
   return launch_check(x.err, "woodbury apply");
 }
 
-// Woodbury solve with <= 2 refinements (pressure_schur.py:116-124) and the final residual.
-int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve) {
+// Woodbury solve with <= 2 refinements (pressure_schur.py:116-124) and the final residual.  The
+// lean form leaves the refinement launches out and computes the first and the final residual in
+// one pass; a step that would have refined is flagged by k_finalize and replayed in full.
+int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bool lean = false) {
   const int shift = pres ? 8 * psolve : 0;
   const int slot0 = pres ? SL_P0 + 8 * psolve : SL_HU;
   int rc;
   if ((rc = enqueue_apply(x, pres, in_id, out_id, 1, -1, shift))) return rc;
+  if (lean) {
+    if (pres) {
+      PresArgs a{in_id, out_id, -1, 3, slot0, SV_RES1, -1};
+      hipLaunchKernelGGL(k_pres, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, a);
+    } else {
+      HelmArgs a{in_id, out_id, -1, -1, 3, SV_RES1, -1};
+      hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
+    }
+    return launch_check(x.err, "lean woodbury solve");
+  }
   for (int it = 0; it < 2; ++it) {
     const int res_slot = it == 0 ? SV_RES1 : SV_RES2;
     const int pred = it == 0 ? -1 : SV_RES1;
@@ -1355,11 +1404,12 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve) {
 }
 
 // project(x, dtmul * dt): div -> pressure solve -> update -> divergence gate (optimized_ghost_cell.py:218-229)
-int enqueue_project(const Ctx& x, int x_id, int out_id, int phi_id, double dtmul, int psolve, int defect_id) {
+int enqueue_project(const Ctx& x, int x_id, int out_id, int phi_id, double dtmul, int psolve, int defect_id,
+                    bool lean = false) {
   int rc;
   DivArgs d{x_id, B_DIVP, dtmul, SL_P0 + 8 * psolve + SV_VAL};
   hipLaunchKernelGGL(k_div, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, d);
-  if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve))) return rc;
+  if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve, lean))) return rc;
   UpdArgs u{x_id, phi_id, out_id, defect_id, dtmul};
   hipLaunchKernelGGL(k_update, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, u);
   DivArgs g{out_id, -1, 0.0, SL_DIV0 + psolve};
@@ -1384,7 +1434,8 @@ int check_ready(mcf_handle* h, double dt) {
 // One AB2/CN step of every case of the context (time_integration.py:7-145,
 // optimized_ghost_cell.py:236-286, main.py:383-415).  Launches only: dt, alpha and the pressure
 // factor are read from each case's parameters (k_set_prm); the caller advances the host state.
-int enqueue_step_kernels(const Ctx& x, int parity, bool first) {
+int enqueue_step_kernels(const Ctx& x, int parity, bool first, bool lean = false, bool force_miss = false) {
+  lean = lean && !first;
   int rc;
   const int S = B_S0 + parity, Snew = B_S0 + 1 - parity;
   const int Hprev = B_H0 + 1 - parity, Hnow = B_H0 + parity;
@@ -1404,11 +1455,11 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first) {
     AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0};
     hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, x.stream, x.dc, r);
   }
-  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0))) return rc;
-  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1))) return rc;
+  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, lean))) return rc;
+  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, lean))) return rc;
   HelmArgs dfa{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1};
   hipLaunchKernelGGL(k_helm, gv, dim3(256), 0, x.stream, x.dc, dfa);
-  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT))) return rc;
+  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, lean))) return rc;
   PressArgs pa{B_PH, B_PHI2, B_PH, B_POUT};
   hipLaunchKernelGGL(k_pressures, gp, dim3(256), 0, x.stream, x.dc, pa);
   FillArgs f{Snew, 1, 1, 1};
@@ -1416,7 +1467,7 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first) {
   ForceArgs fa{Snew, B_POUT, -1.0};
   hipLaunchKernelGGL(k_forces, dim3((x.maxsurf + 63) / 64, x.n), dim3(64), 0, x.stream, x.dc, fa);
   hipLaunchKernelGGL(k_force_total, dim3(1, x.n), dim3(64), 0, x.stream, x.dc);
-  FinArgs fin{nproj};
+  FinArgs fin{nproj, lean ? 1 : 0, force_miss ? 1 : 0};
   hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, fin);
   return launch_check(x.err, "step");
 }
@@ -1436,11 +1487,11 @@ struct StepGraphs {
     }
     valid = false;
   }
-  int build(const Ctx& x) {
+  int build(const Ctx& x, bool lean = false, bool force_miss = false) {
     drop();
     for (int p = 0; p < 2; ++p) {
       HIPCHK(x.err, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_step_kernels(x, p, false);
+      const int rc = enqueue_step_kernels(x, p, false, lean, force_miss);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(x.stream, &g);
       if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -1464,6 +1515,7 @@ void advance_host_state(mcf_handle* h, double dt) {
 
 void drop_graphs(mcf_handle* h) {
   if (h->graphs) h->graphs->drop();
+  if (h->lean_graphs) h->lean_graphs->drop();
   h->graph_dt = -1.0;
   h->prm_valid = false;
 }
@@ -1517,9 +1569,10 @@ int ensure_tuned(mcf_handle* h) {
   return 0;
 }
 
-int enqueue_steps(mcf_handle* h, int n, double dt) {
+int enqueue_steps(mcf_handle* h, int n, double dt, bool lean = false) {
   if (int rc = ensure_tuned(h)) return rc;
   const Ctx x = ctx_of(h);
+  StepGraphs* gs = lean ? h->lean_graphs : h->graphs;
   hipLaunchKernelGGL(k_set_clock, dim3(1, 1), dim3(64), 0, h->stream, h->dc, h->time);
   for (int s = 0; s < n; ++s) {
     const bool steady = h->has_history && h->prev_dt == dt;
@@ -1532,11 +1585,11 @@ int enqueue_steps(mcf_handle* h, int n, double dt) {
     if (!steady || !h->use_graphs) {
       if (int rc = enqueue_step_kernels(x, h->parity, !h->has_history)) return rc;
     } else {
-      if (h->graph_dt != dt || !h->graphs->valid) {
-        if (int rc = h->graphs->build(x)) return rc;
-        h->graph_dt = dt;
-      }
-      HIPCHK(h, hipGraphLaunch(h->graphs->exec[h->parity], h->stream));
+      if (h->graph_dt != dt) drop_graphs(h);
+      if (!gs->valid)
+        if (int rc = gs->build(x, lean, h->spec == 2)) return rc;
+      h->graph_dt = dt;
+      HIPCHK(h, hipGraphLaunch(gs->exec[h->parity], h->stream));
     }
     advance_host_state(h, dt);
   }
@@ -1645,6 +1698,7 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   HIPCHK(h, hipMalloc(&h->prm, 4 * sizeof(double)));
   HIPCHK(h, hipMemset(h->prm, 0, 4 * sizeof(double)));
   h->graphs = new StepGraphs();
+  h->lean_graphs = new StepGraphs();
   HIPCHK(h, hipMalloc(&h->dc, sizeof(CaseDev)));
   HIPCHK(h, hipMalloc(&h->ds, sizeof(h->hs)));
   CaseDev& c = h->hc;
@@ -1672,7 +1726,9 @@ void mcf_destroy(mcf_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graphs(h);
   delete h->graphs;
+  delete h->lean_graphs;
   h->graphs = nullptr;
+  h->lean_graphs = nullptr;
   (void)hipFree(h->clock_step); (void)hipFree(h->clock_time); (void)hipFree(h->prm);
   for (auto& kv : h->owned) (void)hipFree(kv.second);
   for (void* p : h->bufs) (void)hipFree(p);
@@ -1986,21 +2042,73 @@ int mcf_get_state(mcf_handle* h, double* vel, double* p, double* prev_adv, doubl
   return 0;
 }
 
+namespace {
+constexpr int SPEC_MIN = 8, SPEC_MAX = 512, BACKOFF_MIN = 16, BACKOFF_MAX = 1024;
+
+// Speculation policy after a chunk: a clean lean chunk doubles the next one; a miss returns to the
+// shortest chunk and runs a full stretch first, doubling while misses persist (start-up
+// transients refine for tens of steps, steady flow rarely).
+void spec_update(int& chunk, int& full, int& backoff, bool miss) {
+  if (miss) {
+    chunk = SPEC_MIN;
+    full = backoff;
+    backoff = std::min(2 * backoff, BACKOFF_MAX);
+  } else {
+    chunk = std::min(2 * chunk, SPEC_MAX);
+    backoff = BACKOFF_MIN;
+  }
+}
+
+// k steps on one handle into d (k rows).  Speculative: steady steps leave the refinement launches
+// out; if any step of the chunk would have refined, the chunk is replayed from the checkpoint with
+// the refinements in place.
+int run_chunk(mcf_handle* h, int k, double dt, bool lean, double* d) {
+  const int parity0 = h->parity;
+  const double prev_dt0 = h->prev_dt, time0 = h->time;
+  const bool hist0 = h->has_history;
+  const Ctx x = ctx_of(h);
+  const size_t bytes = (size_t)k * MCF_DIAG_LEN * sizeof(double);
+  if (lean) hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, 1), dim3(256), 0, h->stream, h->dc, parity0, 1);
+  if (int rc = enqueue_steps(h, k, dt, lean)) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(d, h->diag, bytes, hipMemcpyDeviceToHost));
+  bool miss = false;
+  for (int s = 0; s < k && lean; ++s) miss = miss || ((int)d[(size_t)s * MCF_DIAG_LEN + MCF_D_STATUS] & 32);
+  if (lean) spec_update(h->spec_chunk, h->spec_full, h->spec_backoff, miss);
+  if (!miss) return 0;
+  h->parity = parity0; h->prev_dt = prev_dt0; h->time = time0; h->has_history = hist0;
+  h->prm_valid = false;
+  ++h->replays;
+  hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, 1), dim3(256), 0, h->stream, h->dc, parity0, 0);
+  if (int rc = enqueue_steps(h, k, dt, false)) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(d, h->diag, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+}  // namespace
+
 int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   if (!h || n <= 0 || !(dt > 0) || !std::isfinite(dt)) return fail(h, 1, "n > 0 and a finite positive dt are required");
   if (int rc = check_ready(h, dt)) return rc;
   if (int rc = ensure_diag(h, n)) return rc;
   h->owned.erase("p_raw_valid");
-  if (int rc = enqueue_steps(h, n, dt)) return rc;
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const bool spec = h->spec > 0 && h->use_graphs;
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
-  HIPCHK(h, hipMemcpy(d.data(), h->diag, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int done = 0; done < n;) {
+    const bool lean = spec && h->spec_full == 0;
+    const int k = !spec ? n - done : std::min(lean ? h->spec_chunk : h->spec_full, n - done);
+    if (spec && !lean) h->spec_full -= k;
+    if (int rc = run_chunk(h, k, dt, lean, d.data() + (size_t)done * MCF_DIAG_LEN)) return rc;
+    for (int s = 0; s < k; ++s)
+      if (int rc = status_error(h, d.data() + (size_t)(done + s) * MCF_DIAG_LEN, done + s)) {
+        if (diag) std::memcpy(diag, d.data(), (size_t)(done + k) * MCF_DIAG_LEN * sizeof(double));
+        return rc;
+      }
+    done += k;
+  }
   if (diag) std::memcpy(diag, d.data(), d.size() * sizeof(double));
-  for (int s = 0; s < n; ++s)
-    if (int rc = status_error(h, d.data() + (size_t)s * MCF_DIAG_LEN, s)) return rc;
   return 0;
 }
-
 int mcf_step(mcf_handle* h, double dt, double* diag) { return mcf_run(h, 1, dt, diag); }
 
 int mcf_project(mcf_handle* h, const double* q, double dt, double* out, double* phi, double* info4) {
@@ -2214,10 +2322,13 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   hipEvent_t e0, e1;
   HIPCHK(h, hipEventCreate(&e0)); HIPCHK(h, hipEventCreate(&e1));
   h->owned.erase("p_raw_valid");
-  // one untimed step settles the history so that the timed steps replay the captured graph
-  if (int rc = enqueue_steps(h, 1, dt)) return rc;
+  // one untimed step settles the history so that the timed steps replay the captured graph; with
+  // speculation on, the timed steps are the lean ones (a step that would have refined is counted
+  // in out5[4], not replayed: the state is not used after profiling)
+  const bool lean = h->spec > 0 && h->use_graphs;
+  if (int rc = enqueue_steps(h, 1, dt, lean)) return rc;
   HIPCHK(h, hipEventRecord(e0, h->stream));
-  if (int rc = enqueue_steps(h, n, dt)) return rc;
+  if (int rc = enqueue_steps(h, n, dt, lean)) return rc;
   HIPCHK(h, hipEventRecord(e1, h->stream));
   HIPCHK(h, hipEventSynchronize(e1));
   float ms = 0;
@@ -2245,8 +2356,11 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
   HIPCHK(h, hipMemcpy(d.data(), h->diag, d.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int s = 0; s < n; ++s)
-    if (int rc = status_error(h, d.data() + (size_t)s * MCF_DIAG_LEN, s)) return rc;
+  for (int s = 0; s < n; ++s) {
+    const double* row = d.data() + (size_t)s * MCF_DIAG_LEN;
+    if ((int)row[MCF_D_STATUS] & 32) { out5[4] += 1; continue; }
+    if (int rc = status_error(h, row, s)) return rc;
+  }
   return launch_check(h, "profile");
 }
 
@@ -2272,6 +2386,20 @@ int mcf_get_launch(mcf_handle* h, int* out5) {
   return 0;
 }
 
+int mcf_set_speculation(mcf_handle* h, int mode) {
+  if (!h || mode < 0 || mode > 2) return fail(h, 1, "speculation mode must be 0, 1 or 2");
+  if (mode != h->spec) drop_graphs(h);
+  h->spec = mode;
+  return 0;
+}
+
+int mcf_get_speculation(mcf_handle* h, int* out2) {
+  if (!h || !out2) return 1;
+  out2[0] = h->spec;
+  out2[1] = h->replays;
+  return 0;
+}
+
 }  // extern "C"
 
 // ======================================================================================
@@ -2288,6 +2416,12 @@ struct mcf_batch {
   bool built = false;
   Launch ln;                    // launch shapes for the whole batch
   bool tuned = false;
+  StepGraphs lean_graphs;       // steady steps without the refinement launches
+  int spec = 1;                 // speculative lean steps (as mcf_set_speculation)
+  int replays = 0;
+  int spec_chunk = 8;           // steps per speculative chunk (as the handle's)
+  int spec_full = 0, spec_backoff = 16;
+  void drop_graphs() { graphs.drop(); lean_graphs.drop(); }
 };
 
 namespace {
@@ -2314,7 +2448,7 @@ int batch_build(mcf_batch* b) {
   x.dc = b->dc; x.dsp = b->ds; x.dsh = b->ds + n;
   x.ln = b->ln;
   b->ctx = x;
-  b->graphs.drop();
+  b->drop_graphs();
   b->built = true;
   return 0;
 }
@@ -2349,7 +2483,7 @@ int mcf_batch_create(mcf_handle* const* cases, const double* dts, int n, mcf_bat
 void mcf_batch_destroy(mcf_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  b->graphs.drop();
+  b->drop_graphs();
   (void)hipFree(b->dc); (void)hipFree(b->ds);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -2374,42 +2508,92 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
     if (int rc = autotune(b->ctx, b->ln)) return rc;
     b->ctx.ln = b->ln;
     b->tuned = true;
-    b->graphs.drop();
+    b->drop_graphs();
   }
   const Ctx& x = b->ctx;
-  for (int c = 0; c < n; ++c) {
-    mcf_handle* h = b->cases[c];
-    const int zero = 0;
-    HIPCHK(e, hipMemcpy(h->clock_step, &zero, sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(e, hipMemcpy(h->clock_time, &h->time, sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(e, hipMemcpy(h->prm, &b->dt[c], sizeof(double), hipMemcpyHostToDevice));
-    h->prm_valid = false;
-  }
-  for (int s = 0; s < nsteps; ++s) {
-    mcf_handle* h0 = b->cases[0];
-    const bool first = !h0->has_history;
-    hipLaunchKernelGGL(k_set_prm, dim3(1, n), dim3(64), 0, b->stream, b->dc, -1.0, first ? 0.0 : 0.5, first ? 1.0 : 0.5);
-    if (first || !e->use_graphs) {
-      if (int rc = enqueue_step_kernels(x, h0->parity, first)) return rc;
-    } else {
-      if (!b->graphs.valid) if (int rc = b->graphs.build(x)) return rc;
-      HIPCHK(e, hipGraphLaunch(b->graphs.exec[h0->parity], b->stream));
+  // speculative chunks over the batch, as mcf_run: lean steady steps in chunks of growing size, a
+  // chunk replayed with the refinements from the checkpoint if any step of any case would refine
+  const bool spec = b->spec > 0 && e->use_graphs;
+  struct Host { int parity; double prev_dt, time; bool hist; };
+  std::vector<Host> saved(n);
+  auto enqueue = [&](int k, bool use_lean) -> int {
+    for (int c = 0; c < n; ++c) {
+      mcf_handle* h = b->cases[c];
+      const int zero = 0;
+      HIPCHK(e, hipMemcpy(h->clock_step, &zero, sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(e, hipMemcpy(h->clock_time, &h->time, sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(e, hipMemcpy(h->prm, &b->dt[c], sizeof(double), hipMemcpyHostToDevice));
+      h->prm_valid = false;
     }
-    for (int c = 0; c < n; ++c) advance_host_state(b->cases[c], b->dt[c]);
-  }
-  HIPCHK(e, hipStreamSynchronize(b->stream));
-  std::vector<double> d((size_t)nsteps * MCF_DIAG_LEN);
-  int first_error = 0;
-  for (int c = 0; c < n; ++c) {
-    mcf_handle* h = b->cases[c];
-    HIPCHK(e, hipMemcpy(d.data(), h->diag, d.size() * sizeof(double), hipMemcpyDeviceToHost));
-    if (diag) std::memcpy(diag + (size_t)c * d.size(), d.data(), d.size() * sizeof(double));
-    for (int s = 0; s < nsteps && !first_error; ++s)
-      if (int rc = status_error(h, d.data() + (size_t)s * MCF_DIAG_LEN, s)) {
-        first_error = rc;
-        e->err = "case " + std::to_string(c) + ": " + h->err;
+    StepGraphs& gs = use_lean ? b->lean_graphs : b->graphs;
+    for (int s = 0; s < k; ++s) {
+      mcf_handle* h0 = b->cases[0];
+      const bool first = !h0->has_history;
+      hipLaunchKernelGGL(k_set_prm, dim3(1, n), dim3(64), 0, b->stream, b->dc, -1.0, first ? 0.0 : 0.5, first ? 1.0 : 0.5);
+      if (first || !e->use_graphs) {
+        if (int rc = enqueue_step_kernels(x, h0->parity, first)) return rc;
+      } else {
+        if (!gs.valid) if (int rc = gs.build(x, use_lean, b->spec == 2)) return rc;
+        HIPCHK(e, hipGraphLaunch(gs.exec[h0->parity], b->stream));
       }
+      for (int c = 0; c < n; ++c) advance_host_state(b->cases[c], b->dt[c]);
+    }
+    HIPCHK(e, hipStreamSynchronize(b->stream));
+    return 0;
+  };
+  const size_t per = (size_t)nsteps * MCF_DIAG_LEN;
+  std::vector<double> d(per * n);
+  // every chunk runs for every case: a case that fails a gate keeps stepping (the caller drops it)
+  // and no longer counts for the speculation
+  int first_error = 0;
+  std::vector<char> dead(n, 0);
+  for (int done = 0; done < nsteps;) {
+    const bool lean = spec && b->spec_full == 0;
+    const int k = !spec ? nsteps - done : std::min(lean ? b->spec_chunk : b->spec_full, nsteps - done);
+    if (spec && !lean) b->spec_full -= k;
+    for (int c = 0; c < n; ++c) {
+      mcf_handle* h = b->cases[c];
+      saved[c] = Host{h->parity, h->prev_dt, h->time, h->has_history};
+    }
+    if (lean) hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, n), dim3(256), 0, b->stream, b->dc, saved[0].parity, 1);
+    if (int rc = enqueue(k, lean)) return rc;
+    const size_t rows = (size_t)k * MCF_DIAG_LEN;
+    auto copy_rows = [&]() -> int {
+      for (int c = 0; c < n; ++c)
+        HIPCHK(e, hipMemcpy(d.data() + per * c + (size_t)done * MCF_DIAG_LEN, b->cases[c]->diag, rows * sizeof(double),
+                            hipMemcpyDeviceToHost));
+      return 0;
+    };
+    if (int rc = copy_rows()) return rc;
+    bool miss = false;
+    for (int c = 0; c < n && lean; ++c)
+      for (int s = 0; s < k && !dead[c]; ++s) miss = miss || ((int)d[per * c + (size_t)(done + s) * MCF_DIAG_LEN + MCF_D_STATUS] & 32);
+    if (lean) spec_update(b->spec_chunk, b->spec_full, b->spec_backoff, miss);
+    if (miss) {
+      for (int c = 0; c < n; ++c) {
+        mcf_handle* h = b->cases[c];
+        h->parity = saved[c].parity; h->prev_dt = saved[c].prev_dt; h->time = saved[c].time; h->has_history = saved[c].hist;
+      }
+      ++b->replays;
+      hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, n), dim3(256), 0, b->stream, b->dc, saved[0].parity, 0);
+      if (int rc = enqueue(k, false)) return rc;
+      if (int rc = copy_rows()) return rc;
+    }
+    for (int c = 0; c < n; ++c) {
+      mcf_handle* h = b->cases[c];
+      for (int s = done; s < done + k && !dead[c]; ++s)
+        if (int rc = status_error(h, d.data() + per * c + (size_t)s * MCF_DIAG_LEN, s)) {
+          dead[c] = 1;
+          if (!first_error) {
+            first_error = rc;
+            e->err = "case " + std::to_string(c) + ": " + h->err;
+          }
+        }
+    }
+    done += k;
   }
+  if (diag)
+    for (int c = 0; c < n; ++c) std::memcpy(diag + per * c, d.data() + per * c, per * sizeof(double));
   return first_error;
 }
 
@@ -2427,7 +2611,7 @@ int mcf_batch_set_launch(mcf_batch* b, int gemm, int tri) {
     b->tuned = true;
   }
   b->ctx.ln = b->ln;
-  b->graphs.drop();
+  b->drop_graphs();
   return 0;
 }
 
@@ -2435,6 +2619,21 @@ int mcf_batch_get_launch(mcf_batch* b, int* out5) {
   if (!b || !out5) return 1;
   out5[0] = b->ln.gemm_p; out5[1] = b->ln.gemm_h; out5[2] = b->ln.tri_p; out5[3] = b->ln.tri_h;
   out5[4] = b->tuned ? 1 : 0;
+  return 0;
+}
+
+int mcf_batch_set_speculation(mcf_batch* b, int mode) {
+  if (!b) return 1;
+  if (mode < 0 || mode > 2) return fail(b->cases[0], 1, "speculation mode must be 0, 1 or 2");
+  if (mode != b->spec) b->drop_graphs();
+  b->spec = mode;
+  return 0;
+}
+
+int mcf_batch_get_speculation(mcf_batch* b, int* out2) {
+  if (!b || !out2) return 1;
+  out2[0] = b->spec;
+  out2[1] = b->replays;
   return 0;
 }
 

@@ -379,6 +379,16 @@ class GhostDevice:
         return dict(gemm_pressure=int(out[0]), gemm_velocity=int(out[1]), tri_pressure=int(out[2]),
                     tri_velocity=int(out[3]), chosen=bool(out[4]))
 
+    def set_speculation(self, mode=1):
+        """Steady steps without the refinement launches, replayed in full when one would refine
+        (1, default); always launch them (0); flag every lean step to exercise the replay (2)."""
+        self.h.call('mcf_set_speculation', int(mode))
+
+    def get_speculation(self):
+        out = np.zeros(2, dtype=np.int32)
+        self.h.call('mcf_get_speculation', out)
+        return dict(mode=int(out[0]), replays=int(out[1]))
+
     # ---------------------------------------------------------------- Schur system
     def set_dump(self, capacity):
         """Record the (c, y) pair of each projection's first pressure solve (angle order)."""

@@ -122,6 +122,7 @@ class Ensemble:
         self._lib = nat.library()
         self.b = None
         self._launch = None
+        self._spec = None
         self._rebuild()
 
     @property
@@ -145,6 +146,8 @@ class Ensemble:
         self.b, self._idx = b, idx
         if self._launch is not None:
             self._call('mcf_batch_set_launch', *self._launch)
+        if self._spec is not None:
+            self._call('mcf_batch_set_speculation', self._spec)
 
     def _call(self, name, *args):
         rc = getattr(self._lib, name)(self.b, *[nat.as_arg(a) for a in args])
@@ -155,6 +158,16 @@ class Ensemble:
         """Pin the batch's transform tile / sweep width variants (-1, -1: autotune at the next run)."""
         self._launch = None if gemm < 0 else (int(gemm), int(tri))
         self._call('mcf_batch_set_launch', int(gemm), int(tri))
+
+    def set_speculation(self, mode=1):
+        """As GhostDevice.set_speculation, for the whole batch."""
+        self._spec = int(mode)
+        self._call('mcf_batch_set_speculation', self._spec)
+
+    def get_speculation(self):
+        out = np.zeros(2, dtype=np.int32)
+        self._call('mcf_batch_get_speculation', out)
+        return dict(mode=int(out[0]), replays=int(out[1]))
 
     def get_launch(self):
         out = np.zeros(5, dtype=np.int32)

@@ -184,6 +184,16 @@ int mcf_net_profile(mcf_net* n, int nsamp, int L, int reps, float* ms_out);
 int mcf_set_launch(mcf_handle* h, int gemm, int tri);
 int mcf_get_launch(mcf_handle* h, int* out5);
 
+/* Speculative steps (no reference counterpart).  mode 1 (default): steady steps of mcf_run leave
+   the Woodbury refinement launches out (pressure_schur.py:116-124 refines at most twice, and only
+   when the first residual misses max(1e-10, 32 eps max|b|)); the device flags any step whose first
+   residual would have refined, and the whole chunk is then replayed from a checkpoint with the
+   refinements in place, so results equal mode 0 bit for bit.  mode 0: always launch them
+   (device-predicated).  mode 2: as 1 with every lean step flagged (tests the replay).
+   mcf_get_speculation: [mode, chunks replayed]. */
+int mcf_set_speculation(mcf_handle* h, int mode);
+int mcf_get_speculation(mcf_handle* h, int* out2);
+
 /* Batched ensemble (no reference counterpart; the reference runs one case per process):
    cases are fully configured handles (tables, state, Helmholtz factors for dts[c]); all share the
    step parity and history state.  Every stage of the step is one launch over all cases and
@@ -197,6 +207,9 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag);
 /* Launch shapes of the batch (as mcf_set_launch / mcf_get_launch; autotuned at the first run). */
 int mcf_batch_set_launch(mcf_batch* b, int gemm, int tri);
 int mcf_batch_get_launch(mcf_batch* b, int* out5);
+/* Speculative steps of the batch (as mcf_set_speculation; a replay re-runs every case). */
+int mcf_batch_set_speculation(mcf_batch* b, int mode);
+int mcf_batch_get_speculation(mcf_batch* b, int* out2);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,8 @@ def test_failed_case_is_dropped(ens):
     batch.set_launch(0, 1)
     d = batch.run(3)
     assert list(batch.failures) == [1] and batch.failures[1]['step'] == 0
+    # the other members ran every step of the call
+    assert np.allclose(d[[0, 2], -1, 0], [3 * cases[0]['dt'], 3 * cases[2]['dt']], rtol=1e-12)
     assert 'CFL' in batch.failures[1]['message'] and d[1, 0, 11] >= 0.5
     assert np.all(np.isnan(d[1, 1:, 0]))
     d2 = batch.run(3)
@@ -128,4 +130,39 @@ def test_autotuned_batch_matches_single_to_roundoff(ens):
         for key in ('u', 'v', 'p'):
             a, b = getattr(batch.flows[c], key), getattr(f, key)
             assert np.max(abs(a - b)) <= 1e-12 * np.max(abs(b)), (c, key)
+    batch.close()
+
+
+def test_batch_speculation_matches_full(ens):
+    cases = small_cases(ens)
+    out = {}
+    for mode in (0, 1, 2):
+        batch = ens.Ensemble(cases, workers=1)
+        batch.set_launch(0, 1)
+        batch.set_speculation(mode)
+        batch.run(4)
+        batch.run(4)
+        batch.pull()
+        out[mode] = ([(f.u.copy(), f.v.copy(), f.p.copy()) for f in batch.flows], batch.get_speculation())
+        batch.close()
+    for mode in (1, 2):
+        for fa, fb in zip(out[mode][0], out[0][0]):
+            for a, b in zip(fa, fb):
+                assert np.array_equal(a, b), mode
+    assert out[2][1]['replays'] >= 1
+
+
+def test_member_failing_mid_run_does_not_stop_the_others(ens):
+    """A member that fails after several speculative chunks (dt too large for its grid) is dropped
+    at its step; the others complete every step of the same call."""
+    good = small_cases(ens)
+    bad = ens.channel_case(5, 0.2, 0.06, 60., nx_range=(40, 400), ny_range=(12, 80), courant=0.12)
+    batch = ens.Ensemble([good[0], bad], workers=1)
+    d = batch.run(60)
+    print('failures', batch.failures)
+    assert np.allclose(d[0, -1, 0], 60 * good[0]['dt'], rtol=1e-12)
+    assert np.all(d[0, :, 26] == 0)
+    if 1 in batch.failures:
+        s = batch.failures[1]['step']
+        assert np.all(np.isnan(d[1, s + 1:, 0]))
     batch.close()

@@ -166,3 +166,27 @@ def test_launch_shapes_agree(pkg):
     ln = f.gg.device.get_launch()
     print('autotuned launch shapes (440x82):', ln)
     assert ln['chosen'] and all(0 <= ln[k] < 3 for k in ('gemm_pressure', 'gemm_velocity', 'tri_pressure', 'tri_velocity'))
+
+
+def test_speculative_steps_match_full(pkg):
+    """Lean steady steps (refinement launches left out, replayed in full on a miss) give the same
+    fields and diagnostics as always launching the refinements; forcing a miss at every lean step
+    replays every chunk and still gives the same result."""
+    runs = {}
+    for mode in (0, 1, 2):
+        f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
+        dev = f.gg.device
+        dev.set_launch(0, 1)          # same sweep width in every run (widths differ in round-off)
+        dev.set_speculation(mode)
+        f._push()
+        d = np.concatenate([dev.run(7, 2e-3), dev.run(5, 2e-3)])
+        f._pull()
+        runs[mode] = (f.u.copy(), f.v.copy(), f.p.copy(), d, dev.get_speculation())
+    for mode in (1, 2):
+        for a, b in zip(runs[mode][:3], runs[0][:3]):
+            assert np.array_equal(a, b), mode
+        for col in (0, 2, 3, 10, 11, 12, 14, 18, 25, 26):
+            assert np.array_equal(runs[mode][3][:, col], runs[0][3][:, col]), (mode, col)
+    print('replays with speculation on:', runs[1][4]['replays'], '(a chunk replays when one of its steps refines)')
+    # after a miss the next stretch runs in full (backoff), so mode 2 replays the first chunk only
+    assert runs[1][4]['replays'] <= 2 and runs[2][4]['replays'] >= 1
