@@ -196,12 +196,11 @@ class MAC_CylinderFlow:
         self.ghost_pressure_extension = ghost_pressure_extension
         self.ghost_velocity_interpolation = ghost_velocity_interpolation
         self.marker_spacing_factor = marker_spacing_factor
-        self.configuration = dict(nx=nx, ny=ny, Lx=Lx, Ly=Ly, D=D, xc=xc, yc=yc, nu=nu, rho=rho, U_inf=U_inf,
-                                  inflow=inflow, U_m=U_m, slip_walls=slip_walls, boundary=boundary,
-                                  transport=transport, pressure_solver=pressure_solver,
-                                  marker_spacing_factor=marker_spacing_factor, time_scheme=time_scheme,
-                                  verbose=False, ghost_pressure_extension=ghost_pressure_extension,
-                                  ghost_velocity_interpolation=ghost_velocity_interpolation)
+        self.configuration = self.configuration_of(
+            nx=nx, ny=ny, Lx=Lx, Ly=Ly, D=D, xc=xc, yc=yc, nu=nu, rho=rho, U_inf=U_inf, inflow=inflow, U_m=U_m,
+            slip_walls=slip_walls, boundary=boundary, transport=transport, pressure_solver=pressure_solver,
+            marker_spacing_factor=marker_spacing_factor, time_scheme=time_scheme,
+            ghost_pressure_extension=ghost_pressure_extension, ghost_velocity_interpolation=ghost_velocity_interpolation)
         self.run_options = {}
         self.last_step_diagnostics = {}
         self.time = 0.0
@@ -254,6 +253,21 @@ class MAC_CylinderFlow:
         self.setup_seconds = time.time() - t0
         if verbose:
             print(f"Ghost AB2/CN device ready in {self.setup_seconds:.3f}s (Re={self.Re:.1f}).")
+
+    CONFIGURATION_KEYS = ('nx', 'ny', 'Lx', 'Ly', 'D', 'xc', 'yc', 'nu', 'rho', 'U_inf', 'inflow', 'U_m', 'slip_walls',
+                          'boundary', 'transport', 'pressure_solver', 'marker_spacing_factor', 'time_scheme', 'verbose',
+                          'ghost_pressure_extension', 'ghost_velocity_interpolation')
+
+    @classmethod
+    def configuration_of(cls, **options):
+        """The ``configuration`` a solver built from these keyword options records (defaults filled,
+        verbose False), as the stage-JSON runner compares it (validation/run_ghost_validation.py:106-111)."""
+        import inspect
+        bound = inspect.signature(cls.__init__).bind(None, **options)
+        bound.apply_defaults()
+        args = dict(bound.arguments)
+        args['verbose'] = False
+        return {k: args[k] for k in cls.CONFIGURATION_KEYS}
 
     # ------------------------------------------------------------------ stepping
     def _push(self):

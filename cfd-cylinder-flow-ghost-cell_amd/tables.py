@@ -155,12 +155,10 @@ def nearest_with_ties(centers, points, k):
     return out
 
 
-def surface_pressure_quadrature(geom):
-    """Points, pressure stencils, weighted normals (cut_cell_solver.py:25-77)."""
-    count = max(360, 4 * int(np.ceil(2 * np.pi * geom.R / min(geom.dx, geom.dy))))
-    theta = np.arange(count) * (2 * np.pi / count)
-    normals = np.column_stack((np.cos(theta), np.sin(theta)))
-    points = np.column_stack((geom.yc + geom.R * normals[:, 1], geom.xc + geom.R * normals[:, 0]))
+def pressure_reconstruction(geom, points):
+    """Pressure values at points (y, x) from the 12 nearest exterior cell centres (ties kept) by a
+    least-squares quadratic (cut_cell_solver.py:25-67): CSR (len(points) x nx*ny)."""
+    points = np.asarray(points, dtype=float)
     padding = 4 * max(geom.dx, geom.dy)
     lo = np.maximum(0, np.floor((points.min(axis=0) - padding) / [geom.dy, geom.dx]).astype(int))
     hi = np.minimum([geom.ny, geom.nx], np.ceil((points.max(axis=0) + padding) / [geom.dy, geom.dx]).astype(int) + 1)
@@ -179,8 +177,16 @@ def surface_pressure_quadrature(geom):
         if rank != 6:
             raise ValueError('Pressure stencil cannot resolve a quadratic polynomial')
         rows.extend([k] * len(near)); cols.extend(indices[near]); vals.extend(w)
-    matrix = sp.csr_matrix((vals, (rows, cols)), shape=(len(points), geom.nx * geom.ny))
-    return points, matrix, normals * (2 * np.pi * geom.R / count)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(len(points), geom.nx * geom.ny))
+
+
+def surface_pressure_quadrature(geom):
+    """Points, pressure stencils, weighted normals (cut_cell_solver.py:70-77)."""
+    count = max(360, 4 * int(np.ceil(2 * np.pi * geom.R / min(geom.dx, geom.dy))))
+    theta = np.arange(count) * (2 * np.pi / count)
+    normals = np.column_stack((np.cos(theta), np.sin(theta)))
+    points = np.column_stack((geom.yc + geom.R * normals[:, 1], geom.xc + geom.R * normals[:, 0]))
+    return points, pressure_reconstruction(geom, points), normals * (2 * np.pi * geom.R / count)
 
 
 def wall_gradient(x, y, active, points, geom, h):

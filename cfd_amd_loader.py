@@ -15,3 +15,11 @@ def load():
     sys.modules['cfd_amd'] = module
     spec.loader.exec_module(module)
     return module
+
+
+if __name__ == '__main__':
+    # python cfd_amd_loader.py run-case --config-path case.json --output out/   (stage-JSON runner)
+    if len(sys.argv) < 2 or sys.argv[1] != 'run-case':
+        sys.exit('usage: python cfd_amd_loader.py run-case --config-path CASE.json --output DIR [--restart CKPT]')
+    import importlib
+    importlib.import_module(load().__name__ + '.runner').main(sys.argv[2:])

@@ -190,3 +190,22 @@ def test_speculative_steps_match_full(pkg):
     print('replays with speculation on:', runs[1][4]['replays'], '(a chunk replays when one of its steps refines)')
     # after a miss the next stretch runs in full (backoff), so mode 2 replays the first chunk only
     assert runs[1][4]['replays'] <= 2 and runs[2][4]['replays'] >= 1
+
+
+@pytest.mark.parametrize('kw', [CH, dict(CFG1, initial='zero')])
+def test_biquadratic_velocity_extension_steps(pkg, kw):
+    """ghost_velocity_interpolation='biquadratic' (9-point probes, optimized_ghost_cell.py:34-44):
+    the wider ghost rows and their override rows through the same kernels match the oracle."""
+    n, dt = 10, 1e-3
+    gpu = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False,
+                               ghost_velocity_interpolation='biquadratic', **kw)
+    t, cd, cl = gpu.solve(n_steps=n, dt=dt, perturb_transient=False, save_artifacts=False, verbose=False)
+    ref = O.CylinderFlow(ghost_velocity_interpolation='biquadratic', **kw)
+    for _ in range(n):
+        ref.step(dt)
+    for key in ('u', 'v', 'p'):
+        assert rel(getattr(gpu, key), getattr(ref, key)) < 1e-11, key
+    d = ref.last_step_diagnostics
+    scale = max(abs(d['cd']), abs(d['cl']))
+    assert abs(cd[-1] - d['cd']) <= 1e-10 * scale and abs(cl[-1] - d['cl']) <= 1e-10 * scale
+    assert gpu.last_step_diagnostics['max_divergence'] < 1e-12

@@ -1,0 +1,65 @@
+"""GPU: wake samples of a live solver and the stage-JSON runner end to end (with a restart)."""
+import importlib
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def mods(pkg):
+    return (importlib.import_module(pkg.__name__ + '.observables'), importlib.import_module(pkg.__name__ + '.runner'))
+
+
+def test_cylinder_samples_of_linear_fields(pkg, mods, tmp_path):
+    """Linear-in-x fields (tests/test_gemini_followup.py:88-98): the quadratic reconstruction makes
+    p_front - p_back = -D exactly and the centre-line velocity crosses zero 0.1 D behind the rear."""
+    ob, _ = mods
+    s = pkg.MAC_CylinderFlow(nx=40, ny=30, Lx=4., Ly=3., xc=1.5, yc=1.5, boundary='ghost_cell',
+                             time_scheme='ab2_cn', verbose=False)
+    path = tmp_path / 'samples.bin'
+    samples = ob.CylinderSamples(s, path, stride=1)
+    for t in (0., .2, .7, 1.1, 1.5):
+        s.time = t
+        s.u[:] = np.arange(s.nx + 1) * s.dx - (s.xc + s.D / 2) - .4 + t
+        s.p[:] = np.broadcast_to(s.gc.xp, s.p.shape) + 2 * t
+        samples.record()
+    samples.close()
+    r = ob.mean_observables(path, .1, .5)
+    assert abs(r['delta_p_mean'] + s.D) < 1e-10, r
+    assert abs(r['recirculation']['length_D'] - .1) < 1e-10, r
+
+
+def test_runner_end_to_end_with_restart(pkg, mods, tmp_path):
+    _, rn = mods
+    solver = dict(nx=88, ny=16, Lx=22., Ly=4.1, xc=2., yc=2., boundary='ghost_cell', time_scheme='ab2_cn')
+    case = dict(dt=1e-3, end_time=0.03, sample_start=0.01, kind='steady', profile_stride=5, solver=solver)
+    cfg = tmp_path / 'case.json'
+    cfg.write_text(json.dumps(case))
+    rn.run(cfg, tmp_path / 'a')
+    rows = [json.loads(x) for x in (tmp_path / 'a' / 'history.jsonl').read_text().splitlines()]
+    assert len(rows) == 30 and rows[-1]['step'] == 30 and abs(rows[-1]['time'] - 0.03) < 1e-12
+    assert all(r['max_divergence'] < 1e-12 for r in rows) and 'delta_p' in rows[0]
+    summary = json.loads((tmp_path / 'a' / 'summary.json').read_text())
+    assert summary['completed'] and summary['steps'] == 30 and summary['sample_window'][0] == 0.01
+    st = summary['statistics']
+    cd = np.array([r['cd'] for r in rows])
+    assert min(cd) - 1e-12 <= st['mean'][0] <= max(cd) + 1e-12
+    assert json.loads((tmp_path / 'a' / 'mean_observables.json').read_text())['window'][1] == pytest.approx(0.03)
+    manifest = json.loads((tmp_path / 'a' / 'manifest.json').read_text())
+    assert manifest['case'] == case and manifest['metadata']['n_steps'] == 30
+    # restart to 0.05 equals a direct run of 50 steps
+    case2 = dict(case, end_time=0.05, sample_start=0.035)   # after the restart time (30 accumulated steps)
+    cfg2 = tmp_path / 'case2.json'
+    cfg2.write_text(json.dumps(case2))
+    rn.run(cfg2, tmp_path / 'b', restart=tmp_path / 'a' / 'checkpoint.npz')
+    rows2 = [json.loads(x) for x in (tmp_path / 'b' / 'history.jsonl').read_text().splitlines()]
+    assert len(rows2) == 20 and abs(rows2[-1]['time'] - 0.05) < 1e-12
+    direct = pkg.MAC_CylinderFlow(verbose=False, **solver)
+    t, cd, cl = direct.solve(n_steps=50, dt=1e-3, perturb_transient=False, save_artifacts=False, verbose=False,
+                             step_callback=lambda state, diag: None)
+    assert abs(cd[-1] - rows2[-1]['cd']) <= 1e-12 * abs(cd[-1])
+    with pytest.raises(FileExistsError):
+        rn.run(cfg2, tmp_path / 'b', restart=tmp_path / 'a' / 'checkpoint.npz')
