@@ -2092,7 +2092,8 @@ int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   if (int rc = check_ready(h, dt)) return rc;
   if (int rc = ensure_diag(h, n)) return rc;
   h->owned.erase("p_raw_valid");
-  const bool spec = h->spec > 0 && h->use_graphs;
+  // no speculation while pairs are dumped: a replayed chunk would record its pairs twice
+  const bool spec = h->spec > 0 && h->use_graphs && h->hs[0].dump_cap == 0;
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
   for (int done = 0; done < n;) {
     const bool lean = spec && h->spec_full == 0;

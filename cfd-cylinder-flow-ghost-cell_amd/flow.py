@@ -54,7 +54,8 @@ class GhostCellIntegrator:
                 ops, pressure = prebuilt
             else:
                 ops = GhostOperators(s.nx, s.ny, s.Lx, s.Ly, s.D, s.xc, s.yc, s.nu, s.rho, s.u_inlet,
-                                     interpolation=s.ghost_velocity_interpolation)
+                                     interpolation=s.ghost_velocity_interpolation,
+                                     pressure_extension=s.ghost_pressure_extension)
                 pressure = None
             self.device = GhostDevice(ops, s.u_inlet, pressure=pressure)
             self.device.set_reference_velocity(s.U_ref)
@@ -69,6 +70,8 @@ class GhostCellIntegrator:
         self.previous_advection = self.previous_dt = self.half_pressure = None
         self.last_diagnostics = {}
         self.reconstruction_info = dict(velocity=s.ghost_velocity_interpolation, pressure=s.ghost_pressure_extension)
+        if s.ghost_pressure_extension == 'quadratic':   # optimized_ghost_cell.py:76-77
+            self.reconstruction_info.update({k: v for k, v in ops.ghost_p.items() if k != 'ghost_nodes'})
 
     # --- packing (optimized_ghost_cell.py:174-184)
     def pack(self, u, v, out=None):
@@ -180,8 +183,6 @@ class MAC_CylinderFlow:
             raise NotImplementedError('This GPU build provides the ghost_cell / ab2_cn path only')
         if pressure_solver != 'direct' or slip_walls:
             raise ValueError('Ghost AB2/CN requires direct/iterative pressure and no-slip channel walls')
-        if ghost_pressure_extension != 'neumann':
-            raise NotImplementedError('Only the Neumann ghost pressure extension is built for the GPU path')
         if any(isinstance(n, bool) or not isinstance(n, (int, np.integer)) or n < 4 for n in (nx, ny)):
             raise ValueError('nx and ny must be integers >= 4')
         if not all(np.isfinite(x) and x > 0 for x in (Lx, Ly, D, nu, rho, U_inf)):

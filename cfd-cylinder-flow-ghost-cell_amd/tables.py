@@ -155,6 +155,51 @@ def nearest_with_ties(centers, points, k):
     return out
 
 
+def exterior_pressure_extension(x, y, sdf, active):
+    """Quadratic continuation of the pressure into the ghost cells (ghost_reconstruction.py:7-45).
+
+    Each ghost cell (-2h <= d <= 1e-13) takes the least-squares quadratic through its 12 nearest
+    active centres within 6h of the surface (ties kept), else 20, else 32, the first fit of rank
+    6 with condition <= 1e8 that reproduces the quadratic basis to 1e-10 after the largest weight
+    absorbs the partition-of-unity defect.  Returns (P, ids, info) like ``extension`` (identity
+    rows on the active cells), with the fit's worst condition, reproduction error and stencil size.
+    """
+    xx, yy = np.meshgrid(x, y)
+    dist = sdf.distance(xx, yy)
+    dx, dy = x[1] - x[0], y[1] - y[0]
+    h = max(dx, dy)
+    ids = np.flatnonzero(active)
+    label = np.full(active.size, -1, dtype=np.int64)
+    label[ids] = np.arange(len(ids))
+    near_set = np.flatnonzero(active.ravel() & (dist.ravel() < 6 * h))
+    if len(near_set) < 12:
+        raise ValueError('Pressure continuation needs twelve exterior centers')
+    ghosts = np.flatnonzero((dist.ravel() <= 1e-13) & (dist.ravel() >= -2 * h))
+    centers = np.column_stack((yy.ravel()[near_set], xx.ravel()[near_set]))
+    target = np.array([1., 0., 0., 0., 0., 0.])
+    rows, cols, vals = [ids], [np.arange(len(ids))], [np.ones(len(ids))]
+    worst_cond, worst_err, widest = 0.0, 0.0, 0
+    for g in ghosts:
+        point = np.array([yy.ravel()[g], xx.ravel()[g]])
+        for count in (12, 20, 32):
+            near = nearest_with_ties(centers, point[None], min(count, len(centers)))[0]
+            sy, sx = ((centers[near] - point) / [dy, dx]).T
+            basis = np.array([np.ones(len(near)), sx, sy, sx * sx, sx * sy, sy * sy])
+            w, _, rank, _ = np.linalg.lstsq(basis, target, rcond=1e-12)
+            cond = float(np.linalg.cond(basis))
+            w[np.argmax(abs(w))] += 1 - w.sum()
+            err = float(np.max(abs(basis @ w - target)))
+            if rank == 6 and np.isfinite(cond) and cond <= 1e8 and err <= 1e-10:
+                break
+        else:
+            raise ValueError('Unresolved exterior pressure polynomial')
+        rows.append(np.full(len(near), g)); cols.append(label[near_set[near]]); vals.append(w)
+        worst_cond, worst_err, widest = max(worst_cond, cond), max(worst_err, err), max(widest, len(near))
+    P = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(active.size, len(ids)))
+    return P, ids, dict(ghost_nodes=ghosts, stencil_condition_max=worst_cond, polynomial_reproduction_linf=worst_err,
+                        neighbor_count_max=widest)
+
+
 def pressure_reconstruction(geom, points):
     """Pressure values at points (y, x) from the 12 nearest exterior cell centres (ties kept) by a
     least-squares quadratic (cut_cell_solver.py:25-67): CSR (len(points) x nx*ny)."""
@@ -215,7 +260,8 @@ def wall_gradient(x, y, active, points, geom, h):
 class GhostOperators:
     """E, P, D, G, A, L and offsets exactly as assembled by the reference integrator."""
 
-    def __init__(self, nx, ny, Lx, Ly, D, xc, yc, nu, rho, u_inlet, interpolation='bilinear'):
+    def __init__(self, nx, ny, Lx, Ly, D, xc, yc, nu, rho, u_inlet, interpolation='bilinear',
+                 pressure_extension='neumann'):
         dx, dy = Lx / nx, Ly / ny
         self.nx, self.ny, self.dx, self.dy, self.nu, self.rho = nx, ny, dx, dy, nu, rho
         self.Lx, self.Ly, self.D, self.xc, self.yc, self.R = Lx, Ly, D, xc, yc, D / 2
@@ -228,7 +274,12 @@ class GhostOperators:
         self.active_u, self.active_v, self.active_p = active_u, active_v, active_p
         self.Eu, self.ui, self.ghost_u = extension(gc.xu, gc.yu, gc.sdf, active_u, interpolation=interpolation)
         self.Ev, self.vi, self.ghost_v = extension(gc.xv, gc.yv, gc.sdf, active_v, interpolation=interpolation)
-        self.P, self.pi, self.ghost_p = extension(gc.xp, gc.yp, gc.sdf, active_p, neumann=True)
+        if pressure_extension == 'quadratic':   # optimized_ghost_cell.py:75-79
+            self.P, self.pi, self.ghost_p = exterior_pressure_extension(gc.xp, gc.yp, gc.sdf, active_p)
+        elif pressure_extension == 'neumann':
+            self.P, self.pi, self.ghost_p = extension(gc.xp, gc.yp, gc.sdf, active_p, neumann=True)
+        else:
+            raise ValueError('Unknown Ghost pressure extension')
         self.E = sp.block_diag((self.Eu, self.Ev), format='csr')
         self.ushape, self.vshape, self.pshape = (ny, nx + 1), (ny + 1, nx), (ny, nx)
         self.usize, self.vsize = ny * (nx + 1), (ny + 1) * nx

@@ -137,7 +137,8 @@ int mcf_set_schur_system(mcf_handle* h, int kind, const double* S, const int* Cp
 /* Record the (c, y) pair of the first pressure solve of every projection of the steps that follow
    (a ring of `capacity` pairs; 0 disables), in angle order, tagged 4*step + projection where step
    counts from the start of each mcf_run / mcf_step call.  mcf_get_dump copies up to max_pairs
-   recorded pairs and empties the ring.  Build-only: the reference solves S directly. */
+   recorded pairs and empties the ring.  Steps run without speculation while dumping.
+   Build-only: the reference solves S directly. */
 int mcf_set_dump(mcf_handle* h, int capacity);
 int mcf_get_dump(mcf_handle* h, int max_pairs, double* c, double* y, int* tags, int* count);
 /* GMRES(restart) on S (angle order) for n right-hand sides (n x r), from x0 (null: zero) until

@@ -155,6 +155,61 @@ def extension(x, y, sdf, active, *, neumann=False, interpolation='bilinear', ret
 # --------------------------------------------------------------------------------------
 # k-nearest neighbours with ties (replaces cKDTree query + query_ball_point)
 # --------------------------------------------------------------------------------------
+def exterior_pressure_extension(x, y, sdf, active):
+    """Quadratic exterior pressure continuation (solver/ghost_reconstruction.py:7-45), one ghost cell
+    at a time: 12, 20 then 32 nearest active centres within 6h (ties kept), least squares on the
+    basis [1, x, y, x^2, xy, y^2], largest weight absorbs the partition-of-unity defect; accepted at
+    rank 6, condition <= 1e8 and reproduction error <= 1e-10.  Returns (P, ids, info)."""
+    xx, yy = np.meshgrid(x, y)
+    distance = sdf.distance(xx, yy)
+    dx, dy = x[1] - x[0], y[1] - y[0]
+    h = max(dx, dy)
+    flat_active = active.ravel()
+    ids = np.flatnonzero(flat_active)
+    labels = -np.ones(active.size, dtype=np.int64)
+    labels[ids] = np.arange(len(ids))
+    pool = [k for k in range(active.size) if flat_active[k] and distance.ravel()[k] < 6 * h]
+    if len(pool) < 12:
+        raise ValueError('Pressure continuation needs twelve exterior centers')
+    pool = np.array(pool)
+    centers = np.column_stack((yy.ravel()[pool], xx.ravel()[pool]))
+    data = {}
+    for k in ids:
+        data[(k, labels[k])] = 1.0
+    cond_max = err_max = 0.0
+    count_max = 0
+    for node in range(active.size):
+        d = distance.ravel()[node]
+        if not (-2 * h <= d <= 1e-13):
+            continue
+        point = np.array([yy.ravel()[node], xx.ravel()[node]])
+        for count in (12, 20, 32):
+            near = knn_with_ties(centers, point, min(count, len(centers)))
+            rel_y = (centers[near, 0] - point[0]) / dy
+            rel_x = (centers[near, 1] - point[1]) / dx
+            basis = np.vstack((np.ones(len(near)), rel_x, rel_y, rel_x ** 2, rel_x * rel_y, rel_y ** 2))
+            want = np.zeros(6)
+            want[0] = 1.0
+            weights = np.linalg.lstsq(basis, want, rcond=1e-12)
+            w, rank = weights[0], weights[2]
+            cond = float(np.linalg.cond(basis))
+            big = int(np.argmax(np.abs(w)))
+            w[big] += 1.0 - np.sum(w)
+            err = float(np.max(np.abs(basis @ w - want)))
+            if rank == 6 and np.isfinite(cond) and cond <= 1e8 and err <= 1e-10:
+                break
+        else:
+            raise ValueError('Unresolved exterior pressure polynomial')
+        for q, c in enumerate(near):
+            data[(node, labels[pool[c]])] = data.get((node, labels[pool[c]]), 0.0) + w[q]
+        cond_max, err_max, count_max = max(cond_max, cond), max(err_max, err), max(count_max, len(near))
+    keys = list(data)
+    P = sp.csr_matrix(([data[k] for k in keys], ([k[0] for k in keys], [k[1] for k in keys])),
+                      shape=(active.size, len(ids)))
+    return P, ids, dict(stencil_condition_max=cond_max, polynomial_reproduction_linf=err_max,
+                        neighbor_count_max=count_max)
+
+
 def knn_with_ties(centers, point, k):
     """All centres within the k-th smallest distance x (1+1e-12), ascending index order.
 
@@ -505,7 +560,10 @@ class GhostIntegrator:
         interp = s.ghost_velocity_interpolation
         eu, self.ui = extension(gc.xu, gc.yu, gc.sdf, active_u, interpolation=interp)
         ev, self.vi = extension(gc.xv, gc.yv, gc.sdf, active_v, interpolation=interp)
-        self.P, self.pi = extension(gc.xp, gc.yp, gc.sdf, active_p, neumann=True)
+        if s.ghost_pressure_extension == 'quadratic':   # optimized_ghost_cell.py:75-79
+            self.P, self.pi, self.reconstruction_fit = exterior_pressure_extension(gc.xp, gc.yp, gc.sdf, active_p)
+        else:
+            self.P, self.pi = extension(gc.xp, gc.yp, gc.sdf, active_p, neumann=True)
         self.Eu, self.Ev = eu, ev
         self.E = sp.block_diag((eu, ev), format='csr')
         self.usize = s.u.size
@@ -711,8 +769,8 @@ class CylinderFlow:
             raise ValueError('The oracle covers boundary=ghost_cell, time_scheme=ab2_cn, centered transport')
         if pressure_solver != 'direct' or slip_walls:
             raise ValueError('Ghost AB2/CN requires the direct pressure solver and no-slip walls')
-        if ghost_pressure_extension != 'neumann':
-            raise ValueError('Only the Neumann pressure extension is restated')
+        if ghost_pressure_extension not in ('neumann', 'quadratic'):
+            raise ValueError('Unknown Ghost pressure extension')
         if any(isinstance(n, bool) or not isinstance(n, (int, np.integer)) or n < 4 for n in (nx, ny)):
             raise ValueError('nx and ny must be integers >= 4')
         if not all(np.isfinite(x) and x > 0 for x in (Lx, Ly, D, nu, rho, U_inf)):

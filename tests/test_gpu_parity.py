@@ -209,3 +209,24 @@ def test_biquadratic_velocity_extension_steps(pkg, kw):
     scale = max(abs(d['cd']), abs(d['cl']))
     assert abs(cd[-1] - d['cd']) <= 1e-10 * scale and abs(cl[-1] - d['cl']) <= 1e-10 * scale
     assert gpu.last_step_diagnostics['max_divergence'] < 1e-12
+
+
+@pytest.mark.parametrize('kw', [CH, dict(CFG1, initial='zero')])
+def test_quadratic_pressure_extension_steps(pkg, kw):
+    """ghost_pressure_extension='quadratic' (ghost_reconstruction.py:7-45): wider pressure ghost rows
+    and the operators built on them through the same kernels match the oracle."""
+    n, dt = 10, 1e-3
+    gpu = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False,
+                               ghost_pressure_extension='quadratic', **kw)
+    t, cd, cl = gpu.solve(n_steps=n, dt=dt, perturb_transient=False, save_artifacts=False, verbose=False)
+    ref = O.CylinderFlow(ghost_pressure_extension='quadratic', **kw)
+    for _ in range(n):
+        ref.step(dt)
+    for key in ('u', 'v', 'p'):
+        assert rel(getattr(gpu, key), getattr(ref, key)) < 1e-11, key
+    d = ref.last_step_diagnostics
+    scale = max(abs(d['cd']), abs(d['cl']))
+    assert abs(cd[-1] - d['cd']) <= 1e-10 * scale and abs(cl[-1] - d['cl']) <= 1e-10 * scale
+    assert gpu.last_step_diagnostics['max_divergence'] < 1e-12
+    assert gpu.gg.reconstruction_info['pressure'] == 'quadratic'
+    assert gpu.gg.reconstruction_info['polynomial_reproduction_linf'] <= 1e-10

@@ -11,10 +11,10 @@ CASES = [dict(nx=88, ny=16, Lx=22., Ly=4.1, D=1., xc=2., yc=2., nu=0.01),
          dict(nx=440, ny=82, Lx=22., Ly=4.1, D=1., xc=2., yc=2., nu=0.05, inflow='parabolic', U_m=1.5)]
 
 
-def build(pkg, case, interpolation='bilinear'):
-    ref = O.CylinderFlow(ghost_velocity_interpolation=interpolation, **case)
+def build(pkg, case, interpolation='bilinear', pressure='neumann'):
+    ref = O.CylinderFlow(ghost_velocity_interpolation=interpolation, ghost_pressure_extension=pressure, **case)
     ops = pkg.GhostOperators(case['nx'], case['ny'], case['Lx'], case['Ly'], case['D'], case['xc'], case['yc'],
-                             case['nu'], 1., ref.u_inlet_base, interpolation=interpolation)
+                             case['nu'], 1., ref.u_inlet_base, interpolation=interpolation, pressure_extension=pressure)
     return ref, ops
 
 
@@ -56,6 +56,27 @@ def test_ghost_probe_records_match(pkg):
 def test_biquadratic_tables_bit_exact(pkg):
     ref, ops = build(pkg, CASES[1], interpolation='biquadratic')
     assert same_csr(ops.E, ref.gg.E) and same_csr(ops.L, ref.gg.L)
+
+
+@pytest.mark.parametrize('case', CASES[1:], ids=['40x30', '440x82'])
+def test_quadratic_pressure_extension_tables(pkg, case):
+    """ghost_pressure_extension='quadratic' (ghost_reconstruction.py:7-45): the product's P and the
+    operators built on it equal the oracle's bit for bit; P reproduces any quadratic polynomial at
+    the ghost cells and its rows sum to one."""
+    ref, ops = build(pkg, case, pressure='quadratic')
+    g = ref.gg
+    for name, a, b in (('P', ops.P, g.P), ('G', ops.G, g.G), ('A', ops.A, g.A)):
+        assert same_csr(a, b), name
+    assert np.array_equal(ops.pi, g.pi)
+    gc = ops.gc
+    X, Y = np.meshgrid(gc.xp, gc.yp)
+    f = 1.3 - 0.7 * X + 0.2 * Y + 0.05 * X * X - 0.11 * X * Y + 0.3 * Y * Y
+    ext = ops.P @ f.ravel()[ops.pi]
+    ghost = ops.ghost_p['ghost_nodes']
+    assert len(ghost) > 0 and np.max(abs(ext[ghost] - f.ravel()[ghost])) < 1e-9 * np.max(abs(f))
+    sums = np.asarray(ops.P.sum(axis=1)).ravel()
+    assert np.max(abs(sums[ghost] - 1.0)) < 1e-12
+    assert ops.ghost_p['polynomial_reproduction_linf'] <= 1e-10 and ops.ghost_p['neighbor_count_max'] >= 12
 
 
 @pytest.mark.parametrize('kind', ['pressure', 'u', 'v'])
