@@ -25,7 +25,13 @@ DIAG_NAMES = ['time', 'dt', 'cd', 'cl', 'cd_pressure', 'cd_viscous', 'cl_pressur
               'unused_rel1', 'pressure_refinement_iterations', 'pressure_residual_target_linf', 'unused_t1',
               'pressure_roundoff_bound_linf', 'unused_r1', 'pressure_backward_error_linf', 'unused_b1',
               'helmholtz_refinement_iterations', 'status', 'rhs_max', 'pressure_solve_count']
-DIAG_LEN = 32
+DIAG_SOLVE0 = 32                     # per-solve records, MCF_SOLVE_LEN doubles each, in call order
+SOLVE_NAMES = ['iterations', 'residual_linf', 'residual_relative_l2', 'refinement_iterations',
+               'residual_target_linf', 'roundoff_bound_linf', 'backward_error_linf', 'status', 'seconds']
+SOLVE_LEN = 10
+MAX_SOLVES = 3
+DIAG_LEN = 64
+PB_DIRECT, PB_GMRES = 0, 1
 
 _D = ctypes.POINTER(ctypes.c_double)
 _F = ctypes.POINTER(ctypes.c_float)
@@ -57,6 +63,10 @@ _SIGNATURES = {
     'mcf_forces': (ctypes.c_int, [_H, _D, _D, ctypes.c_double, _D]),
     'mcf_rect_inverse': (ctypes.c_int, [_H, ctypes.c_int, _D, _D]),
     'mcf_schur_solve': (ctypes.c_int, [_H, ctypes.c_int, _D, _D, _D]),
+    'mcf_set_pressure_backend': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                ctypes.c_double]),
+    'mcf_get_pressure_backend': (ctypes.c_int, [_H, _I]),
+    'mcf_pressure_solve': (ctypes.c_int, [_H, _D, _D, _D]),
     'mcf_apply_operator': (ctypes.c_int, [_H, ctypes.c_int, _D, _D, ctypes.c_double]),
     'mcf_divergence_stats': (ctypes.c_int, [_H, _D, _U8, _D]),
     'mcf_profile': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_double, _D]),

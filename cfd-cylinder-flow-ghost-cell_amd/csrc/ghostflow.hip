@@ -25,6 +25,7 @@
 #include <map>
 #include <algorithm>
 #include <cstdlib>
+#include <chrono>
 #include "ghostflow.h"
 
 #define DEV __device__ __forceinline__
@@ -45,6 +46,8 @@ enum BufId {
   // p-sized
   B_PH, B_POUT, B_DIVP, B_PHI, B_RESP, B_PHI2, B_TP0, B_TP1,
   B_CKP,               // checkpoint of the half pressure
+  B_KA, B_KB, B_KX, B_KR,   // GMRES work vectors (operator output, preconditioned vector, correction, residual)
+  B_KV0,               // first GMRES basis vector: the head of the basis block, null until GMRES is selected
   NBUF
 };
 constexpr int kFirstPBuf = B_PH;
@@ -56,6 +59,7 @@ enum Slot {
   SL_HU = 8, SL_HV = 16,             // Helmholtz u / v solve blocks
   SL_P0 = 24, SL_P1 = 32, SL_P2 = 40, // pressure solve blocks (projection 1, 2, startup)
   SL_FORCE = 48,                     // 6 doubles (written by one block, stripe 0)
+  SL_GM_IT = 54, SL_GM_REF = 57, SL_GM_ST = 60,   // GMRES iterations / refinements / status of pressure solves 0-2 (stripe 0)
   NSLOT = 64
 };
 // Every slot is striped over NSTRIPE words so that the blocks of a reduction hit different
@@ -540,34 +544,40 @@ __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a)
 // Pressure residual b - A phi and the monitored-solve statistics (pressure_schur.py:250-273)
 struct PresArgs { int b_id, x_id, out_id; int mode; int slot0; int res_slot; int pred_slot; };
 
+// Row (j, i) of the pressure operator A applied to X: (A X)_row, sum |a||x| over the row and the
+// row's nonzero count.  Override rows come from their CSR rows; the others are the rectangle's
+// 5-point stencil (outlet column Dirichlet, other walls Neumann).
+DEV void p_row(const CaseDev& c, const double* X, int idx, int j, int i, double& ax, double& aax, int& nnz) {
+  const int r = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
+  if (r >= 0) { ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz); return; }
+  const int nx = c.nx, ny = c.ny;
+  const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
+  const double ox = idx2 / c.rho, oy = idy2 / c.rho;
+  const double x = X[idx];
+  const double cx = (i == 0) ? -1.0 : ((i == nx - 1) ? -3.0 : -2.0);
+  const double cy = (j == 0 || j == ny - 1) ? -1.0 : -2.0;
+  const double dg = (cx * idx2 + cy * idy2) / c.rho;
+  ax = dg * x; aax = fabs(dg) * fabs(x); nnz = 1;
+  if (i > 0) { double t = X[idx - 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
+  if (i < nx - 1) { double t = X[idx + 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
+  if (j > 0) { double t = X[idx - nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
+  if (j < ny - 1) { double t = X[idx + nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
+}
+
 __global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   if (a.pred_slot >= 0 && !pred_refine(c.sl, a.slot0, a.pred_slot)) return;
-  const int nx = c.nx, ny = c.ny;
+  const int nx = c.nx;
   const double* B = c.buf[a.b_id];
   const double* X = c.buf[a.x_id];
   double* O = a.mode == 0 ? c.buf[a.out_id] : nullptr;
-  const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
-  const double ox = idx2 / c.rho, oy = idy2 / c.rho;
   double mres = 0.0, mrnd = 0.0, msc = 0.0, ssr = 0.0, ssv = 0.0;
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double res = 0.0;
     if (p_act(c, j, i)) {
-      const int r = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
       double ax, aax; int nnz;
-      if (r >= 0) ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz);
-      else {
-        const double x = X[idx];
-        const double cx = (i == 0) ? -1.0 : ((i == nx - 1) ? -3.0 : -2.0);
-        const double cy = (j == 0 || j == ny - 1) ? -1.0 : -2.0;
-        const double dg = (cx * idx2 + cy * idy2) / c.rho;
-        ax = dg * x; aax = fabs(dg) * fabs(x); nnz = 1;
-        if (i > 0) { double t = X[idx - 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
-        if (i < nx - 1) { double t = X[idx + 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
-        if (j > 0) { double t = X[idx - nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
-        if (j < ny - 1) { double t = X[idx + nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
-      }
+      p_row(c, X, idx, j, i, ax, aax, nnz);
       const double b = B[idx];
       res = b - ax;
       const double sc = aax + fabs(b);
@@ -608,6 +618,7 @@ struct GemmArgs {
   int epi;       // 0 store, 1 masked store, 2 masked accumulate
   int pred_slot; // -1: always
   int slot_shift; // added to the desc's slot block (pressure solve index)
+  const int* gate = nullptr;   // when set, the launch runs only while *gate != 0
 };
 
 DEV const double* desc_in(const SolveDesc& d, int id) { return d.bufs[id] + d.off; }
@@ -641,6 +652,7 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g
   const int M = d.yy, N = d.xx, K = d.yy;
   const int row0 = ry * TM, col0 = cx * TN;
   if (row0 >= M || col0 >= N) return;
+  if (g.gate && *g.gate == 0) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
   const double* AT = g.forward ? d.V : d.VT;
   const double* Bm;
@@ -730,7 +742,7 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g
   }
 }
 
-struct PcrArgs { int mode; int pred_slot; int slot_shift; };
+struct PcrArgs { int mode; int pred_slot; int slot_shift; const int* gate = nullptr; };
 
 // Block-wide exclusive scan of affine maps y -> A y + B (composition in thread order, or in
 // reverse order when `reverse`); returns the value reached from 0 before this thread's chunk.
@@ -771,6 +783,7 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p
   const SolveDesc& d = descs[blockIdx.y];
   const int k = blockIdx.x;
   if (k >= d.yy) return;
+  if (p.gate && *p.gate == 0) return;
   if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
   const int xx = d.xx, yy = d.yy, tid = threadIdx.x;
   const int C = (xx + TRI_T - 1) / TRI_T;
@@ -1000,7 +1013,29 @@ __global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
 }
 
 // Gather the step's scalars into the diagnostics ring and clear the slots (one wave per case)
-struct FinArgs { int nproj; int lean; int force_miss; };
+// Record of one monitored pressure solve (pressure_schur.py:58-70) from its slot values: the direct
+// backend's refinements follow from its first two residuals, GMRES reports its own counts.
+struct SolveSlots { double val, r1, r2, fin, rnd, asc, ssr, ssv, it, ref, st; };
+__host__ __device__ inline void solve_record(const SolveSlots& s, bool gmres, double* rec) {
+  const double t1 = fmax(1e-10, 32.0 * kEps * fmax(1.0, s.val));
+  const double target = fmax(fmax(1e-10, 64.0 * kEps * fmax(1.0, s.val)), s.rnd);
+  rec[MCF_S_ITERATIONS] = gmres ? s.it : 0.0;
+  rec[MCF_S_RES_LINF] = s.fin;
+  rec[MCF_S_REL_L2] = sqrt(s.ssr) / fmax(sqrt(s.ssv), kTiny);
+  rec[MCF_S_REFINE] = gmres ? s.ref : (double)((s.r1 > t1 ? 1 : 0) + (s.r1 > t1 && s.r2 > t1 ? 1 : 0));
+  rec[MCF_S_TARGET] = target;
+  rec[MCF_S_ROUND] = s.rnd;
+  rec[MCF_S_BACKWARD] = s.fin / fmax(s.asc, kTiny);
+  rec[MCF_S_STATUS] = gmres ? s.st : 0.0;
+  rec[MCF_S_SECONDS] = 0.0;
+  rec[MCF_S_SPARE] = 0.0;
+}
+// the gate of pressure_schur.py:69-70
+__host__ __device__ inline bool solve_ok(const double* rec) {
+  return rec[MCF_S_STATUS] == 0.0 && rec[MCF_S_RES_LINF] <= rec[MCF_S_TARGET];
+}
+
+struct FinArgs { int nproj; int lean; int force_miss; int gmres; };
 constexpr int FIN_N = 34;
 __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a) {
   const CaseDev& c = cases[blockIdx.y];
@@ -1028,6 +1063,9 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
   for (int q = 0; q < 4; ++q) vs[q] = wave_sum(sq[q]);
   double f[6];
   for (int q = 0; q < 6; ++q) f[q] = slot_d(s, (SL_FORCE + q) * NSTRIPE);
+  double gm[9];
+  #pragma unroll
+  for (int q = 0; q < 9; ++q) gm[q] = slot_max(s, SL_GM_IT + q);
   if (lane == 0) {
     const double dt = c.prm[0];
     const int row = *c.clock_step;
@@ -1062,20 +1100,23 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
     if (!(mres <= 1e-10 * fmax(1.0, rhs_max))) status |= 1;
     int refine = 0;
     double worst = -1.0;
-    for (int b = 0; b < a.nproj; ++b) {
-      const double t1 = woodbury_target(pval[b]);
-      refine += (pr1[b] > t1 ? 1 : 0) + (pr1[b] > t1 && pr2[b] > t1 ? 1 : 0);
-      double target = fmax(1e-10, 64.0 * kEps * fmax(1.0, pval[b]));
-      target = fmax(target, prnd[b]);
-      if (!(pfin[b] <= target)) status |= 2;
-      d[MCF_D_P_REL0] = fmax(d[MCF_D_P_REL0], sqrt(pssr[b]) / fmax(sqrt(pssv[b]), kTiny));
+    // one record per pressure solve in call order: the startup projection comes first
+    const int order3[3] = {2, 0, 1};
+    for (int q = 0; q < a.nproj; ++q) {
+      const int b = a.nproj == 3 ? order3[q] : q;
+      const SolveSlots ss{pval[b], pr1[b], pr2[b], pfin[b], prnd[b], pasc[b], pssr[b], pssv[b], gm[b], gm[3 + b], gm[6 + b]};
+      double* rec = d + MCF_D_SOLVE0 + MCF_SOLVE_LEN * q;
+      solve_record(ss, a.gmres != 0, rec);
+      refine += (int)rec[MCF_S_REFINE];
+      if (!solve_ok(rec)) status |= 2;
+      d[MCF_D_P_REL0] = fmax(d[MCF_D_P_REL0], rec[MCF_S_REL_L2]);
       if (b == 0) d[MCF_D_P_RES1] = pfin[b];
       if (pfin[b] > worst || pfin[b] != pfin[b]) {
         worst = pfin[b];
         d[MCF_D_P_RES0] = pfin[b];
-        d[MCF_D_P_TARGET0] = target;
+        d[MCF_D_P_TARGET0] = rec[MCF_S_TARGET];
         d[MCF_D_P_ROUND0] = prnd[b];
-        d[MCF_D_P_BACKWARD0] = pfin[b] / fmax(pasc[b], kTiny);
+        d[MCF_D_P_BACKWARD0] = rec[MCF_S_BACKWARD];
       }
     }
     const double tu = woodbury_target(hu_val), tv = woodbury_target(hv_val);
@@ -1177,6 +1218,434 @@ __global__ __launch_bounds__(256) void k_divstats(const double* u, const double*
 // ======================================================================================
 // Launch shapes of the transform (tile TM x TN, K chunk) and of the tridiagonal sweep (threads per
 // mode), per problem class; picked by the autotuner or pinned by the caller.
+namespace {
+
+// ------------------------------------------------------------------------------------
+// Iterative pressure backend (pressure_schur.py:16-72 with iterative=True): restarted GMRES on the
+// pressure operator A, left-preconditioned by the rectangle inverse M (zero-filled rectangle
+// right-hand side, separable solve, restricted to the active cells).  The control flow is SciPy's
+// gmres, which the reference calls (scipy/sparse/linalg/_isolve/iterative.py, SciPy 1.15): cycles
+// of at most `restart` Arnoldi steps; an inner stop when the preconditioned residual |S[col+1]|
+// falls to ptol, with ptol adapted after every cycle; Givens rotations as LAPACK's dlartg; the
+// outer stop on the true residual ||b - A x||_2 <= max(atol, rtol ||b||_2).  Orthogonalisation is
+// classical Gram-Schmidt applied twice (CGS2): each pass is one kernel over all basis vectors
+// instead of one dependent launch per vector (modified Gram-Schmidt), with the same stability in
+// practice.  Vectors live on the full pressure rectangle and are zero off the active cells.  Sums
+// are deterministic: every block writes its partial sums and one workgroup adds them in a fixed
+// order.  Basis vectors are stored unnormalised with a scale each (vector k = scale[k] * raw k),
+// which saves a normalisation pass per step.
+// ------------------------------------------------------------------------------------
+constexpr int KR_MAXR = 64;                            // longest restart cycle
+constexpr int KR_T = 256, KR_E = 4, KR_CHUNK = KR_T * KR_E;   // vector kernels: elements per block
+constexpr int KR_CT = 1024;                            // threads of the control kernel
+
+struct KryCtl {
+  // read back by the host after each group of launches
+  int active;      // 1 while the current cycle's Arnoldi steps run (gates their launches)
+  int done;        // the GMRES call has ended
+  int info;        // 0 converged, maxiter otherwise (SciPy's info)
+  int inner;       // Arnoldi steps of this call (the reference's callback count)
+  int cycle, col, breakdown, ncols;
+  // parameters
+  int restart, maxiter;
+  double rtol, atol_in;
+  // state
+  double bnrm2, atol, ptol, pmax, presid, rnorm, h0;
+  double h[KR_MAXR][KR_MAXR + 1];   // h[col][k]: column col of the Hessenberg matrix
+  double giv[KR_MAXR][2];
+  double S[KR_MAXR + 1];
+  double hk[KR_MAXR + 1];           // first-pass projections of the current column
+  double coef[KR_MAXR + 1];         // coefficients of the pending update, on the raw vectors
+  double scale[KR_MAXR + 1];
+};
+constexpr int KR_STAT_INTS = 8;
+
+struct KryArgs {
+  KryCtl* ctl;
+  double* V;       // restart + 1 raw basis vectors, ldv apart (vector 0 is buffer B_KV0)
+  size_t ldv;
+  double* part;    // block partial sums, part[q * nblk + block]
+  int nblk;
+};
+
+// Sum of v over the block into red[q][wave]; kr_flush writes the block's nq sums to the partials.
+DEV void kr_wave_sum(double v, int q, double (*red)[KR_T / 64]) {
+  const double s = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = s;
+}
+DEV void kr_flush(double (*red)[KR_T / 64], int nq, const KryArgs& a) {
+  __syncthreads();
+  for (int q = threadIdx.x; q < nq; q += KR_T) {
+    double s = 0.0;
+    #pragma unroll
+    for (int w = 0; w < KR_T / 64; ++w) s += red[q][w];
+    a.part[(size_t)q * a.nblk + blockIdx.x] = s;
+  }
+}
+DEV bool kr_active_cell(const CaseDev& c, int idx) {
+  const int j = idx / c.nx, i = idx - j * c.nx;
+  return p_act(c, j, i);
+}
+
+// r = b on the active cells (0 elsewhere), x = 0; partial sums of b^2
+__global__ __launch_bounds__(KR_T) void k_kry_begin(const CaseDev* cases, KryArgs a, int b_id, int x_id, int r_id) {
+  const CaseDev& c = cases[0];
+  __shared__ double red[1][KR_T / 64];
+  const double* B = c.buf[b_id];
+  double* X = c.buf[x_id];
+  double* R = c.buf[r_id];
+  double ss = 0.0;
+  const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
+  #pragma unroll
+  for (int e = 0; e < KR_E; ++e) {
+    const int idx = base + e * KR_T;
+    if (idx < c.psize) {
+      const double b = kr_active_cell(c, idx) ? B[idx] : 0.0;
+      R[idx] = b;
+      X[idx] = 0.0;
+      ss += b * b;
+    }
+  }
+  kr_wave_sum(ss, 0, red);
+  kr_flush(red, 1, a);
+}
+
+// out = A (scale[col] * raw V[col]) on the active cells
+__global__ __launch_bounds__(KR_T) void k_kry_matvec(const CaseDev* cases, KryArgs a, int out_id) {
+  const KryCtl& k = *a.ctl;
+  if (!k.active) return;
+  const CaseDev& c = cases[0];
+  const int col = k.col;
+  const double sc = k.scale[col];
+  const double* X = a.V + (size_t)col * a.ldv;
+  double* O = c.buf[out_id];
+  const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
+  #pragma unroll
+  for (int e = 0; e < KR_E; ++e) {
+    const int idx = base + e * KR_T;
+    if (idx >= c.psize) continue;
+    const int j = idx / c.nx, i = idx - j * c.nx;
+    double y = 0.0;
+    if (p_act(c, j, i)) {
+      double ax, aax; int nnz;
+      p_row(c, X, idx, j, i, ax, aax, nnz);
+      y = sc * ax;
+    }
+    O[idx] = y;
+  }
+}
+
+// First Gram-Schmidt pass: partial sums of raw V[q] . w (q <= col) and of w . w
+__global__ __launch_bounds__(KR_T) void k_kry_dots(const CaseDev* cases, KryArgs a, int w_id) {
+  const KryCtl& k = *a.ctl;
+  if (!k.active) return;
+  const CaseDev& c = cases[0];
+  __shared__ double red[KR_MAXR + 2][KR_T / 64];
+  const int col = k.col, n = c.psize;
+  const double* W = c.buf[w_id];
+  const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
+  double w[KR_E];
+  double ss = 0.0;
+  #pragma unroll
+  for (int e = 0; e < KR_E; ++e) {
+    const int idx = base + e * KR_T;
+    w[e] = idx < n ? W[idx] : 0.0;
+    ss += w[e] * w[e];
+  }
+  for (int q = 0; q <= col; ++q) {
+    const double* Vq = a.V + (size_t)q * a.ldv;
+    double d = 0.0;
+    #pragma unroll
+    for (int e = 0; e < KR_E; ++e) {
+      const int idx = base + e * KR_T;
+      if (idx < n) d += Vq[idx] * w[e];
+    }
+    kr_wave_sum(d, q, red);
+  }
+  kr_wave_sum(ss, col + 1, red);
+  kr_flush(red, col + 2, a);
+}
+
+// Gram-Schmidt update w -= sum_q coef[q] raw V[q].  pass 1: store w and the partial sums of the
+// second projections V[q] . w; pass 2: store w as raw V[col + 1] with the partial sum of w . w.
+__global__ __launch_bounds__(KR_T) void k_kry_orth(const CaseDev* cases, KryArgs a, int w_id, int pass) {
+  const KryCtl& k = *a.ctl;
+  if (!k.active) return;
+  const CaseDev& c = cases[0];
+  __shared__ double red[KR_MAXR + 2][KR_T / 64];
+  __shared__ double cf[KR_MAXR + 1];
+  const int col = k.col, n = c.psize;
+  for (int q = threadIdx.x; q <= col; q += KR_T) cf[q] = k.coef[q];
+  __syncthreads();
+  double* W = c.buf[w_id];
+  const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
+  double w[KR_E];
+  #pragma unroll
+  for (int e = 0; e < KR_E; ++e) {
+    const int idx = base + e * KR_T;
+    w[e] = idx < n ? W[idx] : 0.0;
+  }
+  for (int q = 0; q <= col; ++q) {
+    const double* Vq = a.V + (size_t)q * a.ldv;
+    const double f = cf[q];
+    #pragma unroll
+    for (int e = 0; e < KR_E; ++e) {
+      const int idx = base + e * KR_T;
+      if (idx < n) w[e] -= f * Vq[idx];
+    }
+  }
+  if (pass == 1) {
+    #pragma unroll
+    for (int e = 0; e < KR_E; ++e) {
+      const int idx = base + e * KR_T;
+      if (idx < n) W[idx] = w[e];
+    }
+    for (int q = 0; q <= col; ++q) {
+      const double* Vq = a.V + (size_t)q * a.ldv;
+      double d = 0.0;
+      #pragma unroll
+      for (int e = 0; e < KR_E; ++e) {
+        const int idx = base + e * KR_T;
+        if (idx < n) d += Vq[idx] * w[e];
+      }
+      kr_wave_sum(d, q, red);
+    }
+    kr_flush(red, col + 1, a);
+  } else {
+    double* Vn = a.V + (size_t)(col + 1) * a.ldv;
+    double ss = 0.0;
+    #pragma unroll
+    for (int e = 0; e < KR_E; ++e) {
+      const int idx = base + e * KR_T;
+      if (idx < n) { Vn[idx] = w[e]; ss += w[e] * w[e]; }
+    }
+    kr_wave_sum(ss, 0, red);
+    kr_flush(red, 1, a);
+  }
+}
+
+// Partial sums of raw V[0] . raw V[0] (a new cycle's first vector)
+__global__ __launch_bounds__(KR_T) void k_kry_norm(const CaseDev* cases, KryArgs a) {
+  const CaseDev& c = cases[0];
+  __shared__ double red[1][KR_T / 64];
+  const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
+  double ss = 0.0;
+  #pragma unroll
+  for (int e = 0; e < KR_E; ++e) {
+    const int idx = base + e * KR_T;
+    if (idx < c.psize) { const double v = a.V[idx]; ss += v * v; }
+  }
+  kr_wave_sum(ss, 0, red);
+  kr_flush(red, 1, a);
+}
+
+// x += sum_{q < ncols} coef[q] raw V[q] (the cycle's update, coef = y * scale)
+__global__ __launch_bounds__(KR_T) void k_kry_xupd(const CaseDev* cases, KryArgs a, int x_id) {
+  const KryCtl& k = *a.ctl;
+  const CaseDev& c = cases[0];
+  __shared__ double cf[KR_MAXR + 1];
+  const int m = k.ncols;
+  for (int q = threadIdx.x; q < m; q += KR_T) cf[q] = k.coef[q];
+  __syncthreads();
+  double* X = c.buf[x_id];
+  const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
+  double acc[KR_E] = {};
+  for (int q = 0; q < m; ++q) {
+    const double* Vq = a.V + (size_t)q * a.ldv;
+    const double f = cf[q];
+    #pragma unroll
+    for (int e = 0; e < KR_E; ++e) {
+      const int idx = base + e * KR_T;
+      if (idx < c.psize) acc[e] += f * Vq[idx];
+    }
+  }
+  #pragma unroll
+  for (int e = 0; e < KR_E; ++e) {
+    const int idx = base + e * KR_T;
+    if (idx < c.psize) X[idx] += acc[e];
+  }
+}
+
+// r = b - A x on the active cells (0 elsewhere); partial sums of r^2
+__global__ __launch_bounds__(KR_T) void k_kry_resid(const CaseDev* cases, KryArgs a, int b_id, int x_id, int r_id) {
+  const CaseDev& c = cases[0];
+  __shared__ double red[1][KR_T / 64];
+  const double* B = c.buf[b_id];
+  const double* X = c.buf[x_id];
+  double* R = c.buf[r_id];
+  const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
+  double ss = 0.0;
+  #pragma unroll
+  for (int e = 0; e < KR_E; ++e) {
+    const int idx = base + e * KR_T;
+    if (idx >= c.psize) continue;
+    const int j = idx / c.nx, i = idx - j * c.nx;
+    double r = 0.0;
+    if (p_act(c, j, i)) {
+      double ax, aax; int nnz;
+      p_row(c, X, idx, j, i, ax, aax, nnz);
+      r = B[idx] - ax;
+    }
+    R[idx] = r;
+    ss += r * r;
+  }
+  kr_wave_sum(ss, 0, red);
+  kr_flush(red, 1, a);
+}
+
+// x += y on the active cells (a refinement's correction)
+__global__ __launch_bounds__(256) void k_kry_axpy(const CaseDev* cases, int x_id, int y_id) {
+  const CaseDev& c = cases[0];
+  double* X = c.buf[x_id];
+  const double* Y = c.buf[y_id];
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x)
+    if (kr_active_cell(c, idx)) X[idx] += Y[idx];
+}
+
+// GMRES counts of pressure solve `psolve` into its slots (stripe 0)
+__global__ void k_kry_record(const CaseDev* cases, int psolve, double iters, double refine, double status) {
+  const CaseDev& c = cases[0];
+  if (threadIdx.x != 0) return;
+  c.sl[(SL_GM_IT + psolve) * NSTRIPE] = (u64)__double_as_longlong(iters);
+  c.sl[(SL_GM_REF + psolve) * NSTRIPE] = (u64)__double_as_longlong(refine);
+  c.sl[(SL_GM_ST + psolve) * NSTRIPE] = (u64)__double_as_longlong(status);
+}
+
+// Plane rotation of LAPACK 3.10's dlartg (E. Anderson, "Algorithm 978: Safe scaling in the Level 1
+// BLAS", ACM TOMS 44, 2017): [c s; -s c] [f; g] = [r; 0].
+DEV void kr_lartg(double f, double g, double& c, double& s, double& r) {
+  const double safmin = 2.2250738585072014e-308, safmax = 1.0 / safmin;
+  const double rtmin = sqrt(safmin), rtmax = sqrt(safmax / 2.0);
+  if (g == 0.0) { c = 1.0; s = 0.0; r = f; return; }
+  if (f == 0.0) { c = 0.0; s = copysign(1.0, g); r = fabs(g); return; }
+  const double f1 = fabs(f), g1 = fabs(g);
+  if (f1 > rtmin && f1 < rtmax && g1 > rtmin && g1 < rtmax) {
+    const double d = sqrt(f * f + g * g);
+    c = f1 / d;
+    r = copysign(d, f);
+    s = g / r;
+  } else {
+    const double u = fmin(safmax, fmax(safmin, fmax(f1, g1)));
+    const double fs = f / u, gs = g / u;
+    const double d = sqrt(fs * fs + gs * gs);
+    c = fabs(fs) / d;
+    r = copysign(d, f);
+    s = gs / r;
+    r *= u;
+  }
+}
+
+enum { KP_BEGIN = 0, KP_START_FIRST, KP_START, KP_DOTS1, KP_DOTS2, KP_NORM, KP_SOLVE, KP_RESID };
+
+// Control step of the GMRES call: adds the block partial sums (one wave per sum, blocks in a fixed
+// order) and updates the scalar state on one thread.
+__global__ __launch_bounds__(KR_CT) void k_kry_ctl(KryArgs a, int phase) {
+  KryCtl& k = *a.ctl;
+  const bool arnoldi = phase == KP_DOTS1 || phase == KP_DOTS2 || phase == KP_NORM;
+  if (arnoldi && !k.active) return;
+  __shared__ double red[KR_MAXR + 2];
+  const int col = k.col;
+  const int nq = phase == KP_DOTS1 ? col + 2 : phase == KP_DOTS2 ? col + 1 : phase == KP_SOLVE ? 0 : 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = wv; q < nq; q += KR_CT / 64) {
+    double s = 0.0;
+    for (int b = lane; b < a.nblk; b += 64) s += a.part[(size_t)q * a.nblk + b];
+    s = wave_sum(s);
+    if (lane == 0) red[q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  switch (phase) {
+    case KP_BEGIN: {
+      const double bn = sqrt(red[0]);
+      k.bnrm2 = bn;
+      k.atol = fmax(k.atol_in, k.rtol * bn);
+      k.inner = 0; k.cycle = 0; k.info = 0; k.done = 0; k.active = 0; k.breakdown = 0; k.ncols = 0; k.col = 0;
+      if (!isfinite(bn)) { k.done = 1; k.info = k.maxiter; }   // a non-finite right-hand side cannot converge
+      else if (bn == 0.0 || bn < k.atol) k.done = 1;          // x = 0 is the answer
+      break;
+    }
+    case KP_START_FIRST:
+    case KP_START: {
+      const double t = sqrt(red[0]);
+      if (phase == KP_START_FIRST) { k.ptol = t * fmin(1.0, k.atol / k.bnrm2); k.pmax = 1.0; }
+      k.scale[0] = 1.0 / t;
+      for (int q = 0; q <= k.restart; ++q) k.S[q] = 0.0;
+      k.S[0] = t;
+      k.col = 0; k.breakdown = 0; k.active = 1;
+      break;
+    }
+    case KP_DOTS1: {
+      for (int q = 0; q <= col; ++q) { k.hk[q] = red[q] * k.scale[q]; k.coef[q] = k.hk[q] * k.scale[q]; }
+      k.h0 = sqrt(red[col + 1]);
+      break;
+    }
+    case KP_DOTS2: {
+      for (int q = 0; q <= col; ++q) {
+        const double d = red[q] * k.scale[q];
+        k.h[col][q] = k.hk[q] + d;
+        k.coef[q] = d * k.scale[q];
+      }
+      break;
+    }
+    case KP_NORM: {
+      const double h1 = sqrt(red[0]);
+      k.h[col][col + 1] = h1;
+      if (h1 <= kEps * k.h0) { k.h[col][col + 1] = 0.0; k.breakdown = 1; }   // exact solution in the span
+      else k.scale[col + 1] = 1.0 / h1;
+      for (int q = 0; q < col; ++q) {
+        const double cq = k.giv[q][0], sq = k.giv[q][1];
+        const double n0 = k.h[col][q], n1 = k.h[col][q + 1];
+        k.h[col][q] = cq * n0 + sq * n1;
+        k.h[col][q + 1] = -sq * n0 + cq * n1;
+      }
+      double cr, sr, rr;
+      kr_lartg(k.h[col][col], k.h[col][col + 1], cr, sr, rr);
+      k.giv[col][0] = cr; k.giv[col][1] = sr;
+      k.h[col][col] = rr; k.h[col][col + 1] = 0.0;
+      const double t = -sr * k.S[col];
+      k.S[col] = cr * k.S[col];
+      k.S[col + 1] = t;
+      k.presid = fabs(t);
+      k.inner += 1;
+      if (k.presid <= k.ptol || k.breakdown || col + 1 == k.restart) { k.active = 0; k.ncols = col + 1; }
+      else k.col = col + 1;
+      break;
+    }
+    case KP_SOLVE: {
+      // back substitution with the triangular Hessenberg factor (singular columns skipped), then
+      // coef = y * scale for the update of x
+      const int m = k.ncols - 1;
+      if (k.h[m][m] == 0.0) k.S[m] = 0.0;
+      for (int q = 0; q <= m; ++q) k.coef[q] = k.S[q];
+      for (int q = m; q > 0; --q) {
+        if (k.coef[q] != 0.0) {
+          k.coef[q] /= k.h[q][q];
+          const double t = k.coef[q];
+          for (int p = 0; p < q; ++p) k.coef[p] -= t * k.h[q][p];
+        }
+      }
+      if (k.coef[0] != 0.0) k.coef[0] /= k.h[0][0];
+      for (int q = 0; q <= m; ++q) k.coef[q] *= k.scale[q];
+      break;
+    }
+    case KP_RESID: {
+      const double rn = sqrt(red[0]);
+      k.rnorm = rn;
+      k.cycle += 1;
+      if (rn <= k.atol) { k.done = 1; k.info = 0; }
+      else if (!isfinite(rn) || k.breakdown || k.cycle >= k.maxiter) { k.done = 1; k.info = k.maxiter; }
+      else {
+        k.pmax = k.presid <= k.ptol ? fmax(kEps, 0.25 * k.pmax) : fmin(1.0, 1.5 * k.pmax);
+        k.ptol = k.presid * fmin(k.pmax, k.atol / rn);
+      }
+      break;
+    }
+  }
+}
+
+}  // namespace
+
 constexpr int N_GEMM_V = 3, N_TRI_V = 3;
 constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64}, GEMM_TN[N_GEMM_V] = {64, 32, 64};
 constexpr int TRI_TH[N_TRI_V] = {128, 256, 512};
@@ -1202,6 +1671,20 @@ void launch_tri(int v, int yy, int count, hipStream_t st, const SolveDesc* D, Pc
 }
 
 struct StepGraphs;
+
+// GMRES workspace of a handle (allocated when the GMRES pressure backend is selected)
+struct Krylov {
+  int restart = 50, maxiter = 100;
+  double rtol = 1e-12, atol = 1e-12;
+  KryCtl* ctl = nullptr;
+  double* V = nullptr;            // restart + 1 basis vectors
+  double* part = nullptr;         // (KR_MAXR + 2) * nblk partial sums
+  size_t ldv = 0;
+  int nblk = 0;
+  int* stat = nullptr;            // pinned host copy of the status words
+  std::vector<double> seconds;    // wall time of each monitored solve since the last read, in call order
+};
+
 struct mcf_handle {
   int nx, ny, usize, vsize, psize, velsize;
   double dx, dy, nu, rho;
@@ -1244,6 +1727,8 @@ struct mcf_handle {
   int spec_chunk = 8;        // steps per speculative chunk: doubles after a clean chunk, back to 8 on a miss
   int spec_full = 0;         // steps to run in full before speculating again (set after a miss)
   int spec_backoff = 16;     // next full stretch: doubles while misses persist, back to 16 after a clean chunk
+  int pbackend = MCF_PB_DIRECT;   // pressure backend
+  Krylov* kry = nullptr;
 };
 
 namespace {
@@ -1301,6 +1786,7 @@ int sv_block(int kind, int psolve) {
 // Everything a launch needs: the case table, the solve problems and the launch shapes (maxima
 // over the cases).  One case and a batch of uneven cases use the same code.
 struct Ctx {
+  mcf_handle* gm = nullptr;   // set when the pressure solves run GMRES (one case)
   Launch ln;             // launch shapes
   bool dump;             // record the (rhs, multiplier) pairs of first pressure solves
   mcf_handle* err;       // receives error messages
@@ -1331,6 +1817,7 @@ Ctx ctx_of(mcf_handle* h) {
   x.err = h; x.stream = h->stream; x.dc = h->dc; x.dsp = h->ds; x.dsh = h->ds + 1; x.n = 1;
   x.ln = h->ln;
   x.dump = h->hs[0].dump_cap > 0;
+  x.gm = h->pbackend == MCF_PB_GMRES ? h : nullptr;
   ctx_shapes(x, h->hc, h->hs);
   return x;
 }
@@ -1362,10 +1849,116 @@ int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int p
   return launch_check(x.err, "woodbury apply");
 }
 
+int read_slots(mcf_handle* h, std::vector<double>& out);
+
+// ---- GMRES pressure backend (host control; the device kernels are above) ----
+constexpr int KR_GROUP = 8;   // Arnoldi steps enqueued between reads of the status words
+
+KryArgs kry_args(const Krylov& K) { return KryArgs{K.ctl, K.V, K.ldv, K.part, K.nblk}; }
+
+int kry_read(mcf_handle* h) {
+  Krylov& K = *h->kry;
+  HIPCHK(h, hipMemcpyAsync(K.stat, K.ctl, KR_STAT_INTS * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// The preconditioner: rectangle inverse of in_id into out_id, zero off the active cells
+void kry_precondition(const Ctx& x, int in_id, int out_id, const int* gate) {
+  GemmArgs g{1, in_id, out_id, 0, -1, 0, gate};
+  launch_gemm(x.ln.gemm_p, x.p_yy, x.p_xx, 1, x.stream, x.dsp, g);
+  PcrArgs p{0, -1, 0, gate};
+  launch_tri(x.ln.tri_p, x.p_yy, 1, x.stream, x.dsp, p);
+  GemmArgs gi{0, in_id, out_id, 1, -1, 0, gate};
+  launch_gemm(x.ln.gemm_p, x.p_yy, x.p_xx, 1, x.stream, x.dsp, gi);
+}
+
+// One GMRES call on the right-hand side b_id from x0 = 0 into x_id (SciPy's gmres with the
+// reference's arguments).  The host enqueues the Arnoldi steps of a cycle in groups and reads the
+// status words after each group; the device ends the cycle (active = 0) on its inner test, after
+// which the remaining launches of the group return at once.  At most maxiter cycles run.
+int kry_call(const Ctx& x, int b_id, int x_id, int& inner, int& info) {
+  mcf_handle* h = x.gm;
+  Krylov& K = *h->kry;
+  const KryArgs a = kry_args(K);
+  const dim3 gb(K.nblk), tb(KR_T), gc(1), tc(KR_CT);
+  hipStream_t st = x.stream;
+  const int* gate = &K.ctl->active;
+  hipLaunchKernelGGL(k_kry_begin, gb, tb, 0, st, x.dc, a, b_id, x_id, (int)B_KR);
+  hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_BEGIN);
+  if (int rc = kry_read(h)) return rc;
+  bool first = true;
+  while (!K.stat[1]) {
+    kry_precondition(x, B_KR, B_KV0, nullptr);
+    hipLaunchKernelGGL(k_kry_norm, gb, tb, 0, st, x.dc, a);
+    hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)(first ? KP_START_FIRST : KP_START));
+    first = false;
+    for (int s0 = 0; s0 < K.restart; s0 += KR_GROUP) {
+      for (int s = s0; s < std::min(s0 + KR_GROUP, K.restart); ++s) {
+        hipLaunchKernelGGL(k_kry_matvec, gb, tb, 0, st, x.dc, a, (int)B_KA);
+        kry_precondition(x, B_KA, B_KB, gate);
+        hipLaunchKernelGGL(k_kry_dots, gb, tb, 0, st, x.dc, a, (int)B_KB);
+        hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_DOTS1);
+        hipLaunchKernelGGL(k_kry_orth, gb, tb, 0, st, x.dc, a, (int)B_KB, 1);
+        hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_DOTS2);
+        hipLaunchKernelGGL(k_kry_orth, gb, tb, 0, st, x.dc, a, (int)B_KB, 2);
+        hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_NORM);
+      }
+      if (int rc = launch_check(h, "gmres arnoldi steps")) return rc;
+      if (int rc = kry_read(h)) return rc;
+      if (!K.stat[0]) break;
+    }
+    if (K.stat[0]) return fail(h, 4, "GMRES cycle did not end after restart steps");
+    hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_SOLVE);
+    hipLaunchKernelGGL(k_kry_xupd, gb, tb, 0, st, x.dc, a, x_id);
+    hipLaunchKernelGGL(k_kry_resid, gb, tb, 0, st, x.dc, a, b_id, x_id, (int)B_KR);
+    hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_RESID);
+    if (int rc = kry_read(h)) return rc;
+  }
+  inner = K.stat[3];
+  info = K.stat[2];
+  return launch_check(h, "gmres");
+}
+
+// Monitored GMRES pressure solve (pressure_schur.py:33-70): GMRES, then while GMRES succeeded and
+// the true residual's max exceeds max(1e-10, 64 eps max(1, max|b|)), at most twice, a GMRES solve
+// on that residual added to x; then the final statistics into the solve's slot block and the
+// GMRES counts (Arnoldi steps of all calls, refinements, status) into its record slots.
+int kry_solve(const Ctx& x, int b_id, int x_id, int psolve) {
+  mcf_handle* h = x.gm;
+  const int slot0 = SL_P0 + 8 * psolve;
+  const auto t0 = std::chrono::steady_clock::now();
+  int inner = 0, info = 0;
+  if (int rc = kry_call(x, b_id, x_id, inner, info)) return rc;
+  int total = inner, status = info, refinements = 0;
+  for (int it = 0; it < 2 && !status; ++it) {
+    const int res_slot = it == 0 ? SV_RES1 : SV_RES2;
+    PresArgs r{b_id, x_id, B_RESP, 0, slot0, res_slot, -1};
+    hipLaunchKernelGGL(k_pres, gridSn(x.maxp, 1), dim3(256), 0, x.stream, x.dc, r);
+    std::vector<double> sv;
+    if (int rc = read_slots(h, sv)) return rc;
+    const double target = std::max(1e-10, 64 * kEps * std::max(1.0, sv[slot0 + SV_VAL]));
+    if (sv[slot0 + res_slot] <= target) break;
+    if (int rc = kry_call(x, B_RESP, B_KX, inner, info)) return rc;
+    total += inner;
+    status = info;
+    ++refinements;
+    hipLaunchKernelGGL(k_kry_axpy, gridSn(x.maxp, 1), dim3(256), 0, x.stream, x.dc, x_id, (int)B_KX);
+  }
+  PresArgs f{b_id, x_id, -1, 1, slot0, SV_FINAL, -1};
+  hipLaunchKernelGGL(k_pres, gridSn(x.maxp, 1), dim3(256), 0, x.stream, x.dc, f);
+  hipLaunchKernelGGL(k_kry_record, dim3(1), dim3(64), 0, x.stream, x.dc, psolve, (double)total,
+                     (double)refinements, (double)status);
+  HIPCHK(h, hipStreamSynchronize(x.stream));
+  h->kry->seconds.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  return launch_check(h, "gmres pressure solve");
+}
+
 // Woodbury solve with <= 2 refinements (pressure_schur.py:116-124) and the final residual.  The
 // lean form leaves the refinement launches out and computes the first and the final residual in
 // one pass; a step that would have refined is flagged by k_finalize and replayed in full.
 int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bool lean = false) {
+  if (pres && x.gm) return kry_solve(x, in_id, out_id, psolve);
   const int shift = pres ? 8 * psolve : 0;
   const int slot0 = pres ? SL_P0 + 8 * psolve : SL_HU;
   int rc;
@@ -1467,7 +2060,7 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, bool lean = false
   ForceArgs fa{Snew, B_POUT, -1.0};
   hipLaunchKernelGGL(k_forces, dim3((x.maxsurf + 63) / 64, x.n), dim3(64), 0, x.stream, x.dc, fa);
   hipLaunchKernelGGL(k_force_total, dim3(1, x.n), dim3(64), 0, x.stream, x.dc);
-  FinArgs fin{nproj, lean ? 1 : 0, force_miss ? 1 : 0};
+  FinArgs fin{nproj, lean ? 1 : 0, force_miss ? 1 : 0, x.gm ? 1 : 0};
   hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, fin);
   return launch_check(x.err, "step");
 }
@@ -1582,7 +2175,8 @@ int enqueue_steps(mcf_handle* h, int n, double dt, bool lean = false) {
       hipLaunchKernelGGL(k_set_prm, dim3(1, 1), dim3(64), 0, h->stream, h->dc, dt, alpha, factor);
       h->prm_valid = true; h->prm_dt = dt; h->prm_alpha = alpha; h->prm_factor = factor;
     }
-    if (!steady || !h->use_graphs) {
+    // GMRES steps read the solver state back between launches: they run eagerly
+    if (!steady || !h->use_graphs || x.gm) {
       if (int rc = enqueue_step_kernels(x, h->parity, !h->has_history)) return rc;
     } else {
       if (h->graph_dt != dt) drop_graphs(h);
@@ -1649,6 +2243,14 @@ int read_slots(mcf_handle* h, std::vector<double>& out) {
   return 0;
 }
 
+// Slot values of pressure solve `psolve` (host view from read_slots)
+SolveSlots slots_of(const std::vector<double>& sv, int psolve) {
+  const int b = SL_P0 + 8 * psolve;
+  return SolveSlots{sv[b + SV_VAL], sv[b + SV_RES1], sv[b + SV_RES2], sv[b + SV_FINAL], sv[b + SV_ROUND],
+                    sv[b + SV_ASCALE], sv[b + SV_SSQR], sv[b + SV_SSQV], sv[SL_GM_IT + psolve],
+                    sv[SL_GM_REF + psolve], sv[SL_GM_ST + psolve]};
+}
+
 int status_error(mcf_handle* h, const double* d, int step) {
   const int st = (int)d[MCF_D_STATUS];
   if (!st) return 0;
@@ -1680,6 +2282,7 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   g_host[h] = HostTables{};
   HIPCHK(h, hipStreamCreate(&h->stream));
   for (int b = 0; b < NBUF; ++b) {
+    if (b == B_KV0) { h->hc.buf[b] = nullptr; continue; }   // set with the GMRES workspace
     void* p = nullptr;
     size_t n = b < kFirstPBuf ? h->velsize : h->psize;
     HIPCHK(h, hipMalloc(&p, n * sizeof(double)));
@@ -1735,6 +2338,11 @@ void mcf_destroy(mcf_handle* h) {
   (void)hipFree(h->slots); (void)hipFree(h->inlet); (void)hipFree(h->dc); (void)hipFree(h->ds);
   if (h->diag) (void)hipFree(h->diag);
   if (h->tmp_rect) (void)hipFree(h->tmp_rect);
+  if (h->kry) {
+    (void)hipFree(h->kry->ctl); (void)hipFree(h->kry->V); (void)hipFree(h->kry->part);
+    (void)hipHostFree(h->kry->stat);
+    delete h->kry;
+  }
   if (h->stream) (void)hipStreamDestroy(h->stream);
   g_host.erase(h);
   delete h;
@@ -2062,6 +2670,20 @@ void spec_update(int& chunk, int& full, int& backoff, bool miss) {
 // k steps on one handle into d (k rows).  Speculative: steady steps leave the refinement launches
 // out; if any step of the chunk would have refined, the chunk is replayed from the checkpoint with
 // the refinements in place.
+// Wall time of the host-driven (GMRES) pressure solves into their records, in call order
+void patch_seconds(mcf_handle* h, double* d, int rows) {
+  if (!h->kry) return;
+  std::vector<double>& sec = h->kry->seconds;
+  size_t t = 0;
+  for (int s = 0; s < rows; ++s) {
+    double* row = d + (size_t)s * MCF_DIAG_LEN;
+    const int n = (int)row[MCF_D_P_SOLVES];
+    for (int q = 0; q < n && q < 3 && t < sec.size(); ++q)
+      row[MCF_D_SOLVE0 + MCF_SOLVE_LEN * q + MCF_S_SECONDS] = sec[t++];
+  }
+  sec.clear();
+}
+
 int run_chunk(mcf_handle* h, int k, double dt, bool lean, double* d) {
   const int parity0 = h->parity;
   const double prev_dt0 = h->prev_dt, time0 = h->time;
@@ -2072,6 +2694,7 @@ int run_chunk(mcf_handle* h, int k, double dt, bool lean, double* d) {
   if (int rc = enqueue_steps(h, k, dt, lean)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(d, h->diag, bytes, hipMemcpyDeviceToHost));
+  patch_seconds(h, d, k);
   bool miss = false;
   for (int s = 0; s < k && lean; ++s) miss = miss || ((int)d[(size_t)s * MCF_DIAG_LEN + MCF_D_STATUS] & 32);
   if (lean) spec_update(h->spec_chunk, h->spec_full, h->spec_backoff, miss);
@@ -2092,8 +2715,9 @@ int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   if (int rc = check_ready(h, dt)) return rc;
   if (int rc = ensure_diag(h, n)) return rc;
   h->owned.erase("p_raw_valid");
-  // no speculation while pairs are dumped: a replayed chunk would record its pairs twice
-  const bool spec = h->spec > 0 && h->use_graphs && h->hs[0].dump_cap == 0;
+  // no speculation while pairs are dumped (a replayed chunk would record its pairs twice) or on the
+  // GMRES backend (its steps have no lean form)
+  const bool spec = h->spec > 0 && h->use_graphs && h->hs[0].dump_cap == 0 && h->pbackend == MCF_PB_DIRECT;
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
   for (int done = 0; done < n;) {
     const bool lean = spec && h->spec_full == 0;
@@ -2124,17 +2748,16 @@ int mcf_project(mcf_handle* h, const double* q, double dt, double* out, double* 
   if (phi) HIPCHK(h, hipMemcpy(phi, h->hc.buf[B_TP0], h->psize * sizeof(double), hipMemcpyDeviceToHost));
   std::vector<double> sv;
   if (int rc = read_slots(h, sv)) return rc;
-  auto sd = [&](int k) { return sv[k]; };
-  const double val = sd(SL_P0 + SV_VAL);
-  const double t1 = std::max(1e-10, 32 * kEps * std::max(1.0, val));
-  const double r1 = sd(SL_P0 + SV_RES1), r2 = sd(SL_P0 + SV_RES2);
-  const double fin = sd(SL_P0 + SV_FINAL), rnd = sd(SL_P0 + SV_ROUND);
-  const double target = std::max(std::max(1e-10, 64 * kEps * std::max(1.0, val)), rnd);
-  const double div = sd(SL_DIV0);
-  if (info4) { info4[0] = fin; info4[1] = (r1 > t1) + (r1 > t1 && r2 > t1); info4[2] = div; info4[3] = target; }
+  double rec[MCF_SOLVE_LEN];
+  solve_record(slots_of(sv, 0), h->pbackend == MCF_PB_GMRES, rec);
+  if (h->kry) h->kry->seconds.clear();
+  const double fin = rec[MCF_S_RES_LINF];
+  const double div = sv[SL_DIV0];
+  if (info4) { info4[0] = fin; info4[1] = rec[MCF_S_REFINE]; info4[2] = div; info4[3] = rec[MCF_S_TARGET]; }
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (!(fin <= target)) return fail(h, 2, "Pressure solve failed: residual " + std::to_string(fin));
+  if (!solve_ok(rec)) return fail(h, 2, "Pressure solve failed: residual " + std::to_string(fin) +
+                                  " status " + std::to_string((int)rec[MCF_S_STATUS]));
   if (!(div <= 1e-10)) {
     char buf[128];
     std::snprintf(buf, sizeof buf, "Ghost projection failed: divergence=%.6g", div);
@@ -2253,6 +2876,85 @@ int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, dou
   return 0;
 }
 
+int mcf_set_pressure_backend(mcf_handle* h, int backend, int restart, int max_cycles, double rtol, double atol) {
+  if (!h) return 1;
+  if ((backend != MCF_PB_DIRECT && backend != MCF_PB_GMRES) || restart < 1 || restart > KR_MAXR || max_cycles < 1 ||
+      !(rtol >= 0) || !(atol >= 0) || !std::isfinite(rtol) || !std::isfinite(atol))
+    return fail(h, 1, "invalid pressure backend settings");
+  if (h->stream) HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (backend == MCF_PB_GMRES) {
+    if (!h->kry) h->kry = new Krylov();
+    Krylov& K = *h->kry;
+    const size_t ldv = ((size_t)h->psize + 63) / 64 * 64;
+    if (!K.V || K.restart != restart) {
+      if (K.V) (void)hipFree(K.V);
+      K.V = nullptr;
+      HIPCHK(h, hipMalloc(&K.V, (size_t)(restart + 1) * ldv * sizeof(double)));
+      HIPCHK(h, hipMemset(K.V, 0, (size_t)(restart + 1) * ldv * sizeof(double)));
+    }
+    K.ldv = ldv;
+    K.nblk = (h->psize + KR_CHUNK - 1) / KR_CHUNK;
+    if (!K.part) HIPCHK(h, hipMalloc(&K.part, (size_t)(KR_MAXR + 2) * K.nblk * sizeof(double)));
+    if (!K.ctl) HIPCHK(h, hipMalloc(&K.ctl, sizeof(KryCtl)));
+    if (!K.stat) HIPCHK(h, hipHostMalloc(&K.stat, KR_STAT_INTS * sizeof(int)));
+    K.restart = restart; K.maxiter = max_cycles; K.rtol = rtol; K.atol = atol;
+    KryCtl init{};
+    init.restart = restart; init.maxiter = max_cycles; init.rtol = rtol; init.atol_in = atol;
+    HIPCHK(h, hipMemcpy(K.ctl, &init, sizeof(KryCtl), hipMemcpyHostToDevice));
+    h->hc.buf[B_KV0] = K.V;
+  }
+  h->pbackend = backend;
+  return sync_descs(h);
+}
+
+int mcf_get_pressure_backend(mcf_handle* h, int* backend) {
+  if (!h || !backend) return 1;
+  *backend = h->pbackend;
+  return 0;
+}
+
+int mcf_pressure_solve(mcf_handle* h, const double* rhs, double* out, double* record) {
+  if (!h || !rhs || !out) return fail(h, 1, "invalid pressure solve arguments");
+  if (!h->has_rect[0] || !h->has_schur[0] || !h->has_solid[2]) return fail(h, 1, "pressure factor missing");
+  // the values on the active cells must be finite (pressure_schur.py:35-36); their max sets the targets
+  const HostTables& ht = g_host[h];
+  double vmax = 0.0;
+  for (int j = 0; j < h->ny; ++j)
+    for (int i = 0; i < h->nx; ++i) {
+      if (ht.lo[2][j] <= i && i <= ht.hi[2][j]) continue;
+      const double v = rhs[(size_t)j * h->nx + i];
+      if (!std::isfinite(v)) return fail(h, 1, "Pressure RHS must be finite");
+      vmax = std::max(vmax, std::fabs(v));
+    }
+  hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
+  HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TP0], rhs, h->psize * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  std::vector<double> s0(NSLOTW, 0.0);
+  s0[(size_t)(SL_P0 + SV_VAL) * NSTRIPE] = vmax;
+  HIPCHK(h, hipMemcpyAsync(h->slots, s0.data(), NSLOTW * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  set_prm_once(h, 1.0);
+  if (int rc = enqueue_solve(ctx_of(h), true, B_TP0, B_TP1, 0)) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(out, h->hc.buf[B_TP1], h->psize * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> sv;
+  if (int rc = read_slots(h, sv)) return rc;
+  double rec[MCF_SOLVE_LEN];
+  solve_record(slots_of(sv, 0), h->pbackend == MCF_PB_GMRES, rec);
+  if (h->kry) {
+    if (!h->kry->seconds.empty()) rec[MCF_S_SECONDS] = h->kry->seconds.back();
+    h->kry->seconds.clear();
+  }
+  if (record) std::memcpy(record, rec, sizeof rec);
+  hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (!solve_ok(rec)) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "Pressure solve failed: residual_linf=%g target=%g status=%d iterations=%d",
+                  rec[MCF_S_RES_LINF], rec[MCF_S_TARGET], (int)rec[MCF_S_STATUS], (int)rec[MCF_S_ITERATIONS]);
+    return fail(h, 2, buf);
+  }
+  return 0;
+}
+
 int mcf_apply_operator(mcf_handle* h, int op, const double* in, double* out, double dt) {
   if (!h || !in || !out || op < 0 || op > 4) return fail(h, 1, "invalid operator arguments");
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
@@ -2326,7 +3028,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   // one untimed step settles the history so that the timed steps replay the captured graph; with
   // speculation on, the timed steps are the lean ones (a step that would have refined is counted
   // in out5[4], not replayed: the state is not used after profiling)
-  const bool lean = h->spec > 0 && h->use_graphs;
+  const bool lean = h->spec > 0 && h->use_graphs && h->pbackend == MCF_PB_DIRECT;
   if (int rc = enqueue_steps(h, 1, dt, lean)) return rc;
   HIPCHK(h, hipEventRecord(e0, h->stream));
   if (int rc = enqueue_steps(h, n, dt, lean)) return rc;
@@ -2469,6 +3171,8 @@ int mcf_batch_create(mcf_handle* const* cases, const double* dts, int n, mcf_bat
     const double dt = dts[c];
     if (!(dt > 0)) return fail(e, 1, "case " + std::to_string(c) + ": dt must be positive");
     if (int rc = check_ready(h, dt)) { e->err = "case " + std::to_string(c) + ": " + h->err; return rc; }
+    if (h->pbackend != MCF_PB_DIRECT)
+      return fail(e, 1, "case " + std::to_string(c) + ": the GMRES pressure backend is not batched");
     if (h->parity != cases[0]->parity || h->has_history != cases[0]->has_history)
       return fail(e, 1, "batched cases must share the step parity and history state");
     if (h->has_history && h->prev_dt != dt) return fail(e, 1, "batched cases must continue at their previous dt");

@@ -60,6 +60,14 @@ def pressure_pieces(ops):
     return factor, SchurFactor(ops.A, ops.pi, factor)
 
 
+def solve_record(rec):
+    """The reference's ``last_solve`` dict (pressure_schur.py:64-68) from one MCF_SOLVE_LEN record."""
+    d = {name: float(rec[k]) for k, name in enumerate(nat.SOLVE_NAMES)}
+    for key in ('iterations', 'refinement_iterations', 'status'):
+        d[key] = int(d[key])
+    return d
+
+
 class PreparedCase:
     """A case's recorded table uploads, prepared on the host without a device (picklable, compact):
     the entry point calls with their arrays, the index layout and the solver info."""
@@ -369,6 +377,47 @@ class GhostDevice:
         return out
 
     # ---------------------------------------------------------------- launch shapes
+    # ---------------------------------------------------------------- pressure backend
+    def set_pressure_backend(self, iterative, restart=50, max_cycles=100, rtol=1e-12, atol=1e-12):
+        """Select the pressure backend (pressure_schur.py:16-31): the boundary-row corrected rectangle
+        inverse, or restarted GMRES on the pressure operator left-preconditioned by the rectangle
+        inverse (the reference's arguments: restart 50, 100 cycles, rtol = atol = 1e-12)."""
+        if not hasattr(self, '_direct_info'):
+            self._direct_info = dict(self.pressure_info)
+        self.h.call('mcf_set_pressure_backend', nat.PB_GMRES if iterative else nat.PB_DIRECT, int(restart),
+                    int(max_cycles), float(rtol), float(atol))
+        # updated in place: the integrator's pressure_inverse_info is this same dict
+        self.pressure_info.clear()
+        if iterative:
+            self.pressure_info.update(backend='gmres_rectangle', preconditioner='separable_rectangle',
+                                      krylov_rtol=float(rtol), krylov_atol=float(atol), restart=int(restart),
+                                      max_cycles=int(max_cycles))
+        else:
+            self.pressure_info.update(self._direct_info)
+
+    @property
+    def pressure_backend(self):
+        out = np.zeros(1, np.int32)
+        self.h.call('mcf_get_pressure_backend', out)
+        return int(out[0])
+
+    def pressure_solve(self, values):
+        """One monitored pressure solve (the reference's ``pressure_solve(values)``,
+        pressure_schur.py:33-71) of the values on the active cells, in ``pi`` order.  Returns the
+        solution; the solve's record is ``pressure_info['last_solve']``, also when the solve fails."""
+        o = self.ops
+        values = np.asarray(values, dtype=float)
+        if values.shape != (len(o.pi),):
+            raise ValueError('Pressure RHS must hold one value per active cell')
+        out = np.zeros(o.nx * o.ny)
+        rec = np.zeros(nat.SOLVE_LEN)
+        try:
+            self.h.call('mcf_pressure_solve', self.q_to_p(values), out, rec)
+        finally:
+            if rec.any():
+                self.pressure_info['last_solve'] = solve_record(rec)
+        return self.p_to_q(out)
+
     def set_launch(self, gemm=-1, tri=-1):
         """Pin the transform tile / sweep width variants (-1, -1: autotune at the next run)."""
         self.h.call('mcf_set_launch', int(gemm), int(tri))

@@ -19,7 +19,7 @@ import numpy as np
 import scipy
 
 from . import _native as nat
-from .device import GhostDevice, PreparedCase
+from .device import GhostDevice, PreparedCase, solve_record
 from .tables import GhostOperators
 
 SOURCE_FILES = ('flow.py', 'device.py', 'tables.py', '_native.py', 'csrc/ghostflow.hip')
@@ -59,13 +59,15 @@ class GhostCellIntegrator:
                 pressure = None
             self.device = GhostDevice(ops, s.u_inlet, pressure=pressure)
             self.device.set_reference_velocity(s.U_ref)
+        # optimized_ghost_cell.py:109-111: GMRES preconditioned by the rectangle inverse when iterative
+        self.device.set_pressure_backend(s.pressure_solver == 'iterative')
         self.ops = ops
         for name in ('ui', 'vi', 'pi', 'qsize', 'E', 'P', 'G', 'A', 'L', 'Dfull', 'offset', 'div_offset',
                      'diff_offset', 'outlet_u', 'du_dn', 'dv_dn', 'pressure_surface', 'normals',
                      'normal_weights', 'surface_points', 'usize'):
             setattr(self, name, getattr(ops, name))
         self.D = ops.Dop
-        self.pressure_inverse_info = dict(self.device.pressure_info)
+        self.pressure_inverse_info = self.device.pressure_info
         self.helmholtz_inverse_info = []
         self.previous_advection = self.previous_dt = self.half_pressure = None
         self.last_diagnostics = {}
@@ -114,6 +116,11 @@ class GhostCellIntegrator:
             return out
         return r
 
+    def pressure_solve(self, values):
+        """Monitored pressure solve of values on the active cells (pressure_schur.py:33-71); the
+        record lands in ``pressure_inverse_info['last_solve']``."""
+        return self.device.pressure_solve(values)
+
     def project(self, q, dt, out=None):
         q_out, phi, info = self.device.project(q, dt)
         if out is not None:
@@ -132,32 +139,33 @@ class GhostCellIntegrator:
         st = self.device.get_state()
         self.previous_advection, self.half_pressure, self.previous_dt = (st['previous_advection'],
                                                                          st['half_pressure'], st['previous_dt'])
-        self.last_diagnostics = diagnostics_from_row(diag)
+        self.last_diagnostics = diagnostics_from_row(diag, self.pressure_inverse_info['backend'])
+        self.pressure_inverse_info['last_solve'] = self.last_diagnostics['pressure_solves'][-1]
         self.helmholtz_inverse_info = self.device.helmholtz_inverse_info
         return st['u'], st['v'], st['p']
 
 
-def diagnostics_from_row(row):
-    """The reference's per-step diagnostics dict (optimized_ghost_cell.py:266-276; main.py:409-413)."""
+def diagnostics_from_row(row, backend='fft_boundary_schur'):
+    """The reference's per-step diagnostics dict (optimized_ghost_cell.py:266-276; main.py:409-413):
+    one ``last_solve`` record per pressure solve of the step, in call order, and their aggregates.
+    Per-solve seconds are wall times of the host-driven GMRES solves; solves replayed inside a step
+    graph report 0."""
     d = {n: float(row[k]) for k, n in enumerate(nat.DIAG_NAMES) if not n.startswith('unused')}
-    out = dict(momentum_iterations=0, momentum_residual_linf=d['momentum_residual_linf'],
-               coupled_momentum_residual_linf=d['coupled_momentum_residual_linf'],
-               pressure_backend='fft_boundary_schur', pressure_solve_count=int(d['pressure_solve_count']),
-               pressure_iterations=0, pressure_refinement_iterations=int(d['pressure_refinement_iterations']),
-               pressure_residual_linf=d['pressure_residual_linf'],
-               pressure_residual_relative_l2=d['pressure_residual_relative_l2'],
-               pressure_seconds=0.0, pressure_solves=[],
-               helmholtz_refinement_iterations=int(d['helmholtz_refinement_iterations']),
-               time=d['time'], dt=d['dt'], max_divergence=d['max_divergence'], cfl=d['cfl'], cd=d['cd'], cl=d['cl'],
-               cd_pressure=d['cd_pressure'], cd_viscous=d['cd_viscous'], cl_pressure=d['cl_pressure'],
-               cl_viscous=d['cl_viscous'])
-    out['pressure_solves'] = [dict(iterations=0, residual_linf=d['pressure_residual_linf'],
-                                   residual_relative_l2=d['pressure_residual_relative_l2'],
-                                   refinement_iterations=int(d['pressure_refinement_iterations']),
-                                   residual_target_linf=d['pressure_residual_target_linf'],
-                                   roundoff_bound_linf=d['pressure_roundoff_bound_linf'],
-                                   backward_error_linf=d['pressure_backward_error_linf'], status=0, seconds=0.0)]
-    return out
+    count = int(d['pressure_solve_count'])
+    solves = [solve_record(row[nat.DIAG_SOLVE0 + nat.SOLVE_LEN * q:nat.DIAG_SOLVE0 + nat.SOLVE_LEN * (q + 1)])
+              for q in range(min(count, nat.MAX_SOLVES))]
+    return dict(momentum_iterations=0, momentum_residual_linf=d['momentum_residual_linf'],
+                coupled_momentum_residual_linf=d['coupled_momentum_residual_linf'],
+                pressure_backend=backend, pressure_solve_count=count,
+                pressure_iterations=sum(c['iterations'] for c in solves),
+                pressure_refinement_iterations=sum(c['refinement_iterations'] for c in solves),
+                pressure_residual_linf=max((c['residual_linf'] for c in solves), default=0.0),
+                pressure_residual_relative_l2=max((c['residual_relative_l2'] for c in solves), default=0.0),
+                pressure_seconds=sum(c['seconds'] for c in solves), pressure_solves=solves,
+                helmholtz_refinement_iterations=int(d['helmholtz_refinement_iterations']),
+                time=d['time'], dt=d['dt'], max_divergence=d['max_divergence'], cfl=d['cfl'], cd=d['cd'], cl=d['cl'],
+                cd_pressure=d['cd_pressure'], cd_viscous=d['cd_viscous'], cl_pressure=d['cl_pressure'],
+                cl_viscous=d['cl_viscous'])
 
 
 class MAC_CylinderFlow:
@@ -181,7 +189,7 @@ class MAC_CylinderFlow:
             raise ValueError('Unknown Ghost pressure extension or velocity interpolation')
         if boundary != 'ghost_cell' or time_scheme != 'ab2_cn':
             raise NotImplementedError('This GPU build provides the ghost_cell / ab2_cn path only')
-        if pressure_solver != 'direct' or slip_walls:
+        if pressure_solver not in ('direct', 'iterative') or slip_walls:
             raise ValueError('Ghost AB2/CN requires direct/iterative pressure and no-slip channel walls')
         if any(isinstance(n, bool) or not isinstance(n, (int, np.integer)) or n < 4 for n in (nx, ny)):
             raise ValueError('nx and ny must be integers >= 4')
@@ -328,14 +336,15 @@ class MAC_CylinderFlow:
                 if step_callback is not None:
                     self._pull()
                     self.time = float(diag[-1, 0])
-                    self.last_step_diagnostics = diagnostics_from_row(diag[-1])
+                    self.last_step_diagnostics = diagnostics_from_row(diag[-1], dev.pressure_info['backend'])
                     step_callback(self, self.last_step_diagnostics.copy())
         finally:
             self._pull()
         diag = np.concatenate(rows) if rows else np.zeros((0, nat.DIAG_LEN))
         self.time = float(diag[-1, 0]) if len(diag) else self.time
-        self.last_step_diagnostics = diagnostics_from_row(diag[-1])
+        self.last_step_diagnostics = diagnostics_from_row(diag[-1], dev.pressure_info['backend'])
         self.gg.last_diagnostics = self.last_step_diagnostics
+        self.gg.pressure_inverse_info['last_solve'] = self.last_step_diagnostics['pressure_solves'][-1]
         self.diagnostics_history = diag
         return diag[:, 0].copy(), diag[:, 2].copy(), diag[:, 3].copy()
 
@@ -407,7 +416,8 @@ class MAC_CylinderFlow:
                     slip_walls=self.slip_walls, dt=dt, n_steps=n_steps, end_time=self.time, boundary=self.boundary,
                     transport=self.transport, time_integrator='AB2_CN_discrete_incremental',
                     time_scheme=self.time_scheme, pressure_solver=self.pressure_solver,
-                    pressure_backend='ghost_fft_boundary_schur', ghost_pressure_extension=self.ghost_pressure_extension,
+                    pressure_backend='ghost_' + self.gg.pressure_inverse_info['backend'],
+                    ghost_pressure_extension=self.ghost_pressure_extension,
                     ghost_velocity_interpolation=self.ghost_velocity_interpolation,
                     pressure_preconditioner={k: v for k, v in self.gg.pressure_inverse_info.items()},
                     device=nat.device_name(), perturb_transient=perturb_transient, perturb_duration=perturb_duration,

@@ -23,6 +23,10 @@
  *   mcf_rect_inverse    rectangle(...)[1] (the separable solve) solver/rectangular_inverse.py:48-53
  *   mcf_schur_solve     boundary_row_inverse(...)[0] / monitored_pressure_solver(...)[0]
  *                                                               solver/pressure_schur.py:16-72, 103-124
+ *   mcf_pressure_solve  monitored_pressure_solver(...)[0] with either backend
+ *                                                               solver/pressure_schur.py:16-72
+ *   mcf_set_pressure_backend  pressure_solver='direct' | 'iterative' for the ghost path
+ *                                                               solver/main.py:40-56, optimized_ghost_cell.py:109-111
  *   mcf_divergence_stats calculate_mac_divergence_cpp           solver/src/bindings.cpp:343-390
  *   mcf_set_* / mcf_get_state   construction of GhostCellIntegrator and its history attributes
  *                                                               solver/optimized_ghost_cell.py:62-150,
@@ -52,7 +56,14 @@ enum {
   MCF_D_P_RES0, MCF_D_P_RES1, MCF_D_P_REL0, MCF_D_P_REL1, MCF_D_P_REFINE, MCF_D_P_TARGET0, MCF_D_P_TARGET1,
   MCF_D_P_ROUND0, MCF_D_P_ROUND1, MCF_D_P_BACKWARD0, MCF_D_P_BACKWARD1, MCF_D_H_REFINE,
   MCF_D_STATUS, MCF_D_RHS_MAX, MCF_D_P_SOLVES,
-  MCF_DIAG_LEN = 32
+  MCF_D_SOLVE0 = 32,   /* MCF_SOLVE_LEN doubles per pressure solve of the step, in call order (up to 3) */
+  MCF_DIAG_LEN = 64
+};
+/* Record of one monitored pressure solve (pressure_schur.py:64-69, `info['last_solve']`). */
+enum {
+  MCF_S_ITERATIONS = 0, MCF_S_RES_LINF, MCF_S_REL_L2, MCF_S_REFINE, MCF_S_TARGET, MCF_S_ROUND,
+  MCF_S_BACKWARD, MCF_S_STATUS, MCF_S_SECONDS, MCF_S_SPARE,
+  MCF_SOLVE_LEN = 10
 };
 
 /* Creation: grid, physics and the inlet profile (u at x=0, length ny). */
@@ -114,6 +125,23 @@ int mcf_rect_inverse(mcf_handle* h, int kind, const double* rhs, double* out);
 /* Boundary-row corrected solve of the embedded operator (values on the active nodes given as
    a q-array in the kind's full array layout); info4 = residual_linf, refinements, roundoff, target. */
 int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, double* info4);
+/* Pressure backends (pressure_schur.py:16-72):
+   MCF_PB_DIRECT  boundary-row (Woodbury) corrected rectangle inverse, <= 2 refinements ('fft_boundary_schur');
+   MCF_PB_GMRES   restarted GMRES on the pressure operator, left-preconditioned by the rectangle inverse,
+                  with <= 2 refinement solves and the same residual gate ('gmres_rectangle').
+   GMRES follows SciPy's gmres (the reference's call: restart 50, at most 100 restart cycles,
+   rtol = atol = 1e-12, the adaptive inner tolerance on the preconditioned residual, stop on the true
+   residual norm); its orthogonalisation is classical Gram-Schmidt applied twice.  restart <= 64,
+   max_cycles >= 1, rtol >= 0, atol >= 0.  Steps of a handle on the GMRES backend run without step
+   graphs or speculation and cannot join a batch. */
+enum { MCF_PB_DIRECT = 0, MCF_PB_GMRES = 1 };
+int mcf_set_pressure_backend(mcf_handle* h, int backend, int restart, int max_cycles, double rtol, double atol);
+int mcf_get_pressure_backend(mcf_handle* h, int* backend);
+/* One monitored pressure solve on the selected backend (the reference's `pressure_solve(values)`):
+   rhs and out are p-arrays (ny, nx), values on the active cells (others ignored / returned 0).
+   record receives MCF_SOLVE_LEN doubles (may be NULL).  Status 1 for a non-finite rhs, 2 when the
+   solve fails its gate or GMRES does not converge (the record is still written). */
+int mcf_pressure_solve(mcf_handle* h, const double* rhs, double* out, double* record);
 /* Operators on packed-equivalent arrays (optimized_ghost_cell.py:186-216, 231-234):
    ADVECTION -N(q), DIFFUSION L q + diff_offset (vel -> vel), GRADIENT G p (p -> vel),
    DIVERGENCE D q + div_offset (vel -> p), HMATVEC q - dt/2 L q (vel -> vel). */

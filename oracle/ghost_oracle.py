@@ -40,6 +40,7 @@ import numpy as np
 import scipy.sparse as sp
 from scipy.fft import dct, idct, dst, idst
 from scipy.linalg import lu_factor, lu_solve, LinAlgWarning
+from scipy.sparse.linalg import LinearOperator, gmres
 
 EPS = np.finfo(float).eps
 
@@ -405,32 +406,69 @@ def boundary_row_inverse(actual, indices, rect, inverse, shape):
     return solve, details
 
 
-def monitored_pressure_solver(actual, indices, rect, inverse, shape):
-    """Direct backend of the monitored pressure solve (pressure_schur.py:16-72)."""
+def monitored_pressure_solver(actual, indices, rect, inverse, shape, iterative=False):
+    """Monitored pressure solve (pressure_schur.py:16-72).
+
+    Direct: the boundary-row corrected rectangle inverse.  Iterative: SciPy's GMRES on ``actual``,
+    left-preconditioned by the rectangle inverse of the zero-filled rectangle right-hand side
+    restricted to ``indices``, with the reference's arguments (restart 50, 100 cycles,
+    rtol = atol = 1e-12, Arnoldi steps counted by the pr_norm callback), then at most two
+    refinement solves on the true residual while it exceeds the target.  Both end with the same
+    true-residual gate.  ``gmres`` is looked up in this module at call time.
+    """
+    indices = np.asarray(indices)
     absolute = abs(actual).tocsr()
     roundoff_weights = (2 * np.diff(absolute.indptr) + 2) * EPS
-    direct, info = boundary_row_inverse(actual, indices, rect, inverse, shape)
-    info['backend'] = 'fft_boundary_schur'
+    if iterative:
+        def precondition(values):
+            full = np.zeros(shape)
+            full.ravel()[indices] = values
+            return inverse(full).ravel()[indices]
+        preconditioner = LinearOperator(actual.shape, matvec=precondition, dtype=float)
+        info = dict(backend='gmres_rectangle', preconditioner='separable_rectangle', krylov_rtol=1e-12,
+                    krylov_atol=1e-12)
+    else:
+        direct, info = boundary_row_inverse(actual, indices, rect, inverse, shape)
+        info['backend'] = 'fft_boundary_schur'
 
     def solve(values):
         started = time.perf_counter()
         if not np.isfinite(values).all():
             raise ValueError('Pressure RHS must be finite')
-        result = direct(values)
+        steps = [0]
+
+        def counted(_):
+            steps[0] += 1
+
+        def krylov(rhs):
+            return gmres(actual, rhs, M=preconditioner, rtol=info['krylov_rtol'], atol=info['krylov_atol'],
+                         restart=50, maxiter=100, callback=counted, callback_type='pr_norm')
+
+        if iterative:
+            result, status = krylov(values)
+        else:
+            result, status = direct(values), 0
         residual = values - actual @ result
         target = max(1e-10, 64 * EPS * max(1., float(np.max(abs(values)))))
         refinements = info.get('last_refinement_iterations', 0)
+        if iterative:
+            refinements = 0
+            while refinements < 2 and not status and np.max(abs(residual)) > target:
+                correction, status = krylov(residual)
+                result = result + correction
+                residual = values - actual @ result
+                refinements += 1
         linf = float(np.max(abs(residual)))
         relative = float(np.linalg.norm(residual) / max(np.linalg.norm(values), np.finfo(float).tiny))
         action_scale = absolute @ abs(result) + abs(values)
         roundoff_bound = float(np.max(roundoff_weights * action_scale))
         target = max(target, roundoff_bound)
-        info['last_solve'] = dict(iterations=0, residual_linf=linf, residual_relative_l2=relative,
+        info['last_solve'] = dict(iterations=steps[0], residual_linf=linf, residual_relative_l2=relative,
                                   refinement_iterations=refinements, residual_target_linf=target,
                                   roundoff_bound_linf=roundoff_bound,
                                   backward_error_linf=linf / max(float(np.max(action_scale)), np.finfo(float).tiny),
-                                  status=0, seconds=time.perf_counter() - started)
-        if not np.isfinite(result).all() or linf > target:
+                                  status=int(status), seconds=time.perf_counter() - started)
+        if status or not np.isfinite(result).all() or linf > target:
             raise RuntimeError(f'Pressure solve failed: {info["last_solve"]}')
         return result
 
@@ -599,7 +637,7 @@ class GhostIntegrator:
         self.A = (self.D @ self.G).tocsc()
         rect, inverse, shape = rectangle(nx, ny, dx, dy, 'pressure', rho=s.rho, workers=fft_workers)
         self.pressure_solve, self.pressure_inverse_info = monitored_pressure_solver(
-            self.A, self.pi, rect, inverse, shape)
+            self.A, self.pi, rect, inverse, shape, iterative=s.pressure_solver == 'iterative')
         self._pressure_calls = []
         # L = nu raw_l E (optimized_ghost_cell.py:113-128)
         rows, cols, values = [], [], []
@@ -767,8 +805,8 @@ class CylinderFlow:
                  fft_workers=None):
         if boundary != 'ghost_cell' or time_scheme != 'ab2_cn' or transport != 'centered':
             raise ValueError('The oracle covers boundary=ghost_cell, time_scheme=ab2_cn, centered transport')
-        if pressure_solver != 'direct' or slip_walls:
-            raise ValueError('Ghost AB2/CN requires the direct pressure solver and no-slip walls')
+        if pressure_solver not in ('direct', 'iterative') or slip_walls:
+            raise ValueError('Ghost AB2/CN requires direct/iterative pressure and no-slip walls')
         if ghost_pressure_extension not in ('neumann', 'quadratic'):
             raise ValueError('Unknown Ghost pressure extension')
         if any(isinstance(n, bool) or not isinstance(n, (int, np.integer)) or n < 4 for n in (nx, ny)):
@@ -781,6 +819,7 @@ class CylinderFlow:
             raise ValueError('Cylinder must lie strictly inside the channel')
         self.ghost_pressure_extension = ghost_pressure_extension
         self.ghost_velocity_interpolation = ghost_velocity_interpolation
+        self.pressure_solver = pressure_solver
         self.configuration = dict(nx=nx, ny=ny, Lx=Lx, Ly=Ly, D=D, xc=xc, yc=yc, nu=nu, rho=rho,
                                   U_inf=U_inf, inflow=inflow, U_m=U_m, slip_walls=slip_walls,
                                   boundary=boundary, transport=transport, pressure_solver=pressure_solver,
