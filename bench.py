@@ -31,8 +31,10 @@ STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB
 GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6}
 
 
-def cpu_baseline(seconds=20.0):
-    """The oracle (NumPy/SciPy restatement) on this host's cores, on a bounded sample of steps."""
+def cpu_baseline(seconds=20.0, gmres_seconds=0.0):
+    """The oracle (NumPy/SciPy restatement) on this host's cores, on a bounded sample of steps; with
+    gmres_seconds, then a bounded sample of the same case on the oracle's GMRES backend (SciPy gmres
+    preconditioned by the rectangle inverse).  Returns (baseline, gmres sample or None)."""
     from oracle.ghost_oracle import CylinderFlow
     cores = len(os.sched_getaffinity(0))
     s = CylinderFlow(initial='profile', v_perturbation=dict(seed=SEED), fft_workers=cores, **WORKLOAD)
@@ -46,9 +48,48 @@ def cpu_baseline(seconds=20.0):
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
     cells = WORKLOAD['nx'] * WORKLOAD['ny']
-    return dict(value=cells / med, unit='cell-steps/s', cores=cores, kind='port',
+    base = dict(value=cells / med, unit='cell-steps/s', cores=cores, kind='port',
                 sample=f'{len(times)} steps of the 1320x246 Re=100 case after 2 warm-up steps, median step time '
                        f'{med * 1e3:.1f} ms (oracle/ghost_oracle.py, scipy.fft workers={cores}, single-threaded SpMV)')
+    if not gmres_seconds:
+        return base, None
+    s.gg.set_pressure_solver('iterative')
+    times, arnoldi = [], []
+    start = time.perf_counter()
+    while time.perf_counter() - start < gmres_seconds and len(times) < 50:
+        t0 = time.perf_counter()
+        s.step(DT)
+        times.append(time.perf_counter() - t0)
+        d = s.last_step_diagnostics
+        arnoldi.append(d['pressure_iterations'] / d['pressure_solve_count'])
+    med = float(np.median(times))
+    return base, dict(ms_per_step=med * 1e3, value=cells / med, steps=len(times), cores=cores,
+                      arnoldi_steps_per_solve=float(np.mean(arnoldi)),
+                      sample=f'{len(times)} GMRES-backend steps of the same case after the baseline sample, median')
+
+
+def gmres_leg(dev, steps):
+    """The bench case on the GMRES pressure backend (pressure_solver='iterative': restarted GMRES on
+    the pressure operator, left-preconditioned by the rectangle inverse, <= 2 refinements).  It
+    continues from the plain leg's state; GMRES steps run eagerly (the host reads the solver state
+    every 8 Arnoldi steps), so the wall clock around the synchronised run is the measure."""
+    import importlib
+    nat = importlib.import_module(type(dev).__module__.rsplit('.', 1)[0] + '._native')
+    dev.set_pressure_backend(True)
+    try:
+        dev.run(2, DT)
+        t0 = time.perf_counter()
+        diag = dev.run(steps, DT)
+        elapsed = time.perf_counter() - t0
+    finally:
+        dev.set_pressure_backend(False)
+    col = nat.DIAG_NAMES.index('pressure_solve_count')
+    its = [diag[s, nat.DIAG_SOLVE0 + nat.SOLVE_LEN * q] for s in range(steps) for q in range(int(diag[s, col]))]
+    secs = [diag[s, nat.DIAG_SOLVE0 + nat.SOLVE_LEN * q + 8] for s in range(steps) for q in range(int(diag[s, col]))]
+    cells = WORKLOAD['nx'] * WORKLOAD['ny']
+    return dict(value=cells * steps / elapsed, unit='cell-steps/s', ms_per_step=elapsed / steps * 1e3, steps=steps,
+                arnoldi_steps_per_solve=float(np.mean(its)), solves_per_step=len(its) / steps,
+                ms_per_solve=float(np.mean(secs)) * 1e3, max_divergence=float(np.max(diag[:, 10])))
 
 
 def ensemble_leg(pkg, steps):
@@ -163,6 +204,8 @@ def main():
                          '--steps steps of the workload; prints nothing else')
     ap.add_argument('--launch', type=int, nargs=2, default=None, metavar=('TILE', 'WIDTH'),
                     help='pin the transform tile and sweep width variants (traffic probe)')
+    ap.add_argument('--gmres-steps', type=int, default=20,
+                    help='steps of the GMRES pressure backend leg (0: skip)')
     ap.add_argument('--ensemble-long', type=int, default=0,
                     help='only run the 64-case ensemble for this many steps and print its summary')
     args = ap.parse_args()
@@ -223,6 +266,9 @@ def main():
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
                kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms, lean_misses=lean_misses),
                speculation=dev.get_speculation(), setup_seconds=setup_s)
+    if args.gmres_steps:
+        print(f'bench: GMRES pressure backend, {args.gmres_steps} steps', file=sys.stderr, flush=True)
+        out['pressure_iterative'] = gmres_leg(dev, args.gmres_steps)
     out['ensemble'] = None if args.no_ensemble else ensemble_leg(pkg, args.ensemble_steps)
     if not args.no_warm_start:
         import importlib
@@ -245,7 +291,9 @@ def main():
             seconds=res['seconds'])
     if not args.no_cpu_baseline:
         print('bench: CPU baseline (oracle)', file=sys.stderr, flush=True)
-        out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)
+        out['cpu_baseline'], cpu_gmres = cpu_baseline(args.cpu_seconds, 15.0 if args.gmres_steps else 0.0)
+        if cpu_gmres is not None:
+            out['pressure_iterative']['cpu'] = cpu_gmres
     else:
         out['cpu_baseline'] = None
     print(json.dumps(out))

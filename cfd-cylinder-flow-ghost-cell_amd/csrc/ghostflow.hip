@@ -1236,7 +1236,7 @@ namespace {
 // which saves a normalisation pass per step.
 // ------------------------------------------------------------------------------------
 constexpr int KR_MAXR = 64;                            // longest restart cycle
-constexpr int KR_T = 256, KR_E = 4, KR_CHUNK = KR_T * KR_E;   // vector kernels: elements per block
+constexpr int KR_T = 256, KR_E = 2, KR_CHUNK = KR_T * KR_E;   // vector kernels: elements per block
 constexpr int KR_CT = 1024;                            // threads of the control kernel
 
 struct KryCtl {
@@ -1683,6 +1683,14 @@ struct Krylov {
   int nblk = 0;
   int* stat = nullptr;            // pinned host copy of the status words
   std::vector<double> seconds;    // wall time of each monitored solve since the last read, in call order
+  hipGraphExec_t group = nullptr; // captured group of Arnoldi steps
+  hipGraph_t group_graph = nullptr;
+  int group_gemm = -1, group_tri = -1;   // launch shapes it was captured with
+  void drop_group() {
+    if (group) (void)hipGraphExecDestroy(group);
+    if (group_graph) (void)hipGraphDestroy(group_graph);
+    group = nullptr; group_graph = nullptr; group_gemm = group_tri = -1;
+  }
 };
 
 struct mcf_handle {
@@ -1873,17 +1881,61 @@ void kry_precondition(const Ctx& x, int in_id, int out_id, const int* gate) {
   launch_gemm(x.ln.gemm_p, x.p_yy, x.p_xx, 1, x.stream, x.dsp, gi);
 }
 
+// KR_GROUP Arnoldi steps: each reads the column index and the activity flag from the control
+// state, so the same launches serve every step of every cycle
+void enqueue_arnoldi_group(const Ctx& x, const Krylov& K) {
+  const KryArgs a = kry_args(K);
+  const dim3 gb(K.nblk), tb(KR_T), gc(1), tc(KR_CT);
+  hipStream_t st = x.stream;
+  const int* gate = &K.ctl->active;
+  for (int s = 0; s < KR_GROUP; ++s) {
+    hipLaunchKernelGGL(k_kry_matvec, gb, tb, 0, st, x.dc, a, (int)B_KA);
+    kry_precondition(x, B_KA, B_KB, gate);
+    hipLaunchKernelGGL(k_kry_dots, gb, tb, 0, st, x.dc, a, (int)B_KB);
+    hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_DOTS1);
+    hipLaunchKernelGGL(k_kry_orth, gb, tb, 0, st, x.dc, a, (int)B_KB, 1);
+    hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_DOTS2);
+    hipLaunchKernelGGL(k_kry_orth, gb, tb, 0, st, x.dc, a, (int)B_KB, 2);
+    hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_NORM);
+  }
+}
+
+// One group of Arnoldi steps: a graph captured once per launch shape and replayed (eager launches
+// when graphs are off)
+int kry_group(const Ctx& x) {
+  mcf_handle* h = x.gm;
+  Krylov& K = *h->kry;
+  if (!h->use_graphs) {
+    enqueue_arnoldi_group(x, K);
+    return launch_check(h, "gmres arnoldi steps");
+  }
+  if (!K.group || K.group_gemm != x.ln.gemm_p || K.group_tri != x.ln.tri_p) {
+    K.drop_group();
+    HIPCHK(h, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_arnoldi_group(x, K);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(x.stream, &g);
+    if (e != hipSuccess) return fail(h, 4, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    K.group_graph = g;
+    HIPCHK(h, hipGraphInstantiate(&K.group, g, nullptr, nullptr, 0));
+    K.group_gemm = x.ln.gemm_p;
+    K.group_tri = x.ln.tri_p;
+  }
+  HIPCHK(h, hipGraphLaunch(K.group, x.stream));
+  return 0;
+}
+
 // One GMRES call on the right-hand side b_id from x0 = 0 into x_id (SciPy's gmres with the
 // reference's arguments).  The host enqueues the Arnoldi steps of a cycle in groups and reads the
-// status words after each group; the device ends the cycle (active = 0) on its inner test, after
-// which the remaining launches of the group return at once.  At most maxiter cycles run.
+// status words after each group; the device ends the cycle (active = 0) on its inner test or after
+// `restart` steps, after which the remaining launches of the group return at once.  At most
+// maxiter cycles run.
 int kry_call(const Ctx& x, int b_id, int x_id, int& inner, int& info) {
   mcf_handle* h = x.gm;
   Krylov& K = *h->kry;
   const KryArgs a = kry_args(K);
   const dim3 gb(K.nblk), tb(KR_T), gc(1), tc(KR_CT);
   hipStream_t st = x.stream;
-  const int* gate = &K.ctl->active;
   hipLaunchKernelGGL(k_kry_begin, gb, tb, 0, st, x.dc, a, b_id, x_id, (int)B_KR);
   hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_BEGIN);
   if (int rc = kry_read(h)) return rc;
@@ -1894,17 +1946,7 @@ int kry_call(const Ctx& x, int b_id, int x_id, int& inner, int& info) {
     hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)(first ? KP_START_FIRST : KP_START));
     first = false;
     for (int s0 = 0; s0 < K.restart; s0 += KR_GROUP) {
-      for (int s = s0; s < std::min(s0 + KR_GROUP, K.restart); ++s) {
-        hipLaunchKernelGGL(k_kry_matvec, gb, tb, 0, st, x.dc, a, (int)B_KA);
-        kry_precondition(x, B_KA, B_KB, gate);
-        hipLaunchKernelGGL(k_kry_dots, gb, tb, 0, st, x.dc, a, (int)B_KB);
-        hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_DOTS1);
-        hipLaunchKernelGGL(k_kry_orth, gb, tb, 0, st, x.dc, a, (int)B_KB, 1);
-        hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_DOTS2);
-        hipLaunchKernelGGL(k_kry_orth, gb, tb, 0, st, x.dc, a, (int)B_KB, 2);
-        hipLaunchKernelGGL(k_kry_ctl, gc, tc, 0, st, a, (int)KP_NORM);
-      }
-      if (int rc = launch_check(h, "gmres arnoldi steps")) return rc;
+      if (int rc = kry_group(x)) return rc;
       if (int rc = kry_read(h)) return rc;
       if (!K.stat[0]) break;
     }
@@ -2109,6 +2151,7 @@ void advance_host_state(mcf_handle* h, double dt) {
 void drop_graphs(mcf_handle* h) {
   if (h->graphs) h->graphs->drop();
   if (h->lean_graphs) h->lean_graphs->drop();
+  if (h->kry) h->kry->drop_group();
   h->graph_dt = -1.0;
   h->prm_valid = false;
 }
@@ -2339,6 +2382,7 @@ void mcf_destroy(mcf_handle* h) {
   if (h->diag) (void)hipFree(h->diag);
   if (h->tmp_rect) (void)hipFree(h->tmp_rect);
   if (h->kry) {
+    h->kry->drop_group();
     (void)hipFree(h->kry->ctl); (void)hipFree(h->kry->V); (void)hipFree(h->kry->part);
     (void)hipHostFree(h->kry->stat);
     delete h->kry;

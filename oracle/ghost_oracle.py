@@ -635,9 +635,8 @@ class GhostIntegrator:
                               shape=(self.qsize, s.p.size))
         self.G = (raw_g @ self.P / s.rho).tocsr()
         self.A = (self.D @ self.G).tocsc()
-        rect, inverse, shape = rectangle(nx, ny, dx, dy, 'pressure', rho=s.rho, workers=fft_workers)
-        self.pressure_solve, self.pressure_inverse_info = monitored_pressure_solver(
-            self.A, self.pi, rect, inverse, shape, iterative=s.pressure_solver == 'iterative')
+        self._pressure_rectangle = rectangle(nx, ny, dx, dy, 'pressure', rho=s.rho, workers=fft_workers)
+        self.set_pressure_solver(s.pressure_solver)
         self._pressure_calls = []
         # L = nu raw_l E (optimized_ghost_cell.py:113-128)
         rows, cols, values = [], [], []
@@ -672,6 +671,12 @@ class GhostIntegrator:
         self.du_dn = wall_gradient(gc.xu, gc.yu, active_u, points, self.geom, h)
         self.dv_dn = wall_gradient(gc.xv, gc.yv, active_v, points, self.geom, h)
         self.last_diagnostics = {}
+
+    def set_pressure_solver(self, pressure_solver):
+        """(Re)build the monitored pressure solve for 'direct' or 'iterative' (optimized_ghost_cell.py:109-111)."""
+        rect, inverse, shape = self._pressure_rectangle
+        self.pressure_solve, self.pressure_inverse_info = monitored_pressure_solver(
+            self.A, self.pi, rect, inverse, shape, iterative=pressure_solver == 'iterative')
 
     # --- packing (optimized_ghost_cell.py:174-184)
     def pack(self, u, v):

@@ -109,3 +109,25 @@ def test_gmres_nonconvergence_is_reported(pkg):
     assert dev.pressure_backend == 1
     dev.set_pressure_backend(False)
     assert dev.pressure_backend == 0 and s.gg.pressure_inverse_info['backend'] == 'fft_boundary_schur'
+
+
+def test_gmres_graph_groups_match_eager(pkg, monkeypatch):
+    """Arnoldi groups replayed from the captured graph equal eagerly launched groups bit for bit,
+    over several solves and steps with host reads (synchronisations) between the replays."""
+    monkeypatch.setenv('GHOSTFLOW_NO_GRAPH', '1')
+    eager = flow(pkg, pressure_solver='iterative', **REF)
+    monkeypatch.setenv('GHOSTFLOW_NO_GRAPH', '0')
+    graph = flow(pkg, pressure_solver='iterative', **REF)
+    for f in (eager, graph):
+        f.gg.device.set_launch(0, 1)      # same sweep width for both (widths differ in round-off)
+    rng = np.random.default_rng(13)
+    for _ in range(3):
+        vals = rng.normal(size=len(eager.gg.pi))
+        assert np.array_equal(eager.gg.pressure_solve(vals), graph.gg.pressure_solve(vals))
+        assert (eager.gg.pressure_inverse_info['last_solve']['iterations']
+                == graph.gg.pressure_inverse_info['last_solve']['iterations'])
+    for chunk in (2, 3):
+        for f in (eager, graph):
+            f.solve(n_steps=chunk, dt=1e-3, perturb_transient=False, save_artifacts=False)
+        for key in ('u', 'v', 'p'):
+            assert np.array_equal(getattr(eager, key), getattr(graph, key)), key
