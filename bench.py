@@ -203,7 +203,7 @@ def main():
                     help='for rocprofv3 --pmc runs: one 256 MiB device copy (the counter calibration), then '
                          '--steps steps of the workload; prints nothing else')
     ap.add_argument('--launch', type=int, nargs=2, default=None, metavar=('TILE', 'WIDTH'),
-                    help='pin the transform tile and sweep width variants (traffic probe)')
+                    help='pin the transform tile and sweep width variants (default: autotuned)')
     ap.add_argument('--gmres-steps', type=int, default=20,
                     help='steps of the GMRES pressure backend leg (0: skip)')
     ap.add_argument('--ensemble-long', type=int, default=0,
@@ -226,6 +226,8 @@ def main():
     dev = flow.gg.device
     flow._push()
     dev.set_dt(DT)
+    if args.launch:
+        dev.set_launch(*args.launch)
     setup_s = time.perf_counter() - t0
     print(f'bench: setup {setup_s:.2f} s; warmup {args.warmup} steps', file=sys.stderr, flush=True)
     if args.warmup:

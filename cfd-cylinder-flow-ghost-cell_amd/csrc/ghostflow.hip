@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <chrono>
+#include <initializer_list>
 #include "ghostflow.h"
 
 #define DEV __device__ __forceinline__
@@ -388,8 +389,7 @@ struct HelmArgs {
   int pred_slot; // -1 always, else run only where that refinement happened
 };
 
-__global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) {
-  const CaseDev& c = cases[blockIdx.y];
+DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
   const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
   const bool run_u = a.pred_slot < 0 || pred_refine(c.sl, SL_HU, a.pred_slot);
   const bool run_v = a.pred_slot < 0 || pred_refine(c.sl, SL_HV, a.pred_slot);
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) 
     const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
     const double h = -0.5 * c.prm[0];
     const int lo = run_u ? 0 : c.usize, hi = run_v ? c.velsize : c.usize;
-    for (int idx = lo + blockIdx.x * blockDim.x + threadIdx.x; idx < hi; idx += gridDim.x * blockDim.x) {
+    for (int idx = lo + bx * blockDim.x + threadIdx.x; idx < hi; idx += gx * blockDim.x) {
       double res = 0.0, r2 = 0.0;
       if (idx < c.usize) {
         const int j = idx / nx1, i = idx - j * nx1;
@@ -464,19 +464,21 @@ __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) 
     block_max_atomic<4>(v4, d4s);
   }
 }
+__global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) {
+  helm_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
+}
 
 // Divergence of a vel q-array: (D q + div_offset) * scale  (optimized_ghost_cell.py:212-216)
 struct DivArgs { int x_id, out_id; double dtmul; int slot; };   // dtmul 0: no scaling, else / (dtmul dt)
 
-__global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
-  const CaseDev& c = cases[blockIdx.y];
+DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
   const int nx = c.nx, nx1 = nx + 1;
   const double* X = c.buf[a.x_id];
   const double* Xv = X + c.usize;
   double* O = a.out_id >= 0 ? c.buf[a.out_id] : nullptr;
   const double scale = a.dtmul == 0.0 ? 1.0 : 1.0 / (a.dtmul * c.prm[0]);
   double m = 0.0;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
+  for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double d = 0.0;
     if (p_act(c, j, i)) {
@@ -496,13 +498,15 @@ __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
   u64* const d1[1] = {c.sl + a.slot * NSTRIPE};
   block_max_atomic<1>(v1, d1);
 }
+__global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
+  div_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
+}
 
 // Projection update out = q - dt G phi (numba_kernels.py:619-623), with the optional
 // coupled momentum residual -defect + dt G phi (time_integration.py:137-142)
 struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dtmul; };   // dt = dtmul * case dt
 
-__global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a) {
-  const CaseDev& c = cases[blockIdx.y];
+DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
   const int nx = c.nx, nx1 = nx + 1;
   const double* X = c.buf[a.x_id];
   const double* Pp = c.buf[a.phi_id];
@@ -510,7 +514,7 @@ __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a)
   const double* Df = a.defect_id >= 0 ? c.buf[a.defect_id] : nullptr;
   const double dt = a.dtmul * c.prm[0];
   double m = 0.0;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
+  for (int idx = bx * blockDim.x + threadIdx.x; idx < c.velsize; idx += gx * blockDim.x) {
     double out = 0.0, g = 0.0;
     bool act;
     if (idx < c.usize) {
@@ -540,6 +544,9 @@ __global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a)
     block_max_atomic<1>(v1, d1);
   }
 }
+__global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a) {
+  upd_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
+}
 
 // Pressure residual b - A phi and the monitored-solve statistics (pressure_schur.py:250-273)
 struct PresArgs { int b_id, x_id, out_id; int mode; int slot0; int res_slot; int pred_slot; };
@@ -564,15 +571,14 @@ DEV void p_row(const CaseDev& c, const double* X, int idx, int j, int i, double&
   if (j < ny - 1) { double t = X[idx + nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
 }
 
-__global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) {
-  const CaseDev& c = cases[blockIdx.y];
+DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
   if (a.pred_slot >= 0 && !pred_refine(c.sl, a.slot0, a.pred_slot)) return;
   const int nx = c.nx;
   const double* B = c.buf[a.b_id];
   const double* X = c.buf[a.x_id];
   double* O = a.mode == 0 ? c.buf[a.out_id] : nullptr;
   double mres = 0.0, mrnd = 0.0, msc = 0.0, ssr = 0.0, ssv = 0.0;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
+  for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double res = 0.0;
     if (p_act(c, j, i)) {
@@ -608,6 +614,9 @@ __global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) 
     block_sum_atomic(ssv, reinterpret_cast<double*>(s + (b0 + SV_SSQV) * NSTRIPE));
   }
 }
+__global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) {
+  pres_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
+}
 
 // ------------------------------------------------------------------------------------
 // Separable rectangle inverse: dense y transform (orthonormal eigenbasis), x tridiagonal
@@ -626,6 +635,34 @@ DEV double* desc_out(const SolveDesc& d, int id) { return d.bufs[id] + d.off; }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// XCD-aware tile order: workgroups go to the 8 XCDs round-robin by linear id, so each XCD is given
+// a contiguous run of tiles (problem slowest, then column panel, row-block fastest) and the A and B
+// panels its tiles share stay in that XCD's L2.  A bijection of [0, T).
+DEV void xcd_tile(int& z, int& cx, int& ry) {
+  const int gx = gridDim.x, gy = gridDim.y, T = gx * gy * gridDim.z;
+  const int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int xcd = id & 7;
+  int t = id >> 3;
+  for (int q = 0; q < xcd; ++q) t += (T - q + 7) >> 3;
+  z = t / (gx * gy);
+  const int rem = t - z * gx * gy;
+  cx = rem / gy;
+  ry = rem - cx * gy;
+}
+
+// One output element of a transform: forward into the scratch F; inverse into the output array
+// with the solid mask (epi 0 store, 1 masked store, 2 masked accumulate)
+DEV void gemm_put(const SolveDesc& d, const GemmArgs& g, int gr, int gc, double v) {
+  if (g.forward) { d.F[(size_t)gr * d.xx + gc] = v; return; }
+  double* O = desc_out(d, g.out_id);
+  const int ai = gc + d.ioff, aj = gr + d.joff;
+  const int slo = d.lo ? d.lo[aj] : 1, shi = d.lo ? d.hi[aj] : 0;
+  double* pp = O + (size_t)gr * d.ld + gc;
+  if (g.epi == 0) *pp = v;
+  else if (slo <= ai && ai <= shi) *pp = 0.0;
+  else *pp = (g.epi == 2) ? (*pp + v) : v;
+}
+
 // Dense FP64 y transform on the matrix cores: C (M x N) = A (M x K) * B (K x N), M = K = yy, N = xx.
 // A^T is read in row-major (k, m) order: forward (A = V^T) reads V, inverse (A = V) reads V^T.
 // Each block stages KC-deep K chunks of its A and B tiles in LDS; the next chunk is fetched into
@@ -638,16 +675,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 template <int TM, int TN, int KC>
 __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g) {
   constexpr int MT = TM / 32, NT = TN / 32;
-  // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by linear id, so each XCD is
-  // given a contiguous run of tiles (problem slowest, then column panel, row-block fastest) and
-  // the A and B panels its tiles share stay in that XCD's L2.  A bijection of [0, T).
-  const int gx = gridDim.x, gy = gridDim.y, T = gx * gy * gridDim.z;
-  const int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  const int xcd = id & 7;
-  int t = id >> 3;
-  for (int q = 0; q < xcd; ++q) t += (T - q + 7) >> 3;
-  const int z = t / (gx * gy), rem = t - z * gx * gy;
-  const int cx = rem / gy, ry = rem - cx * gy;
+  int z, cx, ry;
+  xcd_tile(z, cx, ry);
   const SolveDesc& d = descs[z];
   const int M = d.yy, N = d.xx, K = d.yy;
   const int row0 = ry * TM, col0 = cx * TN;
@@ -716,29 +745,92 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g
     for (int ni = 0; ni < NT; ++ni) {
       const int gc = col0 + (TN / 2) * wn + 16 * ni + l16;
       if (gc >= N) continue;
-      if (g.forward) {
-        #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int gr = rbase + 4 * r;
-          if (gr < M) d.F[(size_t)gr * N + gc] = acc[mi][ni][r];
-        }
-      } else {
-        double* O = desc_out(d, g.out_id);
-        const int ai = gc + d.ioff;
-        #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int gr = rbase + 4 * r;
-          if (gr >= M) continue;
-          const int aj = gr + d.joff;
-          const int slo = d.lo ? d.lo[aj] : 1, shi = d.lo ? d.hi[aj] : 0;
-          const double v = acc[mi][ni][r];
-          double* pp = O + (size_t)gr * d.ld + gc;
-          if (g.epi == 0) *pp = v;
-          else if (slo <= ai && ai <= shi) *pp = 0.0;
-          else *pp = (g.epi == 2) ? (*pp + v) : v;
-        }
+      #pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = rbase + 4 * r;
+        if (gr < M) gemm_put(d, g, gr, gc, acc[mi][ni][r]);
       }
     }
+  }
+}
+
+// Split-K transform (launch variant 3): a block of SK_S waves computes a 32 x 32 output tile; wave w
+// takes the w-th of SK_S consecutive K slices, holds the whole tile in four independent 16 x 16
+// accumulators and reads its MFMA operands straight from global memory (the panels are L2
+// resident).  A slice's loads are issued together in batches of SK_KB k-groups (one batch covers
+// the slice for K <= 4 SK_S SK_KB), so a wave waits for memory once and then runs its MFMAs back
+// to back -- no LDS staging and no barrier in the K loop.  The slices' partial tiles are added in
+// LDS in slice order.  The sum order differs from the staged tiles', so this variant agrees with
+// them to round-off.
+constexpr int SK_S = 8, SK_KB = 8;
+__global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* descs, GemmArgs g) {
+  int z, cx, ry;
+  xcd_tile(z, cx, ry);
+  const SolveDesc& d = descs[z];
+  const int M = d.yy, N = d.xx, K = d.yy;
+  const int row0 = ry * 32, col0 = cx * 32;
+  if (row0 >= M || col0 >= N) return;
+  if (g.gate && *g.gate == 0) return;
+  if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
+  const double* AT = g.forward ? d.V : d.VT;
+  const double* Bm;
+  int ldb;
+  if (g.forward) { Bm = desc_in(d, g.in_id); ldb = d.ld; }
+  else { Bm = d.F; ldb = d.xx; }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, l16 = lane & 15;
+  const int G = (K + 3) >> 2, per = (G + SK_S - 1) / SK_S;
+  const int g0 = min(G, w * per), g1 = min(G, g0 + per);
+  const int ra[2] = {row0 + l16, row0 + 16 + l16}, cb[2] = {col0 + l16, col0 + 16 + l16};
+  const bool va[2] = {ra[0] < M, ra[1] < M}, vb[2] = {cb[0] < N, cb[1] < N};
+  d4 acc[2][2];
+  #pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+    #pragma unroll
+    for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = d4{0.0, 0.0, 0.0, 0.0};
+  // k-group q covers k = 4q .. 4q+3; lane l holds A[m = l%16][k = 4q + l/16] and B[k][n = l%16]
+  for (int gb = g0; gb < g1; gb += SK_KB) {
+    double a[SK_KB][2], b[SK_KB][2];
+    #pragma unroll
+    for (int t = 0; t < SK_KB; ++t) {
+      const int kk = 4 * (gb + t) + kq;
+      const bool ok = gb + t < g1 && kk < K;
+      const double* Ar = AT + (size_t)(ok ? kk : 0) * M;
+      const double* Br = Bm + (size_t)(ok ? kk : 0) * ldb;
+      #pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        a[t][q] = ok && va[q] ? Ar[ra[q]] : 0.0;
+        b[t][q] = ok && vb[q] ? Br[cb[q]] : 0.0;
+      }
+    }
+    #pragma unroll
+    for (int t = 0; t < SK_KB; ++t)
+      #pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+        #pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][mi], b[t][ni], acc[mi][ni], 0, 0, 0);
+  }
+  // partial tiles of the slices: red[slice][subtile][element][lane]; wave w then finishes
+  // elements 2 (w / 4) and 2 (w / 4) + 1 of subtile w % 4
+  __shared__ double red[SK_S][4][4][64];
+  #pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+    #pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+      #pragma unroll
+      for (int r = 0; r < 4; ++r) red[w][2 * mi + ni][r][lane] = acc[mi][ni][r];
+  __syncthreads();
+  const int st = w & 3, mi = st >> 1, ni = st & 1;
+  const int gc = col0 + 16 * ni + l16;
+  if (gc >= N) return;
+  #pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int r = 2 * (w >> 2) + e;
+    const int gr = row0 + 16 * mi + kq + 4 * r;
+    double v = red[0][st][r][lane];
+    #pragma unroll
+    for (int q = 1; q < SK_S; ++q) v += red[q][st][r][lane];
+    if (gr < M) gemm_put(d, g, gr, gc, v);
   }
 }
 
@@ -909,10 +1001,9 @@ __global__ __launch_bounds__(256) void k_dump(const SolveDesc* descs, int proj) 
 // ------------------------------------------------------------------------------------
 struct FillArgs { int x_id; int vel; int inlet; int track; };
 
-__global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) {
-  const CaseDev& c = cases[blockIdx.y];
+DEV void fill_body(const CaseDev& c, const FillArgs& a, int bx, int gx) {
   double* X = c.buf[a.x_id];
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = bx * blockDim.x + threadIdx.x;
   double mu = 0.0, mv = 0.0;
   auto row = [&](const GhostTab& g, int k, const double* base) {
     double s = 0.0;
@@ -949,21 +1040,58 @@ __global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) 
     block_max_atomic<2>(v2, d2);
   }
 }
+__global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) {
+  fill_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
+}
 
 // half_pressure = pressure + increment; output = half + factor * increment
 struct PressArgs { int p_id, inc_id, half_id, out_id; };
-__global__ __launch_bounds__(256) void k_pressures(const CaseDev* cases, PressArgs a) {
-  const CaseDev& c = cases[blockIdx.y];
+DEV void press_body(const CaseDev& c, const PressArgs& a, int bx, int gx) {
   const double* P = c.buf[a.p_id];
   const double* I = c.buf[a.inc_id];
   double* H = c.buf[a.half_id];
   double* O = c.buf[a.out_id];
   const double factor = c.prm[2];
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.psize; idx += gridDim.x * blockDim.x) {
+  for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const double inc = I[idx];
     const double half = P[idx] + inc;
     H[idx] = half;
     O[idx] = half + factor * inc;
+  }
+}
+__global__ __launch_bounds__(256) void k_pressures(const CaseDev* cases, PressArgs a) {
+  press_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
+}
+
+// Independent stages of a step in one launch: the first body's nblk blocks, then the second's,
+// then the third's.  Each body keeps its own block count and grid-stride loop, so every element
+// is computed exactly as by its own kernel; a launch costs the same few microseconds whatever its
+// size, so merging stages that do not read each other's output shortens the step.
+enum BodyKind { BK_NONE = 0, BK_DIV, BK_HELM, BK_PRES, BK_UPD, BK_PRESS, BK_FILL };
+struct Body {
+  int kind, nblk;
+  union { DivArgs d; HelmArgs h; PresArgs p; UpdArgs u; PressArgs q; FillArgs f; };
+};
+struct Fused { Body b[3]; int n; };
+DEV void run_body(const CaseDev& c, const Body& b, int bx) {
+  switch (b.kind) {
+    case BK_DIV: div_body(c, b.d, bx, b.nblk); break;
+    case BK_HELM: helm_body(c, b.h, bx, b.nblk); break;
+    case BK_PRES: pres_body(c, b.p, bx, b.nblk); break;
+    case BK_UPD: upd_body(c, b.u, bx, b.nblk); break;
+    case BK_PRESS: press_body(c, b.q, bx, b.nblk); break;
+    case BK_FILL: fill_body(c, b.f, bx, b.nblk); break;
+    default: break;
+  }
+}
+__global__ __launch_bounds__(256) void k_fused(const CaseDev* cases, Fused f) {
+  const CaseDev& c = cases[blockIdx.y];
+  int bx = blockIdx.x;
+  #pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (q >= f.n) return;
+    if (bx < f.b[q].nblk) { run_body(c, f.b[q], bx); return; }
+    bx -= f.b[q].nblk;
   }
 }
 
@@ -998,17 +1126,22 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a
   }
 }
 
-__global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
-  const CaseDev& c = cases[blockIdx.y];
-  if (threadIdx.x != 0) return;
+// (Fx, Fy, Fx_p, Fy_p, Fx_v, Fy_v) from the per-wave partial sums, added in wave order
+DEV void force_totals(const CaseDev& c, double (&vals)[6]) {
   const int nb = (c.nsurf + 63) / 64;
   double t[4] = {0.0, 0.0, 0.0, 0.0};
   for (int b = 0; b < nb; ++b)
     for (int q = 0; q < 4; ++q) t[q] += c.fpart[b * 4 + q];
   const double scale = M_PI * c.diameter / c.nsurf;
-  double* f = reinterpret_cast<double*>(c.sl + SL_FORCE * NSTRIPE);
   const double fxp = -t[0], fyp = -t[1], fxv = t[2] * scale, fyv = t[3] * scale;
-  const double vals[6] = {fxp + fxv, fyp + fyv, fxp, fyp, fxv, fyv};
+  vals[0] = fxp + fxv; vals[1] = fyp + fyv; vals[2] = fxp; vals[3] = fyp; vals[4] = fxv; vals[5] = fyv;
+}
+__global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
+  const CaseDev& c = cases[blockIdx.y];
+  if (threadIdx.x != 0) return;
+  double vals[6];
+  force_totals(c, vals);
+  double* f = reinterpret_cast<double*>(c.sl + SL_FORCE * NSTRIPE);
   for (int q = 0; q < 6; ++q) f[q * NSTRIPE] = vals[q];
 }
 
@@ -1062,7 +1195,7 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
   #pragma unroll
   for (int q = 0; q < 4; ++q) vs[q] = wave_sum(sq[q]);
   double f[6];
-  for (int q = 0; q < 6; ++q) f[q] = slot_d(s, (SL_FORCE + q) * NSTRIPE);
+  force_totals(c, f);   // the step's forces, from k_forces' per-wave partial sums
   double gm[9];
   #pragma unroll
   for (int q = 0; q < 9; ++q) gm[q] = slot_max(s, SL_GM_IT + q);
@@ -1646,8 +1779,8 @@ __global__ __launch_bounds__(KR_CT) void k_kry_ctl(KryArgs a, int phase) {
 
 }  // namespace
 
-constexpr int N_GEMM_V = 3, N_TRI_V = 3;
-constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64}, GEMM_TN[N_GEMM_V] = {64, 32, 64};
+constexpr int N_GEMM_V = 4, N_TRI_V = 3;
+constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64, 32}, GEMM_TN[N_GEMM_V] = {64, 32, 64, 32};
 constexpr int TRI_TH[N_TRI_V] = {128, 256, 512};
 struct Launch { int gemm_p = 0, gemm_h = 0, tri_p = 1, tri_h = 1; };
 
@@ -1659,6 +1792,7 @@ void launch_gemm(int v, int yy, int xx, int count, hipStream_t st, const SolveDe
   switch (v) {
     case 1: hipLaunchKernelGGL((k_gemm<32, 32, 64>), gg, dim3(256), 0, st, D, g); break;
     case 2: hipLaunchKernelGGL((k_gemm<64, 64, 32>), gg, dim3(256), 0, st, D, g); break;
+    case 3: hipLaunchKernelGGL(k_gemm_sk, gg, dim3(64 * SK_S), 0, st, D, g); break;
     default: hipLaunchKernelGGL((k_gemm<32, 64, 64>), gg, dim3(256), 0, st, D, g); break;
   }
 }
@@ -1859,6 +1993,42 @@ int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int p
 
 int read_slots(mcf_handle* h, std::vector<double>& out);
 
+// ---- stage bodies and merged launches ----
+Body body_of(int kind, int nblk) { Body b; std::memset(&b, 0, sizeof b); b.kind = kind; b.nblk = nblk; return b; }
+Body body_div(const Ctx& x, DivArgs a) { Body b = body_of(BK_DIV, gridSn(x.maxp, 1).x); b.d = a; return b; }
+Body body_helm(const Ctx& x, HelmArgs a) { Body b = body_of(BK_HELM, gridSn(x.maxvel, 1).x); b.h = a; return b; }
+Body body_pres(const Ctx& x, PresArgs a) { Body b = body_of(BK_PRES, gridSn(x.maxp, 1).x); b.p = a; return b; }
+Body body_upd(const Ctx& x, UpdArgs a) { Body b = body_of(BK_UPD, gridSn(x.maxvel, 1).x); b.u = a; return b; }
+Body body_press(const Ctx& x, PressArgs a) { Body b = body_of(BK_PRESS, gridSn(x.maxp, 1).x); b.q = a; return b; }
+Body body_fill(const Ctx& x, FillArgs a) { Body b = body_of(BK_FILL, grid1n(x.maxfill, 1).x); b.f = a; return b; }
+Body no_body() { return body_of(BK_NONE, 0); }
+
+// Launch the given stages together (one body: its own kernel; several: k_fused)
+void launch_bodies(const Ctx& x, std::initializer_list<Body> bodies) {
+  Fused f;
+  std::memset(&f, 0, sizeof f);
+  int total = 0;
+  for (const Body& b : bodies)
+    if (b.kind != BK_NONE) { f.b[f.n++] = b; total += b.nblk; }
+  if (f.n == 0) return;
+  const dim3 blk(256);
+  if (f.n == 1) {
+    const Body& b = f.b[0];
+    const dim3 g(b.nblk, x.n);
+    switch (b.kind) {
+      case BK_DIV: hipLaunchKernelGGL(k_div, g, blk, 0, x.stream, x.dc, b.d); break;
+      case BK_HELM: hipLaunchKernelGGL(k_helm, g, blk, 0, x.stream, x.dc, b.h); break;
+      case BK_PRES: hipLaunchKernelGGL(k_pres, g, blk, 0, x.stream, x.dc, b.p); break;
+      case BK_UPD: hipLaunchKernelGGL(k_update, g, blk, 0, x.stream, x.dc, b.u); break;
+      case BK_PRESS: hipLaunchKernelGGL(k_pressures, g, blk, 0, x.stream, x.dc, b.q); break;
+      case BK_FILL: hipLaunchKernelGGL(k_fill, g, blk, 0, x.stream, x.dc, b.f); break;
+      default: break;
+    }
+    return;
+  }
+  hipLaunchKernelGGL(k_fused, dim3(total, x.n), blk, 0, x.stream, x.dc, f);
+}
+
 // ---- GMRES pressure backend (host control; the device kernels are above) ----
 constexpr int KR_GROUP = 8;   // Arnoldi steps enqueued between reads of the status words
 
@@ -1999,20 +2169,21 @@ int kry_solve(const Ctx& x, int b_id, int x_id, int psolve) {
 // Woodbury solve with <= 2 refinements (pressure_schur.py:116-124) and the final residual.  The
 // lean form leaves the refinement launches out and computes the first and the final residual in
 // one pass; a step that would have refined is flagged by k_finalize and replayed in full.
-int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bool lean = false) {
+// With `tail`, the final-statistics stage is returned there instead of launched, for the caller to
+// merge into its next launch.
+int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bool lean = false,
+                  Body* tail = nullptr) {
+  if (tail) *tail = no_body();
   if (pres && x.gm) return kry_solve(x, in_id, out_id, psolve);
   const int shift = pres ? 8 * psolve : 0;
   const int slot0 = pres ? SL_P0 + 8 * psolve : SL_HU;
   int rc;
   if ((rc = enqueue_apply(x, pres, in_id, out_id, 1, -1, shift))) return rc;
   if (lean) {
-    if (pres) {
-      PresArgs a{in_id, out_id, -1, 3, slot0, SV_RES1, -1};
-      hipLaunchKernelGGL(k_pres, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, a);
-    } else {
-      HelmArgs a{in_id, out_id, -1, -1, 3, SV_RES1, -1};
-      hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
-    }
+    const Body fin = pres ? body_pres(x, PresArgs{in_id, out_id, -1, 3, slot0, SV_RES1, -1})
+                          : body_helm(x, HelmArgs{in_id, out_id, -1, -1, 3, SV_RES1, -1});
+    if (tail) *tail = fin;
+    else launch_bodies(x, {fin});
     return launch_check(x.err, "lean woodbury solve");
   }
   for (int it = 0; it < 2; ++it) {
@@ -2028,27 +2199,27 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bo
       if ((rc = enqueue_apply(x, pres, B_RES, out_id, 2, res_slot, shift))) return rc;
     }
   }
-  if (pres) {
-    PresArgs a{in_id, out_id, -1, 1, slot0, SV_FINAL, -1};
-    hipLaunchKernelGGL(k_pres, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, a);
-  } else {
-    HelmArgs a{in_id, out_id, -1, -1, 1, SV_FINAL, -1};
-    hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
-  }
+  const Body fin = pres ? body_pres(x, PresArgs{in_id, out_id, -1, 1, slot0, SV_FINAL, -1})
+                        : body_helm(x, HelmArgs{in_id, out_id, -1, -1, 1, SV_FINAL, -1});
+  if (tail) *tail = fin;
+  else launch_bodies(x, {fin});
   return launch_check(x.err, "woodbury solve");
 }
 
-// project(x, dtmul * dt): div -> pressure solve -> update -> divergence gate (optimized_ghost_cell.py:218-229)
+// project(x, dtmul * dt): div -> pressure solve -> update -> divergence gate (optimized_ghost_cell.py:218-229).
+// `head` (an independent stage of the caller) is launched with the divergence; the solve's final
+// statistics are launched with the update; with `gate`, the divergence gate is returned there for
+// the caller to merge into its next launch.
 int enqueue_project(const Ctx& x, int x_id, int out_id, int phi_id, double dtmul, int psolve, int defect_id,
-                    bool lean = false) {
+                    bool lean = false, const Body* head = nullptr, Body* gate = nullptr) {
   int rc;
-  DivArgs d{x_id, B_DIVP, dtmul, SL_P0 + 8 * psolve + SV_VAL};
-  hipLaunchKernelGGL(k_div, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, d);
-  if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve, lean))) return rc;
-  UpdArgs u{x_id, phi_id, out_id, defect_id, dtmul};
-  hipLaunchKernelGGL(k_update, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, u);
-  DivArgs g{out_id, -1, 0.0, SL_DIV0 + psolve};
-  hipLaunchKernelGGL(k_div, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, g);
+  launch_bodies(x, {head ? *head : no_body(), body_div(x, DivArgs{x_id, B_DIVP, dtmul, SL_P0 + 8 * psolve + SV_VAL})});
+  Body fin;
+  if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve, lean, &fin))) return rc;
+  launch_bodies(x, {fin, body_upd(x, UpdArgs{x_id, phi_id, out_id, defect_id, dtmul})});
+  const Body g = body_div(x, DivArgs{out_id, -1, 0.0, SL_DIV0 + psolve});
+  if (gate) *gate = g;
+  else launch_bodies(x, {g});
   return launch_check(x.err, "project");
 }
 
@@ -2090,18 +2261,17 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, bool lean = false
     AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0};
     hipLaunchKernelGGL(k_adv_rhs, gv, dim3(256), 0, x.stream, x.dc, r);
   }
-  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, lean))) return rc;
-  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, lean))) return rc;
-  HelmArgs dfa{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1};
-  hipLaunchKernelGGL(k_helm, gv, dim3(256), 0, x.stream, x.dc, dfa);
-  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, lean))) return rc;
-  PressArgs pa{B_PH, B_PHI2, B_PH, B_POUT};
-  hipLaunchKernelGGL(k_pressures, gp, dim3(256), 0, x.stream, x.dc, pa);
-  FillArgs f{Snew, 1, 1, 1};
-  hipLaunchKernelGGL(k_fill, grid1n(x.maxfill, x.n), dim3(256), 0, x.stream, x.dc, f);
+  // Stages that do not read each other's output share a launch: the Helmholtz statistics with the
+  // first projection's divergence, the first gate with the defect, the second gate with the
+  // pressures and the ghost fill of the new state.
+  Body hfin, gate1, gate2;
+  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, lean, &hfin))) return rc;
+  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, lean, &hfin, &gate1))) return rc;
+  launch_bodies(x, {gate1, body_helm(x, HelmArgs{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1})});
+  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, lean, nullptr, &gate2))) return rc;
+  launch_bodies(x, {gate2, body_press(x, PressArgs{B_PH, B_PHI2, B_PH, B_POUT}), body_fill(x, FillArgs{Snew, 1, 1, 1})});
   ForceArgs fa{Snew, B_POUT, -1.0};
   hipLaunchKernelGGL(k_forces, dim3((x.maxsurf + 63) / 64, x.n), dim3(64), 0, x.stream, x.dc, fa);
-  hipLaunchKernelGGL(k_force_total, dim3(1, x.n), dim3(64), 0, x.stream, x.dc);
   FinArgs fin{nproj, lean ? 1 : 0, force_miss ? 1 : 0, x.gm ? 1 : 0};
   hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, fin);
   return launch_check(x.err, "step");
