@@ -1097,20 +1097,35 @@ __global__ __launch_bounds__(256) void k_fused(const CaseDev* cases, Fused f) {
 
 // Surface forces: one lane per quadrature point, one partial sum per wave (summed in a fixed
 // order by k_force_total, so the forces are bit-reproducible)
-constexpr int FORCE_MAXB = 64;   // up to 64 * 64 surface points
+// Surface forces: 16 lanes per quadrature point, four points per wave.  Each lane takes every 16th
+// nonzero of the point's three CSR rows (surface pressure reconstruction, du/dn, dv/dn), so a
+// row's loads are in flight together instead of one dependent index-then-value load per nonzero;
+// the lane groups add their products by a butterfly, and each wave writes one partial sum of its
+// four points, added in wave order by force_totals (bit-reproducible).
+constexpr int FORCE_PPW = 4;      // points per wave
+constexpr int FORCE_MAXP = 4096;  // surface points
+__host__ __device__ inline int force_blocks(int nsurf) { return (nsurf + FORCE_PPW - 1) / FORCE_PPW; }
 struct ForceArgs { int vel_id, p_id; double mu; };
 __global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const double* Vel = c.buf[a.vel_id];
   const double* Pp = c.buf[a.p_id];
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  if (blockIdx.x * 64 >= c.nsurf) return;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};  // pressure x, y; traction x, y
+  const int sub = threadIdx.x & 15;
+  const int k = blockIdx.x * FORCE_PPW + (threadIdx.x >> 4);
+  if (blockIdx.x * FORCE_PPW >= c.nsurf) return;
+  double ps = 0.0, du = 0.0, dv = 0.0;
   if (k < c.nsurf) {
-    double ps = 0.0, du = 0.0, dv = 0.0;
-    for (int t = c.pp[k]; t < c.pp[k + 1]; ++t) ps += c.pv[t] * Pp[c.pc[t]];
-    for (int t = c.up[k]; t < c.up[k + 1]; ++t) du += c.uv[t] * Vel[c.uc[t]];
-    for (int t = c.vp[k]; t < c.vp[k + 1]; ++t) dv += c.vv[t] * Vel[c.usize + c.vc[t]];
+    for (int t = c.pp[k] + sub; t < c.pp[k + 1]; t += 16) ps += c.pv[t] * Pp[c.pc[t]];
+    for (int t = c.up[k] + sub; t < c.up[k + 1]; t += 16) du += c.uv[t] * Vel[c.uc[t]];
+    for (int t = c.vp[k] + sub; t < c.vp[k + 1]; t += 16) dv += c.vv[t] * Vel[c.usize + c.vc[t]];
+  }
+  for (int o = 8; o > 0; o >>= 1) {
+    ps += __shfl_xor(ps, o, 64);
+    du += __shfl_xor(du, o, 64);
+    dv += __shfl_xor(dv, o, 64);
+  }
+  double s[4] = {0.0, 0.0, 0.0, 0.0};  // pressure x, y; traction x, y
+  if (k < c.nsurf && sub == 0) {
     const double nxk = c.nrm[2 * k], nyk = c.nrm[2 * k + 1];
     const double dot = du * nxk + dv * nyk;
     s[0] = ps * c.nw[2 * k];
@@ -1126,21 +1141,30 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a
   }
 }
 
-// (Fx, Fy, Fx_p, Fy_p, Fx_v, Fy_v) from the per-wave partial sums, added in wave order
+// (Fx, Fy, Fx_p, Fy_p, Fx_v, Fy_v) from k_forces' per-wave partial sums.  Called by all 64 lanes of a
+// one-wave block: lane l adds partials l, l + 64, ... in order, then every lane adds the 64 lane
+// sums in lane order, so the totals are bit-reproducible.
 DEV void force_totals(const CaseDev& c, double (&vals)[6]) {
-  const int nb = (c.nsurf + 63) / 64;
+  __shared__ double lt[4][64];
+  const int lane = threadIdx.x & 63;
+  const int nb = force_blocks(c.nsurf);
+  double p[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = lane; b < nb; b += 64)
+    for (int q = 0; q < 4; ++q) p[q] += c.fpart[b * 4 + q];
+  for (int q = 0; q < 4; ++q) lt[q][lane] = p[q];
+  __syncthreads();
   double t[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = 0; b < nb; ++b)
-    for (int q = 0; q < 4; ++q) t[q] += c.fpart[b * 4 + q];
+  for (int l = 0; l < 64; ++l)
+    for (int q = 0; q < 4; ++q) t[q] += lt[q][l];
   const double scale = M_PI * c.diameter / c.nsurf;
   const double fxp = -t[0], fyp = -t[1], fxv = t[2] * scale, fyv = t[3] * scale;
   vals[0] = fxp + fxv; vals[1] = fyp + fyv; vals[2] = fxp; vals[3] = fyp; vals[4] = fxv; vals[5] = fyv;
 }
 __global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
   const CaseDev& c = cases[blockIdx.y];
-  if (threadIdx.x != 0) return;
   double vals[6];
   force_totals(c, vals);
+  if (threadIdx.x != 0) return;
   double* f = reinterpret_cast<double*>(c.sl + SL_FORCE * NSTRIPE);
   for (int q = 0; q < 6; ++q) f[q * NSTRIPE] = vals[q];
 }
@@ -1169,61 +1193,62 @@ __host__ __device__ inline bool solve_ok(const double* rec) {
 }
 
 struct FinArgs { int nproj; int lean; int force_miss; int gmres; };
-constexpr int FIN_N = 34;
+static_assert(MCF_DIAG_LEN == 64, "one diagnostics entry per lane of k_finalize");
+// Slots gathered by k_finalize: FIN_MAX maxima over the stripes, then FIN_SUM sums
+constexpr int FIN_MAX = 41, FIN_SUM = 6, FIN_NQ = FIN_MAX + FIN_SUM;
+__constant__ const int kFinSlots[FIN_NQ] = {
+    SL_UMAX, SL_VMAX, SL_HU + SV_VAL, SL_HV + SV_VAL, SL_HU + SV_FINAL, SL_HV + SV_FINAL,
+    SL_HU + SV_RES1, SL_HU + SV_RES2, SL_HV + SV_RES1, SL_HV + SV_RES2, SL_COUPLED,
+    SL_DIV0, SL_DIV1, SL_DIV2,
+    SL_P0 + SV_VAL, SL_P0 + SV_RES1, SL_P0 + SV_RES2, SL_P0 + SV_FINAL, SL_P0 + SV_ROUND, SL_P0 + SV_ASCALE,
+    SL_P1 + SV_VAL, SL_P1 + SV_RES1, SL_P1 + SV_RES2, SL_P1 + SV_FINAL, SL_P1 + SV_ROUND, SL_P1 + SV_ASCALE,
+    SL_P2 + SV_VAL, SL_P2 + SV_RES1, SL_P2 + SV_RES2, SL_P2 + SV_FINAL, SL_P2 + SV_ROUND, SL_P2 + SV_ASCALE,
+    SL_GM_IT, SL_GM_IT + 1, SL_GM_IT + 2, SL_GM_REF, SL_GM_REF + 1, SL_GM_REF + 2, SL_GM_ST, SL_GM_ST + 1, SL_GM_ST + 2,
+    SL_P0 + SV_SSQR, SL_P0 + SV_SSQV, SL_P1 + SV_SSQR, SL_P1 + SV_SSQV, SL_P2 + SV_SSQR, SL_P2 + SV_SSQV};
+
+// Gather the step's scalars into the diagnostics ring and clear the slots (one wave per case).
+// Every lane stores its stripe of every slot in LDS; lane q then reduces slot q over the stripes
+// (maxima of |.|, which do not depend on the order, or sums in stripe order); lane 0 evaluates the
+// gates and the per-solve records; the row is stored once, one entry per lane.
 __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const u64* s = c.sl;
   const int lane = threadIdx.x;
-  // slots read by this kernel; all loads are issued before any reduction
-  const int ids[FIN_N] = {SL_UMAX, SL_VMAX, SL_HU + SV_VAL, SL_HV + SV_VAL, SL_HU + SV_FINAL, SL_HV + SV_FINAL,
-                          SL_HU + SV_RES1, SL_HU + SV_RES2, SL_HV + SV_RES1, SL_HV + SV_RES2, SL_COUPLED,
-                          SL_DIV0, SL_DIV1, SL_DIV2,
-                          SL_P0 + SV_VAL, SL_P0 + SV_RES1, SL_P0 + SV_RES2, SL_P0 + SV_FINAL, SL_P0 + SV_ROUND, SL_P0 + SV_ASCALE,
-                          SL_P1 + SV_VAL, SL_P1 + SV_RES1, SL_P1 + SV_RES2, SL_P1 + SV_FINAL, SL_P1 + SV_ROUND, SL_P1 + SV_ASCALE,
-                          SL_P2 + SV_VAL, SL_P2 + SV_RES1, SL_P2 + SV_RES2, SL_P2 + SV_FINAL, SL_P2 + SV_ROUND, SL_P2 + SV_ASCALE,
-                          SL_P0 + SV_SSQR, SL_P0 + SV_SSQV};
-  double raw[FIN_N], sq[4];
+  // step parameters and clock, loaded together with the slots
+  const double dt = c.prm[0];
+  const int row = *c.clock_step;
+  const double time = *c.clock_time + dt;
+  __shared__ double raw[FIN_NQ][65];   // padded: lane q reads row q
+  __shared__ double v[FIN_NQ];
+  __shared__ double d[MCF_DIAG_LEN];
   #pragma unroll
-  for (int q = 0; q < FIN_N; ++q) raw[q] = __longlong_as_double((long long)s[ids[q] * NSTRIPE + lane]);
-  const int sids[4] = {SL_P1 + SV_SSQR, SL_P1 + SV_SSQV, SL_P2 + SV_SSQR, SL_P2 + SV_SSQV};
-  #pragma unroll
-  for (int q = 0; q < 4; ++q) sq[q] = __longlong_as_double((long long)s[sids[q] * NSTRIPE + lane]);
-  double v[FIN_N];
-  #pragma unroll
-  for (int q = 0; q < FIN_N; ++q) v[q] = (q >= FIN_N - 2) ? wave_sum(raw[q]) : wave_max_abs(raw[q]);
-  double vs[4];
-  #pragma unroll
-  for (int q = 0; q < 4; ++q) vs[q] = wave_sum(sq[q]);
+  for (int q = 0; q < FIN_NQ; ++q) raw[q][lane] = __longlong_as_double((long long)s[kFinSlots[q] * NSTRIPE + lane]);
+  d[lane] = 0.0;
   double f[6];
   force_totals(c, f);   // the step's forces, from k_forces' per-wave partial sums
-  double gm[9];
-  #pragma unroll
-  for (int q = 0; q < 9; ++q) gm[q] = slot_max(s, SL_GM_IT + q);
+  __syncthreads();
+  if (lane < FIN_NQ) {
+    double r = 0.0;
+    if (lane < FIN_MAX) { for (int t = 0; t < 64; ++t) r = nanmax(r, fabs(raw[lane][t])); }
+    else { for (int t = 0; t < 64; ++t) r += raw[lane][t]; }
+    v[lane] = r;
+  }
+  __syncthreads();
   if (lane == 0) {
-    const double dt = c.prm[0];
-    const int row = *c.clock_step;
-    const double time = *c.clock_time + dt;
-    *c.clock_step = row + 1;
-    *c.clock_time = time;
     const double umax = v[0], vmax = v[1], hu_val = v[2], hv_val = v[3], hu_fin = v[4], hv_fin = v[5];
     const double hu_r1 = v[6], hu_r2 = v[7], hv_r1 = v[8], hv_r2 = v[9], coupled = v[10];
     const double div[3] = {v[11], v[12], v[13]};
-    double pval[3], pr1[3], pr2[3], pfin[3], prnd[3], pasc[3];
-    for (int q = 0; q < 3; ++q) {
-      pval[q] = v[14 + 6 * q]; pr1[q] = v[15 + 6 * q]; pr2[q] = v[16 + 6 * q];
-      pfin[q] = v[17 + 6 * q]; prnd[q] = v[18 + 6 * q]; pasc[q] = v[19 + 6 * q];
-    }
-    const double pssr[3] = {v[32], vs[0], vs[2]}, pssv[3] = {v[33], vs[1], vs[3]};
-    double* d = c.diag + (size_t)row * MCF_DIAG_LEN;
     const double denom = 0.5 * c.rho * c.uref * c.uref * c.diameter;
-    for (int k = 0; k < MCF_DIAG_LEN; ++k) d[k] = 0.0;
+    const double cd = f[0] / denom, cl = f[1] / denom;
+    const double maxdiv = div[0];
+    const double cfl = dt * (umax / c.dx + vmax / c.dy);
     d[MCF_D_TIME] = time; d[MCF_D_DT] = dt;
     d[MCF_D_FX] = f[0]; d[MCF_D_FY] = f[1];
-    d[MCF_D_CD] = f[0] / denom; d[MCF_D_CL] = f[1] / denom;
+    d[MCF_D_CD] = cd; d[MCF_D_CL] = cl;
     d[MCF_D_CD_P] = f[2] / denom; d[MCF_D_CL_P] = f[3] / denom;
     d[MCF_D_CD_V] = f[4] / denom; d[MCF_D_CL_V] = f[5] / denom;
-    d[MCF_D_MAXDIV] = div[0];
-    d[MCF_D_CFL] = dt * (umax / c.dx + vmax / c.dy);
+    d[MCF_D_MAXDIV] = maxdiv;
+    d[MCF_D_CFL] = cfl;
     const double rhs_max = fmax(hu_val, hv_val);
     const double mres = nanmax(hu_fin, hv_fin);
     d[MCF_D_MOMENTUM_RES] = mres;
@@ -1232,26 +1257,30 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
     int status = 0;
     if (!(mres <= 1e-10 * fmax(1.0, rhs_max))) status |= 1;
     int refine = 0;
-    double worst = -1.0;
+    double worst = -1.0, relmax = 0.0;
     // one record per pressure solve in call order: the startup projection comes first
     const int order3[3] = {2, 0, 1};
     for (int q = 0; q < a.nproj; ++q) {
       const int b = a.nproj == 3 ? order3[q] : q;
-      const SolveSlots ss{pval[b], pr1[b], pr2[b], pfin[b], prnd[b], pasc[b], pssr[b], pssv[b], gm[b], gm[3 + b], gm[6 + b]};
-      double* rec = d + MCF_D_SOLVE0 + MCF_SOLVE_LEN * q;
+      const double pfin = v[17 + 6 * b], prnd = v[18 + 6 * b];
+      const SolveSlots ss{v[14 + 6 * b], v[15 + 6 * b], v[16 + 6 * b], pfin, prnd, v[19 + 6 * b],
+                          v[FIN_MAX + 2 * b], v[FIN_MAX + 2 * b + 1], v[32 + b], v[35 + b], v[38 + b]};
+      double rec[MCF_SOLVE_LEN];
       solve_record(ss, a.gmres != 0, rec);
+      for (int k = 0; k < MCF_SOLVE_LEN; ++k) d[MCF_D_SOLVE0 + MCF_SOLVE_LEN * q + k] = rec[k];
       refine += (int)rec[MCF_S_REFINE];
       if (!solve_ok(rec)) status |= 2;
-      d[MCF_D_P_REL0] = fmax(d[MCF_D_P_REL0], rec[MCF_S_REL_L2]);
-      if (b == 0) d[MCF_D_P_RES1] = pfin[b];
-      if (pfin[b] > worst || pfin[b] != pfin[b]) {
-        worst = pfin[b];
-        d[MCF_D_P_RES0] = pfin[b];
+      relmax = fmax(relmax, rec[MCF_S_REL_L2]);
+      if (b == 0) d[MCF_D_P_RES1] = pfin;
+      if (pfin > worst || pfin != pfin) {
+        worst = pfin;
+        d[MCF_D_P_RES0] = pfin;
         d[MCF_D_P_TARGET0] = rec[MCF_S_TARGET];
-        d[MCF_D_P_ROUND0] = prnd[b];
+        d[MCF_D_P_ROUND0] = prnd;
         d[MCF_D_P_BACKWARD0] = rec[MCF_S_BACKWARD];
       }
     }
+    d[MCF_D_P_REL0] = relmax;
     const double tu = woodbury_target(hu_val), tv = woodbury_target(hv_val);
     const int hrefine = (hu_r1 > tu) + (hu_r1 > tu && hu_r2 > tu) + (hv_r1 > tv) + (hv_r1 > tv && hv_r2 > tv);
     d[MCF_D_P_REFINE] = refine;
@@ -1260,12 +1289,18 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
     double divg = 0.0;
     for (int q = 0; q < 3; ++q) divg = nanmax(divg, div[q]);
     if (!(divg <= 1e-10)) status |= 4;
-    if (!(d[MCF_D_MAXDIV] < 1e-10) || !(d[MCF_D_CFL] < 0.5)) status |= 8;
-    if (!isfinite(d[MCF_D_CD]) || !isfinite(d[MCF_D_CL])) status |= 16;
+    if (!(maxdiv < 1e-10) || !(cfl < 0.5)) status |= 8;
+    if (!isfinite(cd) || !isfinite(cl)) status |= 16;
     // a lean step (refinement launches left out) whose first residual called for a refinement:
     // the host replays the chunk with the refinements in place
     if (a.lean && (refine > 0 || hrefine > 0 || a.force_miss)) status |= 32;
     d[MCF_D_STATUS] = status;
+  }
+  __syncthreads();
+  c.diag[(size_t)row * MCF_DIAG_LEN + lane] = d[lane];
+  if (lane == 0) {
+    *c.clock_step = row + 1;
+    *c.clock_time = time;
   }
   u64* wsl = c.sl;
   for (int k = lane; k < NSLOTW; k += 64) wsl[k] = 0ull;
@@ -2271,7 +2306,7 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, bool lean = false
   if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, lean, nullptr, &gate2))) return rc;
   launch_bodies(x, {gate2, body_press(x, PressArgs{B_PH, B_PHI2, B_PH, B_POUT}), body_fill(x, FillArgs{Snew, 1, 1, 1})});
   ForceArgs fa{Snew, B_POUT, -1.0};
-  hipLaunchKernelGGL(k_forces, dim3((x.maxsurf + 63) / 64, x.n), dim3(64), 0, x.stream, x.dc, fa);
+  hipLaunchKernelGGL(k_forces, dim3(force_blocks(x.maxsurf), x.n), dim3(64), 0, x.stream, x.dc, fa);
   FinArgs fin{nproj, lean ? 1 : 0, force_miss ? 1 : 0, x.gm ? 1 : 0};
   hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, fin);
   return launch_check(x.err, "step");
@@ -2805,9 +2840,9 @@ int mcf_set_surface(mcf_handle* h, int n, const int* pp, const int* pc, const do
   if (int rc = upload(h, "snw", normal_weights, 2 * n, &nw)) return rc;
   c.nsurf = n; c.pp = a; c.pc = b; c.pv = x; c.up = d; c.uc = e; c.uv = y; c.vp = f; c.vc = g; c.vv = z;
   c.nrm = nr; c.nw = nw; c.diameter = diameter; c.uref = u_ref;
-  if (n > 64 * FORCE_MAXB) return fail(h, 1, "too many surface points");
+  if (n > FORCE_MAXP) return fail(h, 1, "too many surface points");
   double* fp;
-  if (int rc = upload<double>(h, "sfp", nullptr, (size_t)4 * ((n + 63) / 64), &fp)) return rc;
+  if (int rc = upload<double>(h, "sfp", nullptr, (size_t)4 * force_blocks(n), &fp)) return rc;
   c.fpart = fp;
   h->has_surface = true;
   return sync_descs(h);
@@ -2996,7 +3031,7 @@ int mcf_forces(mcf_handle* h, const double* vel, const double* p, double mu, dou
   HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TV0], vel, h->velsize * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TP0], p, h->psize * sizeof(double), hipMemcpyHostToDevice, h->stream));
   ForceArgs fa{B_TV0, B_TP0, mu};
-  hipLaunchKernelGGL(k_forces, dim3((h->hc.nsurf + 63) / 64, 1), dim3(64), 0, h->stream, h->dc, fa);
+  hipLaunchKernelGGL(k_forces, dim3(force_blocks(h->hc.nsurf), 1), dim3(64), 0, h->stream, h->dc, fa);
   hipLaunchKernelGGL(k_force_total, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   std::vector<double> sv;
