@@ -28,7 +28,7 @@ FP64_PEAK = 78.6e12        # FLOP/s
 STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB2/CN step
 # HBM bytes per forward launch of the pressure transform by tile variant (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 from rocprofv3 --pmc, scaled by a 256 MiB copy; profiles/k_gemm_1320x246_traffic.md)
-GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6}
+GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6}
 
 
 def cpu_baseline(seconds=20.0, gmres_seconds=0.0):
@@ -262,7 +262,8 @@ def main():
                    bound='mfma', achieved=gemm_achieved / 1e12, peak=FP64_PEAK / 1e12, unit='TFLOP/s',
                    frac=gemm_achieved / FP64_PEAK, traffic=GEMM_TRAFFIC.get(tile),
                    traffic_unit='bytes per launch', algorithmic_bytes=8.0 * (yy * yy + 2 * yy * xx), tile=tile,
-                   kernel='k_gemm (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)', launch_ms=gemm_ms),
+                   kernel=('k_gemm_sk' if tile == 3 else 'k_gemm') + ' (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
+                   launch_ms=gemm_ms),
                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
