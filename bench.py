@@ -92,7 +92,7 @@ def gmres_leg(dev, steps):
                 ms_per_solve=float(np.mean(secs)) * 1e3, max_divergence=float(np.max(diag[:, 10])))
 
 
-def ensemble_leg(pkg, steps):
+def ensemble_leg(pkg, steps, launch=None):
     """The seeded 64-case ensemble (uneven 330x64 .. 1320x246 channels, Re 40..200, zero start,
     dt = 0.05 h/U_m): aggregate cell-steps/s of the batched path (one launch per stage over all
     cases, graph-captured steps) against the same cases stepped one after another on the plain
@@ -105,8 +105,11 @@ def ensemble_leg(pkg, steps):
     t0 = time.perf_counter()
     batch = ens.Ensemble(cases)
     setup_s = time.perf_counter() - t0
+    if launch:
+        batch.set_launch(*launch)
     plain_flows = ens.build_flows(cases, failures={})
     batch.run(3)
+    shapes = batch.get_launch()
     print(f'bench: ensemble of 64 cases ready in {setup_s:.1f} s; timing {steps} batched steps', file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     d = batch.run(steps)
@@ -132,7 +135,7 @@ def ensemble_leg(pkg, steps):
                 cells=int(cells.sum()), grids='330x64 .. 1320x246, seeds 4000..4063', steps=steps,
                 window='steps 3 .. 3+steps of both legs, fresh copies', batched_value=batched, plain_value=plain,
                 ratio=batched / plain, max_divergence=float(np.nanmax(d[ran][:, :, 10])), setup_seconds=setup_s,
-                unit='cell-steps/s', batched_speculation=spec)
+                unit='cell-steps/s', batched_speculation=spec, batched_launch=shapes)
 
 
 def ensemble_long(pkg, total, chunk=500):
@@ -204,6 +207,8 @@ def main():
                          '--steps steps of the workload; prints nothing else')
     ap.add_argument('--launch', type=int, nargs=2, default=None, metavar=('TILE', 'WIDTH'),
                     help='pin the transform tile and sweep width variants (default: autotuned)')
+    ap.add_argument('--ensemble-launch', type=int, nargs=2, default=None, metavar=('TILE', 'WIDTH'),
+                    help="pin the ensemble batch's transform tile and sweep width (default: autotuned)")
     ap.add_argument('--gmres-steps', type=int, default=20,
                     help='steps of the GMRES pressure backend leg (0: skip)')
     ap.add_argument('--ensemble-long', type=int, default=0,
@@ -272,7 +277,7 @@ def main():
     if args.gmres_steps:
         print(f'bench: GMRES pressure backend, {args.gmres_steps} steps', file=sys.stderr, flush=True)
         out['pressure_iterative'] = gmres_leg(dev, args.gmres_steps)
-    out['ensemble'] = None if args.no_ensemble else ensemble_leg(pkg, args.ensemble_steps)
+    out['ensemble'] = None if args.no_ensemble else ensemble_leg(pkg, args.ensemble_steps, args.ensemble_launch)
     if not args.no_warm_start:
         import importlib
         ws = importlib.import_module(pkg.__name__ + '.warmstart')

@@ -1094,6 +1094,27 @@ __global__ __launch_bounds__(256) void k_fused(const CaseDev* cases, Fused f) {
     bx -= f.b[q].nblk;
   }
 }
+// The combinations the step uses, instantiated with only their own bodies: a kernel's registers
+// are those of its largest body, so each keeps the occupancy of its stages' own kernels.
+template <int K>
+DEV void run_kind(const CaseDev& c, const Body& b, int bx) {
+  if constexpr (K == BK_DIV) div_body(c, b.d, bx, b.nblk);
+  else if constexpr (K == BK_HELM) helm_body(c, b.h, bx, b.nblk);
+  else if constexpr (K == BK_PRES) pres_body(c, b.p, bx, b.nblk);
+  else if constexpr (K == BK_UPD) upd_body(c, b.u, bx, b.nblk);
+  else if constexpr (K == BK_PRESS) press_body(c, b.q, bx, b.nblk);
+  else if constexpr (K == BK_FILL) fill_body(c, b.f, bx, b.nblk);
+}
+template <int K0, int K1, int K2>
+__global__ __launch_bounds__(256) void k_fused_of(const CaseDev* cases, Fused f) {
+  const CaseDev& c = cases[blockIdx.y];
+  int bx = blockIdx.x;
+  if (bx < f.b[0].nblk) { run_kind<K0>(c, f.b[0], bx); return; }
+  bx -= f.b[0].nblk;
+  if (K2 == BK_NONE || bx < f.b[1].nblk) { run_kind<K1>(c, f.b[1], bx); return; }
+  bx -= f.b[1].nblk;
+  run_kind<K2>(c, f.b[2], bx);
+}
 
 // Surface forces: one lane per quadrature point, one partial sum per wave (summed in a fixed
 // order by k_force_total, so the forces are bit-reproducible)
@@ -2038,7 +2059,23 @@ Body body_press(const Ctx& x, PressArgs a) { Body b = body_of(BK_PRESS, gridSn(x
 Body body_fill(const Ctx& x, FillArgs a) { Body b = body_of(BK_FILL, grid1n(x.maxfill, 1).x); b.f = a; return b; }
 Body no_body() { return body_of(BK_NONE, 0); }
 
-// Launch the given stages together (one body: its own kernel; several: k_fused)
+// Launch one stage body with its own kernel
+void launch_body(const Ctx& x, const Body& b) {
+  const dim3 g(b.nblk, x.n), blk(256);
+  switch (b.kind) {
+    case BK_DIV: hipLaunchKernelGGL(k_div, g, blk, 0, x.stream, x.dc, b.d); break;
+    case BK_HELM: hipLaunchKernelGGL(k_helm, g, blk, 0, x.stream, x.dc, b.h); break;
+    case BK_PRES: hipLaunchKernelGGL(k_pres, g, blk, 0, x.stream, x.dc, b.p); break;
+    case BK_UPD: hipLaunchKernelGGL(k_update, g, blk, 0, x.stream, x.dc, b.u); break;
+    case BK_PRESS: hipLaunchKernelGGL(k_pressures, g, blk, 0, x.stream, x.dc, b.q); break;
+    case BK_FILL: hipLaunchKernelGGL(k_fill, g, blk, 0, x.stream, x.dc, b.f); break;
+    default: break;
+  }
+}
+
+// Launch the given stages together.  One case: one launch (k_fused); a batch of cases: one launch
+// per stage, since the launch cost is shared by the cases and the stages' own kernels keep their
+// higher occupancy on the larger grids.
 void launch_bodies(const Ctx& x, std::initializer_list<Body> bodies) {
   Fused f;
   std::memset(&f, 0, sizeof f);
@@ -2046,22 +2083,22 @@ void launch_bodies(const Ctx& x, std::initializer_list<Body> bodies) {
   for (const Body& b : bodies)
     if (b.kind != BK_NONE) { f.b[f.n++] = b; total += b.nblk; }
   if (f.n == 0) return;
-  const dim3 blk(256);
-  if (f.n == 1) {
-    const Body& b = f.b[0];
-    const dim3 g(b.nblk, x.n);
-    switch (b.kind) {
-      case BK_DIV: hipLaunchKernelGGL(k_div, g, blk, 0, x.stream, x.dc, b.d); break;
-      case BK_HELM: hipLaunchKernelGGL(k_helm, g, blk, 0, x.stream, x.dc, b.h); break;
-      case BK_PRES: hipLaunchKernelGGL(k_pres, g, blk, 0, x.stream, x.dc, b.p); break;
-      case BK_UPD: hipLaunchKernelGGL(k_update, g, blk, 0, x.stream, x.dc, b.u); break;
-      case BK_PRESS: hipLaunchKernelGGL(k_pressures, g, blk, 0, x.stream, x.dc, b.q); break;
-      case BK_FILL: hipLaunchKernelGGL(k_fill, g, blk, 0, x.stream, x.dc, b.f); break;
-      default: break;
-    }
+  if (f.n == 1 || x.n > 1) {
+    for (int q = 0; q < f.n; ++q) launch_body(x, f.b[q]);
     return;
   }
-  hipLaunchKernelGGL(k_fused, dim3(total, x.n), blk, 0, x.stream, x.dc, f);
+  const dim3 g(total, x.n), blk(256);
+  const int k0 = f.b[0].kind, k1 = f.b[1].kind, k2 = f.n > 2 ? f.b[2].kind : BK_NONE;
+  if (k0 == BK_HELM && k1 == BK_DIV && k2 == BK_NONE)
+    hipLaunchKernelGGL((k_fused_of<BK_HELM, BK_DIV, BK_NONE>), g, blk, 0, x.stream, x.dc, f);
+  else if (k0 == BK_PRES && k1 == BK_UPD && k2 == BK_NONE)
+    hipLaunchKernelGGL((k_fused_of<BK_PRES, BK_UPD, BK_NONE>), g, blk, 0, x.stream, x.dc, f);
+  else if (k0 == BK_DIV && k1 == BK_HELM && k2 == BK_NONE)
+    hipLaunchKernelGGL((k_fused_of<BK_DIV, BK_HELM, BK_NONE>), g, blk, 0, x.stream, x.dc, f);
+  else if (k0 == BK_DIV && k1 == BK_PRESS && k2 == BK_FILL)
+    hipLaunchKernelGGL((k_fused_of<BK_DIV, BK_PRESS, BK_FILL>), g, blk, 0, x.stream, x.dc, f);
+  else
+    hipLaunchKernelGGL(k_fused, g, blk, 0, x.stream, x.dc, f);
 }
 
 // ---- GMRES pressure backend (host control; the device kernels are above) ----

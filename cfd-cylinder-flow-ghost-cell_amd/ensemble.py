@@ -7,7 +7,9 @@ process pool.  The reference runs one case per process; this module has no
 counterpart there beyond the per-case construction it reuses (solver/main.py:40-242).
 """
 import ctypes
+import multiprocessing
 import os
+import sys
 from concurrent.futures import ProcessPoolExecutor
 
 import numpy as np
@@ -69,6 +71,20 @@ def solver_kwargs(case):
     return {k: case[k] for k in ('nx', 'ny', 'Lx', 'Ly', 'D', 'xc', 'yc', 'nu', 'rho', 'inflow', 'U_m')}
 
 
+def _worker_pool(workers):
+    """A process pool whose workers do not inherit this process's GPU runtime: they fork from a
+    fresh forkserver process, which loads the package by path (``cfd_amd_loader``).  None when the
+    package was not loaded through that module."""
+    loader = sys.modules.get('cfd_amd_loader')
+    main = sys.modules.get('__main__')
+    main_file = getattr(main, '__file__', None)
+    if loader is None or (getattr(main, '__spec__', None) is None and not (main_file and os.path.isfile(main_file))):
+        return None   # workers re-import the main module; without a file to import, prepare serially
+    ctx = multiprocessing.get_context('forkserver')
+    ctx.set_forkserver_preload(['cfd_amd_loader'])
+    return ProcessPoolExecutor(max_workers=workers, mp_context=ctx, initializer=loader.load)
+
+
 def build_flows(cases, workers=None, initial='zero', failures=None):
     """One device-resident ``MAC_CylinderFlow`` per case, its Helmholtz factors built for its dt;
     the host tables are prepared in a process pool.  With a ``failures`` dict, a case whose start
@@ -76,8 +92,9 @@ def build_flows(cases, workers=None, initial='zero', failures=None):
     without one the error propagates."""
     cases = list(cases)
     workers = workers or min(16, len(os.sched_getaffinity(0)), len(cases))
-    if workers > 1:
-        with ProcessPoolExecutor(max_workers=workers) as pool:
+    pool = _worker_pool(workers) if workers > 1 else None
+    if pool is not None:
+        with pool:
             prepared = list(pool.map(prepare, cases))
     else:
         prepared = [prepare(c) for c in cases]
