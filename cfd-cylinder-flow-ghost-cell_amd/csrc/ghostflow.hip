@@ -164,14 +164,41 @@ DEV double woodbury_target(double vmax) { return fmax(1e-10, 32.0 * kEps * fmax(
 
 // max that propagates NaN (a NaN anywhere must fail the gates that read the maximum)
 DEV double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+// Cross-lane move of a double through DPP (its two 32-bit halves); lanes without a source lane, and
+// rows outside ROW_MASK, read 0
+template <int CTRL, int ROW_MASK = 0xf>
+DEV double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+DEV double from_lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// Reduction over the 64 lanes of a wave in a fixed order, on the VALU (DPP) instead of LDS
+// permutes: within each 16-lane row by shifts of 1, 2, 4 and 8 (lane 15 then holds the row's
+// total), rows 0+1 and 2+3 by broadcasting lane 15 into the next row, then rows 0-1 into rows 2-3
+// by broadcasting lane 31; lane 63 holds the total, which every lane receives.  Lanes without a
+// source contribute 0, the identity of both reductions below.  All 64 lanes must be active.
+template <class Op>
+DEV double wave_reduce(double v, Op op) {
+  v = op(v, dpp_move<0x111>(v));        // row_shr:1
+  v = op(v, dpp_move<0x112>(v));        // row_shr:2
+  v = op(v, dpp_move<0x114>(v));        // row_shr:4
+  v = op(v, dpp_move<0x118>(v));        // row_shr:8
+  v = op(v, dpp_move<0x142, 0xa>(v));   // row_bcast:15 into rows 1 and 3
+  v = op(v, dpp_move<0x143, 0xc>(v));   // row_bcast:31 into rows 2 and 3
+  return from_lane63(v);
+}
 DEV double wave_max_abs(double v) {
-  v = fabs(v);
-  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
-  return v;
+  return wave_reduce(fabs(v), [](double a, double b) { return nanmax(a, b); });
 }
 DEV double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  return wave_reduce(v, [](double a, double b) { return a + b; });
 }
 // Slot value (max over stripes, or sum); must be called by all 64 lanes of a wave.
 DEV double slot_max(const u64* sl, int k) {
@@ -393,8 +420,10 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
   const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
   const bool run_u = a.pred_slot < 0 || pred_refine(c.sl, SL_HU, a.pred_slot);
   const bool run_v = a.pred_slot < 0 || pred_refine(c.sl, SL_HV, a.pred_slot);
+  const int lo = run_u ? 0 : c.usize, hi = run_v ? c.velsize : c.usize;
+  if (!(run_u || run_v) || lo + bx * (int)blockDim.x >= hi) return;   // no element: nothing to reduce
   double mu = 0.0, mv = 0.0, xu = 0.0, xv = 0.0;
-  if (run_u || run_v) {
+  {
     const double* B = c.buf[a.b_id];
     const double* X = c.buf[a.x_id];
     const double* Xv = X + c.usize;
@@ -402,7 +431,6 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
     double* O2 = a.out2_id >= 0 ? c.buf[a.out2_id] : nullptr;
     const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
     const double h = -0.5 * c.prm[0];
-    const int lo = run_u ? 0 : c.usize, hi = run_v ? c.velsize : c.usize;
     for (int idx = lo + bx * blockDim.x + threadIdx.x; idx < hi; idx += gx * blockDim.x) {
       double res = 0.0, r2 = 0.0;
       if (idx < c.usize) {
@@ -472,6 +500,7 @@ __global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) 
 struct DivArgs { int x_id, out_id; double dtmul; int slot; };   // dtmul 0: no scaling, else / (dtmul dt)
 
 DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
+  if (bx * (int)blockDim.x >= c.psize) return;   // no cell of this case for this block
   const int nx = c.nx, nx1 = nx + 1;
   const double* X = c.buf[a.x_id];
   const double* Xv = X + c.usize;
@@ -507,6 +536,7 @@ __global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
 struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dtmul; };   // dt = dtmul * case dt
 
 DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
+  if (bx * (int)blockDim.x >= c.velsize) return;   // no face of this case for this block
   const int nx = c.nx, nx1 = nx + 1;
   const double* X = c.buf[a.x_id];
   const double* Pp = c.buf[a.phi_id];
@@ -572,6 +602,7 @@ DEV void p_row(const CaseDev& c, const double* X, int idx, int j, int i, double&
 }
 
 DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
+  if (bx * (int)blockDim.x >= c.psize) return;   // no cell of this case for this block
   if (a.pred_slot >= 0 && !pred_refine(c.sl, a.slot0, a.pred_slot)) return;
   const int nx = c.nx;
   const double* B = c.buf[a.b_id];

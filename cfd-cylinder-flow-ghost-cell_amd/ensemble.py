@@ -9,6 +9,7 @@ counterpart there beyond the per-case construction it reuses (solver/main.py:40-
 import ctypes
 import multiprocessing
 import os
+import signal
 import sys
 from concurrent.futures import ProcessPoolExecutor
 
@@ -80,6 +81,8 @@ def _worker_pool(workers):
     main_file = getattr(main, '__file__', None)
     if loader is None or (getattr(main, '__spec__', None) is None and not (main_file and os.path.isfile(main_file))):
         return None   # workers re-import the main module; without a file to import, prepare serially
+    if signal.getsignal(signal.SIGINT) is None or signal.getsignal(signal.SIGCHLD) is None:
+        return None   # a handler installed outside Python (a profiler's): the forkserver cannot restore it
     ctx = multiprocessing.get_context('forkserver')
     ctx.set_forkserver_preload(['cfd_amd_loader'])
     return ProcessPoolExecutor(max_workers=workers, mp_context=ctx, initializer=loader.load)
