@@ -6,7 +6,7 @@ parabolic inflow U_m = 1.5, Re = 100 (nu = 0.01), dt = 5e-4, FP64, forces every 
 started from the inlet profile with a seeded 1e-3 Gaussian transverse v perturbation
 (seed 20261203).  The state stays resident in HBM for the whole timed region.
 
-Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: the benchmark runs one GPU.
 """
 import argparse
 import json
@@ -215,7 +215,7 @@ def main():
                     help='only run the 64-case ensemble for this many steps and print its summary')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f'bench.py: this build runs one case on one GPU; --gpus {args.gpus} is not supported', file=sys.stderr)
+        print(f'bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported', file=sys.stderr)
         sys.exit(2)
     import cfd_amd_loader
     pkg = cfd_amd_loader.load()
