@@ -2051,7 +2051,17 @@ Ctx ctx_of(mcf_handle* h) {
   return x;
 }
 
-inline dim3 gridSn(int n, int cases) { dim3 g = gridS(n); g.y = cases; return g; }
+// Grid of the grid-stride kernels over `cases` cases.  One case: one element per thread.  A batch:
+// at most GSB_BATCH blocks per case, so that each block pays its prologue (the case's parameters
+// and buffer pointers, several dependent loads) over several elements; the batch has parallelism
+// to spare.
+constexpr int GSB_BATCH = 512;
+inline dim3 gridSn(int n, int cases) {
+  dim3 g = gridS(n);
+  if (cases > 1) g.x = std::min<unsigned>(g.x, GSB_BATCH);
+  g.y = cases;
+  return g;
+}
 inline dim3 grid1n(int n, int cases) { dim3 g = grid1(std::max(n, 1)); g.y = cases; return g; }
 
 // One Woodbury apply (in -> out): epi 1 store masked, 2 accumulate masked.
@@ -2082,11 +2092,11 @@ int read_slots(mcf_handle* h, std::vector<double>& out);
 
 // ---- stage bodies and merged launches ----
 Body body_of(int kind, int nblk) { Body b; std::memset(&b, 0, sizeof b); b.kind = kind; b.nblk = nblk; return b; }
-Body body_div(const Ctx& x, DivArgs a) { Body b = body_of(BK_DIV, gridSn(x.maxp, 1).x); b.d = a; return b; }
-Body body_helm(const Ctx& x, HelmArgs a) { Body b = body_of(BK_HELM, gridSn(x.maxvel, 1).x); b.h = a; return b; }
-Body body_pres(const Ctx& x, PresArgs a) { Body b = body_of(BK_PRES, gridSn(x.maxp, 1).x); b.p = a; return b; }
-Body body_upd(const Ctx& x, UpdArgs a) { Body b = body_of(BK_UPD, gridSn(x.maxvel, 1).x); b.u = a; return b; }
-Body body_press(const Ctx& x, PressArgs a) { Body b = body_of(BK_PRESS, gridSn(x.maxp, 1).x); b.q = a; return b; }
+Body body_div(const Ctx& x, DivArgs a) { Body b = body_of(BK_DIV, gridSn(x.maxp, x.n).x); b.d = a; return b; }
+Body body_helm(const Ctx& x, HelmArgs a) { Body b = body_of(BK_HELM, gridSn(x.maxvel, x.n).x); b.h = a; return b; }
+Body body_pres(const Ctx& x, PresArgs a) { Body b = body_of(BK_PRES, gridSn(x.maxp, x.n).x); b.p = a; return b; }
+Body body_upd(const Ctx& x, UpdArgs a) { Body b = body_of(BK_UPD, gridSn(x.maxvel, x.n).x); b.u = a; return b; }
+Body body_press(const Ctx& x, PressArgs a) { Body b = body_of(BK_PRESS, gridSn(x.maxp, x.n).x); b.q = a; return b; }
 Body body_fill(const Ctx& x, FillArgs a) { Body b = body_of(BK_FILL, grid1n(x.maxfill, 1).x); b.f = a; return b; }
 Body no_body() { return body_of(BK_NONE, 0); }
 
