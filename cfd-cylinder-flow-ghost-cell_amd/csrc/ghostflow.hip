@@ -3456,6 +3456,31 @@ struct mcf_batch {
 
 namespace {
 
+// Order of a batch's transform problems.  The XCD-aware tile order gives each XCD a contiguous
+// eighth of the problems, so the problems are arranged in eight consecutive groups of equal count
+// and nearly equal transform work (largest first, each into the lightest group with room; ties by
+// index): with cases in their given order, uneven sizes left some XCDs 1.4x the mean work.
+std::vector<int> xcd_balanced_order(const std::vector<double>& work) {
+  const int n = (int)work.size(), G = std::min(8, n);
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return work[a] > work[b]; });
+  std::vector<std::vector<int>> groups(G);
+  std::vector<double> load(G, 0.0);
+  for (int i : idx) {
+    int best = -1;
+    for (int g = 0; g < G; ++g) {
+      const int cap = n / G + (g < n % G ? 1 : 0);
+      if ((int)groups[g].size() < cap && (best < 0 || load[g] < load[best])) best = g;
+    }
+    groups[best].push_back(i);
+    load[best] += work[i];
+  }
+  std::vector<int> order;
+  for (auto& g : groups) order.insert(order.end(), g.begin(), g.end());
+  return order;
+}
+
 int batch_build(mcf_batch* b) {
   mcf_handle* e = b->cases[0];
   const int n = (int)b->cases.size();
@@ -3463,12 +3488,19 @@ int batch_build(mcf_batch* b) {
   std::vector<SolveDesc> hs(3 * n);
   Ctx x{};
   x.err = e; x.stream = b->stream; x.n = n;
+  std::vector<double> wp(n), wh(2 * n);
+  for (int c = 0; c < n; ++c) {
+    const SolveDesc* s3 = b->cases[c]->hs;
+    wp[c] = (double)s3[0].yy * s3[0].yy * s3[0].xx;
+    for (int q = 0; q < 2; ++q) wh[2 * c + q] = (double)s3[1 + q].yy * s3[1 + q].yy * s3[1 + q].xx;
+  }
+  const std::vector<int> op = xcd_balanced_order(wp), oh = xcd_balanced_order(wh);
   for (int c = 0; c < n; ++c) {
     mcf_handle* h = b->cases[c];
     hc[c] = h->hc;
-    hs[c] = h->hs[0];
-    hs[n + 2 * c] = h->hs[1];
-    hs[n + 2 * c + 1] = h->hs[2];
+    hs[c] = b->cases[op[c]]->hs[0];
+    hs[n + c] = b->cases[oh[c] / 2]->hs[1 + oh[c] % 2];
+    hs[2 * n + c] = b->cases[oh[n + c] / 2]->hs[1 + oh[n + c] % 2];
     ctx_shapes(x, h->hc, h->hs);
   }
   if (!b->dc) HIPCHK(e, hipMalloc(&b->dc, n * sizeof(CaseDev)));
