@@ -31,6 +31,11 @@
 
 #define DEV __device__ __forceinline__
 typedef unsigned long long u64;
+// Device memory through the global address space.  Pointers read from a descriptor table are
+// generic to the compiler, and generic accesses compile to flat_* instructions, which also count
+// against the scalar-load wait counter; G() gives the global_* forms.
+template <class T> using gptr = T __attribute__((address_space(1)))*;
+template <class T> DEV gptr<T> G(T* p) { return (gptr<T>)p; }
 
 namespace {
 
@@ -203,37 +208,65 @@ DEV double wave_sum(double v) {
 // Slot value (max over stripes, or sum); must be called by all 64 lanes of a wave.
 DEV double slot_max(const u64* sl, int k) {
   const int lane = threadIdx.x & 63;
-  return wave_max_abs(__longlong_as_double((long long)sl[k * NSTRIPE + lane]));
+  return wave_max_abs(__longlong_as_double((long long)G(sl)[k * NSTRIPE + lane]));
 }
 DEV double slot_sum(const u64* sl, int k) {
   const int lane = threadIdx.x & 63;
-  return wave_sum(__longlong_as_double((long long)sl[k * NSTRIPE + lane]));
+  return wave_sum(__longlong_as_double((long long)G(sl)[k * NSTRIPE + lane]));
 }
 DEV bool pred_refine(const u64* sl, int slot0, int res_slot) {
   const double res = slot_max(sl, slot0 + res_slot);
   const double val = slot_max(sl, slot0 + SV_VAL);
   return res > woodbury_target(val) || res != res;
 }
-// Block-wide max |value| of up to 4 values, one atomic per block and value. blockDim <= 1024.
-template <int N>
-DEV void block_max_atomic(double (&v)[N], u64* const (&dst)[N]) {
-  __shared__ double red[16][4];
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+// Block statistics of a 256-thread block (every thread calls): the maxima of |v[k]| and the sums of
+// sv[q] over the block, one atomic per block and statistic, into dst[k] (and dst2[k] when set) and
+// sdst[q].  Every thread stores its values in LDS once; wave 0 then reduces each statistic: lane l
+// combines entries l, l + 64, l + 128 and l + 192, then the wave by DPP, in a fixed order.  The
+// maxima are taken on the bit patterns of |v|: non-negative doubles order like their patterns and a
+// NaN (sign cleared) lies above +inf, so the u64 maximum propagates NaN as the gates require.
+constexpr int RED_T = 256;
+DEV double max_bits(double a, double b) {
+  return (u64)__double_as_longlong(a) > (u64)__double_as_longlong(b) ? a : b;
+}
+template <int N, int S = 0>
+DEV void block_stats_atomic(const double (&v)[N], u64* const (&dst)[N], u64* const (&dst2)[N],
+                            const double* sv = nullptr, double* const* sdst = nullptr) {
+  __shared__ double red[N + S][RED_T];
+  const int t = threadIdx.x, lane = t & 63;
   #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    double m = wave_max_abs(v[k]);
-    if (lane == 0) red[w][k] = m;
-  }
+  for (int k = 0; k < N; ++k) red[k][t] = fabs(v[k]);
+  #pragma unroll
+  for (int q = 0; q < S; ++q) red[N + q][t] = sv[q];
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (t < 64) {
+    const int sid = stripe_id();
     #pragma unroll
     for (int k = 0; k < N; ++k) {
-      if (!dst[k]) continue;
-      double m = 0.0;
-      for (int q = 0; q < nw; ++q) m = nanmax(red[q][k], m);
-      if (m != 0.0 || m != m) atomic_max_abs(dst[k] + stripe_id(), m);
+      double m = red[k][lane];
+      #pragma unroll
+      for (int e = 1; e < RED_T / 64; ++e) m = max_bits(m, red[k][lane + 64 * e]);
+      m = wave_reduce(m, [](double a, double b) { return max_bits(a, b); });
+      const u64 bits = (u64)__double_as_longlong(m);
+      if (lane == 0 && bits != 0) {
+        if (dst[k]) atomicMax(dst[k] + sid, bits);
+        if (dst2[k]) atomicMax(dst2[k] + sid, bits);
+      }
+    }
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+      double m = red[N + q][lane];
+      #pragma unroll
+      for (int e = 1; e < RED_T / 64; ++e) m += red[N + q][lane + 64 * e];
+      m = wave_sum(m);
+      if (lane == 0 && m != 0.0) unsafeAtomicAdd(sdst[q] + sid, m);
     }
   }
+}
+template <int N>
+DEV void block_max_atomic(const double (&v)[N], u64* const (&dst)[N]) {
+  u64* const none[N] = {};
+  block_stats_atomic<N>(v, dst, none);
 }
 DEV void block_sum_atomic(double v, double* dst) {
   __shared__ double reds[16];
@@ -252,7 +285,8 @@ DEV void block_sum_atomic(double v, double* dst) {
 DEV bool solid(const CaseDev& c, int field, int j, int i) {
   const int* b = c.sbox[field];
   if (j < b[0] || j > b[1] || i < b[2] || i > b[3]) return false;
-  return c.lo[field][j] <= i && i <= c.hi[field][j];
+  const int lo = G(c.lo[field])[j], hi = G(c.hi[field])[j];   // both loads in flight together
+  return (lo <= i) & (i <= hi);
 }
 DEV bool u_act(const CaseDev& c, int j, int i) { return i >= 1 && !solid(c, 0, j, i); }
 DEV bool v_act(const CaseDev& c, int j, int i) { return j >= 1 && j <= c.ny - 1 && !solid(c, 1, j, i); }
@@ -262,21 +296,50 @@ DEV int ov_row(const Override& o, int j, int i) {
   if (!o.map) return -1;
   int a = j - o.j0, b = i - o.i0;
   if (a < 0 || b < 0 || a >= o.bh || b >= o.bw) return -1;
-  return o.map[a * o.bw + b];
+  return G(o.map)[a * o.bw + b];
 }
-DEV double ov_dot(const Override& o, int r, const double* x) {
+// Override rows hold up to ~16 nonzeros.  A row is read in batches of OV_B entries: the batch's
+// column and value loads go out together, then its x loads, so a batch costs two memory round trips
+// instead of two per nonzero.  Indices past the row's end are clamped to its last entry (the loads
+// stay in the row and need no predicate) and their products are not added; the products are added
+// in nonzero order, as a plain loop over the row adds them.
+constexpr int OV_B = 8;
+template <class P, class F>
+DEV void ov_batches(const Override& o, int r, P x, F use) {
+  const auto ptr = G(o.ptr), col = G(o.col);
+  const auto val = G(o.val);
+  const int k0 = ptr[r], k1 = ptr[r + 1];
+  for (int k = k0; k < k1; k += OV_B) {
+    int cc[OV_B];
+    double vv[OV_B], xv[OV_B];
+    #pragma unroll
+    for (int u = 0; u < OV_B; ++u) {
+      const int kk = min(k + u, k1 - 1);
+      cc[u] = col[kk];
+      vv[u] = val[kk];
+    }
+    #pragma unroll
+    for (int u = 0; u < OV_B; ++u) xv[u] = x[cc[u]];
+    #pragma unroll
+    for (int u = 0; u < OV_B; ++u)
+      if (k + u < k1) use(vv[u], xv[u]);
+  }
+}
+template <class P>
+DEV double ov_dot(const Override& o, int r, P x) {
   double acc = 0.0;
-  for (int k = o.ptr[r]; k < o.ptr[r + 1]; ++k) acc += o.val[k] * x[o.col[k]];
+  ov_batches(o, r, x, [&](double v, double xv) { acc += v * xv; });
   return acc;
 }
-DEV void ov_dot_abs(const Override& o, int r, const double* x, double& acc, double& aacc, int& nnz) {
+template <class P>
+DEV void ov_dot_abs(const Override& o, int r, P x, double& acc, double& aacc, int& nnz) {
   acc = 0.0; aacc = 0.0; nnz = 0;
-  for (int k = o.ptr[r]; k < o.ptr[r + 1]; ++k) {
-    double t = o.val[k] * x[o.col[k]];
+  ov_batches(o, r, x, [&](double v, double xv) {
+    double t = v * xv;
     acc += t;
-    aacc += fabs(o.val[k]) * fabs(x[o.col[k]]);
-    nnz += (o.val[k] != 0.0);
-  }
+    aacc += fabs(v) * fabs(xv);
+    nnz += (v != 0.0);
+  });
 }
 
 // ------------------------------------------------------------------------------------
@@ -292,25 +355,26 @@ struct AdvArgs {
   int mode;  // 0: AB2 rhs, 1: projected-midpoint, 2: rhs with explicit = adv(adv_id), 3: out = L q + offset
 };
 
-__global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a) {
+__global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* __restrict__ cases, AdvArgs a) {
   const CaseDev& c = cases[blockIdx.y];
   const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
-  const double* Q = c.buf[a.q_id];
-  const double* A = c.buf[a.adv_id];
-  const double* Pp = c.buf[a.p_id];
-  const double* prev = a.prev_id >= 0 ? c.buf[a.prev_id] : nullptr;
-  double* now = a.now_id >= 0 ? c.buf[a.now_id] : nullptr;
-  double* out = c.buf[a.out_id];
+  const gptr<const double> Q = G(c.buf[a.q_id]);
+  const gptr<const double> A = G(c.buf[a.adv_id]);
+  const gptr<const double> Pp = G(c.buf[a.p_id]);
+  const gptr<const double> prev = a.prev_id >= 0 ? G(c.buf[a.prev_id]) : nullptr;
+  const gptr<double> now = a.now_id >= 0 ? G(c.buf[a.now_id]) : nullptr;
+  const gptr<double> out = G(c.buf[a.out_id]);
   const double dx = c.dx, dy = c.dy, idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy);
-  const double* in = c.inlet;
-  const double* Av = A + c.usize;
-  const double* Qv = Q + c.usize;
+  const gptr<const double> in = G(c.inlet);
+  const gptr<const double> Av = A + c.usize;
+  const gptr<const double> Qv = Q + c.usize;
   const double dt = c.prm[0], alpha = c.prm[1];
   double mu = 0.0, mv = 0.0;
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
     double res = 0.0, nw = 0.0;
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
+      const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
       if (u_act(c, j, i)) {
         auto AU = [&](int jj, int ii) { return ii == 0 ? in[jj] : A[jj * nx1 + ii]; };
         auto AV = [&](int jj, int ii) { return Av[jj * nx + ii]; };
@@ -337,7 +401,6 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
         if (j < ny - 1) lap += idy2 * QU(j + 1, i);
         if (j > 0) lap += idy2 * QU(j - 1, i);
         const double diff = c.nu * lap;
-        const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);
         const double grad = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp)
                                    : ((i < nx) ? (Pp[j * nx + i] - Pp[j * nx + i - 1]) / dx : -2.0 * Pp[j * nx + nx - 1] / dx) / c.rho;
         const double doff = (i == 1) ? c.nu * in[j] * idx2 : 0.0;
@@ -359,6 +422,7 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
     } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
+      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
       if (v_act(c, j, i)) {
         auto AU = [&](int jj, int ii) { return ii == 0 ? in[jj] : A[jj * nx1 + ii]; };
         auto AV = [&](int jj, int ii) { return Av[jj * nx + ii]; };
@@ -379,7 +443,6 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* cases, AdvArgs a
         if (i < nx - 1) lap += idx2 * QV(j, i + 1);
         if (i > 0) lap += idx2 * QV(j, i - 1);
         const double diff = c.nu * lap;
-        const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
         const double grad = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp)
                                    : (Pp[j * nx + i] - Pp[(j - 1) * nx + i]) / dy / c.rho;
         if (a.mode == 0) {
@@ -424,20 +487,20 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
   if (!(run_u || run_v) || lo + bx * (int)blockDim.x >= hi) return;   // no element: nothing to reduce
   double mu = 0.0, mv = 0.0, xu = 0.0, xv = 0.0;
   {
-    const double* B = c.buf[a.b_id];
-    const double* X = c.buf[a.x_id];
-    const double* Xv = X + c.usize;
-    double* O = a.out_id >= 0 ? c.buf[a.out_id] : nullptr;
-    double* O2 = a.out2_id >= 0 ? c.buf[a.out2_id] : nullptr;
+    const gptr<const double> B = G(c.buf[a.b_id]);
+    const gptr<const double> X = G(c.buf[a.x_id]);
+    const gptr<const double> Xv = X + c.usize;
+    const gptr<double> O = a.out_id >= 0 ? G(c.buf[a.out_id]) : nullptr;
+    const gptr<double> O2 = a.out2_id >= 0 ? G(c.buf[a.out2_id]) : nullptr;
     const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
     const double h = -0.5 * c.prm[0];
     for (int idx = lo + bx * blockDim.x + threadIdx.x; idx < hi; idx += gx * blockDim.x) {
       double res = 0.0, r2 = 0.0;
       if (idx < c.usize) {
         const int j = idx / nx1, i = idx - j * nx1;
+        const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);   // looked up together with the solid test
         if (u_act(c, j, i)) {
           auto XU = [&](int jj, int ii) { return X[jj * nx1 + ii]; };
-          const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);
           double hx;
           const double x = XU(j, i);
           if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_U], r, X);
@@ -456,9 +519,9 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
       } else {
         const int t = idx - c.usize;
         const int j = t / nx, i = t - j * nx;
+        const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
         if (v_act(c, j, i)) {
           auto XV = [&](int jj, int ii) { return Xv[jj * nx + ii]; };
-          const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
           double hx;
           const double x = XV(j, i);
           if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_V], r, X);
@@ -484,15 +547,15 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
     block_max_atomic<2>(v2, d2);
   } else {
     // mode 3 (lean solve, no refinement): the same residual is the first and the final one
-    double v4[4] = {mu, mv, mu, mv};
-    u64* const d4s[4] = {run_u ? c.sl + (SL_HU + a.res_slot) * NSTRIPE : nullptr,
-                         run_v ? c.sl + (SL_HV + a.res_slot) * NSTRIPE : nullptr,
-                         a.mode == 3 ? c.sl + (SL_HU + SV_FINAL) * NSTRIPE : nullptr,
-                         a.mode == 3 ? c.sl + (SL_HV + SV_FINAL) * NSTRIPE : nullptr};
-    block_max_atomic<4>(v4, d4s);
+    const double v2[2] = {mu, mv};
+    u64* const d2[2] = {run_u ? c.sl + (SL_HU + a.res_slot) * NSTRIPE : nullptr,
+                        run_v ? c.sl + (SL_HV + a.res_slot) * NSTRIPE : nullptr};
+    u64* const f2[2] = {a.mode == 3 ? c.sl + (SL_HU + SV_FINAL) * NSTRIPE : nullptr,
+                        a.mode == 3 ? c.sl + (SL_HV + SV_FINAL) * NSTRIPE : nullptr};
+    block_stats_atomic<2>(v2, d2, f2);
   }
 }
-__global__ __launch_bounds__(256) void k_helm(const CaseDev* cases, HelmArgs a) {
+__global__ __launch_bounds__(256) void k_helm(const CaseDev* __restrict__ cases, HelmArgs a) {
   helm_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
 }
 
@@ -502,20 +565,21 @@ struct DivArgs { int x_id, out_id; double dtmul; int slot; };   // dtmul 0: no s
 DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
   if (bx * (int)blockDim.x >= c.psize) return;   // no cell of this case for this block
   const int nx = c.nx, nx1 = nx + 1;
-  const double* X = c.buf[a.x_id];
-  const double* Xv = X + c.usize;
-  double* O = a.out_id >= 0 ? c.buf[a.out_id] : nullptr;
+  const gptr<const double> X = G(c.buf[a.x_id]);
+  const gptr<const double> Xv = X + c.usize;
+  const gptr<double> O = a.out_id >= 0 ? G(c.buf[a.out_id]) : nullptr;
+  const gptr<const double> inl = G(c.inlet);
   const double scale = a.dtmul == 0.0 ? 1.0 : 1.0 / (a.dtmul * c.prm[0]);
   double m = 0.0;
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double d = 0.0;
+    const int r = ov_row(c.ov[MCF_OP_DIV], j, i);   // looked up together with the solid test
     if (p_act(c, j, i)) {
-      const int r = ov_row(c.ov[MCF_OP_DIV], j, i);
-      const double off = (i == 0) ? -c.inlet[j] / c.dx : 0.0;
+      const double off = (i == 0) ? -inl[j] / c.dx : 0.0;
       if (r >= 0) d = ov_dot(c.ov[MCF_OP_DIV], r, X) + off;
       else {
-        const double ul = (i == 0) ? c.inlet[j] : X[j * nx1 + i];
+        const double ul = (i == 0) ? inl[j] : X[j * nx1 + i];
         d = (X[j * nx1 + i + 1] - ul) / c.dx + (Xv[(j + 1) * nx + i] - Xv[j * nx + i]) / c.dy;
       }
       d *= scale;
@@ -527,7 +591,7 @@ DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
   u64* const d1[1] = {c.sl + a.slot * NSTRIPE};
   block_max_atomic<1>(v1, d1);
 }
-__global__ __launch_bounds__(256) void k_div(const CaseDev* cases, DivArgs a) {
+__global__ __launch_bounds__(256) void k_div(const CaseDev* __restrict__ cases, DivArgs a) {
   div_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
 }
 
@@ -538,10 +602,10 @@ struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dtmul; };   // dt =
 DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
   if (bx * (int)blockDim.x >= c.velsize) return;   // no face of this case for this block
   const int nx = c.nx, nx1 = nx + 1;
-  const double* X = c.buf[a.x_id];
-  const double* Pp = c.buf[a.phi_id];
-  double* O = c.buf[a.out_id];
-  const double* Df = a.defect_id >= 0 ? c.buf[a.defect_id] : nullptr;
+  const gptr<const double> X = G(c.buf[a.x_id]);
+  const gptr<const double> Pp = G(c.buf[a.phi_id]);
+  const gptr<double> O = G(c.buf[a.out_id]);
+  const gptr<const double> Df = a.defect_id >= 0 ? G(c.buf[a.defect_id]) : nullptr;
   const double dt = a.dtmul * c.prm[0];
   double m = 0.0;
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.velsize; idx += gx * blockDim.x) {
@@ -549,18 +613,18 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
     bool act;
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
+      const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
       act = u_act(c, j, i);
       if (act) {
-        const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);
         g = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp)
                    : ((i < nx) ? (Pp[j * nx + i] - Pp[j * nx + i - 1]) / c.dx : -2.0 * Pp[j * nx + nx - 1] / c.dx) / c.rho;
       }
     } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
+      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
       act = v_act(c, j, i);
       if (act) {
-        const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
         g = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp) : (Pp[j * nx + i] - Pp[(j - 1) * nx + i]) / c.dy / c.rho;
       }
     }
@@ -574,7 +638,7 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
     block_max_atomic<1>(v1, d1);
   }
 }
-__global__ __launch_bounds__(256) void k_update(const CaseDev* cases, UpdArgs a) {
+__global__ __launch_bounds__(256) void k_update(const CaseDev* __restrict__ cases, UpdArgs a) {
   upd_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
 }
 
@@ -584,8 +648,8 @@ struct PresArgs { int b_id, x_id, out_id; int mode; int slot0; int res_slot; int
 // Row (j, i) of the pressure operator A applied to X: (A X)_row, sum |a||x| over the row and the
 // row's nonzero count.  Override rows come from their CSR rows; the others are the rectangle's
 // 5-point stencil (outlet column Dirichlet, other walls Neumann).
-DEV void p_row(const CaseDev& c, const double* X, int idx, int j, int i, double& ax, double& aax, int& nnz) {
-  const int r = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
+template <class P>
+DEV void p_row(const CaseDev& c, P X, int idx, int j, int i, int r, double& ax, double& aax, int& nnz) {
   if (r >= 0) { ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz); return; }
   const int nx = c.nx, ny = c.ny;
   const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
@@ -605,16 +669,17 @@ DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
   if (bx * (int)blockDim.x >= c.psize) return;   // no cell of this case for this block
   if (a.pred_slot >= 0 && !pred_refine(c.sl, a.slot0, a.pred_slot)) return;
   const int nx = c.nx;
-  const double* B = c.buf[a.b_id];
-  const double* X = c.buf[a.x_id];
-  double* O = a.mode == 0 ? c.buf[a.out_id] : nullptr;
+  const gptr<const double> B = G(c.buf[a.b_id]);
+  const gptr<const double> X = G(c.buf[a.x_id]);
+  const gptr<double> O = a.mode == 0 ? G(c.buf[a.out_id]) : nullptr;
   double mres = 0.0, mrnd = 0.0, msc = 0.0, ssr = 0.0, ssv = 0.0;
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double res = 0.0;
+    const int orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);   // looked up together with the solid test
     if (p_act(c, j, i)) {
       double ax, aax; int nnz;
-      p_row(c, X, idx, j, i, ax, aax, nnz);
+      p_row(c, X, idx, j, i, orow, ax, aax, nnz);
       const double b = B[idx];
       res = b - ax;
       const double sc = aax + fabs(b);
@@ -635,17 +700,16 @@ DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
   } else {
     // mode 1: final statistics; mode 3 (lean solve, no refinement): first residual and final
     // statistics of the same x in one pass
-    double v4[4] = {mres, mrnd, msc, mres};
-    u64* const d4s[4] = {s + (b0 + SV_FINAL) * NSTRIPE, s + (b0 + SV_ROUND) * NSTRIPE, s + (b0 + SV_ASCALE) * NSTRIPE,
-                         a.mode == 3 ? s + (b0 + a.res_slot) * NSTRIPE : nullptr};
-    block_max_atomic<4>(v4, d4s);
-    __syncthreads();
-    block_sum_atomic(ssr, reinterpret_cast<double*>(s + (b0 + SV_SSQR) * NSTRIPE));
-    __syncthreads();
-    block_sum_atomic(ssv, reinterpret_cast<double*>(s + (b0 + SV_SSQV) * NSTRIPE));
+    const double v3[3] = {mres, mrnd, msc};
+    u64* const d3[3] = {s + (b0 + SV_FINAL) * NSTRIPE, s + (b0 + SV_ROUND) * NSTRIPE, s + (b0 + SV_ASCALE) * NSTRIPE};
+    u64* const r3[3] = {a.mode == 3 ? s + (b0 + a.res_slot) * NSTRIPE : nullptr, nullptr, nullptr};
+    const double sums[2] = {ssr, ssv};
+    double* const sd[2] = {reinterpret_cast<double*>(s + (b0 + SV_SSQR) * NSTRIPE),
+                           reinterpret_cast<double*>(s + (b0 + SV_SSQV) * NSTRIPE)};
+    block_stats_atomic<3, 2>(v3, d3, r3, sums, sd);
   }
 }
-__global__ __launch_bounds__(256) void k_pres(const CaseDev* cases, PresArgs a) {
+__global__ __launch_bounds__(256) void k_pres(const CaseDev* __restrict__ cases, PresArgs a) {
   pres_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
 }
 
@@ -661,8 +725,8 @@ struct GemmArgs {
   const int* gate = nullptr;   // when set, the launch runs only while *gate != 0
 };
 
-DEV const double* desc_in(const SolveDesc& d, int id) { return d.bufs[id] + d.off; }
-DEV double* desc_out(const SolveDesc& d, int id) { return d.bufs[id] + d.off; }
+DEV gptr<const double> desc_in(const SolveDesc& d, int id) { return G(G(d.bufs)[id]) + d.off; }
+DEV gptr<double> desc_out(const SolveDesc& d, int id) { return G(G(d.bufs)[id]) + d.off; }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -684,11 +748,11 @@ DEV void xcd_tile(int& z, int& cx, int& ry) {
 // One output element of a transform: forward into the scratch F; inverse into the output array
 // with the solid mask (epi 0 store, 1 masked store, 2 masked accumulate)
 DEV void gemm_put(const SolveDesc& d, const GemmArgs& g, int gr, int gc, double v) {
-  if (g.forward) { d.F[(size_t)gr * d.xx + gc] = v; return; }
-  double* O = desc_out(d, g.out_id);
+  if (g.forward) { G(d.F)[(size_t)gr * d.xx + gc] = v; return; }
+  const gptr<double> O = desc_out(d, g.out_id);
   const int ai = gc + d.ioff, aj = gr + d.joff;
-  const int slo = d.lo ? d.lo[aj] : 1, shi = d.lo ? d.hi[aj] : 0;
-  double* pp = O + (size_t)gr * d.ld + gc;
+  const int slo = d.lo ? G(d.lo)[aj] : 1, shi = d.lo ? G(d.hi)[aj] : 0;
+  const gptr<double> pp = O + (size_t)gr * d.ld + gc;
   if (g.epi == 0) *pp = v;
   else if (slo <= ai && ai <= shi) *pp = 0.0;
   else *pp = (g.epi == 2) ? (*pp + v) : v;
@@ -704,7 +768,7 @@ DEV void gemm_put(const SolveDesc& d, const GemmArgs& g, int gr, int gc, double 
 // of KC staged in LDS; the autotuner picks the shape per context.  Every shape accumulates each
 // output element over k in the same order, so all shapes give identical results.
 template <int TM, int TN, int KC>
-__global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g) {
+__global__ __launch_bounds__(256) void k_gemm(const SolveDesc* __restrict__ descs, GemmArgs g) {
   constexpr int MT = TM / 32, NT = TN / 32;
   int z, cx, ry;
   xcd_tile(z, cx, ry);
@@ -714,11 +778,11 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g
   if (row0 >= M || col0 >= N) return;
   if (g.gate && *g.gate == 0) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
-  const double* AT = g.forward ? d.V : d.VT;
-  const double* Bm;
+  const gptr<const double> AT = G(g.forward ? d.V : d.VT);
+  gptr<const double> Bm;
   int ldb;
   if (g.forward) { Bm = desc_in(d, g.in_id); ldb = d.ld; }
-  else { Bm = d.F; ldb = d.xx; }
+  else { Bm = G(d.F); ldb = d.xx; }
   __shared__ double As[KC][TM];
   __shared__ double Bs[KC][TN];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -794,7 +858,7 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* descs, GemmArgs g
 // LDS in slice order.  The sum order differs from the staged tiles', so this variant agrees with
 // them to round-off.
 constexpr int SK_S = 8, SK_KB = 8;
-__global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* descs, GemmArgs g) {
+__global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restrict__ descs, GemmArgs g) {
   int z, cx, ry;
   xcd_tile(z, cx, ry);
   const SolveDesc& d = descs[z];
@@ -803,14 +867,14 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* descs, G
   if (row0 >= M || col0 >= N) return;
   if (g.gate && *g.gate == 0) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
-  const double* AT = g.forward ? d.V : d.VT;
-  const double* Bm;
+  const gptr<const double> AT = G(g.forward ? d.V : d.VT);
+  gptr<const double> Bm;
   int ldb;
   if (g.forward) { Bm = desc_in(d, g.in_id); ldb = d.ld; }
-  else { Bm = d.F; ldb = d.xx; }
+  else { Bm = G(d.F); ldb = d.xx; }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, l16 = lane & 15;
-  const int G = (K + 3) >> 2, per = (G + SK_S - 1) / SK_S;
-  const int g0 = min(G, w * per), g1 = min(G, g0 + per);
+  const int NG = (K + 3) >> 2, per = (NG + SK_S - 1) / SK_S;
+  const int g0 = min(NG, w * per), g1 = min(NG, g0 + per);
   const int ra[2] = {row0 + l16, row0 + 16 + l16}, cb[2] = {col0 + l16, col0 + 16 + l16};
   const bool va[2] = {ra[0] < M, ra[1] < M}, vb[2] = {cb[0] < N, cb[1] < N};
   d4 acc[2][2];
@@ -825,8 +889,8 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* descs, G
     for (int t = 0; t < SK_KB; ++t) {
       const int kk = 4 * (gb + t) + kq;
       const bool ok = gb + t < g1 && kk < K;
-      const double* Ar = AT + (size_t)(ok ? kk : 0) * M;
-      const double* Br = Bm + (size_t)(ok ? kk : 0) * ldb;
+      const gptr<const double> Ar = AT + (size_t)(ok ? kk : 0) * M;
+      const gptr<const double> Br = Bm + (size_t)(ok ? kk : 0) * ldb;
       #pragma unroll
       for (int q = 0; q < 2; ++q) {
         a[t][q] = ok && va[q] ? Ar[ra[q]] : 0.0;
@@ -901,7 +965,7 @@ DEV double block_affine_scan(double A, double B, bool reverse) {
 // points per thread; every shape covers rows up to TRI_MAXX = 4096 points.
 constexpr int TRI_MAXX = 4096;
 template <int TRI_T>
-__global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p) {
+__global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ descs, PcrArgs p) {
   constexpr int TRI_C = TRI_MAXX / TRI_T;
   const SolveDesc& d = descs[blockIdx.y];
   const int k = blockIdx.x;
@@ -911,23 +975,12 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p
   const int xx = d.xx, yy = d.yy, tid = threadIdx.x;
   const int C = (xx + TRI_T - 1) / TRI_T;
   const int i0 = min(tid * C, xx), n = min(i0 + C, xx) - i0;
-  double* Frow = d.F + (size_t)k * xx;
-  const double* Lrow = d.luL + (size_t)k * xx;
-  const double* Irow = d.luI + (size_t)k * xx;
-  const double cu = d.b * d.upper[0];
+  const gptr<double> Frow = G(d.F) + (size_t)k * xx;
+  const gptr<const double> Lrow = G(d.luL) + (size_t)k * xx;
+  const gptr<const double> Irow = G(d.luI) + (size_t)k * xx;
+  const double cu = d.b * G(d.upper)[0];
   __shared__ double zl[TRI_MAXX];
-  if (p.mode == 1) {
-    // z_k = V^T (rows of the capacitance correction scattered by y), dense only on the
-    // x columns that hold a correction row
-    for (int i = tid; i < xx; i += TRI_T) zl[i] = 0.0;
-    __syncthreads();
-    for (int gq = tid; gq < d.rg_n; gq += TRI_T) {
-      double s = 0.0;
-      for (int t = d.rg_ptr[gq]; t < d.rg_ptr[gq + 1]; ++t) s += d.V[(size_t)d.rc_j[t] * yy + k] * d.y[d.rc_id[t]];
-      zl[d.rg_x[gq]] = s;
-    }
-    __syncthreads();
-  }
+  // the factor rows first: their loads stay in flight while mode 1 gathers its right-hand side
   double r[TRI_C], l[TRI_C], id[TRI_C];
   #pragma unroll
   for (int c = 0; c < TRI_C; ++c) {
@@ -935,8 +988,45 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p
     if (c < n) {
       l[c] = Lrow[i];
       id[c] = Irow[i];
-      r[c] = (p.mode == 0) ? Frow[i] : zl[i];
+      r[c] = (p.mode == 0) ? Frow[i] : 0.0;
     } else { l[c] = 0.0; id[c] = 0.0; r[c] = 0.0; }
+  }
+  if (p.mode == 1) {
+    // z_k = V^T (rows of the capacitance correction scattered by y), dense only on the
+    // x columns that hold a correction row.  A column's rows are read in batches of ZB (indices
+    // past the group clamped to its last row, not added), products added in row order.
+    constexpr int ZB = 4;
+    for (int i = tid; i < xx; i += TRI_T) zl[i] = 0.0;
+    __syncthreads();
+    const gptr<const int> rg_ptr = G(d.rg_ptr), rg_x = G(d.rg_x), rc_j = G(d.rc_j), rc_id = G(d.rc_id);
+    const gptr<const double> V = G(d.V), yv = G(d.y);
+    for (int gq = tid; gq < d.rg_n; gq += TRI_T) {
+      const int t0 = rg_ptr[gq], t1 = rg_ptr[gq + 1], xg = rg_x[gq];
+      double s = 0.0;
+      for (int t = t0; t < t1; t += ZB) {
+        int jr[ZB], ir[ZB];
+        double vr[ZB], yr[ZB];
+        #pragma unroll
+        for (int u = 0; u < ZB; ++u) {
+          const int tt = min(t + u, t1 - 1);
+          jr[u] = rc_j[tt];
+          ir[u] = rc_id[tt];
+        }
+        #pragma unroll
+        for (int u = 0; u < ZB; ++u) {
+          vr[u] = V[(size_t)jr[u] * yy + k];
+          yr[u] = yv[ir[u]];
+        }
+        #pragma unroll
+        for (int u = 0; u < ZB; ++u)
+          if (t + u < t1) s += vr[u] * yr[u];
+      }
+      zl[xg] = s;
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int c = 0; c < TRI_C; ++c)
+      if (c < n) r[c] = zl[i0 + c];
   }
   // forward: y_i = r_i - l_i y_{i-1}
   double A = 1.0, B = 0.0;
@@ -973,40 +1063,68 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* descs, PcrArgs p
 }
 
 // first = R^{-1} b at the capacitance points: one wave per point, lanes over the y modes
-__global__ __launch_bounds__(256) void k_points(const SolveDesc* descs, PcrArgs p) {
+__global__ __launch_bounds__(256) void k_points(const SolveDesc* __restrict__ descs, PcrArgs p) {
   const SolveDesc& d = descs[blockIdx.y];
   const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blockIdx.x * 4 >= d.npts) return;
   if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
   if (q >= d.npts) return;
   const int yy = d.yy, xx = d.xx, lane = threadIdx.x & 63;
-  const int x = d.pt_x[q];
-  const double* Vr = d.V + (size_t)d.pt_j[q] * yy;
+  const int x = G(d.pt_x)[q];
+  const gptr<const double> Vr = G(d.V) + (size_t)G(d.pt_j)[q] * yy;
+  const gptr<const double> F = G(d.F);
+  // batches of PT_U modes per lane: the batch's loads go out together; products added in mode order
+  constexpr int PT_U = 8;
   double s = 0.0;
-  for (int kk = lane; kk < yy; kk += 64) s += Vr[kk] * d.F[(size_t)kk * xx + x];
+  for (int k0 = lane; k0 < yy; k0 += 64 * PT_U) {
+    double a[PT_U], b[PT_U];
+    #pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const int kk = min(k0 + 64 * u, yy - 1);
+      a[u] = Vr[kk];
+      b[u] = F[(size_t)kk * xx + x];
+    }
+    #pragma unroll
+    for (int u = 0; u < PT_U; ++u)
+      if (k0 + 64 * u < yy) s += a[u] * b[u];
+  }
   s = wave_sum(s);
-  if (lane == 0) d.firstK[q] = s;
+  if (lane == 0) G(d.firstK)[q] = s;
 }
 
 // y = (S^{-1} C) first: one wave per row of the precomputed r x npts matrix S^{-1} C
-__global__ __launch_bounds__(256) void k_capacitance(const SolveDesc* descs, PcrArgs p) {
+__global__ __launch_bounds__(256) void k_capacitance(const SolveDesc* __restrict__ descs, PcrArgs p) {
   const SolveDesc& d = descs[blockIdx.y];
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blockIdx.x * 4 >= d.r) return;
   if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
   if (row >= d.r) return;
   const int lane = threadIdx.x & 63, np = d.npts;
-  const double* Sr = d.SC + (size_t)row * np;
+  const gptr<const double> Sr = G(d.SC) + (size_t)row * np;
+  const gptr<const double> fk = G(d.firstK);
+  // batches of CAP_U points per lane: the batch's loads go out together; products added in point order
+  constexpr int CAP_U = 16;
   double s = 0.0;
-  for (int q = lane; q < np; q += 64) s += Sr[q] * d.firstK[q];
+  for (int q0 = lane; q0 < np; q0 += 64 * CAP_U) {
+    double a[CAP_U], b[CAP_U];
+    #pragma unroll
+    for (int u = 0; u < CAP_U; ++u) {
+      const int q = min(q0 + 64 * u, np - 1);
+      a[u] = Sr[q];
+      b[u] = fk[q];
+    }
+    #pragma unroll
+    for (int u = 0; u < CAP_U; ++u)
+      if (q0 + 64 * u < np) s += a[u] * b[u];
+  }
   s = wave_sum(s);
-  if (lane == 0) d.y[row] = s;
+  if (lane == 0) G(d.y)[row] = s;
 }
 
 // Records the pair of a first pressure Schur solve: rhs = C first (first = R^{-1} b at the
 // capacitance points) and the multiplier y = S^{-1} rhs, both in boundary-angle order, tagged with
 // the step and the projection.  One block per problem; a full ring stops recording.
-__global__ __launch_bounds__(256) void k_dump(const SolveDesc* descs, int proj) {
+__global__ __launch_bounds__(256) void k_dump(const SolveDesc* __restrict__ descs, int proj) {
   const SolveDesc& d = descs[blockIdx.y];
   if (!d.dump_y) return;
   __shared__ int slot;
@@ -1033,13 +1151,13 @@ __global__ __launch_bounds__(256) void k_dump(const SolveDesc* descs, int proj) 
 struct FillArgs { int x_id; int vel; int inlet; int track; };
 
 DEV void fill_body(const CaseDev& c, const FillArgs& a, int bx, int gx) {
-  double* X = c.buf[a.x_id];
+  const gptr<double> X = G(c.buf[a.x_id]);
   int t = bx * blockDim.x + threadIdx.x;
   double mu = 0.0, mv = 0.0;
-  auto row = [&](const GhostTab& g, int k, const double* base) {
+  auto row = [&](const GhostTab& g, int k, gptr<const double> base) {
     double s = 0.0;
-    const int* cl = g.col + (size_t)k * g.width;
-    const double* wt = g.w + (size_t)k * g.width;
+    const gptr<const int> cl = G(g.col) + (size_t)k * g.width;
+    const gptr<const double> wt = G(g.w) + (size_t)k * g.width;
     if (g.width == 8) {
       #pragma unroll
       for (int q = 0; q < 8; ++q) s += wt[q] * base[cl[q]];
@@ -1053,17 +1171,17 @@ DEV void fill_body(const CaseDev& c, const FillArgs& a, int bx, int gx) {
     const GhostTab& gv = c.gt[1];
     if (t < gu.n) {
       const double s = row(gu, t, X);
-      X[gu.node[t]] = s; mu = s;
+      X[G(gu.node)[t]] = s; mu = s;
     } else if ((t -= gu.n) < gv.n) {
       const double s = row(gv, t, X + c.usize);
-      X[c.usize + gv.node[t]] = s; mv = s;
+      X[c.usize + G(gv.node)[t]] = s; mv = s;
     } else if ((t -= gv.n) < c.ny) {
-      const double v = a.inlet ? c.inlet[t] : 0.0;
+      const double v = a.inlet ? G(c.inlet)[t] : 0.0;
       X[t * (c.nx + 1)] = v; mu = v;
     }
   } else {
     const GhostTab& gp = c.gt[2];
-    if (t < gp.n) X[gp.node[t]] = row(gp, t, X);
+    if (t < gp.n) X[G(gp.node)[t]] = row(gp, t, X);
   }
   if (a.vel && a.track) {
     double v2[2] = {mu, mv};
@@ -1071,17 +1189,17 @@ DEV void fill_body(const CaseDev& c, const FillArgs& a, int bx, int gx) {
     block_max_atomic<2>(v2, d2);
   }
 }
-__global__ __launch_bounds__(256) void k_fill(const CaseDev* cases, FillArgs a) {
+__global__ __launch_bounds__(256) void k_fill(const CaseDev* __restrict__ cases, FillArgs a) {
   fill_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
 }
 
 // half_pressure = pressure + increment; output = half + factor * increment
 struct PressArgs { int p_id, inc_id, half_id, out_id; };
 DEV void press_body(const CaseDev& c, const PressArgs& a, int bx, int gx) {
-  const double* P = c.buf[a.p_id];
-  const double* I = c.buf[a.inc_id];
-  double* H = c.buf[a.half_id];
-  double* O = c.buf[a.out_id];
+  const gptr<const double> P = G(c.buf[a.p_id]);
+  const gptr<const double> I = G(c.buf[a.inc_id]);
+  const gptr<double> H = G(c.buf[a.half_id]);
+  const gptr<double> O = G(c.buf[a.out_id]);
   const double factor = c.prm[2];
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const double inc = I[idx];
@@ -1090,7 +1208,7 @@ DEV void press_body(const CaseDev& c, const PressArgs& a, int bx, int gx) {
     O[idx] = half + factor * inc;
   }
 }
-__global__ __launch_bounds__(256) void k_pressures(const CaseDev* cases, PressArgs a) {
+__global__ __launch_bounds__(256) void k_pressures(const CaseDev* __restrict__ cases, PressArgs a) {
   press_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
 }
 
@@ -1115,7 +1233,7 @@ DEV void run_body(const CaseDev& c, const Body& b, int bx) {
     default: break;
   }
 }
-__global__ __launch_bounds__(256) void k_fused(const CaseDev* cases, Fused f) {
+__global__ __launch_bounds__(256) void k_fused(const CaseDev* __restrict__ cases, Fused f) {
   const CaseDev& c = cases[blockIdx.y];
   int bx = blockIdx.x;
   #pragma unroll
@@ -1137,7 +1255,7 @@ DEV void run_kind(const CaseDev& c, const Body& b, int bx) {
   else if constexpr (K == BK_FILL) fill_body(c, b.f, bx, b.nblk);
 }
 template <int K0, int K1, int K2>
-__global__ __launch_bounds__(256) void k_fused_of(const CaseDev* cases, Fused f) {
+__global__ __launch_bounds__(256) void k_fused_of(const CaseDev* __restrict__ cases, Fused f) {
   const CaseDev& c = cases[blockIdx.y];
   int bx = blockIdx.x;
   if (bx < f.b[0].nblk) { run_kind<K0>(c, f.b[0], bx); return; }
@@ -1158,18 +1276,20 @@ constexpr int FORCE_PPW = 4;      // points per wave
 constexpr int FORCE_MAXP = 4096;  // surface points
 __host__ __device__ inline int force_blocks(int nsurf) { return (nsurf + FORCE_PPW - 1) / FORCE_PPW; }
 struct ForceArgs { int vel_id, p_id; double mu; };
-__global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a) {
+__global__ __launch_bounds__(64) void k_forces(const CaseDev* __restrict__ cases, ForceArgs a) {
   const CaseDev& c = cases[blockIdx.y];
-  const double* Vel = c.buf[a.vel_id];
-  const double* Pp = c.buf[a.p_id];
+  const gptr<const double> Vel = G(c.buf[a.vel_id]);
+  const gptr<const double> Pp = G(c.buf[a.p_id]);
   const int sub = threadIdx.x & 15;
   const int k = blockIdx.x * FORCE_PPW + (threadIdx.x >> 4);
   if (blockIdx.x * FORCE_PPW >= c.nsurf) return;
   double ps = 0.0, du = 0.0, dv = 0.0;
   if (k < c.nsurf) {
-    for (int t = c.pp[k] + sub; t < c.pp[k + 1]; t += 16) ps += c.pv[t] * Pp[c.pc[t]];
-    for (int t = c.up[k] + sub; t < c.up[k + 1]; t += 16) du += c.uv[t] * Vel[c.uc[t]];
-    for (int t = c.vp[k] + sub; t < c.vp[k + 1]; t += 16) dv += c.vv[t] * Vel[c.usize + c.vc[t]];
+    const gptr<const int> pp = G(c.pp), pc = G(c.pc), up = G(c.up), uc = G(c.uc), vp = G(c.vp), vc = G(c.vc);
+    const gptr<const double> pv = G(c.pv), uv = G(c.uv), vv = G(c.vv);
+    for (int t = pp[k] + sub; t < pp[k + 1]; t += 16) ps += pv[t] * Pp[pc[t]];
+    for (int t = up[k] + sub; t < up[k + 1]; t += 16) du += uv[t] * Vel[uc[t]];
+    for (int t = vp[k] + sub; t < vp[k + 1]; t += 16) dv += vv[t] * Vel[c.usize + vc[t]];
   }
   for (int o = 8; o > 0; o >>= 1) {
     ps += __shfl_xor(ps, o, 64);
@@ -1178,10 +1298,11 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a
   }
   double s[4] = {0.0, 0.0, 0.0, 0.0};  // pressure x, y; traction x, y
   if (k < c.nsurf && sub == 0) {
-    const double nxk = c.nrm[2 * k], nyk = c.nrm[2 * k + 1];
+    const gptr<const double> nrm = G(c.nrm), nw = G(c.nw);
+    const double nxk = nrm[2 * k], nyk = nrm[2 * k + 1];
     const double dot = du * nxk + dv * nyk;
-    s[0] = ps * c.nw[2 * k];
-    s[1] = ps * c.nw[2 * k + 1];
+    s[0] = ps * nw[2 * k];
+    s[1] = ps * nw[2 * k + 1];
     const double mu = a.mu > 0.0 ? a.mu : c.nu * c.rho;
     s[2] = mu * (du + nxk * dot);
     s[3] = mu * (dv + nyk * dot);
@@ -1189,7 +1310,7 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* cases, ForceArgs a
   #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const double v = wave_sum(s[q]);
-    if (threadIdx.x == 0) c.fpart[blockIdx.x * 4 + q] = v;
+    if (threadIdx.x == 0) G(c.fpart)[blockIdx.x * 4 + q] = v;
   }
 }
 
@@ -1201,18 +1322,20 @@ DEV void force_totals(const CaseDev& c, double (&vals)[6]) {
   const int lane = threadIdx.x & 63;
   const int nb = force_blocks(c.nsurf);
   double p[4] = {0.0, 0.0, 0.0, 0.0};
+  const gptr<const double> fp = G(c.fpart);
   for (int b = lane; b < nb; b += 64)
-    for (int q = 0; q < 4; ++q) p[q] += c.fpart[b * 4 + q];
+    for (int q = 0; q < 4; ++q) p[q] += fp[b * 4 + q];
   for (int q = 0; q < 4; ++q) lt[q][lane] = p[q];
   __syncthreads();
   double t[4] = {0.0, 0.0, 0.0, 0.0};
+  #pragma unroll 8
   for (int l = 0; l < 64; ++l)
     for (int q = 0; q < 4; ++q) t[q] += lt[q][l];
   const double scale = M_PI * c.diameter / c.nsurf;
   const double fxp = -t[0], fyp = -t[1], fxv = t[2] * scale, fyv = t[3] * scale;
   vals[0] = fxp + fxv; vals[1] = fyp + fyv; vals[2] = fxp; vals[3] = fyp; vals[4] = fxv; vals[5] = fyv;
 }
-__global__ __launch_bounds__(64) void k_force_total(const CaseDev* cases) {
+__global__ __launch_bounds__(64) void k_force_total(const CaseDev* __restrict__ cases) {
   const CaseDev& c = cases[blockIdx.y];
   double vals[6];
   force_totals(c, vals);
@@ -1262,14 +1385,14 @@ __constant__ const int kFinSlots[FIN_NQ] = {
 // Every lane stores its stripe of every slot in LDS; lane q then reduces slot q over the stripes
 // (maxima of |.|, which do not depend on the order, or sums in stripe order); lane 0 evaluates the
 // gates and the per-solve records; the row is stored once, one entry per lane.
-__global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a) {
+__global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cases, FinArgs a) {
   const CaseDev& c = cases[blockIdx.y];
-  const u64* s = c.sl;
+  const gptr<const u64> s = G(c.sl);
   const int lane = threadIdx.x;
   // step parameters and clock, loaded together with the slots
-  const double dt = c.prm[0];
-  const int row = *c.clock_step;
-  const double time = *c.clock_time + dt;
+  const double dt = G(c.prm)[0];
+  const int row = *G(c.clock_step);
+  const double time = *G(c.clock_time) + dt;
   __shared__ double raw[FIN_NQ][65];   // padded: lane q reads row q
   __shared__ double v[FIN_NQ];
   __shared__ double d[MCF_DIAG_LEN];
@@ -1281,8 +1404,13 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
   __syncthreads();
   if (lane < FIN_NQ) {
     double r = 0.0;
-    if (lane < FIN_MAX) { for (int t = 0; t < 64; ++t) r = nanmax(r, fabs(raw[lane][t])); }
-    else { for (int t = 0; t < 64; ++t) r += raw[lane][t]; }
+    if (lane < FIN_MAX) {
+      #pragma unroll 8
+      for (int t = 0; t < 64; ++t) r = nanmax(r, fabs(raw[lane][t]));
+    } else {
+      #pragma unroll 8
+      for (int t = 0; t < 64; ++t) r += raw[lane][t];
+    }
     v[lane] = r;
   }
   __syncthreads();
@@ -1310,10 +1438,9 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
     if (!(mres <= 1e-10 * fmax(1.0, rhs_max))) status |= 1;
     int refine = 0;
     double worst = -1.0, relmax = 0.0;
-    // one record per pressure solve in call order: the startup projection comes first
-    const int order3[3] = {2, 0, 1};
+    // one record per pressure solve in call order: the startup projection (block 2) comes first
     for (int q = 0; q < a.nproj; ++q) {
-      const int b = a.nproj == 3 ? order3[q] : q;
+      const int b = a.nproj == 3 ? (q == 0 ? 2 : q - 1) : q;
       const double pfin = v[17 + 6 * b], prnd = v[18 + 6 * b];
       const SolveSlots ss{v[14 + 6 * b], v[15 + 6 * b], v[16 + 6 * b], pfin, prnd, v[19 + 6 * b],
                           v[FIN_MAX + 2 * b], v[FIN_MAX + 2 * b + 1], v[32 + b], v[35 + b], v[38 + b]};
@@ -1349,17 +1476,17 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* cases, FinArgs a
     d[MCF_D_STATUS] = status;
   }
   __syncthreads();
-  c.diag[(size_t)row * MCF_DIAG_LEN + lane] = d[lane];
+  G(c.diag)[(size_t)row * MCF_DIAG_LEN + lane] = d[lane];
   if (lane == 0) {
-    *c.clock_step = row + 1;
-    *c.clock_time = time;
+    *G(c.clock_step) = row + 1;
+    *G(c.clock_time) = time;
   }
-  u64* wsl = c.sl;
+  const gptr<u64> wsl = G(c.sl);
   for (int k = lane; k < NSLOTW; k += 64) wsl[k] = 0ull;
 }
 
 // Step parameters of every case of the launch; dt <= 0 keeps each case's own dt
-__global__ void k_set_prm(const CaseDev* cases, double dt, double alpha, double factor) {
+__global__ void k_set_prm(const CaseDev* __restrict__ cases, double dt, double alpha, double factor) {
   const CaseDev& c = cases[blockIdx.y];
   if (threadIdx.x == 0) {
     if (dt > 0.0) c.prm[0] = dt;
@@ -1368,19 +1495,19 @@ __global__ void k_set_prm(const CaseDev* cases, double dt, double alpha, double 
   }
 }
 
-__global__ void k_set_clock(const CaseDev* cases, double t0) {
+__global__ void k_set_clock(const CaseDev* __restrict__ cases, double t0) {
   const CaseDev& c = cases[blockIdx.y];
   if (threadIdx.x == 0) { *c.clock_step = 0; *c.clock_time = t0; }
 }
 
-__global__ void k_clear_slots(const CaseDev* cases) {
+__global__ void k_clear_slots(const CaseDev* __restrict__ cases) {
   const CaseDev& c = cases[blockIdx.y];
   for (int k = threadIdx.x; k < NSLOTW; k += blockDim.x) c.sl[k] = 0ull;
 }
 
 // Checkpoint of what a steady step reads from earlier steps: the state S[parity], the advection
 // history H[1 - parity] and the half pressure (save = 1: copy out, 0: copy back)
-__global__ __launch_bounds__(256) void k_ckpt(const CaseDev* cases, int parity, int save) {
+__global__ __launch_bounds__(256) void k_ckpt(const CaseDev* __restrict__ cases, int parity, int save) {
   const CaseDev& c = cases[blockIdx.y];
   double* S = c.buf[B_S0 + parity];
   double* H = c.buf[B_H0 + 1 - parity];
@@ -1398,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_ckpt(const CaseDev* cases, int parity, 
   }
 }
 
-__global__ __launch_bounds__(256) void k_copy(const CaseDev* cases, int src, int dst, int n) {
+__global__ __launch_bounds__(256) void k_copy(const CaseDev* __restrict__ cases, int src, int dst, int n) {
   const CaseDev& c = cases[blockIdx.y];
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < n) c.buf[dst][idx] = c.buf[src][idx];
@@ -1508,7 +1635,7 @@ DEV bool kr_active_cell(const CaseDev& c, int idx) {
 }
 
 // r = b on the active cells (0 elsewhere), x = 0; partial sums of b^2
-__global__ __launch_bounds__(KR_T) void k_kry_begin(const CaseDev* cases, KryArgs a, int b_id, int x_id, int r_id) {
+__global__ __launch_bounds__(KR_T) void k_kry_begin(const CaseDev* __restrict__ cases, KryArgs a, int b_id, int x_id, int r_id) {
   const CaseDev& c = cases[0];
   __shared__ double red[1][KR_T / 64];
   const double* B = c.buf[b_id];
@@ -1531,7 +1658,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_begin(const CaseDev* cases, KryArg
 }
 
 // out = A (scale[col] * raw V[col]) on the active cells
-__global__ __launch_bounds__(KR_T) void k_kry_matvec(const CaseDev* cases, KryArgs a, int out_id) {
+__global__ __launch_bounds__(KR_T) void k_kry_matvec(const CaseDev* __restrict__ cases, KryArgs a, int out_id) {
   const KryCtl& k = *a.ctl;
   if (!k.active) return;
   const CaseDev& c = cases[0];
@@ -1546,9 +1673,10 @@ __global__ __launch_bounds__(KR_T) void k_kry_matvec(const CaseDev* cases, KryAr
     if (idx >= c.psize) continue;
     const int j = idx / c.nx, i = idx - j * c.nx;
     double y = 0.0;
+    const int orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
     if (p_act(c, j, i)) {
       double ax, aax; int nnz;
-      p_row(c, X, idx, j, i, ax, aax, nnz);
+      p_row(c, X, idx, j, i, orow, ax, aax, nnz);
       y = sc * ax;
     }
     O[idx] = y;
@@ -1556,7 +1684,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_matvec(const CaseDev* cases, KryAr
 }
 
 // First Gram-Schmidt pass: partial sums of raw V[q] . w (q <= col) and of w . w
-__global__ __launch_bounds__(KR_T) void k_kry_dots(const CaseDev* cases, KryArgs a, int w_id) {
+__global__ __launch_bounds__(KR_T) void k_kry_dots(const CaseDev* __restrict__ cases, KryArgs a, int w_id) {
   const KryCtl& k = *a.ctl;
   if (!k.active) return;
   const CaseDev& c = cases[0];
@@ -1588,7 +1716,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_dots(const CaseDev* cases, KryArgs
 
 // Gram-Schmidt update w -= sum_q coef[q] raw V[q].  pass 1: store w and the partial sums of the
 // second projections V[q] . w; pass 2: store w as raw V[col + 1] with the partial sum of w . w.
-__global__ __launch_bounds__(KR_T) void k_kry_orth(const CaseDev* cases, KryArgs a, int w_id, int pass) {
+__global__ __launch_bounds__(KR_T) void k_kry_orth(const CaseDev* __restrict__ cases, KryArgs a, int w_id, int pass) {
   const KryCtl& k = *a.ctl;
   if (!k.active) return;
   const CaseDev& c = cases[0];
@@ -1645,7 +1773,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_orth(const CaseDev* cases, KryArgs
 }
 
 // Partial sums of raw V[0] . raw V[0] (a new cycle's first vector)
-__global__ __launch_bounds__(KR_T) void k_kry_norm(const CaseDev* cases, KryArgs a) {
+__global__ __launch_bounds__(KR_T) void k_kry_norm(const CaseDev* __restrict__ cases, KryArgs a) {
   const CaseDev& c = cases[0];
   __shared__ double red[1][KR_T / 64];
   const int base = blockIdx.x * KR_CHUNK + threadIdx.x;
@@ -1660,7 +1788,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_norm(const CaseDev* cases, KryArgs
 }
 
 // x += sum_{q < ncols} coef[q] raw V[q] (the cycle's update, coef = y * scale)
-__global__ __launch_bounds__(KR_T) void k_kry_xupd(const CaseDev* cases, KryArgs a, int x_id) {
+__global__ __launch_bounds__(KR_T) void k_kry_xupd(const CaseDev* __restrict__ cases, KryArgs a, int x_id) {
   const KryCtl& k = *a.ctl;
   const CaseDev& c = cases[0];
   __shared__ double cf[KR_MAXR + 1];
@@ -1687,7 +1815,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_xupd(const CaseDev* cases, KryArgs
 }
 
 // r = b - A x on the active cells (0 elsewhere); partial sums of r^2
-__global__ __launch_bounds__(KR_T) void k_kry_resid(const CaseDev* cases, KryArgs a, int b_id, int x_id, int r_id) {
+__global__ __launch_bounds__(KR_T) void k_kry_resid(const CaseDev* __restrict__ cases, KryArgs a, int b_id, int x_id, int r_id) {
   const CaseDev& c = cases[0];
   __shared__ double red[1][KR_T / 64];
   const double* B = c.buf[b_id];
@@ -1701,9 +1829,10 @@ __global__ __launch_bounds__(KR_T) void k_kry_resid(const CaseDev* cases, KryArg
     if (idx >= c.psize) continue;
     const int j = idx / c.nx, i = idx - j * c.nx;
     double r = 0.0;
+    const int orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
     if (p_act(c, j, i)) {
       double ax, aax; int nnz;
-      p_row(c, X, idx, j, i, ax, aax, nnz);
+      p_row(c, X, idx, j, i, orow, ax, aax, nnz);
       r = B[idx] - ax;
     }
     R[idx] = r;
@@ -1714,7 +1843,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_resid(const CaseDev* cases, KryArg
 }
 
 // x += y on the active cells (a refinement's correction)
-__global__ __launch_bounds__(256) void k_kry_axpy(const CaseDev* cases, int x_id, int y_id) {
+__global__ __launch_bounds__(256) void k_kry_axpy(const CaseDev* __restrict__ cases, int x_id, int y_id) {
   const CaseDev& c = cases[0];
   double* X = c.buf[x_id];
   const double* Y = c.buf[y_id];
@@ -1723,7 +1852,7 @@ __global__ __launch_bounds__(256) void k_kry_axpy(const CaseDev* cases, int x_id
 }
 
 // GMRES counts of pressure solve `psolve` into its slots (stripe 0)
-__global__ void k_kry_record(const CaseDev* cases, int psolve, double iters, double refine, double status) {
+__global__ void k_kry_record(const CaseDev* __restrict__ cases, int psolve, double iters, double refine, double status) {
   const CaseDev& c = cases[0];
   if (threadIdx.x != 0) return;
   c.sl[(SL_GM_IT + psolve) * NSTRIPE] = (u64)__double_as_longlong(iters);
