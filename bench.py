@@ -267,7 +267,7 @@ def main():
                    bound='mfma', achieved=gemm_achieved / 1e12, peak=FP64_PEAK / 1e12, unit='TFLOP/s',
                    frac=gemm_achieved / FP64_PEAK, traffic=GEMM_TRAFFIC.get(tile),
                    traffic_unit='bytes per launch', algorithmic_bytes=8.0 * (yy * yy + 2 * yy * xx), tile=tile,
-                   kernel=('k_gemm_sk' if tile == 3 else 'k_gemm') + ' (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
+                   kernel=('k_gemm_sk' if tile >= 3 else 'k_gemm') + ' (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
                    launch_ms=gemm_ms),
                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',

@@ -849,21 +849,24 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* __restrict__ desc
   }
 }
 
-// Split-K transform (launch variant 3): a block of SK_S waves computes a 32 x 32 output tile; wave w
-// takes the w-th of SK_S consecutive K slices, holds the whole tile in four independent 16 x 16
-// accumulators and reads its MFMA operands straight from global memory (the panels are L2
+// Split-K transform (launch variants 3 and 4): a block of SK_S waves computes a (16 MT) x 32 output
+// tile; wave w takes the w-th of SK_S consecutive K slices, holds the whole tile in 2 MT independent
+// 16 x 16 accumulators and reads its MFMA operands straight from global memory (the panels are L2
 // resident).  A slice's loads are issued together in batches of SK_KB k-groups (one batch covers
 // the slice for K <= 4 SK_S SK_KB), so a wave waits for memory once and then runs its MFMAs back
 // to back -- no LDS staging and no barrier in the K loop.  The slices' partial tiles are added in
-// LDS in slice order.  The sum order differs from the staged tiles', so this variant agrees with
-// them to round-off.
+// LDS in slice order.  Both tile heights slice K alike and give identical results; the sum order
+// differs from the staged tiles', so these variants agree with those to round-off.  The 16-row
+// tile doubles the number of blocks, which spreads the matrix-core work more evenly over the CUs
+// when the 32-row grid holds only a little more than one block per CU.
 constexpr int SK_S = 8, SK_KB = 8;
+template <int MT>
 __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restrict__ descs, GemmArgs g) {
   int z, cx, ry;
   xcd_tile(z, cx, ry);
   const SolveDesc& d = descs[z];
   const int M = d.yy, N = d.xx, K = d.yy;
-  const int row0 = ry * 32, col0 = cx * 32;
+  const int row0 = ry * (16 * MT), col0 = cx * 32;
   if (row0 >= M || col0 >= N) return;
   if (g.gate && *g.gate == 0) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
@@ -875,52 +878,62 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restri
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, l16 = lane & 15;
   const int NG = (K + 3) >> 2, per = (NG + SK_S - 1) / SK_S;
   const int g0 = min(NG, w * per), g1 = min(NG, g0 + per);
-  const int ra[2] = {row0 + l16, row0 + 16 + l16}, cb[2] = {col0 + l16, col0 + 16 + l16};
-  const bool va[2] = {ra[0] < M, ra[1] < M}, vb[2] = {cb[0] < N, cb[1] < N};
-  d4 acc[2][2];
+  // operand rows / columns past the matrix are read at the last one (in bounds, no predicate): they
+  // feed only output rows / columns that are not stored
+  int ra[MT];
   #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+  for (int mi = 0; mi < MT; ++mi) ra[mi] = min(row0 + 16 * mi + l16, M - 1);
+  const int cb[2] = {min(col0 + l16, N - 1), min(col0 + 16 + l16, N - 1)};
+  d4 acc[MT][2];
+  #pragma unroll
+  for (int mi = 0; mi < MT; ++mi)
     #pragma unroll
     for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = d4{0.0, 0.0, 0.0, 0.0};
   // k-group q covers k = 4q .. 4q+3; lane l holds A[m = l%16][k = 4q + l/16] and B[k][n = l%16]
   for (int gb = g0; gb < g1; gb += SK_KB) {
-    double a[SK_KB][2], b[SK_KB][2];
+    double a[SK_KB][MT], b[SK_KB][2];
     #pragma unroll
     for (int t = 0; t < SK_KB; ++t) {
+      // k past the slice or past K contributes zero: the load reads a valid k, the select zeroes it
       const int kk = 4 * (gb + t) + kq;
       const bool ok = gb + t < g1 && kk < K;
-      const gptr<const double> Ar = AT + (size_t)(ok ? kk : 0) * M;
-      const gptr<const double> Br = Bm + (size_t)(ok ? kk : 0) * ldb;
+      const int kc = min(kk, K - 1);
+      const gptr<const double> Ar = AT + kc * M;
+      const gptr<const double> Br = Bm + kc * ldb;
       #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        a[t][q] = ok && va[q] ? Ar[ra[q]] : 0.0;
-        b[t][q] = ok && vb[q] ? Br[cb[q]] : 0.0;
-      }
+      for (int q = 0; q < MT; ++q) { const double v = Ar[ra[q]]; a[t][q] = ok ? v : 0.0; }
+      #pragma unroll
+      for (int q = 0; q < 2; ++q) { const double v = Br[cb[q]]; b[t][q] = ok ? v : 0.0; }
     }
+    // keep every load of the batch ahead of the first MFMA (the scheduler would otherwise interleave
+    // them and expose a memory round trip per k-group)
+    __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
     for (int t = 0; t < SK_KB; ++t)
       #pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
+      for (int mi = 0; mi < MT; ++mi)
         #pragma unroll
         for (int ni = 0; ni < 2; ++ni)
           acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][mi], b[t][ni], acc[mi][ni], 0, 0, 0);
   }
   // partial tiles of the slices: red[slice][subtile][element][lane]; wave w then finishes
-  // elements 2 (w / 4) and 2 (w / 4) + 1 of subtile w % 4
-  __shared__ double red[SK_S][4][4][64];
+  // elements MT (w / NS) .. MT (w / NS) + MT - 1 of subtile w % NS
+  constexpr int NS = 2 * MT;
+  static_assert(NS * 4 == SK_S * MT, "each wave finishes MT (subtile, element) pairs");
+  __shared__ double red[SK_S][NS][4][64];
   #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+  for (int mi = 0; mi < MT; ++mi)
     #pragma unroll
     for (int ni = 0; ni < 2; ++ni)
       #pragma unroll
       for (int r = 0; r < 4; ++r) red[w][2 * mi + ni][r][lane] = acc[mi][ni][r];
   __syncthreads();
-  const int st = w & 3, mi = st >> 1, ni = st & 1;
+  const int st = w % NS, mi = st >> 1, ni = st & 1;
   const int gc = col0 + 16 * ni + l16;
   if (gc >= N) return;
   #pragma unroll
-  for (int e = 0; e < 2; ++e) {
-    const int r = 2 * (w >> 2) + e;
+  for (int e = 0; e < MT; ++e) {
+    const int r = MT * (w / NS) + e;
     const int gr = row0 + 16 * mi + kq + 4 * r;
     double v = red[0][st][r][lane];
     #pragma unroll
@@ -1053,13 +1066,20 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ des
       x = al * x + r[c] * id[c];
       r[c] = x;
     }
-  #pragma unroll
-  for (int c = 0; c < TRI_C; ++c)
-    if (c < n) {
-      const int i = i0 + c;
-      if (p.mode == 0) Frow[i] = r[c];
-      else Frow[i] -= r[c];
-    }
+  if (p.mode == 0) {
+    #pragma unroll
+    for (int c = 0; c < TRI_C; ++c)
+      if (c < n) Frow[i0 + c] = r[c];
+  } else {
+    // F <- F - T^{-1} z: the chunk's loads of F all go out before the first store
+    double f[TRI_C];
+    #pragma unroll
+    for (int c = 0; c < TRI_C; ++c) f[c] = (c < n) ? Frow[i0 + c] : 0.0;
+    __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+    for (int c = 0; c < TRI_C; ++c)
+      if (c < n) Frow[i0 + c] = f[c] - r[c];
+  }
 }
 
 // first = R^{-1} b at the capacitance points: one wave per point, lanes over the y modes
@@ -1113,6 +1133,7 @@ __global__ __launch_bounds__(256) void k_capacitance(const SolveDesc* __restrict
       a[u] = Sr[q];
       b[u] = fk[q];
     }
+    __builtin_amdgcn_sched_barrier(0);   // the batch's loads all go out before the first product
     #pragma unroll
     for (int u = 0; u < CAP_U; ++u)
       if (q0 + 64 * u < np) s += a[u] * b[u];
@@ -1995,8 +2016,8 @@ __global__ __launch_bounds__(KR_CT) void k_kry_ctl(KryArgs a, int phase) {
 
 }  // namespace
 
-constexpr int N_GEMM_V = 4, N_TRI_V = 3;
-constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64, 32}, GEMM_TN[N_GEMM_V] = {64, 32, 64, 32};
+constexpr int N_GEMM_V = 5, N_TRI_V = 3;
+constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64, 32, 16}, GEMM_TN[N_GEMM_V] = {64, 32, 64, 32, 32};
 constexpr int TRI_TH[N_TRI_V] = {128, 256, 512};
 struct Launch { int gemm_p = 0, gemm_h = 0, tri_p = 1, tri_h = 1; };
 
@@ -2008,7 +2029,8 @@ void launch_gemm(int v, int yy, int xx, int count, hipStream_t st, const SolveDe
   switch (v) {
     case 1: hipLaunchKernelGGL((k_gemm<32, 32, 64>), gg, dim3(256), 0, st, D, g); break;
     case 2: hipLaunchKernelGGL((k_gemm<64, 64, 32>), gg, dim3(256), 0, st, D, g); break;
-    case 3: hipLaunchKernelGGL(k_gemm_sk, gg, dim3(64 * SK_S), 0, st, D, g); break;
+    case 3: hipLaunchKernelGGL(k_gemm_sk<2>, gg, dim3(64 * SK_S), 0, st, D, g); break;
+    case 4: hipLaunchKernelGGL(k_gemm_sk<1>, gg, dim3(64 * SK_S), 0, st, D, g); break;
     default: hipLaunchKernelGGL((k_gemm<32, 64, 64>), gg, dim3(256), 0, st, D, g); break;
   }
 }

@@ -148,10 +148,10 @@ def test_graph_replays_match_eager(pkg, monkeypatch):
 
 
 def test_launch_shapes_agree(pkg):
-    """Every staged transform tile gives bit-identical steps; the split-K transform and every sweep
-    width agree to round-off."""
+    """Every staged transform tile gives bit-identical steps, and so do the two split-K tile heights;
+    the split-K transforms and every sweep width agree with the staged tiles to round-off."""
     runs = {}
-    for g, t in [(0, 1), (1, 1), (2, 1), (0, 0), (0, 2), (3, 1)]:
+    for g, t in [(0, 1), (1, 1), (2, 1), (0, 0), (0, 2), (3, 1), (4, 1)]:
         f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
         f.gg.device.set_launch(g, t)
         f.solve(n_steps=6, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
@@ -164,22 +164,25 @@ def test_launch_shapes_agree(pkg):
     for key in [(0, 0), (0, 2), (3, 1)]:
         for a, b in zip(runs[key], ref):
             assert np.max(abs(a - b)) <= 1e-12 * np.max(abs(b)), key
+    for a, b in zip(runs[(4, 1)], runs[(3, 1)]):
+        assert np.array_equal(a, b)
     f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
     f.solve(n_steps=2, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
     ln = f.gg.device.get_launch()
     print('autotuned launch shapes (440x82):', ln)
-    assert ln['chosen'] and all(0 <= ln[k] < 4 for k in ('gemm_pressure', 'gemm_velocity'))
+    assert ln['chosen'] and all(0 <= ln[k] < 5 for k in ('gemm_pressure', 'gemm_velocity'))
     assert all(0 <= ln[k] < 3 for k in ('tri_pressure', 'tri_velocity'))
 
 
+@pytest.mark.parametrize('tile', [3, 4], ids=['32-row', '16-row'])
 @pytest.mark.parametrize('kind', ['pressure', 'u', 'v'])
-def test_split_k_transform_inverse(pair88, kind):
-    """The split-K transform inside the rectangle inverse against the oracle's DCT/DST inverse."""
+def test_split_k_transform_inverse(pair88, kind, tile):
+    """The split-K transforms inside the rectangle inverse against the oracle's DCT/DST inverse."""
     ref, gpu = pair88
     dev = gpu.gg.device
     dev.set_dt(1e-3)
     before = dev.get_launch()
-    dev.set_launch(3, 1)
+    dev.set_launch(tile, 1)
     code = {'pressure': 0, 'u': 1, 'v': 2}[kind]
     factor = dev.pressure_factor if kind == 'pressure' else dev.helmholtz_factors[code - 1][0]
     b = np.random.default_rng(31).normal(size=factor.shape)
