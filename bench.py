@@ -245,7 +245,8 @@ def main():
     value = cells * args.steps / elapsed
     max_div = float(np.max(diag[:, 10]))
     # dominant kernels timed with HIP events on the library's stream
-    tile = dev.get_launch()['gemm_pressure']
+    launch = dev.get_launch()
+    tile = launch['gemm_pressure']
     lean_misses = None
     if args.no_kernel_timing:
         gemm_ms = pcr_ms = step_ms = None
@@ -272,7 +273,7 @@ def main():
                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
-               kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms, lean_misses=lean_misses),
+               kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms, lean_misses=lean_misses), launch=launch,
                speculation=dev.get_speculation(), setup_seconds=setup_s)
     if args.gmres_steps:
         print(f'bench: GMRES pressure backend, {args.gmres_steps} steps', file=sys.stderr, flush=True)
