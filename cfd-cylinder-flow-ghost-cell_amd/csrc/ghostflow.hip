@@ -977,6 +977,9 @@ DEV double block_affine_scan(double A, double B, bool reverse) {
 // Launch shapes (chosen per context by the autotuner): TRI_T threads per mode, up to TRI_C
 // points per thread; every shape covers rows up to TRI_MAXX = 4096 points.
 constexpr int TRI_MAXX = 4096;
+// column groups of capacitance rows per thread in the correction gather: a problem may hold up to
+// TRI_GPT x 128 such groups (the narrowest sweep has 128 threads)
+constexpr int TRI_GPT = 4, TRI_MAXG = TRI_GPT * 128;
 template <int TRI_T>
 __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ descs, PcrArgs p) {
   constexpr int TRI_C = TRI_MAXX / TRI_T;
@@ -1005,37 +1008,59 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ des
     } else { l[c] = 0.0; id[c] = 0.0; r[c] = 0.0; }
   }
   if (p.mode == 1) {
-    // z_k = V^T (rows of the capacitance correction scattered by y), dense only on the
-    // x columns that hold a correction row.  A column's rows are read in batches of ZB (indices
-    // past the group clamped to its last row, not added), products added in row order.
-    constexpr int ZB = 4;
-    for (int i = tid; i < xx; i += TRI_T) zl[i] = 0.0;
-    __syncthreads();
+    // z_k = V^T (rows of the capacitance correction scattered by y), dense only on the x columns
+    // that hold a correction row.  The rows (in x-column groups) are staged in passes of two per
+    // thread: the factors V[j_t][k] and y[t] of a pass load together into LDS.  The thread of a
+    // group (up to TRI_GPT groups per thread) then adds its rows' products in row order, and the
+    // group sums are scattered into the zeroed row, which reuses the staging array.
+    constexpr int P = 2 * TRI_T;   // rows per pass
+    static_assert(2 * P <= TRI_MAXX, "a pass's (V, y) pairs fit the row array");
     const gptr<const int> rg_ptr = G(d.rg_ptr), rg_x = G(d.rg_x), rc_j = G(d.rc_j), rc_id = G(d.rc_id);
     const gptr<const double> V = G(d.V), yv = G(d.y);
-    for (int gq = tid; gq < d.rg_n; gq += TRI_T) {
-      const int t0 = rg_ptr[gq], t1 = rg_ptr[gq + 1], xg = rg_x[gq];
-      double s = 0.0;
-      for (int t = t0; t < t1; t += ZB) {
-        int jr[ZB], ir[ZB];
-        double vr[ZB], yr[ZB];
-        #pragma unroll
-        for (int u = 0; u < ZB; ++u) {
-          const int tt = min(t + u, t1 - 1);
-          jr[u] = rc_j[tt];
-          ir[u] = rc_id[tt];
-        }
-        #pragma unroll
-        for (int u = 0; u < ZB; ++u) {
-          vr[u] = V[(size_t)jr[u] * yy + k];
-          yr[u] = yv[ir[u]];
-        }
-        #pragma unroll
-        for (int u = 0; u < ZB; ++u)
-          if (t + u < t1) s += vr[u] * yr[u];
-      }
-      zl[xg] = s;
+    const int ng = d.rg_n, nr = d.r;
+    int t0[TRI_GPT], t1[TRI_GPT], xg[TRI_GPT];
+    double sg[TRI_GPT];
+    #pragma unroll
+    for (int m = 0; m < TRI_GPT; ++m) {
+      const int gq = tid + TRI_T * m;
+      const bool has = gq < ng;
+      t0[m] = has ? rg_ptr[gq] : 0;
+      t1[m] = has ? rg_ptr[gq + 1] : 0;
+      xg[m] = has ? rg_x[gq] : -1;
+      sg[m] = 0.0;
     }
+    for (int base = 0; base < nr; base += P) {
+      int jr[2], ir[2];
+      #pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int t = min(base + tid + TRI_T * u, nr - 1);
+        jr[u] = rc_j[t];
+        ir[u] = rc_id[t];
+      }
+      double vr[2], yr[2];
+      #pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        vr[u] = V[(size_t)jr[u] * yy + k];
+        yr[u] = yv[ir[u]];
+      }
+      #pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int q = tid + TRI_T * u;
+        if (base + q < nr) { zl[2 * q] = vr[u]; zl[2 * q + 1] = yr[u]; }
+      }
+      __syncthreads();
+      #pragma unroll
+      for (int m = 0; m < TRI_GPT; ++m) {
+        const int a = max(t0[m], base), b = min(t1[m], base + P);
+        for (int t = a; t < b; ++t) sg[m] += zl[2 * (t - base)] * zl[2 * (t - base) + 1];
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < xx; i += TRI_T) zl[i] = 0.0;
+    __syncthreads();
+    #pragma unroll
+    for (int m = 0; m < TRI_GPT; ++m)
+      if (xg[m] >= 0) zl[xg[m]] = sg[m];
     __syncthreads();
     #pragma unroll
     for (int c = 0; c < TRI_C; ++c)
@@ -3018,6 +3043,8 @@ int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, con
   for (int x = 0; x < xx; ++x)
     if (cnt[x]) { rg_x.push_back(x); rg_ptr.push_back(acc); start[x] = acc; acc += cnt[x]; }
   rg_ptr.push_back(acc);
+  if ((int)rg_x.size() > TRI_MAXG)
+    return fail(h, 1, "capacitance rows lie on more than " + std::to_string(TRI_MAXG) + " x columns");
   std::vector<int> rc_j(r), rc_id(r);
   for (int k = 0; k < r; ++k) {
     const int x = rows[k] % xx, t = start[x]++;
