@@ -69,9 +69,15 @@ enum Slot {
   NSLOT = 64
 };
 // Every slot is striped over NSTRIPE words so that the blocks of a reduction hit different
-// addresses (same-address L2 atomics serialise); readers reduce over the stripes.
+// addresses (same-address atomics serialise); readers reduce over the stripes.  Stripe s of slot k
+// is word s * SLOT_STRIDE + k: the stripes of one slot lie 4 KB apart, so a reduction's
+// device-scope atomics spread over the memory channels instead of queueing on the one that holds
+// a contiguous 512-byte run, while the NSLOT slots of a stripe stay together for the readers.
 constexpr int NSTRIPE = 64;
-constexpr int NSLOTW = NSLOT * NSTRIPE;
+constexpr int SLOT_STRIDE = 512;
+constexpr int NSLOTW = NSTRIPE * SLOT_STRIDE;
+static_assert(NSLOT == 64 && NSLOT <= SLOT_STRIDE, "one slot per lane of a wave, all within a stripe");
+__host__ __device__ constexpr size_t slot_word(int k, int s) { return (size_t)s * SLOT_STRIDE + k; }
 
 struct Override {
   int j0, i0, bh, bw;
@@ -158,13 +164,7 @@ struct SolveDesc {
   const int* clock;     // the case's device step counter
 };
 
-DEV double slot_d(const u64* s, int k) { return __longlong_as_double((long long)s[k]); }
 DEV int stripe_id() { return (blockIdx.x + 7 * blockIdx.y) & (NSTRIPE - 1); }
-DEV void atomic_max_abs(u64* slot, double v) {
-  double a = fabs(v);
-  u64 bits = (u64)__double_as_longlong(a);
-  atomicMax(slot, bits);  // non-negative doubles order like their bit patterns; NaN sorts above inf
-}
 DEV double woodbury_target(double vmax) { return fmax(1e-10, 32.0 * kEps * fmax(1.0, vmax)); }
 
 // max that propagates NaN (a NaN anywhere must fail the gates that read the maximum)
@@ -208,11 +208,11 @@ DEV double wave_sum(double v) {
 // Slot value (max over stripes, or sum); must be called by all 64 lanes of a wave.
 DEV double slot_max(const u64* sl, int k) {
   const int lane = threadIdx.x & 63;
-  return wave_max_abs(__longlong_as_double((long long)G(sl)[k * NSTRIPE + lane]));
+  return wave_max_abs(__longlong_as_double((long long)G(sl)[slot_word(k, lane)]));
 }
 DEV double slot_sum(const u64* sl, int k) {
   const int lane = threadIdx.x & 63;
-  return wave_sum(__longlong_as_double((long long)G(sl)[k * NSTRIPE + lane]));
+  return wave_sum(__longlong_as_double((long long)G(sl)[slot_word(k, lane)]));
 }
 DEV bool pred_refine(const u64* sl, int slot0, int res_slot) {
   const double res = slot_max(sl, slot0 + res_slot);
@@ -249,8 +249,8 @@ DEV void block_stats_atomic(const double (&v)[N], u64* const (&dst)[N], u64* con
       m = wave_reduce(m, [](double a, double b) { return max_bits(a, b); });
       const u64 bits = (u64)__double_as_longlong(m);
       if (lane == 0 && bits != 0) {
-        if (dst[k]) atomicMax(dst[k] + sid, bits);
-        if (dst2[k]) atomicMax(dst2[k] + sid, bits);
+        if (dst[k]) atomicMax(dst[k] + slot_word(0, sid), bits);
+        if (dst2[k]) atomicMax(dst2[k] + slot_word(0, sid), bits);
       }
     }
     #pragma unroll
@@ -259,7 +259,7 @@ DEV void block_stats_atomic(const double (&v)[N], u64* const (&dst)[N], u64* con
       #pragma unroll
       for (int e = 1; e < RED_T / 64; ++e) m += red[N + q][lane + 64 * e];
       m = wave_sum(m);
-      if (lane == 0 && m != 0.0) unsafeAtomicAdd(sdst[q] + sid, m);
+      if (lane == 0 && m != 0.0) unsafeAtomicAdd(sdst[q] + slot_word(0, sid), m);
     }
   }
 }
@@ -277,7 +277,7 @@ DEV void block_sum_atomic(double v, double* dst) {
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int q = 0; q < nw; ++q) s += reds[q];
-    if (s != 0.0) unsafeAtomicAdd(dst + stripe_id(), s);
+    if (s != 0.0) unsafeAtomicAdd(dst + slot_word(0, stripe_id()), s);
   }
 }
 
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* __restrict__ cas
   }
   double v2[2] = {mu, mv};
   const bool rhs_mode = a.mode == 0 || a.mode == 2;
-  u64* const d2[2] = {rhs_mode ? c.sl + (SL_HU + SV_VAL) * NSTRIPE : nullptr, rhs_mode ? c.sl + (SL_HV + SV_VAL) * NSTRIPE : nullptr};
+  u64* const d2[2] = {rhs_mode ? c.sl + (SL_HU + SV_VAL) : nullptr, rhs_mode ? c.sl + (SL_HV + SV_VAL) : nullptr};
   block_max_atomic<2>(v2, d2);
 }
 
@@ -543,15 +543,15 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
   }
   if (a.mode == 2) {
     double v2[2] = {xu, xv};
-    u64* const d2[2] = {c.sl + SL_UMAX * NSTRIPE, c.sl + SL_VMAX * NSTRIPE};
+    u64* const d2[2] = {c.sl + SL_UMAX, c.sl + SL_VMAX};
     block_max_atomic<2>(v2, d2);
   } else {
     // mode 3 (lean solve, no refinement): the same residual is the first and the final one
     const double v2[2] = {mu, mv};
-    u64* const d2[2] = {run_u ? c.sl + (SL_HU + a.res_slot) * NSTRIPE : nullptr,
-                        run_v ? c.sl + (SL_HV + a.res_slot) * NSTRIPE : nullptr};
-    u64* const f2[2] = {a.mode == 3 ? c.sl + (SL_HU + SV_FINAL) * NSTRIPE : nullptr,
-                        a.mode == 3 ? c.sl + (SL_HV + SV_FINAL) * NSTRIPE : nullptr};
+    u64* const d2[2] = {run_u ? c.sl + (SL_HU + a.res_slot) : nullptr,
+                        run_v ? c.sl + (SL_HV + a.res_slot) : nullptr};
+    u64* const f2[2] = {a.mode == 3 ? c.sl + (SL_HU + SV_FINAL) : nullptr,
+                        a.mode == 3 ? c.sl + (SL_HV + SV_FINAL) : nullptr};
     block_stats_atomic<2>(v2, d2, f2);
   }
 }
@@ -588,7 +588,7 @@ DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
     m = nanmax(m, fabs(d));
   }
   double v1[1] = {m};
-  u64* const d1[1] = {c.sl + a.slot * NSTRIPE};
+  u64* const d1[1] = {c.sl + a.slot};
   block_max_atomic<1>(v1, d1);
 }
 __global__ __launch_bounds__(256) void k_div(const CaseDev* __restrict__ cases, DivArgs a) {
@@ -634,7 +634,7 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
   }
   if (Df) {
     double v1[1] = {m};
-    u64* const d1[1] = {c.sl + SL_COUPLED * NSTRIPE};
+    u64* const d1[1] = {c.sl + SL_COUPLED};
     block_max_atomic<1>(v1, d1);
   }
 }
@@ -695,17 +695,17 @@ DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
   const int b0 = a.slot0;
   if (a.mode == 0) {
     double v1[1] = {mres};
-    u64* const d1[1] = {s + (b0 + a.res_slot) * NSTRIPE};
+    u64* const d1[1] = {s + (b0 + a.res_slot)};
     block_max_atomic<1>(v1, d1);
   } else {
     // mode 1: final statistics; mode 3 (lean solve, no refinement): first residual and final
     // statistics of the same x in one pass
     const double v3[3] = {mres, mrnd, msc};
-    u64* const d3[3] = {s + (b0 + SV_FINAL) * NSTRIPE, s + (b0 + SV_ROUND) * NSTRIPE, s + (b0 + SV_ASCALE) * NSTRIPE};
-    u64* const r3[3] = {a.mode == 3 ? s + (b0 + a.res_slot) * NSTRIPE : nullptr, nullptr, nullptr};
+    u64* const d3[3] = {s + (b0 + SV_FINAL), s + (b0 + SV_ROUND), s + (b0 + SV_ASCALE)};
+    u64* const r3[3] = {a.mode == 3 ? s + (b0 + a.res_slot) : nullptr, nullptr, nullptr};
     const double sums[2] = {ssr, ssv};
-    double* const sd[2] = {reinterpret_cast<double*>(s + (b0 + SV_SSQR) * NSTRIPE),
-                           reinterpret_cast<double*>(s + (b0 + SV_SSQV) * NSTRIPE)};
+    double* const sd[2] = {reinterpret_cast<double*>(s + (b0 + SV_SSQR)),
+                           reinterpret_cast<double*>(s + (b0 + SV_SSQV))};
     block_stats_atomic<3, 2>(v3, d3, r3, sums, sd);
   }
 }
@@ -1231,7 +1231,7 @@ DEV void fill_body(const CaseDev& c, const FillArgs& a, int bx, int gx) {
   }
   if (a.vel && a.track) {
     double v2[2] = {mu, mv};
-    u64* const d2[2] = {c.sl + SL_UMAX * NSTRIPE, c.sl + SL_VMAX * NSTRIPE};
+    u64* const d2[2] = {c.sl + SL_UMAX, c.sl + SL_VMAX};
     block_max_atomic<2>(v2, d2);
   }
 }
@@ -1386,8 +1386,8 @@ __global__ __launch_bounds__(64) void k_force_total(const CaseDev* __restrict__ 
   double vals[6];
   force_totals(c, vals);
   if (threadIdx.x != 0) return;
-  double* f = reinterpret_cast<double*>(c.sl + SL_FORCE * NSTRIPE);
-  for (int q = 0; q < 6; ++q) f[q * NSTRIPE] = vals[q];
+  double* f = reinterpret_cast<double*>(c.sl + SL_FORCE);   // stripe 0 of slots SL_FORCE .. SL_FORCE + 5
+  for (int q = 0; q < 6; ++q) f[q] = vals[q];
 }
 
 // Gather the step's scalars into the diagnostics ring and clear the slots (one wave per case)
@@ -1443,7 +1443,7 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
   __shared__ double v[FIN_NQ];
   __shared__ double d[MCF_DIAG_LEN];
   #pragma unroll
-  for (int q = 0; q < FIN_NQ; ++q) raw[q][lane] = __longlong_as_double((long long)s[kFinSlots[q] * NSTRIPE + lane]);
+  for (int q = 0; q < FIN_NQ; ++q) raw[q][lane] = __longlong_as_double((long long)s[slot_word(kFinSlots[q], lane)]);
   d[lane] = 0.0;
   double f[6];
   force_totals(c, f);   // the step's forces, from k_forces' per-wave partial sums
@@ -1528,7 +1528,7 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
     *G(c.clock_time) = time;
   }
   const gptr<u64> wsl = G(c.sl);
-  for (int k = lane; k < NSLOTW; k += 64) wsl[k] = 0ull;
+  for (int st = 0; st < NSTRIPE; ++st) wsl[slot_word(lane, st)] = 0ull;   // the used words of every stripe
 }
 
 // Step parameters of every case of the launch; dt <= 0 keeps each case's own dt
@@ -1548,7 +1548,7 @@ __global__ void k_set_clock(const CaseDev* __restrict__ cases, double t0) {
 
 __global__ void k_clear_slots(const CaseDev* __restrict__ cases) {
   const CaseDev& c = cases[blockIdx.y];
-  for (int k = threadIdx.x; k < NSLOTW; k += blockDim.x) c.sl[k] = 0ull;
+  for (int w = threadIdx.x; w < NSTRIPE * NSLOT; w += blockDim.x) c.sl[slot_word(w % NSLOT, w / NSLOT)] = 0ull;
 }
 
 // Checkpoint of what a steady step reads from earlier steps: the state S[parity], the advection
@@ -1598,10 +1598,10 @@ __global__ __launch_bounds__(256) void k_divstats(const double* u, const double*
   double v1[1] = {m};
   u64* const d1[1] = {reinterpret_cast<u64*>(out)};
   block_max_atomic<1>(v1, d1);
-  __syncthreads(); block_sum_atomic(s, out + 1 * NSTRIPE);
-  __syncthreads(); block_sum_atomic(cnt, out + 2 * NSTRIPE);
-  __syncthreads(); block_sum_atomic(cs, out + 3 * NSTRIPE);
-  __syncthreads(); block_sum_atomic(cc, out + 4 * NSTRIPE);
+  __syncthreads(); block_sum_atomic(s, out + 1);
+  __syncthreads(); block_sum_atomic(cnt, out + 2);
+  __syncthreads(); block_sum_atomic(cs, out + 3);
+  __syncthreads(); block_sum_atomic(cc, out + 4);
 }
 
 }  // namespace
@@ -1901,9 +1901,9 @@ __global__ __launch_bounds__(256) void k_kry_axpy(const CaseDev* __restrict__ ca
 __global__ void k_kry_record(const CaseDev* __restrict__ cases, int psolve, double iters, double refine, double status) {
   const CaseDev& c = cases[0];
   if (threadIdx.x != 0) return;
-  c.sl[(SL_GM_IT + psolve) * NSTRIPE] = (u64)__double_as_longlong(iters);
-  c.sl[(SL_GM_REF + psolve) * NSTRIPE] = (u64)__double_as_longlong(refine);
-  c.sl[(SL_GM_ST + psolve) * NSTRIPE] = (u64)__double_as_longlong(status);
+  c.sl[SL_GM_IT + psolve] = (u64)__double_as_longlong(iters);    // stripe 0
+  c.sl[SL_GM_REF + psolve] = (u64)__double_as_longlong(refine);
+  c.sl[SL_GM_ST + psolve] = (u64)__double_as_longlong(status);
 }
 
 // Plane rotation of LAPACK 3.10's dlartg (E. Anderson, "Algorithm 978: Safe scaling in the Level 1
@@ -2724,19 +2724,33 @@ int ensure_diag(mcf_handle* h, int rows) {
   return sync_descs(h);
 }
 
+// Host images of the used slot words, [stripe][slot] (NSTRIPE x NSLOT doubles): 2D copies of the
+// first NSLOT words of every stripe.
+int download_slots(mcf_handle* h, std::vector<double>& img) {
+  img.assign((size_t)NSTRIPE * NSLOT, 0.0);
+  HIPCHK(h, hipMemcpy2D(img.data(), NSLOT * sizeof(double), h->slots, SLOT_STRIDE * sizeof(u64),
+                        NSLOT * sizeof(double), NSTRIPE, hipMemcpyDeviceToHost));
+  return 0;
+}
+int upload_slots(mcf_handle* h, const std::vector<double>& img) {
+  HIPCHK(h, hipMemcpy2DAsync(h->slots, SLOT_STRIDE * sizeof(u64), img.data(), NSLOT * sizeof(double),
+                             NSLOT * sizeof(double), NSTRIPE, hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
 // Host view of the slots: max |.| over stripes, sums for the sum slots, stripe 0 for the forces.
 int read_slots(mcf_handle* h, std::vector<double>& out) {
-  std::vector<double> raw(NSLOTW);
+  std::vector<double> raw;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(raw.data(), h->slots, NSLOTW * sizeof(double), hipMemcpyDeviceToHost));
+  if (int rc = download_slots(h, raw)) return rc;
   out.assign(NSLOT, 0.0);
   for (int k = 0; k < NSLOT; ++k) {
     const bool force = k >= SL_FORCE && k < SL_FORCE + 6;
     const bool sum = k >= SL_HU && k < SL_FORCE && ((k - SL_HU) % 8 == SV_SSQR || (k - SL_HU) % 8 == SV_SSQV);
     double v = 0.0;
     for (int t = 0; t < NSTRIPE; ++t) {
-      const double x = raw[(size_t)k * NSTRIPE + t];
-      if (force) { v = raw[(size_t)k * NSTRIPE]; break; }
+      const double x = raw[(size_t)t * NSLOT + k];
+      if (force) { v = raw[k]; break; }
       if (sum) v += x;
       else v = (std::fabs(x) > v || x != x) ? std::fabs(x) : v;
     }
@@ -3347,10 +3361,10 @@ int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, dou
     const int comp = pres ? 0 : (k < h->usize ? 0 : 1);
     vm[comp] = std::max(vm[comp], std::fabs(hv[k]));
   }
-  std::vector<double> s0(NSLOTW, 0.0);
-  if (pres) s0[(size_t)(SL_P0 + SV_VAL) * NSTRIPE] = vm[0];
-  else { s0[(size_t)(SL_HU + SV_VAL) * NSTRIPE] = vm[0]; s0[(size_t)(SL_HV + SV_VAL) * NSTRIPE] = vm[1]; }
-  HIPCHK(h, hipMemcpyAsync(h->slots, s0.data(), NSLOTW * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  std::vector<double> s0((size_t)NSTRIPE * NSLOT, 0.0);   // stripe 0 carries the right-hand side maxima
+  if (pres) s0[SL_P0 + SV_VAL] = vm[0];
+  else { s0[SL_HU + SV_VAL] = vm[0]; s0[SL_HV + SV_VAL] = vm[1]; }
+  if (int rc = upload_slots(h, s0)) return rc;
   const double dt = pres ? 1.0 : -2.0 * h->hs[1].b / h->nu;
   set_prm_once(h, dt);
   if (int rc = enqueue_solve(ctx_of(h), pres, in_id, out_id, 0)) return rc;
@@ -3433,9 +3447,9 @@ int mcf_pressure_solve(mcf_handle* h, const double* rhs, double* out, double* re
     }
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TP0], rhs, h->psize * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  std::vector<double> s0(NSLOTW, 0.0);
-  s0[(size_t)(SL_P0 + SV_VAL) * NSTRIPE] = vmax;
-  HIPCHK(h, hipMemcpyAsync(h->slots, s0.data(), NSLOTW * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  std::vector<double> s0((size_t)NSTRIPE * NSLOT, 0.0);
+  s0[SL_P0 + SV_VAL] = vmax;
+  if (int rc = upload_slots(h, s0)) return rc;
   set_prm_once(h, 1.0);
   if (int rc = enqueue_solve(ctx_of(h), true, B_TP0, B_TP1, 0)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -3505,18 +3519,18 @@ int mcf_divergence_stats(mcf_handle* h, const double* vel, const unsigned char* 
   unsigned char* dm = nullptr;
   double* acc = nullptr;
   HIPCHK(h, hipMalloc(&dm, h->psize));
-  HIPCHK(h, hipMalloc(&acc, 5 * NSTRIPE * sizeof(double)));
+  HIPCHK(h, hipMalloc(&acc, NSLOTW * sizeof(double)));   // five slots in the slot layout
   HIPCHK(h, hipMemcpy(dm, mask, h->psize, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemset(acc, 0, 5 * NSTRIPE * sizeof(double)));
+  HIPCHK(h, hipMemset(acc, 0, NSLOTW * sizeof(double)));
   HIPCHK(h, hipMemcpy(h->hc.buf[B_TV0], vel, h->velsize * sizeof(double), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_divstats, grid1(h->psize), dim3(256), 0, h->stream, h->hc.buf[B_TV0],
                      h->hc.buf[B_TV0] + h->usize, dm, h->nx, h->ny, h->dx, h->dy, acc);
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  std::vector<double> raw(5 * NSTRIPE);
+  std::vector<double> raw(NSLOTW);
   HIPCHK(h, hipMemcpy(raw.data(), acc, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int q = 0; q < 5; ++q) {
     double v = 0.0;
-    for (int t = 0; t < NSTRIPE; ++t) v = q == 0 ? std::max(v, raw[q * NSTRIPE + t]) : v + raw[q * NSTRIPE + t];
+    for (int t = 0; t < NSTRIPE; ++t) v = q == 0 ? std::max(v, raw[slot_word(q, t)]) : v + raw[slot_word(q, t)];
     out5[q] = v;
   }
   (void)hipFree(dm); (void)hipFree(acc);
