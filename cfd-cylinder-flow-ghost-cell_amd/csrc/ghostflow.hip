@@ -894,27 +894,28 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restri
     double a[SK_KB][MT], b[SK_KB][2];
     #pragma unroll
     for (int t = 0; t < SK_KB; ++t) {
-      // k past the slice or past K contributes zero: the load reads a valid k, the select zeroes it
-      const int kk = 4 * (gb + t) + kq;
-      const bool ok = gb + t < g1 && kk < K;
-      const int kc = min(kk, K - 1);
+      const int kc = min(4 * (gb + t) + kq, K - 1);   // a valid k; past the slice or K it is zeroed below
       const gptr<const double> Ar = AT + kc * M;
       const gptr<const double> Br = Bm + kc * ldb;
       #pragma unroll
-      for (int q = 0; q < MT; ++q) { const double v = Ar[ra[q]]; a[t][q] = ok ? v : 0.0; }
+      for (int q = 0; q < MT; ++q) a[t][q] = Ar[ra[q]];
       #pragma unroll
-      for (int q = 0; q < 2; ++q) { const double v = Br[cb[q]]; b[t][q] = ok ? v : 0.0; }
+      for (int q = 0; q < 2; ++q) b[t][q] = Br[cb[q]];
     }
-    // keep every load of the batch ahead of the first MFMA (the scheduler would otherwise interleave
-    // them and expose a memory round trip per k-group)
+    // every load of the batch goes out before the first MFMA (the scheduler would otherwise
+    // interleave them and expose a memory round trip per k-group); each k-group's MFMAs then wait
+    // only for that group's loads, so they run while the later loads arrive
     __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
-    for (int t = 0; t < SK_KB; ++t)
+    for (int t = 0; t < SK_KB; ++t) {
+      const int kk = 4 * (gb + t) + kq;
+      const bool ok = gb + t < g1 && kk < K;   // k past the slice or past K contributes zero
       #pragma unroll
       for (int mi = 0; mi < MT; ++mi)
         #pragma unroll
         for (int ni = 0; ni < 2; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][mi], b[t][ni], acc[mi][ni], 0, 0, 0);
+          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? a[t][mi] : 0.0, b[t][ni], acc[mi][ni], 0, 0, 0);
+    }
   }
   // partial tiles of the slices: red[slice][subtile][element][lane]; wave w then finishes
   // elements MT (w / NS) .. MT (w / NS) + MT - 1 of subtile w % NS
