@@ -1362,22 +1362,19 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* __restrict__ cases
 }
 
 // (Fx, Fy, Fx_p, Fy_p, Fx_v, Fy_v) from k_forces' per-wave partial sums.  Called by all 64 lanes of a
-// one-wave block: lane l adds partials l, l + 64, ... in order, then every lane adds the 64 lane
-// sums in lane order, so the totals are bit-reproducible.
+// one-wave block: lane l adds partials l, l + 64, ... in order, then the wave adds the 64 lane sums
+// by its DPP reduction (a fixed order), so the totals are bit-reproducible.
 DEV void force_totals(const CaseDev& c, double (&vals)[6]) {
-  __shared__ double lt[4][64];
   const int lane = threadIdx.x & 63;
   const int nb = force_blocks(c.nsurf);
-  double p[4] = {0.0, 0.0, 0.0, 0.0};
   const gptr<const double> fp = G(c.fpart);
+  double p[4] = {0.0, 0.0, 0.0, 0.0};
   for (int b = lane; b < nb; b += 64)
+    #pragma unroll
     for (int q = 0; q < 4; ++q) p[q] += fp[b * 4 + q];
-  for (int q = 0; q < 4; ++q) lt[q][lane] = p[q];
-  __syncthreads();
-  double t[4] = {0.0, 0.0, 0.0, 0.0};
-  #pragma unroll 8
-  for (int l = 0; l < 64; ++l)
-    for (int q = 0; q < 4; ++q) t[q] += lt[q][l];
+  double t[4];
+  #pragma unroll
+  for (int q = 0; q < 4; ++q) t[q] = wave_sum(p[q]);
   const double scale = M_PI * c.diameter / c.nsurf;
   const double fxp = -t[0], fyp = -t[1], fxv = t[2] * scale, fyv = t[3] * scale;
   vals[0] = fxp + fxv; vals[1] = fyp + fyv; vals[2] = fxp; vals[3] = fyp; vals[4] = fxv; vals[5] = fyv;
@@ -1452,8 +1449,17 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
   if (lane < FIN_NQ) {
     double r = 0.0;
     if (lane < FIN_MAX) {
-      #pragma unroll 8
-      for (int t = 0; t < 64; ++t) r = nanmax(r, fabs(raw[lane][t]));
+      // max |.| over the stripes on the bit patterns (NaN above +inf), four independent chains
+      u64 m[4] = {0ull, 0ull, 0ull, 0ull};
+      #pragma unroll 4
+      for (int t = 0; t < 64; t += 4)
+        #pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const u64 b = (u64)__double_as_longlong(fabs(raw[lane][t + e]));
+          m[e] = b > m[e] ? b : m[e];
+        }
+      const u64 m01 = m[0] > m[1] ? m[0] : m[1], m23 = m[2] > m[3] ? m[2] : m[3];
+      r = __longlong_as_double((long long)(m01 > m23 ? m01 : m23));
     } else {
       #pragma unroll 8
       for (int t = 0; t < 64; ++t) r += raw[lane][t];
