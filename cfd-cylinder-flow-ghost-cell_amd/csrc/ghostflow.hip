@@ -945,29 +945,86 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restri
 
 struct PcrArgs { int mode; int pred_slot; int slot_shift; const int* gate = nullptr; };
 
-// Block-wide exclusive scan of affine maps y -> A y + B (composition in thread order, or in
-// reverse order when `reverse`); returns the value reached from 0 before this thread's chunk.
-template <int W>
-DEV double block_affine_scan(double A, double B, bool reverse) {
+// Cross-lane move through DPP in which lanes without a source lane (and rows outside ROW_MASK) keep
+// `old` instead of reading 0
+template <int CTRL, int ROW_MASK = 0xf>
+DEV double dpp_or(double v, double old) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)(o & 0xffffffffll), (int)(unsigned)(b & 0xffffffffll),
+                                             CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int L>
+DEV double lane_value(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), L);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), L);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// (A, B) <- (A, B) o (the map CTRL brings in): apply the incoming map first, then this lane's;
+// lanes with no incoming map compose with the identity (1, 0)
+template <int CTRL, int ROW_MASK = 0xf>
+DEV void affine_step(double& A, double& B) {
+  const double Ai = dpp_or<CTRL, ROW_MASK>(A, 1.0), Bi = dpp_or<CTRL, ROW_MASK>(B, 0.0);
+  B = A * Bi + B;
+  A = A * Ai;
+}
+// Inclusive scans of affine maps y -> A y + B over the 64 lanes of a wave on the VALU (DPP), all
+// lanes active.  Forward: lane l ends with the maps of lanes 0..l composed in lane order (row
+// shifts by 1, 2, 4, 8, then the row broadcasts of lanes 15 and 31).  Reverse: lane l ends with the
+// maps of lanes 63..l composed from lane 63 down (row shifts left, then the row totals of the rows
+// above, read from lanes 16, 32 and 48).
+DEV void wave_affine_fwd(double& A, double& B) {
+  affine_step<0x111>(A, B); affine_step<0x112>(A, B); affine_step<0x114>(A, B); affine_step<0x118>(A, B);
+  affine_step<0x142, 0xa>(A, B);   // row_bcast:15 into rows 1 and 3
+  affine_step<0x143, 0xc>(A, B);   // row_bcast:31 into rows 2 and 3
+}
+DEV void wave_affine_rev(double& A, double& B) {
+  affine_step<0x101>(A, B); affine_step<0x102>(A, B); affine_step<0x104>(A, B); affine_step<0x108>(A, B);
+  const double A1 = lane_value<16>(A), B1 = lane_value<16>(B), A2 = lane_value<32>(A), B2 = lane_value<32>(B);
+  const double A3 = lane_value<48>(A), B3 = lane_value<48>(B);
+  // the rows above row r, composed from row 3 down: row 2 <- R3, row 1 <- R2 o R3, row 0 <- R1 o R2 o R3
+  const double e1A = A2 * A3, e1B = A2 * B3 + B2;
+  const double e0A = A1 * e1A, e0B = A1 * e1B + B1;
+  const int row = (threadIdx.x & 63) >> 4;
+  const double EA = row == 0 ? e0A : row == 1 ? e1A : row == 2 ? A3 : 1.0;
+  const double EB = row == 0 ? e0B : row == 1 ? e1B : row == 2 ? B3 : 0.0;
+  B = A * EB + B;
+  A = A * EA;
+}
+
+// Block-wide exclusive scan of affine maps y -> A y + B (composition in thread order, or in reverse
+// order when REVERSE); returns the value reached from 0 before this thread's chunk.  Waves scan on
+// the VALU; the W wave totals are composed through LDS.
+template <int W, bool REVERSE>
+DEV double block_affine_scan(double A, double B) {
   __shared__ double sa[W], sb[W];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int off = 1; off < 64; off <<= 1) {
-    const double Ap = reverse ? __shfl_down(A, off, 64) : __shfl_up(A, off, 64);
-    const double Bp = reverse ? __shfl_down(B, off, 64) : __shfl_up(B, off, 64);
-    const bool ok = reverse ? (lane + off < 64) : (lane >= off);
-    if (ok) { B = A * Bp + B; A = A * Ap; }
-  }
-  if (lane == (reverse ? 0 : 63)) { sa[w] = A; sb[w] = B; }
+  if (REVERSE) wave_affine_rev(A, B);
+  else wave_affine_fwd(A, B);
+  if (lane == (REVERSE ? 0 : 63)) { sa[w] = A; sb[w] = B; }
   __syncthreads();
   // value entering this wave: compose the totals of the waves before it (after it, reversed)
   double vin = 0.0;
-  if (!reverse) { for (int q = 0; q < w; ++q) vin = sa[q] * vin + sb[q]; }
+  if (!REVERSE) { for (int q = 0; q < w; ++q) vin = sa[q] * vin + sb[q]; }
   else { for (int q = W - 1; q > w; --q) vin = sa[q] * vin + sb[q]; }
-  // inclusive value of the neighbouring lane
-  const double Bn = reverse ? __shfl_down(A * vin + B, 1, 64) : __shfl_up(A * vin + B, 1, 64);
-  const bool edge = reverse ? (lane == 63) : (lane == 0);
+  // value after each lane; a lane's entering value is its neighbour's (the wave's edge lane: vin)
+  const double after = A * vin + B;
+  const int r16 = lane & 15;
+  double enter;
+  if (!REVERSE) {
+    const double inrow = dpp_or<0x111>(after, vin);            // row_shr:1; lane 0 of a row keeps vin
+    const double b15 = dpp_or<0x142, 0xa>(after, vin);         // lanes 15, 47 into rows 1, 3
+    const double b31 = dpp_or<0x143, 0x4>(after, vin);         // lane 31 into row 2
+    enter = (r16 != 0 || lane == 0) ? inrow : ((lane >> 4) == 2 ? b31 : b15);
+  } else {
+    const double inrow = dpp_or<0x101>(after, vin);            // row_shl:1; lane 15 of a row keeps vin
+    const double a16 = lane_value<16>(after), a32 = lane_value<32>(after), a48 = lane_value<48>(after);
+    enter = (r16 != 15 || lane == 63) ? inrow : (lane == 15 ? a16 : lane == 31 ? a32 : a48);
+  }
   __syncthreads();
-  return edge ? vin : Bn;
+  return enter;
 }
 
 // Per-mode tridiagonal solve along x with the precomputed LU factors of (a + b lam_k) I + b lx
@@ -1072,7 +1129,7 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ des
   #pragma unroll
   for (int c = 0; c < TRI_C; ++c)
     if (c < n) { const double al = -l[c]; A *= al; B = al * B + r[c]; }
-  double y = block_affine_scan<TRI_T / 64>(A, B, false);
+  double y = block_affine_scan<TRI_T / 64, false>(A, B);
   #pragma unroll
   for (int c = 0; c < TRI_C; ++c)
     if (c < n) { y = r[c] - l[c] * y; r[c] = y; }
@@ -1084,7 +1141,7 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ des
       const double al = (i0 + c < xx - 1) ? -cu * id[c] : 0.0;
       A *= al; B = al * B + r[c] * id[c];
     }
-  double x = block_affine_scan<TRI_T / 64>(A, B, true);
+  double x = block_affine_scan<TRI_T / 64, true>(A, B);
   #pragma unroll
   for (int c = TRI_C - 1; c >= 0; --c)
     if (c < n) {
