@@ -355,11 +355,13 @@ struct AdvArgs {
   int mode;  // 0: AB2 rhs, 1: projected-midpoint, 2: rhs with explicit = adv(adv_id), 3: out = L q + offset
 };
 
-__global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* __restrict__ cases, AdvArgs a) {
-  const CaseDev& c = cases[blockIdx.y];
+// SAME: the advected and the advecting fields are one buffer (the AB2 and midpoint steps), so their
+// reads at one index are one load
+template <bool SAME>
+DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
   const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
-  const gptr<const double> Q = G(c.buf[a.q_id]);
   const gptr<const double> A = G(c.buf[a.adv_id]);
+  const gptr<const double> Q = SAME ? A : G(c.buf[a.q_id]);
   const gptr<const double> Pp = G(c.buf[a.p_id]);
   const gptr<const double> prev = a.prev_id >= 0 ? G(c.buf[a.prev_id]) : nullptr;
   const gptr<double> now = a.now_id >= 0 ? G(c.buf[a.now_id]) : nullptr;
@@ -468,6 +470,11 @@ __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* __restrict__ cas
   const bool rhs_mode = a.mode == 0 || a.mode == 2;
   u64* const d2[2] = {rhs_mode ? c.sl + (SL_HU + SV_VAL) : nullptr, rhs_mode ? c.sl + (SL_HV + SV_VAL) : nullptr};
   block_max_atomic<2>(v2, d2);
+}
+__global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* __restrict__ cases, AdvArgs a) {
+  const CaseDev& c = cases[blockIdx.y];
+  if (a.q_id == a.adv_id) adv_body<true>(c, a);
+  else adv_body<false>(c, a);
 }
 
 // Helmholtz residual / defect on vel q-arrays: H x = x - dt/2 L x
