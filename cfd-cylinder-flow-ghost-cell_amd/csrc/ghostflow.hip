@@ -1617,6 +1617,25 @@ __global__ void k_set_clock(const CaseDev* __restrict__ cases, double t0) {
   if (threadIdx.x == 0) { *c.clock_step = 0; *c.clock_time = t0; }
 }
 
+// Host view of the slots, one lane per slot (one wave): max |.| over the stripes, the sum for the
+// residual sum-of-squares slots, stripe 0 for the forces; the 64 values go to `out`.
+__global__ __launch_bounds__(64) void k_slot_values(const CaseDev* __restrict__ cases, double* out) {
+  const CaseDev& c = cases[0];
+  const int k = threadIdx.x;
+  const gptr<const u64> sl = G(c.sl);
+  const bool force = k >= SL_FORCE && k < SL_FORCE + 6;
+  const bool sum = k >= SL_HU && k < SL_FORCE && ((k - SL_HU) % 8 == SV_SSQR || (k - SL_HU) % 8 == SV_SSQV);
+  double v = 0.0;
+  if (force) v = __longlong_as_double((long long)sl[slot_word(k, 0)]);
+  else
+    for (int t = 0; t < NSTRIPE; ++t) {
+      const double x = __longlong_as_double((long long)sl[slot_word(k, t)]);
+      if (sum) v += x;
+      else v = (fabs(x) > v || x != x) ? fabs(x) : v;
+    }
+  G(out)[k] = v;
+}
+
 __global__ void k_clear_slots(const CaseDev* __restrict__ cases) {
   const CaseDev& c = cases[blockIdx.y];
   for (int w = threadIdx.x; w < NSTRIPE * NSLOT; w += blockDim.x) c.sl[slot_word(w % NSLOT, w / NSLOT)] = 0ull;
@@ -2180,6 +2199,7 @@ struct mcf_handle {
   std::vector<int> pt_pos[3];   // capacitance point (caller order) -> group order
   double* inlet = nullptr;
   u64* slots = nullptr;
+  double* slot_vals = nullptr;   // NSLOT host-view slot values (k_slot_values)
   double* diag = nullptr;
   int diag_cap = 0;
   double prev_dt = 0.0, time = 0.0;
@@ -2797,38 +2817,20 @@ int ensure_diag(mcf_handle* h, int rows) {
   return sync_descs(h);
 }
 
-// Host images of the used slot words, [stripe][slot] (NSTRIPE x NSLOT doubles): 2D copies of the
-// first NSLOT words of every stripe.
-int download_slots(mcf_handle* h, std::vector<double>& img) {
-  img.assign((size_t)NSTRIPE * NSLOT, 0.0);
-  HIPCHK(h, hipMemcpy2D(img.data(), NSLOT * sizeof(double), h->slots, SLOT_STRIDE * sizeof(u64),
-                        NSLOT * sizeof(double), NSTRIPE, hipMemcpyDeviceToHost));
-  return 0;
-}
-int upload_slots(mcf_handle* h, const std::vector<double>& img) {
-  HIPCHK(h, hipMemcpy2DAsync(h->slots, SLOT_STRIDE * sizeof(u64), img.data(), NSLOT * sizeof(double),
-                             NSLOT * sizeof(double), NSTRIPE, hipMemcpyHostToDevice, h->stream));
+// Clear every slot and set stripe 0 (the first NSLOT words) to `vals`
+int upload_slots(mcf_handle* h, const std::vector<double>& vals) {
+  hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
+  HIPCHK(h, hipMemcpyAsync(h->slots, vals.data(), NSLOT * sizeof(double), hipMemcpyHostToDevice, h->stream));
   return 0;
 }
 
-// Host view of the slots: max |.| over stripes, sums for the sum slots, stripe 0 for the forces.
+// Host view of the slots (max |.| over stripes, sums for the sum slots, stripe 0 for the forces),
+// reduced on the device so that only NSLOT values cross to the host.
 int read_slots(mcf_handle* h, std::vector<double>& out) {
-  std::vector<double> raw;
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (int rc = download_slots(h, raw)) return rc;
+  hipLaunchKernelGGL(k_slot_values, dim3(1), dim3(NSLOT), 0, h->stream, h->dc, h->slot_vals);
   out.assign(NSLOT, 0.0);
-  for (int k = 0; k < NSLOT; ++k) {
-    const bool force = k >= SL_FORCE && k < SL_FORCE + 6;
-    const bool sum = k >= SL_HU && k < SL_FORCE && ((k - SL_HU) % 8 == SV_SSQR || (k - SL_HU) % 8 == SV_SSQV);
-    double v = 0.0;
-    for (int t = 0; t < NSTRIPE; ++t) {
-      const double x = raw[(size_t)t * NSLOT + k];
-      if (force) { v = raw[k]; break; }
-      if (sum) v += x;
-      else v = (std::fabs(x) > v || x != x) ? std::fabs(x) : v;
-    }
-    out[k] = v;
-  }
+  HIPCHK(h, hipMemcpyAsync(out.data(), h->slot_vals, NSLOT * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 
@@ -2881,6 +2883,7 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   }
   HIPCHK(h, hipMalloc(&h->slots, NSLOTW * sizeof(u64)));
   HIPCHK(h, hipMemset(h->slots, 0, NSLOTW * sizeof(u64)));
+  HIPCHK(h, hipMalloc(&h->slot_vals, NSLOT * sizeof(double)));
   HIPCHK(h, hipMalloc(&h->inlet, ny * sizeof(double)));
   HIPCHK(h, hipMemset(h->inlet, 0, ny * sizeof(double)));
   HIPCHK(h, hipMalloc(&h->clock_step, sizeof(int)));
@@ -2924,7 +2927,7 @@ void mcf_destroy(mcf_handle* h) {
   (void)hipFree(h->clock_step); (void)hipFree(h->clock_time); (void)hipFree(h->prm);
   for (auto& kv : h->owned) (void)hipFree(kv.second);
   for (void* p : h->bufs) (void)hipFree(p);
-  (void)hipFree(h->slots); (void)hipFree(h->inlet); (void)hipFree(h->dc); (void)hipFree(h->ds);
+  (void)hipFree(h->slots); (void)hipFree(h->slot_vals); (void)hipFree(h->inlet); (void)hipFree(h->dc); (void)hipFree(h->ds);
   if (h->diag) (void)hipFree(h->diag);
   if (h->tmp_rect) (void)hipFree(h->tmp_rect);
   if (h->kry) {
@@ -3434,7 +3437,7 @@ int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, dou
     const int comp = pres ? 0 : (k < h->usize ? 0 : 1);
     vm[comp] = std::max(vm[comp], std::fabs(hv[k]));
   }
-  std::vector<double> s0((size_t)NSTRIPE * NSLOT, 0.0);   // stripe 0 carries the right-hand side maxima
+  std::vector<double> s0(NSLOT, 0.0);   // stripe 0 carries the right-hand side maxima
   if (pres) s0[SL_P0 + SV_VAL] = vm[0];
   else { s0[SL_HU + SV_VAL] = vm[0]; s0[SL_HV + SV_VAL] = vm[1]; }
   if (int rc = upload_slots(h, s0)) return rc;
@@ -3520,7 +3523,7 @@ int mcf_pressure_solve(mcf_handle* h, const double* rhs, double* out, double* re
     }
   hipLaunchKernelGGL(k_clear_slots, dim3(1, 1), dim3(64), 0, h->stream, h->dc);
   HIPCHK(h, hipMemcpyAsync(h->hc.buf[B_TP0], rhs, h->psize * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  std::vector<double> s0((size_t)NSTRIPE * NSLOT, 0.0);
+  std::vector<double> s0(NSLOT, 0.0);
   s0[SL_P0 + SV_VAL] = vmax;
   if (int rc = upload_slots(h, s0)) return rc;
   set_prm_once(h, 1.0);
