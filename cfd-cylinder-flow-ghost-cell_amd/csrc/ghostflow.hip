@@ -2323,8 +2323,9 @@ Ctx ctx_of(mcf_handle* h) {
 // velocity-sized ones about two, so that their waves fit the CUs' wave slots in one round (measured
 // 0.6-1.2 us less per velocity-sized launch at 1320x246).  A batch: at most GSB_BATCH blocks per
 // case, so that each block pays its prologue (the case's parameters and buffer pointers, several
-// dependent loads) over several elements; the batch has parallelism to spare.
-constexpr int GSB_SINGLE = 1280, GSB_BATCH = 512;
+// dependent loads) over several elements; the batch has parallelism to spare (the 64-case
+// ensemble measured 1.668 / 1.681 / 1.646 / 1.586e9 cell-steps/s at 128 / 256 / 512 / 1024).
+constexpr int GSB_SINGLE = 1280, GSB_BATCH = 256;
 inline dim3 gridSn(int n, int cases) {
   dim3 g = gridS(n);
   g.x = std::min<unsigned>(g.x, cases > 1 ? GSB_BATCH : GSB_SINGLE);
