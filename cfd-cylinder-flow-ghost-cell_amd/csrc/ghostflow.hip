@@ -2221,8 +2221,7 @@ struct mcf_handle {
   int spec = 1;              // speculative lean steps: 0 off, 1 on, 2 on with every lean step flagged
   int replays = 0;           // chunks replayed with the refinements in place
   int spec_chunk = 8;        // steps per speculative chunk: doubles after a clean chunk, back to 8 on a miss
-  int spec_full = 0;         // steps to run in full before speculating again (set after a miss)
-  int spec_backoff = 16;     // next full stretch: doubles while misses persist, back to 16 after a clean chunk
+  int spec_full = 0;         // length of the next chunk when it runs in full (0: the next chunk is lean)
   int pbackend = MCF_PB_DIRECT;   // pressure backend
   Krylov* kry = nullptr;
 };
@@ -3247,20 +3246,29 @@ int mcf_get_state(mcf_handle* h, double* vel, double* p, double* prev_adv, doubl
 }
 
 namespace {
-constexpr int SPEC_MIN = 8, SPEC_MAX = 512, BACKOFF_MIN = 16, BACKOFF_MAX = 1024;
+constexpr int SPEC_MIN = 8, SPEC_MAX = 512, SPEC_FULL = 16, SPEC_QUIET = 4;
 
-// Speculation policy after a chunk: a clean lean chunk doubles the next one; a miss returns to the
-// shortest chunk and runs a full stretch first, doubling while misses persist (start-up
-// transients refine for tens of steps, steady flow rarely).
-void spec_update(int& chunk, int& full, int& backoff, bool miss) {
-  if (miss) {
-    chunk = SPEC_MIN;
-    full = backoff;
-    backoff = std::min(2 * backoff, BACKOFF_MAX);
-  } else {
+// Speculation policy after a chunk.  A clean lean chunk doubles the next one.  After a lean chunk
+// that missed (and was replayed in full) or a full chunk, the next chunk is lean again, from the
+// shortest length, once the chunk's last SPEC_QUIET steps refined nothing; otherwise it is a full
+// chunk of SPEC_FULL steps.  Start-up transients refine for tens of steps and steady flow rarely,
+// so the full steps end within SPEC_FULL steps of the transient.
+void spec_update(int& chunk, int& full, bool lean, bool miss, bool quiet) {
+  if (lean && !miss) {
     chunk = std::min(2 * chunk, SPEC_MAX);
-    backoff = BACKOFF_MIN;
+    full = 0;
+    return;
   }
+  chunk = SPEC_MIN;
+  full = quiet ? 0 : SPEC_FULL;
+}
+// No refinement (pressure or velocity) in the last SPEC_QUIET of k diagnostics rows
+bool quiet_tail(const double* rows, int k) {
+  for (int s = std::max(0, k - SPEC_QUIET); s < k; ++s) {
+    const double* r = rows + (size_t)s * MCF_DIAG_LEN;
+    if (r[MCF_D_P_REFINE] > 0 || r[MCF_D_H_REFINE] > 0) return false;
+  }
+  return true;
 }
 
 // k steps on one handle into d (k rows).  Speculative: steady steps leave the refinement launches
@@ -3280,7 +3288,7 @@ void patch_seconds(mcf_handle* h, double* d, int rows) {
   sec.clear();
 }
 
-int run_chunk(mcf_handle* h, int k, double dt, bool lean, double* d) {
+int run_chunk(mcf_handle* h, int k, double dt, bool spec, bool lean, double* d) {
   const int parity0 = h->parity;
   const double prev_dt0 = h->prev_dt, time0 = h->time;
   const bool hist0 = h->has_history;
@@ -3293,15 +3301,16 @@ int run_chunk(mcf_handle* h, int k, double dt, bool lean, double* d) {
   patch_seconds(h, d, k);
   bool miss = false;
   for (int s = 0; s < k && lean; ++s) miss = miss || ((int)d[(size_t)s * MCF_DIAG_LEN + MCF_D_STATUS] & 32);
-  if (lean) spec_update(h->spec_chunk, h->spec_full, h->spec_backoff, miss);
-  if (!miss) return 0;
-  h->parity = parity0; h->prev_dt = prev_dt0; h->time = time0; h->has_history = hist0;
-  h->prm_valid = false;
-  ++h->replays;
-  hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, 1), dim3(256), 0, h->stream, h->dc, parity0, 0);
-  if (int rc = enqueue_steps(h, k, dt, false)) return rc;
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(d, h->diag, bytes, hipMemcpyDeviceToHost));
+  if (miss) {
+    h->parity = parity0; h->prev_dt = prev_dt0; h->time = time0; h->has_history = hist0;
+    h->prm_valid = false;
+    ++h->replays;
+    hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, 1), dim3(256), 0, h->stream, h->dc, parity0, 0);
+    if (int rc = enqueue_steps(h, k, dt, false)) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(d, h->diag, bytes, hipMemcpyDeviceToHost));
+  }
+  if (spec) spec_update(h->spec_chunk, h->spec_full, lean, miss, quiet_tail(d, k));
   return 0;
 }
 }  // namespace
@@ -3318,8 +3327,7 @@ int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   for (int done = 0; done < n;) {
     const bool lean = spec && h->spec_full == 0;
     const int k = !spec ? n - done : std::min(lean ? h->spec_chunk : h->spec_full, n - done);
-    if (spec && !lean) h->spec_full -= k;
-    if (int rc = run_chunk(h, k, dt, lean, d.data() + (size_t)done * MCF_DIAG_LEN)) return rc;
+    if (int rc = run_chunk(h, k, dt, spec, lean, d.data() + (size_t)done * MCF_DIAG_LEN)) return rc;
     for (int s = 0; s < k; ++s)
       if (int rc = status_error(h, d.data() + (size_t)(done + s) * MCF_DIAG_LEN, done + s)) {
         if (diag) std::memcpy(diag, d.data(), (size_t)(done + k) * MCF_DIAG_LEN * sizeof(double));
@@ -3719,7 +3727,7 @@ struct mcf_batch {
   int spec = 1;                 // speculative lean steps (as mcf_set_speculation)
   int replays = 0;
   int spec_chunk = 8;           // steps per speculative chunk (as the handle's)
-  int spec_full = 0, spec_backoff = 16;
+  int spec_full = 0;
   void drop_graphs() { graphs.drop(); lean_graphs.drop(); }
 };
 
@@ -3883,7 +3891,6 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
   for (int done = 0; done < nsteps;) {
     const bool lean = spec && b->spec_full == 0;
     const int k = !spec ? nsteps - done : std::min(lean ? b->spec_chunk : b->spec_full, nsteps - done);
-    if (spec && !lean) b->spec_full -= k;
     for (int c = 0; c < n; ++c) {
       mcf_handle* h = b->cases[c];
       saved[c] = Host{h->parity, h->prev_dt, h->time, h->has_history};
@@ -3901,7 +3908,6 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
     bool miss = false;
     for (int c = 0; c < n && lean; ++c)
       for (int s = 0; s < k && !dead[c]; ++s) miss = miss || ((int)d[per * c + (size_t)(done + s) * MCF_DIAG_LEN + MCF_D_STATUS] & 32);
-    if (lean) spec_update(b->spec_chunk, b->spec_full, b->spec_backoff, miss);
     if (miss) {
       for (int c = 0; c < n; ++c) {
         mcf_handle* h = b->cases[c];
@@ -3911,6 +3917,12 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, n), dim3(256), 0, b->stream, b->dc, saved[0].parity, 0);
       if (int rc = enqueue(k, false)) return rc;
       if (int rc = copy_rows()) return rc;
+    }
+    if (spec) {
+      bool quiet = true;   // over the members that still count
+      for (int c = 0; c < n && quiet; ++c)
+        if (!dead[c]) quiet = quiet_tail(d.data() + per * c + (size_t)done * MCF_DIAG_LEN, k);
+      spec_update(b->spec_chunk, b->spec_full, lean, miss, quiet);
     }
     for (int c = 0; c < n; ++c) {
       mcf_handle* h = b->cases[c];
