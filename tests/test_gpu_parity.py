@@ -212,7 +212,8 @@ def test_speculative_steps_match_full(pkg):
         for col in (0, 2, 3, 10, 11, 12, 14, 18, 25, 26):
             assert np.array_equal(runs[mode][3][:, col], runs[0][3][:, col]), (mode, col)
     print('replays with speculation on:', runs[1][4]['replays'], '(a chunk replays when one of its steps refines)')
-    # after a miss the next stretch runs in full (backoff), so mode 2 replays the first chunk only
+    # after a miss the next chunk runs in full until a chunk's last steps refine nothing, so the
+    # start-up refinements cost at most a couple of replays; mode 2 replays every lean chunk
     assert runs[1][4]['replays'] <= 2 and runs[2][4]['replays'] >= 1
 
 
