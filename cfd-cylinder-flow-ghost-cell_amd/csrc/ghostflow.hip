@@ -138,13 +138,13 @@ struct SolveDesc {
   const int* pt_j;      // npts (ordered by x column, then y)
   const int* pt_x;      // npts (group order)
   double* firstK;       // npts
-  const double* SC;     // r*npts: S^{-1} C over the points (group order)
-  double* y;            // r
+  const double* SC;     // r*npts: S^{-1} C, rows grouped by x column, points in group order
+  double* y;            // r: the multipliers, rows grouped by x column
   int rg_n;             // number of x columns holding capacitance rows
-  const int* rg_ptr;    // rg_n+1 into rc_j / rc_id
+  const int* rg_ptr;    // rg_n+1: each column's rows (group order)
   const int* rg_x;      // rg_n
-  const int* rc_j;      // r: y index of each row (grouped by x)
-  const int* rc_id;     // r: index into y
+  const double* Vsel;   // yy*r: V at the rows' y index, [mode][row in group order]
+  const int* gpos;      // r: group position of each capacitance row (caller order)
   const int* lo;        // solid intervals of the array rows
   const int* hi;
   double* const* bufs;  // the case's buffer table (device)
@@ -1075,13 +1075,13 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ des
   if (p.mode == 1) {
     // z_k = V^T (rows of the capacitance correction scattered by y), dense only on the x columns
     // that hold a correction row.  The rows (in x-column groups) are staged in passes of two per
-    // thread: the factors V[j_t][k] and y[t] of a pass load together into LDS.  The thread of a
+    // thread: the factors Vsel[k][t] = V[j_t][k] and y[t] of a pass load together into LDS.  The thread of a
     // group (up to TRI_GPT groups per thread) then adds its rows' products in row order, and the
     // group sums are scattered into the zeroed row, which reuses the staging array.
     constexpr int P = 2 * TRI_T;   // rows per pass
     static_assert(2 * P <= TRI_MAXX, "a pass's (V, y) pairs fit the row array");
-    const gptr<const int> rg_ptr = G(d.rg_ptr), rg_x = G(d.rg_x), rc_j = G(d.rc_j), rc_id = G(d.rc_id);
-    const gptr<const double> V = G(d.V), yv = G(d.y);
+    const gptr<const int> rg_ptr = G(d.rg_ptr), rg_x = G(d.rg_x);
+    const gptr<const double> Vk = G(d.Vsel) + (size_t)k * d.r, yv = G(d.y);
     const int ng = d.rg_n, nr = d.r;
     int t0[TRI_GPT], t1[TRI_GPT], xg[TRI_GPT];
     double sg[TRI_GPT];
@@ -1095,18 +1095,12 @@ __global__ __launch_bounds__(TRI_T) void k_tri(const SolveDesc* __restrict__ des
       sg[m] = 0.0;
     }
     for (int base = 0; base < nr; base += P) {
-      int jr[2], ir[2];
-      #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int t = min(base + tid + TRI_T * u, nr - 1);
-        jr[u] = rc_j[t];
-        ir[u] = rc_id[t];
-      }
       double vr[2], yr[2];
       #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        vr[u] = V[(size_t)jr[u] * yy + k];
-        yr[u] = yv[ir[u]];
+        const int t = min(base + tid + TRI_T * u, nr - 1);
+        vr[u] = Vk[t];
+        yr[u] = yv[t];
       }
       #pragma unroll
       for (int u = 0; u < 2; ++u) {
@@ -1248,7 +1242,7 @@ __global__ __launch_bounds__(256) void k_dump(const SolveDesc* __restrict__ desc
     double s = 0.0;
     for (int q = d.Cptr[m]; q < d.Cptr[m + 1]; ++q) s += d.Cval[q] * d.firstK[d.Ccol[q]];
     c[d.apos[m]] = s;
-    y[d.apos[m]] = d.y[m];
+    y[d.apos[m]] = d.y[d.gpos[m]];
   }
   if (threadIdx.x == 0) {
     d.dump_tag[slot] = 4 * (*d.clock) + proj;
@@ -1376,13 +1370,11 @@ __global__ __launch_bounds__(256) void k_fused_of(const CaseDev* __restrict__ ca
   run_kind<K2>(c, f.b[2], bx);
 }
 
-// Surface forces: one lane per quadrature point, one partial sum per wave (summed in a fixed
-// order by k_force_total, so the forces are bit-reproducible)
 // Surface forces: 16 lanes per quadrature point, four points per wave.  Each lane takes every 16th
 // nonzero of the point's three CSR rows (surface pressure reconstruction, du/dn, dv/dn), so a
 // row's loads are in flight together instead of one dependent index-then-value load per nonzero;
 // the lane groups add their products by a butterfly, and each wave writes one partial sum of its
-// four points, added in wave order by force_totals (bit-reproducible).
+// four points, which force_totals adds in a fixed order (bit-reproducible).
 constexpr int FORCE_PPW = 4;      // points per wave
 constexpr int FORCE_MAXP = 4096;  // surface points
 __host__ __device__ inline int force_blocks(int nsurf) { return (nsurf + FORCE_PPW - 1) / FORCE_PPW; }
@@ -2197,6 +2189,7 @@ struct mcf_handle {
   std::map<std::string, void*> owned;
   std::vector<void*> bufs;
   std::vector<int> pt_pos[3];   // capacitance point (caller order) -> group order
+  const int* rc_j[3] = {nullptr, nullptr, nullptr};   // y index of each capacitance row (group order)
   double* inlet = nullptr;
   u64* slots = nullptr;
   double* slot_vals = nullptr;   // NSLOT host-view slot values (k_slot_values)
@@ -2263,6 +2256,29 @@ int sync_descs(mcf_handle* h) {
 }
 
 inline dim3 grid1(int n, int nb = 256) { return dim3((n + nb - 1) / nb, 1, 1); }
+
+// Vsel[k][t] = V[j_t][k]: the y eigenbasis at the capacitance rows' y indices, laid out so that the
+// correction sweep of mode k reads its r factors contiguously.
+__global__ __launch_bounds__(256) void k_vsel(const double* V, const int* rj, int yy, int r, double* out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)yy * r) return;
+  const int k = (int)(e / r), t = (int)(e % r);
+  out[e] = V[(size_t)rj[t] * yy + k];
+}
+
+// (Re)builds a problem's Vsel from its current eigenbasis and capacitance rows; synchronous.
+int build_vsel(mcf_handle* h, int kind) {
+  SolveDesc& s = h->hs[kind];
+  double* dv = nullptr;
+  const size_t n = (size_t)s.yy * s.r;
+  if (int rc = upload<double>(h, "schur" + std::to_string(kind) + "vs", nullptr, n, &dv)) return rc;
+  s.Vsel = dv;
+  if (!n) return 0;
+  hipLaunchKernelGGL(k_vsel, grid1((int)n), dim3(256), 0, h->stream, s.V, h->rc_j[kind], s.yy, s.r, dv);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
 // grid of the grid-stride kernels
 inline dim3 gridS(int n) { return dim3(std::min((n + 255) / 256, GSB), 1, 1); }
 
@@ -3090,6 +3106,8 @@ int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double 
   s.luL = dL; s.luI = dI;
   s.yy = yy; s.xx = xx; s.a = a; s.b = b;
   s.V = dV; s.VT = dVT; s.lam = dl; s.lower = dlo; s.diag = ddg; s.upper = dup; s.F = dF;
+  if (h->has_schur[kind])
+    if (int rc = build_vsel(h, kind)) return rc;
   if (kind == MCF_PRESSURE) { s.off = 0; s.ld = h->nx; s.joff = 0; s.ioff = 0; s.field = MCF_FIELD_P; }
   else if (kind == MCF_U) { s.off = 1; s.ld = h->nx + 1; s.joff = 0; s.ioff = 1; s.field = MCF_FIELD_U; }
   else { s.off = h->usize + h->nx; s.ld = h->nx; s.joff = 1; s.ioff = 0; s.field = MCF_FIELD_V; }
@@ -3122,9 +3140,6 @@ int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, con
     pt_j[t] = points[k] / xx;
     pt_x[t] = points[k] % xx;
   }
-  std::vector<double> scg((size_t)r * npts);
-  for (int m = 0; m < r; ++m)
-    for (int k = 0; k < npts; ++k) scg[(size_t)m * npts + pos[k]] = SC[(size_t)m * npts + k];
   // capacitance rows grouped by x column
   std::vector<int> cnt(xx, 0);
   for (int k = 0; k < r; ++k) cnt[rows[k] % xx]++;
@@ -3135,25 +3150,31 @@ int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, con
   rg_ptr.push_back(acc);
   if ((int)rg_x.size() > TRI_MAXG)
     return fail(h, 1, "capacitance rows lie on more than " + std::to_string(TRI_MAXG) + " x columns");
-  std::vector<int> rc_j(r), rc_id(r);
+  // row k of the caller's S^{-1} C becomes row gpos[k] (group order), its columns in point order
+  std::vector<int> rc_j(r), gpos(r);
   for (int k = 0; k < r; ++k) {
     const int x = rows[k] % xx, t = start[x]++;
-    rc_j[t] = rows[k] / xx; rc_id[t] = k;
+    rc_j[t] = rows[k] / xx; gpos[k] = t;
   }
+  std::vector<double> scg((size_t)r * npts);
+  for (int m = 0; m < r; ++m)
+    for (int k = 0; k < npts; ++k) scg[(size_t)gpos[m] * npts + pos[k]] = SC[(size_t)m * npts + k];
   std::string key = "schur" + std::to_string(kind);
-  int *dpj, *dpx, *drp, *drx, *drj, *dri; double *dS, *dfk, *dy;
+  int *dpj, *dpx, *drp, *drx, *drj, *dgp; double *dS, *dfk, *dy;
   if (int rc = upload(h, key + "pj", pt_j.data(), pt_j.size(), &dpj)) return rc;
   if (int rc = upload(h, key + "px", pt_x.data(), pt_x.size(), &dpx)) return rc;
   if (int rc = upload(h, key + "SC", scg.data(), scg.size(), &dS)) return rc;
   if (int rc = upload(h, key + "rp", rg_ptr.data(), rg_ptr.size(), &drp)) return rc;
   if (int rc = upload(h, key + "rx", rg_x.data(), rg_x.size(), &drx)) return rc;
   if (int rc = upload(h, key + "rj", rc_j.data(), rc_j.size(), &drj)) return rc;
-  if (int rc = upload(h, key + "ri", rc_id.data(), rc_id.size(), &dri)) return rc;
+  if (int rc = upload(h, key + "gp", gpos.data(), gpos.size(), &dgp)) return rc;
   if (int rc = upload<double>(h, key + "fk", nullptr, std::max(npts, 1), &dfk)) return rc;
   if (int rc = upload<double>(h, key + "y", nullptr, std::max(r, 1), &dy)) return rc;
   s.r = r; s.npts = npts;
   s.pt_j = dpj; s.pt_x = dpx; s.firstK = dfk; s.SC = dS; s.y = dy;
-  s.rg_n = (int)rg_x.size(); s.rg_ptr = drp; s.rg_x = drx; s.rc_j = drj; s.rc_id = dri;
+  s.rg_n = (int)rg_x.size(); s.rg_ptr = drp; s.rg_x = drx; s.gpos = dgp;
+  h->rc_j[kind] = drj;
+  if (int rc = build_vsel(h, kind)) return rc;
   // a new capacitance set invalidates the Schur system and any pair dump built on the old one
   s.Sth = nullptr; s.Cptr = nullptr; s.Ccol = nullptr; s.Cval = nullptr; s.apos = nullptr;
   s.dump_c = s.dump_y = nullptr; s.dump_tag = s.dump_count = nullptr; s.dump_cap = 0;
