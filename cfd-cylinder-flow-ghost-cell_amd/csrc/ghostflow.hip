@@ -372,95 +372,115 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
   const gptr<const double> Qv = Q + c.usize;
   const double dt = c.prm[0], alpha = c.prm[1];
   double mu = 0.0, mv = 0.0;
+  // Each face loads all of its operands in one batch, at indices clamped into the arrays, and forms
+  // its value without branches around the loads (inactive faces compute a value that is then
+  // discarded): one memory round trip per face after the lookups, where per-term loads under the
+  // boundary tests took one round trip per test.  The arithmetic is the stencil's, term for term.
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
     double res = 0.0, nw = 0.0;
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
       const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
-      if (u_act(c, j, i)) {
-        auto AU = [&](int jj, int ii) { return ii == 0 ? in[jj] : A[jj * nx1 + ii]; };
-        auto AV = [&](int jj, int ii) { return Av[jj * nx + ii]; };
-        auto QU = [&](int jj, int ii) { return ii == 0 ? in[jj] : Q[jj * nx1 + ii]; };
-        auto fxx = [&](int ii) { double uf = 0.5 * (AU(j, ii) + AU(j, ii + 1)); return -(uf * uf * dy); };
-        auto fxy = [&](int jj) {
-          if (jj == 0 || jj == ny) return 0.0;
-          double vf = (i == nx) ? AV(jj, nx - 1) : 0.5 * (AV(jj, i - 1) + AV(jj, i));
-          double ua = 0.5 * (AU(jj - 1, i) + AU(jj, i));
-          return -(vf * ua * dx);
-        };
-        double ru;
-        if (i < nx) ru = (fxx(i) - fxx(i - 1)) + (fxy(j + 1) - fxy(j));
-        else {
-          double ue = AU(j, nx);
-          ru = -(ue * ue * dy) - fxx(nx - 1) + 0.5 * (fxy(j + 1) - fxy(j));
-        }
-        double adv = ru / (dx * dy);
-        if (i == nx) adv *= 2.0;
-        const double q = QU(j, i);
-        double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * q
-                     + ((i == nx) ? 2.0 : 1.0) * idx2 * QU(j, i - 1);
-        if (i < nx) lap += idx2 * QU(j, i + 1);
-        if (j < ny - 1) lap += idy2 * QU(j + 1, i);
-        if (j > 0) lap += idy2 * QU(j - 1, i);
-        const double diff = c.nu * lap;
-        const double grad = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp)
-                                   : ((i < nx) ? (Pp[j * nx + i] - Pp[j * nx + i - 1]) / dx : -2.0 * Pp[j * nx + nx - 1] / dx) / c.rho;
-        const double doff = (i == 1) ? c.nu * in[j] * idx2 : 0.0;
-        if (a.mode == 0) {
-          nw = adv;
-          const double ex = (1.0 + alpha) * adv - alpha * prev[idx];
-          res = q + dt * (ex + 0.5 * diff - grad) + 0.5 * dt * doff;
-        } else if (a.mode == 1) {
-          nw = adv;
-          res = ((adv + diff) - grad) * (0.5 * dt) + q;
-        } else if (a.mode == 3) {
-          nw = adv;
-          res = diff;
-        } else {
-          res = q + dt * (adv + 0.5 * diff - grad) + 0.5 * dt * doff;
-        }
+      const bool act = u_act(c, j, i);
+      const int iw = max(i - 1, 0), ie = min(i + 1, nx), ic = min(i, nx - 1);
+      const int js = max(j - 1, 0), jn = min(j + 1, ny - 1);
+      // u at (j, i-1 / i / i+1) and (j-1 / j+1, i); v at rows j and j+1 beside the face (v has ny+1 rows)
+      const double inj = in[j];
+      const double a_c = A[j * nx1 + i], a_w0 = A[j * nx1 + iw], a_e = A[j * nx1 + ie];
+      const double a_s = A[js * nx1 + i], a_n = A[jn * nx1 + i];
+      const double vs_w = Av[j * nx + iw], vs_e = Av[j * nx + ic];
+      const double vn_w = Av[(j + 1) * nx + iw], vn_e = Av[(j + 1) * nx + ic];
+      const double q = Q[j * nx1 + i], q_w0 = Q[j * nx1 + iw], q_e = Q[j * nx1 + ie];
+      const double q_s = Q[js * nx1 + i], q_n = Q[jn * nx1 + i];
+      const double p_e = Pp[j * nx + ic], p_w = Pp[j * nx + iw];
+      const double pv = a.mode == 0 ? prev[idx] : 0.0;
+      // u at column 0 is the inlet profile
+      const double a_w = (i == 1) ? inj : a_w0, q_w = (i == 1) ? inj : q_w0;
+      auto fxx = [&](double l, double h) { double uf = 0.5 * (l + h); return -(uf * uf * dy); };
+      auto fxy = [&](bool wall, double vw, double ve, double ulo, double uhi) {
+        if (wall) return 0.0;
+        double vf = (i == nx) ? vw : 0.5 * (vw + ve);
+        double ua = 0.5 * (ulo + uhi);
+        return -(vf * ua * dx);
+      };
+      const double fn = fxy(j + 1 == ny, vn_w, vn_e, a_c, a_n), fs = fxy(j == 0, vs_w, vs_e, a_s, a_c);
+      double ru;
+      if (i < nx) ru = (fxx(a_c, a_e) - fxx(a_w, a_c)) + (fn - fs);
+      else ru = -(a_c * a_c * dy) - fxx(a_w, a_c) + 0.5 * (fn - fs);
+      double adv = ru / (dx * dy);
+      if (i == nx) adv *= 2.0;
+      double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * q
+                   + ((i == nx) ? 2.0 : 1.0) * idx2 * q_w;
+      if (i < nx) lap += idx2 * q_e;
+      if (j < ny - 1) lap += idy2 * q_n;
+      if (j > 0) lap += idy2 * q_s;
+      const double diff = c.nu * lap;
+      double grad = ((i < nx) ? (p_e - p_w) / dx : -2.0 * p_w / dx) / c.rho;
+      if (r >= 0) grad = ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp);
+      const double doff = (i == 1) ? c.nu * inj * idx2 : 0.0;
+      if (a.mode == 0) {
+        nw = adv;
+        const double ex = (1.0 + alpha) * adv - alpha * pv;
+        res = q + dt * (ex + 0.5 * diff - grad) + 0.5 * dt * doff;
+      } else if (a.mode == 1) {
+        nw = adv;
+        res = ((adv + diff) - grad) * (0.5 * dt) + q;
+      } else if (a.mode == 3) {
+        nw = adv;
+        res = diff;
+      } else {
+        res = q + dt * (adv + 0.5 * diff - grad) + 0.5 * dt * doff;
       }
+      res = act ? res : 0.0;
+      nw = act ? nw : 0.0;
       mu = nanmax(mu, fabs(res));
     } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
       const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
-      if (v_act(c, j, i)) {
-        auto AU = [&](int jj, int ii) { return ii == 0 ? in[jj] : A[jj * nx1 + ii]; };
-        auto AV = [&](int jj, int ii) { return Av[jj * nx + ii]; };
-        auto QV = [&](int jj, int ii) { return Qv[jj * nx + ii]; };
-        auto fyy = [&](int jj) { double vf = 0.5 * (AV(jj, i) + AV(jj + 1, i)); return -(vf * vf * dx); };
-        auto fyx = [&](int ii) {
-          if (ii == 0) return 0.0;
-          if (ii == nx) { double uo = 0.5 * (AU(j - 1, nx) + AU(j, nx)); return -(uo * AV(j, nx - 1) * dy); }
-          double uf = 0.5 * (AU(j - 1, ii) + AU(j, ii));
-          double va = 0.5 * (AV(j, ii - 1) + AV(j, ii));
-          return -(uf * va * dy);
-        };
-        const double rv = (fyx(i + 1) - fyx(i)) + (fyy(j) - fyy(j - 1));
-        const double adv = rv / (dx * dy);
-        const double q = QV(j, i);
-        double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * q
-                     + idy2 * (QV(j + 1, i) + QV(j - 1, i));
-        if (i < nx - 1) lap += idx2 * QV(j, i + 1);
-        if (i > 0) lap += idx2 * QV(j, i - 1);
-        const double diff = c.nu * lap;
-        const double grad = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp)
-                                   : (Pp[j * nx + i] - Pp[(j - 1) * nx + i]) / dy / c.rho;
-        if (a.mode == 0) {
-          nw = adv;
-          const double ex = (1.0 + alpha) * adv - alpha * prev[idx];
-          res = q + dt * (ex + 0.5 * diff - grad);
-        } else if (a.mode == 1) {
-          nw = adv;
-          res = ((adv + diff) - grad) * (0.5 * dt) + q;
-        } else if (a.mode == 3) {
-          nw = adv;
-          res = diff;
-        } else {
-          res = q + dt * (adv + 0.5 * diff - grad);
-        }
+      const bool act = v_act(c, j, i);
+      // rows clamped for the wall faces j = 0 and j = ny (inactive): v has ny+1 rows, u and p ny
+      const int js = max(j - 1, 0), jn = min(j + 1, ny), ju = min(j, ny - 1);
+      const int iw = max(i - 1, 0), ie = min(i + 1, nx - 1);
+      // v at (j, i-1 / i / i+1) and (j-1 / j+1, i); u at columns i and i+1 of rows j-1 and j
+      const double v_c = Av[j * nx + i], v_n = Av[jn * nx + i], v_s = Av[js * nx + i];
+      const double v_e = Av[j * nx + ie], v_w = Av[j * nx + iw];
+      const double ue_s = A[js * nx1 + i + 1], ue_n = A[ju * nx1 + i + 1];
+      const double uw_s = A[js * nx1 + i], uw_n = A[ju * nx1 + i];
+      const double q = Qv[j * nx + i], q_n = Qv[jn * nx + i], q_s = Qv[js * nx + i];
+      const double q_e = Qv[j * nx + ie], q_w = Qv[j * nx + iw];
+      const double p_c = Pp[ju * nx + i], p_s = Pp[js * nx + i];
+      const double pv = a.mode == 0 ? prev[idx] : 0.0;
+      auto fyy = [&](double lo, double hi) { double vf = 0.5 * (lo + hi); return -(vf * vf * dx); };
+      double fe;
+      if (i + 1 == nx) { double uo = 0.5 * (ue_s + ue_n); fe = -(uo * v_c * dy); }
+      else { double uf = 0.5 * (ue_s + ue_n); double va = 0.5 * (v_c + v_e); fe = -(uf * va * dy); }
+      double fw = 0.0;
+      if (i != 0) { double uf = 0.5 * (uw_s + uw_n); double va = 0.5 * (v_w + v_c); fw = -(uf * va * dy); }
+      const double rv = (fe - fw) + (fyy(v_c, v_n) - fyy(v_s, v_c));
+      const double adv = rv / (dx * dy);
+      double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * q
+                   + idy2 * (q_n + q_s);
+      if (i < nx - 1) lap += idx2 * q_e;
+      if (i > 0) lap += idx2 * q_w;
+      const double diff = c.nu * lap;
+      double grad = (p_c - p_s) / dy / c.rho;
+      if (r >= 0) grad = ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp);
+      if (a.mode == 0) {
+        nw = adv;
+        const double ex = (1.0 + alpha) * adv - alpha * pv;
+        res = q + dt * (ex + 0.5 * diff - grad);
+      } else if (a.mode == 1) {
+        nw = adv;
+        res = ((adv + diff) - grad) * (0.5 * dt) + q;
+      } else if (a.mode == 3) {
+        nw = adv;
+        res = diff;
+      } else {
+        res = q + dt * (adv + 0.5 * diff - grad);
       }
+      res = act ? res : 0.0;
+      nw = act ? nw : 0.0;
       mv = nanmax(mv, fabs(res));
     }
     if (now) now[idx] = nw;
@@ -501,25 +521,26 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
     const gptr<double> O2 = a.out2_id >= 0 ? G(c.buf[a.out2_id]) : nullptr;
     const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
     const double h = -0.5 * c.prm[0];
+    // a face's stencil operands and right-hand side load in one batch at clamped indices, ahead of
+    // the activity test (as in the advection stage); override rows replace the stencil value
     for (int idx = lo + bx * blockDim.x + threadIdx.x; idx < hi; idx += gx * blockDim.x) {
       double res = 0.0, r2 = 0.0;
+      const double bv = B[idx];
       if (idx < c.usize) {
         const int j = idx / nx1, i = idx - j * nx1;
         const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);   // looked up together with the solid test
-        if (u_act(c, j, i)) {
-          auto XU = [&](int jj, int ii) { return X[jj * nx1 + ii]; };
-          double hx;
-          const double x = XU(j, i);
-          if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_U], r, X);
-          else {
-            double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * x
-                         + ((i == nx) ? 2.0 : 1.0) * idx2 * XU(j, i - 1);
-            if (i < nx) lap += idx2 * XU(j, i + 1);
-            if (j < ny - 1) lap += idy2 * XU(j + 1, i);
-            if (j > 0) lap += idy2 * XU(j - 1, i);
-            hx = x + h * (c.nu * lap);
-          }
-          res = B[idx] - hx;
+        const bool act = u_act(c, j, i);
+        const double x = X[j * nx1 + i], xw = X[j * nx1 + max(i - 1, 0)], xe = X[j * nx1 + min(i + 1, nx)];
+        const double xn = X[min(j + 1, ny - 1) * nx1 + i], xs = X[max(j - 1, 0) * nx1 + i];
+        double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * x
+                     + ((i == nx) ? 2.0 : 1.0) * idx2 * xw;
+        if (i < nx) lap += idx2 * xe;
+        if (j < ny - 1) lap += idy2 * xn;
+        if (j > 0) lap += idy2 * xs;
+        double hx = x + h * (c.nu * lap);
+        if (act && r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_U], r, X);
+        if (act) {
+          res = bv - hx;
           if (a.mode == 2) { r2 = x + res; xu = nanmax(xu, fabs(x)); }
         }
         mu = nanmax(mu, fabs(res));
@@ -527,19 +548,18 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
         const int t = idx - c.usize;
         const int j = t / nx, i = t - j * nx;
         const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
-        if (v_act(c, j, i)) {
-          auto XV = [&](int jj, int ii) { return Xv[jj * nx + ii]; };
-          double hx;
-          const double x = XV(j, i);
-          if (r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_V], r, X);
-          else {
-            double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * x
-                         + idy2 * (XV(j + 1, i) + XV(j - 1, i));
-            if (i < nx - 1) lap += idx2 * XV(j, i + 1);
-            if (i > 0) lap += idx2 * XV(j, i - 1);
-            hx = x + h * (c.nu * lap);
-          }
-          res = B[idx] - hx;
+        const bool act = v_act(c, j, i);
+        // rows clamped for the wall faces j = 0 and j = ny (inactive)
+        const double x = Xv[j * nx + i], xn = Xv[min(j + 1, ny) * nx + i], xs = Xv[max(j - 1, 0) * nx + i];
+        const double xe = Xv[j * nx + min(i + 1, nx - 1)], xw = Xv[j * nx + max(i - 1, 0)];
+        double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * x
+                     + idy2 * (xn + xs);
+        if (i < nx - 1) lap += idx2 * xe;
+        if (i > 0) lap += idx2 * xw;
+        double hx = x + h * (c.nu * lap);
+        if (act && r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_V], r, X);
+        if (act) {
+          res = bv - hx;
           if (a.mode == 2) { r2 = x + res; xv = nanmax(xv, fabs(x)); }
         }
         mv = nanmax(mv, fabs(res));
@@ -582,15 +602,17 @@ DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
     const int j = idx / nx, i = idx - j * nx;
     double d = 0.0;
     const int r = ov_row(c.ov[MCF_OP_DIV], j, i);   // looked up together with the solid test
-    if (p_act(c, j, i)) {
-      const double off = (i == 0) ? -inl[j] / c.dx : 0.0;
-      if (r >= 0) d = ov_dot(c.ov[MCF_OP_DIV], r, X) + off;
-      else {
-        const double ul = (i == 0) ? inl[j] : X[j * nx1 + i];
-        d = (X[j * nx1 + i + 1] - ul) / c.dx + (Xv[(j + 1) * nx + i] - Xv[j * nx + i]) / c.dy;
-      }
-      d *= scale;
+    const bool act = p_act(c, j, i);
+    // the cell's four faces (and the inlet value) load in one batch ahead of the activity test
+    const double inj = inl[j], xl = X[j * nx1 + i], xr = X[j * nx1 + i + 1];
+    const double vt = Xv[(j + 1) * nx + i], vb = Xv[j * nx + i];
+    const double ul = (i == 0) ? inj : xl;
+    double dc = (xr - ul) / c.dx + (vt - vb) / c.dy;
+    if (act && r >= 0) {
+      const double off = (i == 0) ? -inj / c.dx : 0.0;
+      dc = ov_dot(c.ov[MCF_OP_DIV], r, X) + off;
     }
+    if (act) d = dc * scale;
     if (O) O[idx] = d;
     m = nanmax(m, fabs(d));
   }
@@ -615,29 +637,33 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
   const gptr<const double> Df = a.defect_id >= 0 ? G(c.buf[a.defect_id]) : nullptr;
   const double dt = a.dtmul * c.prm[0];
   double m = 0.0;
+  const gptr<const double> Dl = Df ? Df : X;   // read unconditionally; used only with a defect
+  // a face's pressure pair, old value and defect load in one batch at clamped indices, ahead of
+  // the activity test; override rows replace the stencil gradient
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.velsize; idx += gx * blockDim.x) {
-    double out = 0.0, g = 0.0;
+    double out = 0.0, g;
     bool act;
+    const double xv = X[idx], dv = Dl[idx];
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
       const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
       act = u_act(c, j, i);
-      if (act) {
-        g = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp)
-                   : ((i < nx) ? (Pp[j * nx + i] - Pp[j * nx + i - 1]) / c.dx : -2.0 * Pp[j * nx + nx - 1] / c.dx) / c.rho;
-      }
+      const double pe = Pp[j * nx + min(i, nx - 1)], pw = Pp[j * nx + max(i - 1, 0)];
+      g = ((i < nx) ? (pe - pw) / c.dx : -2.0 * pw / c.dx) / c.rho;
+      if (act && r >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp);
     } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
       const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
       act = v_act(c, j, i);
-      if (act) {
-        g = r >= 0 ? ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp) : (Pp[j * nx + i] - Pp[(j - 1) * nx + i]) / c.dy / c.rho;
-      }
+      // rows clamped for the wall faces j = 0 and j = ny (inactive)
+      const double pc = Pp[min(j, c.ny - 1) * nx + i], ps = Pp[max(j - 1, 0) * nx + i];
+      g = (pc - ps) / c.dy / c.rho;
+      if (act && r >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp);
     }
-    if (act) out = X[idx] - dt * g;
+    if (act) out = xv - dt * g;
     O[idx] = out;
-    if (Df && act) m = nanmax(m, fabs(-Df[idx] + dt * g));
+    if (Df && act) m = nanmax(m, fabs(-dv + dt * g));
   }
   if (Df) {
     double v1[1] = {m};
@@ -655,21 +681,25 @@ struct PresArgs { int b_id, x_id, out_id; int mode; int slot0; int res_slot; int
 // Row (j, i) of the pressure operator A applied to X: (A X)_row, sum |a||x| over the row and the
 // row's nonzero count.  Override rows come from their CSR rows; the others are the rectangle's
 // 5-point stencil (outlet column Dirichlet, other walls Neumann).
+// The stencil's five values load in one batch (a missing neighbour reads the centre, unused) and
+// the stencil row is formed for every row, so that the loads go out ahead of the override test.
 template <class P>
 DEV void p_row(const CaseDev& c, P X, int idx, int j, int i, int r, double& ax, double& aax, int& nnz) {
-  if (r >= 0) { ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz); return; }
   const int nx = c.nx, ny = c.ny;
+  const double x = X[idx];
+  const double tw = X[i > 0 ? idx - 1 : idx], te = X[i < nx - 1 ? idx + 1 : idx];
+  const double ts = X[j > 0 ? idx - nx : idx], tn = X[j < ny - 1 ? idx + nx : idx];
   const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
   const double ox = idx2 / c.rho, oy = idy2 / c.rho;
-  const double x = X[idx];
   const double cx = (i == 0) ? -1.0 : ((i == nx - 1) ? -3.0 : -2.0);
   const double cy = (j == 0 || j == ny - 1) ? -1.0 : -2.0;
   const double dg = (cx * idx2 + cy * idy2) / c.rho;
   ax = dg * x; aax = fabs(dg) * fabs(x); nnz = 1;
-  if (i > 0) { double t = X[idx - 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
-  if (i < nx - 1) { double t = X[idx + 1]; ax += ox * t; aax += ox * fabs(t); ++nnz; }
-  if (j > 0) { double t = X[idx - nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
-  if (j < ny - 1) { double t = X[idx + nx]; ax += oy * t; aax += oy * fabs(t); ++nnz; }
+  if (i > 0) { ax += ox * tw; aax += ox * fabs(tw); ++nnz; }
+  if (i < nx - 1) { ax += ox * te; aax += ox * fabs(te); ++nnz; }
+  if (j > 0) { ax += oy * ts; aax += oy * fabs(ts); ++nnz; }
+  if (j < ny - 1) { ax += oy * tn; aax += oy * fabs(tn); ++nnz; }
+  if (r >= 0) ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz);
 }
 
 DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
@@ -684,10 +714,11 @@ DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
     const int j = idx / nx, i = idx - j * nx;
     double res = 0.0;
     const int orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);   // looked up together with the solid test
-    if (p_act(c, j, i)) {
-      double ax, aax; int nnz;
-      p_row(c, X, idx, j, i, orow, ax, aax, nnz);
-      const double b = B[idx];
+    const bool act = p_act(c, j, i);
+    const double b = B[idx];
+    double ax, aax; int nnz;
+    p_row(c, X, idx, j, i, orow, ax, aax, nnz);   // ahead of the activity test: one batch of loads
+    if (act) {
       res = b - ax;
       const double sc = aax + fabs(b);
       mres = nanmax(mres, fabs(res));
