@@ -1498,6 +1498,9 @@ __host__ __device__ inline bool solve_ok(const double* rec) {
   return rec[MCF_S_STATUS] == 0.0 && rec[MCF_S_RES_LINF] <= rec[MCF_S_TARGET];
 }
 
+// lean: which solves ran without their refinement launches (LEAN_VEL: the velocity solve,
+// LEAN_PRES: the pressure solves)
+enum { LEAN_VEL = 1, LEAN_PRES = 2, LEAN_ALL = 3 };
 struct FinArgs { int nproj; int lean; int force_miss; int gmres; };
 static_assert(MCF_DIAG_LEN == 64, "one diagnostics entry per lane of k_finalize");
 // Slots gathered by k_finalize: FIN_MAX maxima over the stripes, then FIN_SUM sums
@@ -1610,9 +1613,10 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
     if (!(divg <= 1e-10)) status |= 4;
     if (!(maxdiv < 1e-10) || !(cfl < 0.5)) status |= 8;
     if (!isfinite(cd) || !isfinite(cl)) status |= 16;
-    // a lean step (refinement launches left out) whose first residual called for a refinement:
-    // the host replays the chunk with the refinements in place
-    if (a.lean && (refine > 0 || hrefine > 0 || a.force_miss)) status |= 32;
+    // a lean step (refinement launches left out) whose first residual called for a refinement of a
+    // solve that ran lean: the host replays the chunk with the refinements in place
+    if (((a.lean & LEAN_PRES) && refine > 0) || ((a.lean & LEAN_VEL) && hrefine > 0) || (a.lean && a.force_miss))
+      status |= 32;
     d[MCF_D_STATUS] = status;
   }
   __syncthreads();
@@ -2242,10 +2246,12 @@ struct mcf_handle {
   Launch ln;                 // launch shapes of this handle's solves
   bool tuned = false;        // ln chosen (by the autotuner or the caller)
   StepGraphs* lean_graphs = nullptr;
+  StepGraphs* vlean_graphs = nullptr;   // velocity solve lean, pressure solves in full
   int spec = 1;              // speculative lean steps: 0 off, 1 on, 2 on with every lean step flagged
   int replays = 0;           // chunks replayed with the refinements in place
   int spec_chunk = 8;        // steps per speculative chunk: doubles after a clean chunk, back to 8 on a miss
   int spec_full = 0;         // length of the next chunk when it runs in full (0: the next chunk is lean)
+  bool spec_vfull = false;   // the last chunk refined a velocity solve: full chunks refine every solve
   int pbackend = MCF_PB_DIRECT;   // pressure backend
   Krylov* kry = nullptr;
 };
@@ -2669,8 +2675,9 @@ int check_ready(mcf_handle* h, double dt) {
 // One AB2/CN step of every case of the context (time_integration.py:7-145,
 // optimized_ghost_cell.py:236-286, main.py:383-415).  Launches only: dt, alpha and the pressure
 // factor are read from each case's parameters (k_set_prm); the caller advances the host state.
-int enqueue_step_kernels(const Ctx& x, int parity, bool first, bool lean = false, bool force_miss = false) {
-  lean = lean && !first;
+int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, bool force_miss = false) {
+  if (first) lean = 0;
+  const bool vlean = lean & LEAN_VEL, plean = lean & LEAN_PRES;
   int rc;
   const int S = B_S0 + parity, Snew = B_S0 + 1 - parity;
   const int Hprev = B_H0 + 1 - parity, Hnow = B_H0 + parity;
@@ -2694,14 +2701,14 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, bool lean = false
   // first projection's divergence, the first gate with the defect, the second gate with the
   // pressures and the ghost fill of the new state.
   Body hfin, gate1, gate2;
-  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, lean, &hfin))) return rc;
-  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, lean, &hfin, &gate1))) return rc;
+  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, vlean, &hfin))) return rc;
+  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, plean, &hfin, &gate1))) return rc;
   launch_bodies(x, {gate1, body_helm(x, HelmArgs{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1})});
-  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, lean, nullptr, &gate2))) return rc;
+  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, plean, nullptr, &gate2))) return rc;
   launch_bodies(x, {gate2, body_press(x, PressArgs{B_PH, B_PHI2, B_PH, B_POUT}), body_fill(x, FillArgs{Snew, 1, 1, 1})});
   ForceArgs fa{Snew, B_POUT, -1.0};
   hipLaunchKernelGGL(k_forces, dim3(force_blocks(x.maxsurf), x.n), dim3(64), 0, x.stream, x.dc, fa);
-  FinArgs fin{nproj, lean ? 1 : 0, force_miss ? 1 : 0, x.gm ? 1 : 0};
+  FinArgs fin{nproj, lean, force_miss ? 1 : 0, x.gm ? 1 : 0};
   hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, fin);
   return launch_check(x.err, "step");
 }
@@ -2721,7 +2728,7 @@ struct StepGraphs {
     }
     valid = false;
   }
-  int build(const Ctx& x, bool lean = false, bool force_miss = false) {
+  int build(const Ctx& x, int lean = 0, bool force_miss = false) {
     drop();
     for (int p = 0; p < 2; ++p) {
       HIPCHK(x.err, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
@@ -2750,6 +2757,7 @@ void advance_host_state(mcf_handle* h, double dt) {
 void drop_graphs(mcf_handle* h) {
   if (h->graphs) h->graphs->drop();
   if (h->lean_graphs) h->lean_graphs->drop();
+  if (h->vlean_graphs) h->vlean_graphs->drop();
   if (h->kry) h->kry->drop_group();
   h->graph_dt = -1.0;
   h->prm_valid = false;
@@ -2804,10 +2812,10 @@ int ensure_tuned(mcf_handle* h) {
   return 0;
 }
 
-int enqueue_steps(mcf_handle* h, int n, double dt, bool lean = false) {
+int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
   if (int rc = ensure_tuned(h)) return rc;
   const Ctx x = ctx_of(h);
-  StepGraphs* gs = lean ? h->lean_graphs : h->graphs;
+  StepGraphs* gs = lean == LEAN_ALL ? h->lean_graphs : lean == LEAN_VEL ? h->vlean_graphs : h->graphs;
   hipLaunchKernelGGL(k_set_clock, dim3(1, 1), dim3(64), 0, h->stream, h->dc, h->time);
   for (int s = 0; s < n; ++s) {
     const bool steady = h->has_history && h->prev_dt == dt;
@@ -2941,6 +2949,7 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   HIPCHK(h, hipMemset(h->prm, 0, 4 * sizeof(double)));
   h->graphs = new StepGraphs();
   h->lean_graphs = new StepGraphs();
+  h->vlean_graphs = new StepGraphs();
   HIPCHK(h, hipMalloc(&h->dc, sizeof(CaseDev)));
   HIPCHK(h, hipMalloc(&h->ds, sizeof(h->hs)));
   CaseDev& c = h->hc;
@@ -2969,8 +2978,10 @@ void mcf_destroy(mcf_handle* h) {
   drop_graphs(h);
   delete h->graphs;
   delete h->lean_graphs;
+  delete h->vlean_graphs;
   h->graphs = nullptr;
   h->lean_graphs = nullptr;
+  h->vlean_graphs = nullptr;
   (void)hipFree(h->clock_step); (void)hipFree(h->clock_time); (void)hipFree(h->prm);
   for (auto& kv : h->owned) (void)hipFree(kv.second);
   for (void* p : h->bufs) (void)hipFree(p);
@@ -3314,6 +3325,16 @@ void spec_update(int& chunk, int& full, bool lean, bool miss, bool quiet) {
   chunk = SPEC_MIN;
   full = quiet ? 0 : SPEC_FULL;
 }
+// Any velocity-solve refinement in k diagnostics rows
+bool vel_refined(const double* rows, int k) {
+  for (int s = 0; s < k; ++s)
+    if (rows[(size_t)s * MCF_DIAG_LEN + MCF_D_H_REFINE] > 0) return true;
+  return false;
+}
+// Lean mask of the next chunk: all solves lean; or, for a full chunk, the velocity solve lean
+// (its refinements are rarer than the pressure solves') unless the last chunk refined it
+int spec_mask(bool spec, int full, bool vfull) { return !spec ? 0 : full == 0 ? LEAN_ALL : vfull ? 0 : LEAN_VEL; }
+
 // No refinement (pressure or velocity) in the last SPEC_QUIET of k diagnostics rows
 bool quiet_tail(const double* rows, int k) {
   for (int s = std::max(0, k - SPEC_QUIET); s < k; ++s) {
@@ -3340,7 +3361,7 @@ void patch_seconds(mcf_handle* h, double* d, int rows) {
   sec.clear();
 }
 
-int run_chunk(mcf_handle* h, int k, double dt, bool spec, bool lean, double* d) {
+int run_chunk(mcf_handle* h, int k, double dt, bool spec, int lean, double* d) {
   const int parity0 = h->parity;
   const double prev_dt0 = h->prev_dt, time0 = h->time;
   const bool hist0 = h->has_history;
@@ -3358,11 +3379,14 @@ int run_chunk(mcf_handle* h, int k, double dt, bool spec, bool lean, double* d) 
     h->prm_valid = false;
     ++h->replays;
     hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, 1), dim3(256), 0, h->stream, h->dc, parity0, 0);
-    if (int rc = enqueue_steps(h, k, dt, false)) return rc;
+    if (int rc = enqueue_steps(h, k, dt, 0)) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(d, h->diag, bytes, hipMemcpyDeviceToHost));
   }
-  if (spec) spec_update(h->spec_chunk, h->spec_full, lean, miss, quiet_tail(d, k));
+  if (spec) {
+    spec_update(h->spec_chunk, h->spec_full, lean == LEAN_ALL, miss, quiet_tail(d, k));
+    h->spec_vfull = vel_refined(d, k);
+  }
   return 0;
 }
 }  // namespace
@@ -3377,8 +3401,8 @@ int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   const bool spec = h->spec > 0 && h->use_graphs && h->hs[0].dump_cap == 0 && h->pbackend == MCF_PB_DIRECT;
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
   for (int done = 0; done < n;) {
-    const bool lean = spec && h->spec_full == 0;
-    const int k = !spec ? n - done : std::min(lean ? h->spec_chunk : h->spec_full, n - done);
+    const int lean = spec_mask(spec, h->spec_full, h->spec_vfull);
+    const int k = !spec ? n - done : std::min(lean == LEAN_ALL ? h->spec_chunk : h->spec_full, n - done);
     if (int rc = run_chunk(h, k, dt, spec, lean, d.data() + (size_t)done * MCF_DIAG_LEN)) return rc;
     for (int s = 0; s < k; ++s)
       if (int rc = status_error(h, d.data() + (size_t)(done + s) * MCF_DIAG_LEN, done + s)) {
@@ -3684,7 +3708,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   // one untimed step settles the history so that the timed steps replay the captured graph; with
   // speculation on, the timed steps are the lean ones (a step that would have refined is counted
   // in out5[4], not replayed: the state is not used after profiling)
-  const bool lean = h->spec > 0 && h->use_graphs && h->pbackend == MCF_PB_DIRECT;
+  const int lean = h->spec > 0 && h->use_graphs && h->pbackend == MCF_PB_DIRECT ? LEAN_ALL : 0;
   if (int rc = enqueue_steps(h, 1, dt, lean)) return rc;
   HIPCHK(h, hipEventRecord(e0, h->stream));
   if (int rc = enqueue_steps(h, n, dt, lean)) return rc;
@@ -3776,11 +3800,13 @@ struct mcf_batch {
   Launch ln;                    // launch shapes for the whole batch
   bool tuned = false;
   StepGraphs lean_graphs;       // steady steps without the refinement launches
+  StepGraphs vlean_graphs;      // steady steps whose velocity solve leaves its refinements out
   int spec = 1;                 // speculative lean steps (as mcf_set_speculation)
   int replays = 0;
   int spec_chunk = 8;           // steps per speculative chunk (as the handle's)
   int spec_full = 0;
-  void drop_graphs() { graphs.drop(); lean_graphs.drop(); }
+  bool spec_vfull = false;
+  void drop_graphs() { graphs.drop(); lean_graphs.drop(); vlean_graphs.drop(); }
 };
 
 namespace {
@@ -3909,7 +3935,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
   const bool spec = b->spec > 0 && e->use_graphs;
   struct Host { int parity; double prev_dt, time; bool hist; };
   std::vector<Host> saved(n);
-  auto enqueue = [&](int k, bool use_lean) -> int {
+  auto enqueue = [&](int k, int lean) -> int {
     for (int c = 0; c < n; ++c) {
       mcf_handle* h = b->cases[c];
       const int zero = 0;
@@ -3918,7 +3944,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       HIPCHK(e, hipMemcpy(h->prm, &b->dt[c], sizeof(double), hipMemcpyHostToDevice));
       h->prm_valid = false;
     }
-    StepGraphs& gs = use_lean ? b->lean_graphs : b->graphs;
+    StepGraphs& gs = lean == LEAN_ALL ? b->lean_graphs : lean == LEAN_VEL ? b->vlean_graphs : b->graphs;
     for (int s = 0; s < k; ++s) {
       mcf_handle* h0 = b->cases[0];
       const bool first = !h0->has_history;
@@ -3926,7 +3952,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       if (first || !e->use_graphs) {
         if (int rc = enqueue_step_kernels(x, h0->parity, first)) return rc;
       } else {
-        if (!gs.valid) if (int rc = gs.build(x, use_lean, b->spec == 2)) return rc;
+        if (!gs.valid) if (int rc = gs.build(x, lean, b->spec == 2)) return rc;
         HIPCHK(e, hipGraphLaunch(gs.exec[h0->parity], b->stream));
       }
       for (int c = 0; c < n; ++c) advance_host_state(b->cases[c], b->dt[c]);
@@ -3941,8 +3967,8 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
   int first_error = 0;
   std::vector<char> dead(n, 0);
   for (int done = 0; done < nsteps;) {
-    const bool lean = spec && b->spec_full == 0;
-    const int k = !spec ? nsteps - done : std::min(lean ? b->spec_chunk : b->spec_full, nsteps - done);
+    const int lean = spec_mask(spec, b->spec_full, b->spec_vfull);
+    const int k = !spec ? nsteps - done : std::min(lean == LEAN_ALL ? b->spec_chunk : b->spec_full, nsteps - done);
     for (int c = 0; c < n; ++c) {
       mcf_handle* h = b->cases[c];
       saved[c] = Host{h->parity, h->prev_dt, h->time, h->has_history};
@@ -3967,14 +3993,19 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       }
       ++b->replays;
       hipLaunchKernelGGL(k_ckpt, gridSn(x.maxvel, n), dim3(256), 0, b->stream, b->dc, saved[0].parity, 0);
-      if (int rc = enqueue(k, false)) return rc;
+      if (int rc = enqueue(k, 0)) return rc;
       if (int rc = copy_rows()) return rc;
     }
     if (spec) {
-      bool quiet = true;   // over the members that still count
-      for (int c = 0; c < n && quiet; ++c)
-        if (!dead[c]) quiet = quiet_tail(d.data() + per * c + (size_t)done * MCF_DIAG_LEN, k);
-      spec_update(b->spec_chunk, b->spec_full, lean, miss, quiet);
+      bool quiet = true, vel = false;   // over the members that still count
+      for (int c = 0; c < n; ++c) {
+        if (dead[c]) continue;
+        const double* rows = d.data() + per * c + (size_t)done * MCF_DIAG_LEN;
+        quiet = quiet && quiet_tail(rows, k);
+        vel = vel || vel_refined(rows, k);
+      }
+      spec_update(b->spec_chunk, b->spec_full, lean == LEAN_ALL, miss, quiet);
+      b->spec_vfull = vel;
     }
     for (int c = 0; c < n; ++c) {
       mcf_handle* h = b->cases[c];
