@@ -1498,9 +1498,10 @@ __host__ __device__ inline bool solve_ok(const double* rec) {
   return rec[MCF_S_STATUS] == 0.0 && rec[MCF_S_RES_LINF] <= rec[MCF_S_TARGET];
 }
 
-// lean: which solves ran without their refinement launches (LEAN_VEL: the velocity solve,
-// LEAN_PRES: the pressure solves)
-enum { LEAN_VEL = 1, LEAN_PRES = 2, LEAN_ALL = 3 };
+// lean: which solves ran without (some of) their refinement launches (LEAN_VEL: the velocity solve
+// without any, LEAN_PRES: the pressure solves without any, ONCE_PRES: the pressure solves without
+// the second)
+enum { LEAN_VEL = 1, LEAN_PRES = 2, LEAN_ALL = 3, ONCE_PRES = 4 };
 struct FinArgs { int nproj; int lean; int force_miss; int gmres; };
 static_assert(MCF_DIAG_LEN == 64, "one diagnostics entry per lane of k_finalize");
 // Slots gathered by k_finalize: FIN_MAX maxima over the stripes, then FIN_SUM sums
@@ -1580,12 +1581,18 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
     int status = 0;
     if (!(mres <= 1e-10 * fmax(1.0, rhs_max))) status |= 1;
     int refine = 0;
+    bool second = false;   // a pressure solve needed the second refinement that ONCE_PRES left out
     double worst = -1.0, relmax = 0.0;
     // one record per pressure solve in call order: the startup projection (block 2) comes first
     for (int q = 0; q < a.nproj; ++q) {
       const int b = a.nproj == 3 ? (q == 0 ? 2 : q - 1) : q;
       const double pfin = v[17 + 6 * b], prnd = v[18 + 6 * b];
-      const SolveSlots ss{v[14 + 6 * b], v[15 + 6 * b], v[16 + 6 * b], pfin, prnd, v[19 + 6 * b],
+      // with one refinement pass the final residual is the one the second pass would have tested
+      const bool once = a.lean & ONCE_PRES;
+      const double pval = v[14 + 6 * b], pr1 = v[15 + 6 * b], pr2 = once ? pfin : v[16 + 6 * b];
+      const double tgt = woodbury_target(pval);
+      second = second || (once && (pr1 > tgt || pr1 != pr1) && (pfin > tgt || pfin != pfin));
+      const SolveSlots ss{pval, pr1, pr2, pfin, prnd, v[19 + 6 * b],
                           v[FIN_MAX + 2 * b], v[FIN_MAX + 2 * b + 1], v[32 + b], v[35 + b], v[38 + b]};
       double rec[MCF_SOLVE_LEN];
       solve_record(ss, a.gmres != 0, rec);
@@ -1615,7 +1622,8 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
     if (!isfinite(cd) || !isfinite(cl)) status |= 16;
     // a lean step (refinement launches left out) whose first residual called for a refinement of a
     // solve that ran lean: the host replays the chunk with the refinements in place
-    if (((a.lean & LEAN_PRES) && refine > 0) || ((a.lean & LEAN_VEL) && hrefine > 0) || (a.lean && a.force_miss))
+    if (((a.lean & LEAN_PRES) && refine > 0) || ((a.lean & LEAN_VEL) && hrefine > 0) || second ||
+        (a.lean && a.force_miss))
       status |= 32;
     d[MCF_D_STATUS] = status;
   }
@@ -2246,12 +2254,13 @@ struct mcf_handle {
   Launch ln;                 // launch shapes of this handle's solves
   bool tuned = false;        // ln chosen (by the autotuner or the caller)
   StepGraphs* lean_graphs = nullptr;
-  StepGraphs* vlean_graphs = nullptr;   // velocity solve lean, pressure solves in full
+  StepGraphs* mixed[8] = {};   // other lean masks (LEAN_VEL, LEAN_VEL | ONCE_PRES), built on first use
   int spec = 1;              // speculative lean steps: 0 off, 1 on, 2 on with every lean step flagged
   int replays = 0;           // chunks replayed with the refinements in place
   int spec_chunk = 8;        // steps per speculative chunk: doubles after a clean chunk, back to 8 on a miss
   int spec_full = 0;         // length of the next chunk when it runs in full (0: the next chunk is lean)
   bool spec_vfull = false;   // the last chunk refined a velocity solve: full chunks refine every solve
+  bool spec_p2 = false;      // the last chunk refined a pressure solve twice: full chunks allow it
   int pbackend = MCF_PB_DIRECT;   // pressure backend
   Krylov* kry = nullptr;
 };
@@ -2601,12 +2610,14 @@ int kry_solve(const Ctx& x, int b_id, int x_id, int psolve) {
   return launch_check(h, "gmres pressure solve");
 }
 
-// Woodbury solve with <= 2 refinements (pressure_schur.py:116-124) and the final residual.  The
-// lean form leaves the refinement launches out and computes the first and the final residual in
-// one pass; a step that would have refined is flagged by k_finalize and replayed in full.
+// Woodbury solve with <= 2 refinements (pressure_schur.py:116-124) and the final residual, with
+// `refs` refinement passes launched (each runs only when the residual before it calls for it).  The
+// lean form (refs = 0) computes the first and the final residual in one pass; refs = 1 leaves the
+// second pass out.  A step whose solve would have needed a pass that was left out is flagged by
+// k_finalize and replayed in full.
 // With `tail`, the final-statistics stage is returned there instead of launched, for the caller to
 // merge into its next launch.
-int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bool lean = false,
+int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, int refs = 2,
                   Body* tail = nullptr) {
   if (tail) *tail = no_body();
   if (pres && x.gm) return kry_solve(x, in_id, out_id, psolve);
@@ -2614,14 +2625,14 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bo
   const int slot0 = pres ? SL_P0 + 8 * psolve : SL_HU;
   int rc;
   if ((rc = enqueue_apply(x, pres, in_id, out_id, 1, -1, shift))) return rc;
-  if (lean) {
+  if (refs == 0) {
     const Body fin = pres ? body_pres(x, PresArgs{in_id, out_id, -1, 3, slot0, SV_RES1, -1})
                           : body_helm(x, HelmArgs{in_id, out_id, -1, -1, 3, SV_RES1, -1});
     if (tail) *tail = fin;
     else launch_bodies(x, {fin});
     return launch_check(x.err, "lean woodbury solve");
   }
-  for (int it = 0; it < 2; ++it) {
+  for (int it = 0; it < refs; ++it) {
     const int res_slot = it == 0 ? SV_RES1 : SV_RES2;
     const int pred = it == 0 ? -1 : SV_RES1;
     if (pres) {
@@ -2646,11 +2657,11 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, bo
 // statistics are launched with the update; with `gate`, the divergence gate is returned there for
 // the caller to merge into its next launch.
 int enqueue_project(const Ctx& x, int x_id, int out_id, int phi_id, double dtmul, int psolve, int defect_id,
-                    bool lean = false, const Body* head = nullptr, Body* gate = nullptr) {
+                    int refs = 2, const Body* head = nullptr, Body* gate = nullptr) {
   int rc;
   launch_bodies(x, {head ? *head : no_body(), body_div(x, DivArgs{x_id, B_DIVP, dtmul, SL_P0 + 8 * psolve + SV_VAL})});
   Body fin;
-  if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve, lean, &fin))) return rc;
+  if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve, refs, &fin))) return rc;
   launch_bodies(x, {fin, body_upd(x, UpdArgs{x_id, phi_id, out_id, defect_id, dtmul})});
   const Body g = body_div(x, DivArgs{out_id, -1, 0.0, SL_DIV0 + psolve});
   if (gate) *gate = g;
@@ -2677,7 +2688,8 @@ int check_ready(mcf_handle* h, double dt) {
 // factor are read from each case's parameters (k_set_prm); the caller advances the host state.
 int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, bool force_miss = false) {
   if (first) lean = 0;
-  const bool vlean = lean & LEAN_VEL, plean = lean & LEAN_PRES;
+  const int vrefs = (lean & LEAN_VEL) ? 0 : 2;
+  const int prefs = (lean & LEAN_PRES) ? 0 : (lean & ONCE_PRES) ? 1 : 2;
   int rc;
   const int S = B_S0 + parity, Snew = B_S0 + 1 - parity;
   const int Hprev = B_H0 + 1 - parity, Hnow = B_H0 + parity;
@@ -2701,10 +2713,10 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, boo
   // first projection's divergence, the first gate with the defect, the second gate with the
   // pressures and the ghost fill of the new state.
   Body hfin, gate1, gate2;
-  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, vlean, &hfin))) return rc;
-  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, plean, &hfin, &gate1))) return rc;
+  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, vrefs, &hfin))) return rc;
+  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, prefs, &hfin, &gate1))) return rc;
   launch_bodies(x, {gate1, body_helm(x, HelmArgs{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1})});
-  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, plean, nullptr, &gate2))) return rc;
+  if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, prefs, nullptr, &gate2))) return rc;
   launch_bodies(x, {gate2, body_press(x, PressArgs{B_PH, B_PHI2, B_PH, B_POUT}), body_fill(x, FillArgs{Snew, 1, 1, 1})});
   ForceArgs fa{Snew, B_POUT, -1.0};
   hipLaunchKernelGGL(k_forces, dim3(force_blocks(x.maxsurf), x.n), dim3(64), 0, x.stream, x.dc, fa);
@@ -2757,7 +2769,8 @@ void advance_host_state(mcf_handle* h, double dt) {
 void drop_graphs(mcf_handle* h) {
   if (h->graphs) h->graphs->drop();
   if (h->lean_graphs) h->lean_graphs->drop();
-  if (h->vlean_graphs) h->vlean_graphs->drop();
+  for (StepGraphs* g : h->mixed)
+    if (g) g->drop();
   if (h->kry) h->kry->drop_group();
   h->graph_dt = -1.0;
   h->prm_valid = false;
@@ -2815,7 +2828,8 @@ int ensure_tuned(mcf_handle* h) {
 int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
   if (int rc = ensure_tuned(h)) return rc;
   const Ctx x = ctx_of(h);
-  StepGraphs* gs = lean == LEAN_ALL ? h->lean_graphs : lean == LEAN_VEL ? h->vlean_graphs : h->graphs;
+  StepGraphs* gs = lean == LEAN_ALL ? h->lean_graphs : lean == 0 ? h->graphs : h->mixed[lean];
+  if (!gs) gs = h->mixed[lean] = new StepGraphs();
   hipLaunchKernelGGL(k_set_clock, dim3(1, 1), dim3(64), 0, h->stream, h->dc, h->time);
   for (int s = 0; s < n; ++s) {
     const bool steady = h->has_history && h->prev_dt == dt;
@@ -2869,7 +2883,10 @@ int ensure_diag(mcf_handle* h, int rows) {
   HIPCHK(h, hipMalloc(&h->diag, (size_t)cap * MCF_DIAG_LEN * sizeof(double)));
   h->diag_cap = cap;
   h->hc.diag = h->diag;
-  return sync_descs(h);
+  // kernels reach the ring through the device descriptor, so the captured step graphs stay valid:
+  // only the descriptor is copied (no graph is rebuilt inside a long run's first call)
+  HIPCHK(h, hipMemcpy(h->dc, &h->hc, sizeof(CaseDev), hipMemcpyHostToDevice));
+  return 0;
 }
 
 // Clear every slot and set stripe 0 (the first NSLOT words) to `vals`
@@ -2949,7 +2966,6 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   HIPCHK(h, hipMemset(h->prm, 0, 4 * sizeof(double)));
   h->graphs = new StepGraphs();
   h->lean_graphs = new StepGraphs();
-  h->vlean_graphs = new StepGraphs();
   HIPCHK(h, hipMalloc(&h->dc, sizeof(CaseDev)));
   HIPCHK(h, hipMalloc(&h->ds, sizeof(h->hs)));
   CaseDev& c = h->hc;
@@ -2978,10 +2994,9 @@ void mcf_destroy(mcf_handle* h) {
   drop_graphs(h);
   delete h->graphs;
   delete h->lean_graphs;
-  delete h->vlean_graphs;
+  for (StepGraphs*& g : h->mixed) { delete g; g = nullptr; }
   h->graphs = nullptr;
   h->lean_graphs = nullptr;
-  h->vlean_graphs = nullptr;
   (void)hipFree(h->clock_step); (void)hipFree(h->clock_time); (void)hipFree(h->prm);
   for (auto& kv : h->owned) (void)hipFree(kv.second);
   for (void* p : h->bufs) (void)hipFree(p);
@@ -3325,15 +3340,22 @@ void spec_update(int& chunk, int& full, bool lean, bool miss, bool quiet) {
   chunk = SPEC_MIN;
   full = quiet ? 0 : SPEC_FULL;
 }
-// Any velocity-solve refinement in k diagnostics rows
-bool vel_refined(const double* rows, int k) {
-  for (int s = 0; s < k; ++s)
-    if (rows[(size_t)s * MCF_DIAG_LEN + MCF_D_H_REFINE] > 0) return true;
-  return false;
+// In k diagnostics rows: any velocity-solve refinement (vel), any pressure solve refined twice (p2)
+void refine_kinds(const double* rows, int k, bool& vel, bool& p2) {
+  for (int s = 0; s < k; ++s) {
+    const double* r = rows + (size_t)s * MCF_DIAG_LEN;
+    vel = vel || r[MCF_D_H_REFINE] > 0;
+    for (int q = 0; q < (int)r[MCF_D_P_SOLVES] && q < 3; ++q)
+      p2 = p2 || r[MCF_D_SOLVE0 + MCF_SOLVE_LEN * q + MCF_S_REFINE] >= 2;
+  }
 }
-// Lean mask of the next chunk: all solves lean; or, for a full chunk, the velocity solve lean
-// (its refinements are rarer than the pressure solves') unless the last chunk refined it
-int spec_mask(bool spec, int full, bool vfull) { return !spec ? 0 : full == 0 ? LEAN_ALL : vfull ? 0 : LEAN_VEL; }
+// Lean mask of the next chunk: all solves lean; or, for a full chunk, the velocity solve lean and
+// the pressure solves with one refinement pass (start-up transients refine the pressure solves
+// once per solve, the velocity solve rarely), each widened to the full form while the last chunk
+// needed it
+int spec_mask(bool spec, int full, bool vfull, bool p2) {
+  return !spec ? 0 : full == 0 ? LEAN_ALL : vfull ? 0 : LEAN_VEL | (p2 ? 0 : ONCE_PRES);
+}
 
 // No refinement (pressure or velocity) in the last SPEC_QUIET of k diagnostics rows
 bool quiet_tail(const double* rows, int k) {
@@ -3385,7 +3407,9 @@ int run_chunk(mcf_handle* h, int k, double dt, bool spec, int lean, double* d) {
   }
   if (spec) {
     spec_update(h->spec_chunk, h->spec_full, lean == LEAN_ALL, miss, quiet_tail(d, k));
-    h->spec_vfull = vel_refined(d, k);
+    bool vel = false, p2 = false;
+    refine_kinds(d, k, vel, p2);
+    h->spec_vfull = vel; h->spec_p2 = p2;
   }
   return 0;
 }
@@ -3401,7 +3425,7 @@ int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   const bool spec = h->spec > 0 && h->use_graphs && h->hs[0].dump_cap == 0 && h->pbackend == MCF_PB_DIRECT;
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
   for (int done = 0; done < n;) {
-    const int lean = spec_mask(spec, h->spec_full, h->spec_vfull);
+    const int lean = spec_mask(spec, h->spec_full, h->spec_vfull, h->spec_p2);
     const int k = !spec ? n - done : std::min(lean == LEAN_ALL ? h->spec_chunk : h->spec_full, n - done);
     if (int rc = run_chunk(h, k, dt, spec, lean, d.data() + (size_t)done * MCF_DIAG_LEN)) return rc;
     for (int s = 0; s < k; ++s)
@@ -3800,13 +3824,16 @@ struct mcf_batch {
   Launch ln;                    // launch shapes for the whole batch
   bool tuned = false;
   StepGraphs lean_graphs;       // steady steps without the refinement launches
-  StepGraphs vlean_graphs;      // steady steps whose velocity solve leaves its refinements out
+  StepGraphs mixed[8];          // steady steps with other lean masks (as the handle's)
   int spec = 1;                 // speculative lean steps (as mcf_set_speculation)
   int replays = 0;
   int spec_chunk = 8;           // steps per speculative chunk (as the handle's)
   int spec_full = 0;
-  bool spec_vfull = false;
-  void drop_graphs() { graphs.drop(); lean_graphs.drop(); vlean_graphs.drop(); }
+  bool spec_vfull = false, spec_p2 = false;
+  void drop_graphs() {
+    graphs.drop(); lean_graphs.drop();
+    for (StepGraphs& g : mixed) g.drop();
+  }
 };
 
 namespace {
@@ -3914,15 +3941,23 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
   if (!b || nsteps <= 0) return 1;
   mcf_handle* e = b->cases[0];
   const int n = (int)b->cases.size();
-  bool changed = !b->built;
+  bool rings = false;
   for (mcf_handle* h : b->cases) {
     (void)hipStreamSynchronize(h->stream);
     const int cap = h->diag_cap;
     if (int rc = ensure_diag(h, nsteps)) return rc;
-    changed = changed || cap != h->diag_cap;
+    rings = rings || cap != h->diag_cap;
     h->owned.erase("p_raw_valid");
   }
-  if (changed) if (int rc = batch_build(b)) return rc;
+  if (!b->built) {
+    if (int rc = batch_build(b)) return rc;
+  } else if (rings) {
+    // new diagnostics rings: the batch's case descriptors are copied again; its graphs reach them
+    // through the same table and stay valid
+    std::vector<CaseDev> hc(n);
+    for (int c = 0; c < n; ++c) hc[c] = b->cases[c]->hc;
+    HIPCHK(e, hipMemcpy(b->dc, hc.data(), n * sizeof(CaseDev), hipMemcpyHostToDevice));
+  }
   if (!b->tuned) {
     if (int rc = autotune(b->ctx, b->ln)) return rc;
     b->ctx.ln = b->ln;
@@ -3944,7 +3979,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       HIPCHK(e, hipMemcpy(h->prm, &b->dt[c], sizeof(double), hipMemcpyHostToDevice));
       h->prm_valid = false;
     }
-    StepGraphs& gs = lean == LEAN_ALL ? b->lean_graphs : lean == LEAN_VEL ? b->vlean_graphs : b->graphs;
+    StepGraphs& gs = lean == LEAN_ALL ? b->lean_graphs : lean == 0 ? b->graphs : b->mixed[lean];
     for (int s = 0; s < k; ++s) {
       mcf_handle* h0 = b->cases[0];
       const bool first = !h0->has_history;
@@ -3967,7 +4002,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
   int first_error = 0;
   std::vector<char> dead(n, 0);
   for (int done = 0; done < nsteps;) {
-    const int lean = spec_mask(spec, b->spec_full, b->spec_vfull);
+    const int lean = spec_mask(spec, b->spec_full, b->spec_vfull, b->spec_p2);
     const int k = !spec ? nsteps - done : std::min(lean == LEAN_ALL ? b->spec_chunk : b->spec_full, nsteps - done);
     for (int c = 0; c < n; ++c) {
       mcf_handle* h = b->cases[c];
@@ -3997,15 +4032,15 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       if (int rc = copy_rows()) return rc;
     }
     if (spec) {
-      bool quiet = true, vel = false;   // over the members that still count
+      bool quiet = true, vel = false, p2 = false;   // over the members that still count
       for (int c = 0; c < n; ++c) {
         if (dead[c]) continue;
         const double* rows = d.data() + per * c + (size_t)done * MCF_DIAG_LEN;
         quiet = quiet && quiet_tail(rows, k);
-        vel = vel || vel_refined(rows, k);
+        refine_kinds(rows, k, vel, p2);
       }
       spec_update(b->spec_chunk, b->spec_full, lean == LEAN_ALL, miss, quiet);
-      b->spec_vfull = vel;
+      b->spec_vfull = vel; b->spec_p2 = p2;
     }
     for (int c = 0; c < n; ++c) {
       mcf_handle* h = b->cases[c];
