@@ -218,7 +218,9 @@ int mcf_get_launch(mcf_handle* h, int* out5);
    the Woodbury refinement launches out (pressure_schur.py:116-124 refines at most twice, and only
    when the first residual misses max(1e-10, 32 eps max|b|)); the device flags any step whose first
    residual would have refined, and the whole chunk is then replayed from a checkpoint with the
-   refinements in place, so results equal mode 0 bit for bit.  mode 0: always launch them
+   refinements in place, so results equal mode 0 bit for bit.  Chunks that follow a refinement
+   launch the pressure solves' first refinement pass and leave the velocity solve's and the
+   second pressure pass out, flagged and replayed the same way.  mode 0: always launch them
    (device-predicated).  mode 2: as 1 with every lean step flagged (tests the replay).
    mcf_get_speculation: [mode, chunks replayed]. */
 int mcf_set_speculation(mcf_handle* h, int mode);
