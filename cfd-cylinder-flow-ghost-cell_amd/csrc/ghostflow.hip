@@ -887,19 +887,21 @@ __global__ __launch_bounds__(256) void k_gemm(const SolveDesc* __restrict__ desc
   }
 }
 
-// Split-K transform (launch variants 3 and 4): a block of SK_S waves computes a (16 MT) x 32 output
-// tile; wave w takes the w-th of SK_S consecutive K slices, holds the whole tile in 2 MT independent
+// Split-K transform (launch variants 3-6): a block of S waves computes a (16 MT) x 32 output tile;
+// wave w takes the w-th of S consecutive K slices, holds the whole tile in 2 MT independent
 // 16 x 16 accumulators and reads its MFMA operands straight from global memory (the panels are L2
 // resident).  A slice's loads are issued together in batches of SK_KB k-groups (one batch covers
-// the slice for K <= 4 SK_S SK_KB), so a wave waits for memory once and then runs its MFMAs back
-// to back -- no LDS staging and no barrier in the K loop.  The slices' partial tiles are added in
-// LDS in slice order.  Both tile heights slice K alike and give identical results; the sum order
-// differs from the staged tiles', so these variants agree with those to round-off.  The 16-row
-// tile doubles the number of blocks, which spreads the matrix-core work more evenly over the CUs
-// when the 32-row grid holds only a little more than one block per CU.
+// the slice for K <= 4 S SK_KB), so a wave waits for memory once and then runs its MFMAs back to
+// back -- no LDS staging and no barrier in the K loop.  The slices' partial tiles are added in LDS
+// in slice order.  Both tile heights slice K alike and give identical results for one S; the sum
+// order differs from the staged tiles' (and between S = 8 and 4), so the variants agree to
+// round-off.  The 16-row tile doubles the number of blocks, which spreads the matrix-core work more
+// evenly over the CUs when the 32-row grid holds only a little more than one block per CU; the
+// 4-wave blocks (S = 4: half the LDS, twice the K per wave) fit twice as many blocks per CU, for
+// grids larger than one round of the 8-wave blocks (the two velocity problems at 1320x246).
 constexpr int SK_S = 8, SK_KB = 8;
-template <int MT>
-__global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restrict__ descs, GemmArgs g) {
+template <int MT, int S>
+__global__ __launch_bounds__(64 * S) void k_gemm_sk(const SolveDesc* __restrict__ descs, GemmArgs g) {
   int z, cx, ry;
   xcd_tile(z, cx, ry);
   const SolveDesc& d = descs[z];
@@ -914,7 +916,7 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restri
   if (g.forward) { Bm = desc_in(d, g.in_id); ldb = d.ld; }
   else { Bm = G(d.F); ldb = d.xx; }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, l16 = lane & 15;
-  const int NG = (K + 3) >> 2, per = (NG + SK_S - 1) / SK_S;
+  const int NG = (K + 3) >> 2, per = (NG + S - 1) / S;
   const int g0 = min(NG, w * per), g1 = min(NG, g0 + per);
   // operand rows / columns past the matrix are read at the last one (in bounds, no predicate): they
   // feed only output rows / columns that are not stored
@@ -956,10 +958,10 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restri
     }
   }
   // partial tiles of the slices: red[slice][subtile][element][lane]; wave w then finishes
-  // elements MT (w / NS) .. MT (w / NS) + MT - 1 of subtile w % NS
-  constexpr int NS = 2 * MT;
-  static_assert(NS * 4 == SK_S * MT, "each wave finishes MT (subtile, element) pairs");
-  __shared__ double red[SK_S][NS][4][64];
+  // elements E (w / NS) .. E (w / NS) + E - 1 of subtile w % NS
+  constexpr int NS = 2 * MT, E = 4 * NS / S;
+  static_assert(S >= NS && E * S == 4 * NS, "the waves split the (subtile, element) pairs evenly");
+  __shared__ double red[S][NS][4][64];
   #pragma unroll
   for (int mi = 0; mi < MT; ++mi)
     #pragma unroll
@@ -971,12 +973,12 @@ __global__ __launch_bounds__(64 * SK_S) void k_gemm_sk(const SolveDesc* __restri
   const int gc = col0 + 16 * ni + l16;
   if (gc >= N) return;
   #pragma unroll
-  for (int e = 0; e < MT; ++e) {
-    const int r = MT * (w / NS) + e;
+  for (int e = 0; e < E; ++e) {
+    const int r = E * (w / NS) + e;
     const int gr = row0 + 16 * mi + kq + 4 * r;
     double v = red[0][st][r][lane];
     #pragma unroll
-    for (int q = 1; q < SK_S; ++q) v += red[q][st][r][lane];
+    for (int q = 1; q < S; ++q) v += red[q][st][r][lane];
     if (gr < M) gemm_put(d, g, gr, gc, v);
   }
 }
@@ -2166,8 +2168,8 @@ __global__ __launch_bounds__(KR_CT) void k_kry_ctl(KryArgs a, int phase) {
 
 }  // namespace
 
-constexpr int N_GEMM_V = 5, N_TRI_V = 3;
-constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64, 32, 16}, GEMM_TN[N_GEMM_V] = {64, 32, 64, 32, 32};
+constexpr int N_GEMM_V = 7, N_TRI_V = 3;
+constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64, 32, 16, 32, 16}, GEMM_TN[N_GEMM_V] = {64, 32, 64, 32, 32, 32, 32};
 constexpr int TRI_TH[N_TRI_V] = {128, 256, 512};
 struct Launch { int gemm_p = 0, gemm_h = 0, tri_p = 1, tri_h = 1; };
 
@@ -2179,8 +2181,10 @@ void launch_gemm(int v, int yy, int xx, int count, hipStream_t st, const SolveDe
   switch (v) {
     case 1: hipLaunchKernelGGL((k_gemm<32, 32, 64>), gg, dim3(256), 0, st, D, g); break;
     case 2: hipLaunchKernelGGL((k_gemm<64, 64, 32>), gg, dim3(256), 0, st, D, g); break;
-    case 3: hipLaunchKernelGGL(k_gemm_sk<2>, gg, dim3(64 * SK_S), 0, st, D, g); break;
-    case 4: hipLaunchKernelGGL(k_gemm_sk<1>, gg, dim3(64 * SK_S), 0, st, D, g); break;
+    case 3: hipLaunchKernelGGL((k_gemm_sk<2, SK_S>), gg, dim3(64 * SK_S), 0, st, D, g); break;
+    case 4: hipLaunchKernelGGL((k_gemm_sk<1, SK_S>), gg, dim3(64 * SK_S), 0, st, D, g); break;
+    case 5: hipLaunchKernelGGL((k_gemm_sk<2, SK_S / 2>), gg, dim3(32 * SK_S), 0, st, D, g); break;
+    case 6: hipLaunchKernelGGL((k_gemm_sk<1, SK_S / 2>), gg, dim3(32 * SK_S), 0, st, D, g); break;
     default: hipLaunchKernelGGL((k_gemm<32, 64, 64>), gg, dim3(256), 0, st, D, g); break;
   }
 }

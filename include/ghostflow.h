@@ -204,11 +204,12 @@ int mcf_net_loss_grad(mcf_net* n, int nsamp, int L, const float* x, const float*
 int mcf_net_profile(mcf_net* n, int nsamp, int L, int reps, float* ms_out);
 
 /* Launch shapes of the rectangle inverse (no reference counterpart): the FP64 transform tile
-   (0: 32x64, 1: 32x32, 2: 64x64 staged in LDS; 3: 32x32, 4: 16x32 split over K) and the tridiagonal
+   (0: 32x64, 1: 32x32, 2: 64x64 staged in LDS; 3: 32x32, 4: 16x32 split over K in 8-wave blocks;
+   5: 32x32, 6: 16x32 split over K in 4-wave blocks) and the tridiagonal
    sweep width (0: 128, 1: 256, 2: 512 threads per mode).  By default the first mcf_run / mcf_step of a handle times every shape on the
    handle's own problems and keeps the fastest per problem class (pressure, velocity); the staged
-   transform shapes give identical results, and so do the two split-K shapes; the two families and
-   the sweep shapes differ in round-off.  mcf_set_launch(h, g, t) pins
+   transform shapes give identical results, and so do the two split-K shapes of one block size; the
+   families (staged, 8-wave and 4-wave split-K) and the sweep shapes differ in round-off.  mcf_set_launch(h, g, t) pins
    both classes to (g, t); (-1, -1) returns to autotuning.  mcf_get_launch: [transform pressure,
    transform velocity, sweep pressure, sweep velocity, chosen (1) or default (0)]. */
 int mcf_set_launch(mcf_handle* h, int gemm, int tri);

@@ -148,10 +148,11 @@ def test_graph_replays_match_eager(pkg, monkeypatch):
 
 
 def test_launch_shapes_agree(pkg):
-    """Every staged transform tile gives bit-identical steps, and so do the two split-K tile heights;
-    the split-K transforms and every sweep width agree with the staged tiles to round-off."""
+    """Every staged transform tile gives bit-identical steps, and so do the two split-K tile heights
+    of one block size (8 or 4 waves); the split-K transforms and every sweep width agree with the
+    staged tiles to round-off."""
     runs = {}
-    for g, t in [(0, 1), (1, 1), (2, 1), (0, 0), (0, 2), (3, 1), (4, 1)]:
+    for g, t in [(0, 1), (1, 1), (2, 1), (0, 0), (0, 2), (3, 1), (4, 1), (5, 1), (6, 1)]:
         f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
         f.gg.device.set_launch(g, t)
         f.solve(n_steps=6, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
@@ -161,20 +162,22 @@ def test_launch_shapes_agree(pkg):
     for key in [(1, 1), (2, 1)]:
         for a, b in zip(runs[key], ref):
             assert np.array_equal(a, b), key
-    for key in [(0, 0), (0, 2), (3, 1)]:
+    for key in [(0, 0), (0, 2), (3, 1), (5, 1)]:
         for a, b in zip(runs[key], ref):
             assert np.max(abs(a - b)) <= 1e-12 * np.max(abs(b)), key
     for a, b in zip(runs[(4, 1)], runs[(3, 1)]):
+        assert np.array_equal(a, b)
+    for a, b in zip(runs[(6, 1)], runs[(5, 1)]):
         assert np.array_equal(a, b)
     f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
     f.solve(n_steps=2, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
     ln = f.gg.device.get_launch()
     print('autotuned launch shapes (440x82):', ln)
-    assert ln['chosen'] and all(0 <= ln[k] < 5 for k in ('gemm_pressure', 'gemm_velocity'))
+    assert ln['chosen'] and all(0 <= ln[k] < 7 for k in ('gemm_pressure', 'gemm_velocity'))
     assert all(0 <= ln[k] < 3 for k in ('tri_pressure', 'tri_velocity'))
 
 
-@pytest.mark.parametrize('tile', [3, 4], ids=['32-row', '16-row'])
+@pytest.mark.parametrize('tile', [3, 4, 5, 6], ids=['32-row', '16-row', '32-row-4-waves', '16-row-4-waves'])
 @pytest.mark.parametrize('kind', ['pressure', 'u', 'v'])
 def test_split_k_transform_inverse(pair88, kind, tile):
     """The split-K transforms inside the rectangle inverse against the oracle's DCT/DST inverse."""
