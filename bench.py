@@ -28,7 +28,7 @@ FP64_PEAK = 78.6e12        # FLOP/s
 STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB2/CN step
 # HBM bytes per forward launch of the pressure transform by tile variant (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 from rocprofv3 --pmc, scaled by a 256 MiB copy; profiles/k_gemm_1320x246_traffic.md)
-GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6, 4: 9.657e6}
+GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6, 4: 9.657e6, 5: 9.754e6, 6: 9.732e6}
 
 
 def cpu_baseline(seconds=20.0, gmres_seconds=0.0):
