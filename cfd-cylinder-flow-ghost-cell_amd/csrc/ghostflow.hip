@@ -374,14 +374,13 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
   double mu = 0.0, mv = 0.0;
   // Each face loads all of its operands in one batch, at indices clamped into the arrays, and forms
   // its value without branches around the loads (inactive faces compute a value that is then
-  // discarded): one memory round trip per face after the lookups, where per-term loads under the
-  // boundary tests took one round trip per test.  The arithmetic is the stencil's, term for term.
+  // discarded); the override and solid lookups are issued after the batch, so the batch is in
+  // flight while they resolve.  Per-term loads under the boundary tests took one round trip per
+  // test.  The arithmetic is the stencil's, term for term.
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < c.velsize; idx += gridDim.x * blockDim.x) {
     double res = 0.0, nw = 0.0;
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
-      const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
-      const bool act = u_act(c, j, i);
       const int iw = max(i - 1, 0), ie = min(i + 1, nx), ic = min(i, nx - 1);
       const int js = max(j - 1, 0), jn = min(j + 1, ny - 1);
       // u at (j, i-1 / i / i+1) and (j-1 / j+1, i); v at rows j and j+1 beside the face (v has ny+1 rows)
@@ -394,6 +393,8 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
       const double q_s = Q[js * nx1 + i], q_n = Q[jn * nx1 + i];
       const double p_e = Pp[j * nx + ic], p_w = Pp[j * nx + iw];
       const double pv = a.mode == 0 ? prev[idx] : 0.0;
+      const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
+      const bool act = u_act(c, j, i);
       // u at column 0 is the inlet profile
       const double a_w = (i == 1) ? inj : a_w0, q_w = (i == 1) ? inj : q_w0;
       auto fxx = [&](double l, double h) { double uf = 0.5 * (l + h); return -(uf * uf * dy); };
@@ -437,8 +438,6 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
     } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
-      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
-      const bool act = v_act(c, j, i);
       // rows clamped for the wall faces j = 0 and j = ny (inactive): v has ny+1 rows, u and p ny
       const int js = max(j - 1, 0), jn = min(j + 1, ny), ju = min(j, ny - 1);
       const int iw = max(i - 1, 0), ie = min(i + 1, nx - 1);
@@ -451,6 +450,8 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
       const double q_e = Qv[j * nx + ie], q_w = Qv[j * nx + iw];
       const double p_c = Pp[ju * nx + i], p_s = Pp[js * nx + i];
       const double pv = a.mode == 0 ? prev[idx] : 0.0;
+      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
+      const bool act = v_act(c, j, i);
       auto fyy = [&](double lo, double hi) { double vf = 0.5 * (lo + hi); return -(vf * vf * dx); };
       double fe;
       if (i + 1 == nx) { double uo = 0.5 * (ue_s + ue_n); fe = -(uo * v_c * dy); }
@@ -528,10 +529,10 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
       const double bv = B[idx];
       if (idx < c.usize) {
         const int j = idx / nx1, i = idx - j * nx1;
-        const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);   // looked up together with the solid test
-        const bool act = u_act(c, j, i);
         const double x = X[j * nx1 + i], xw = X[j * nx1 + max(i - 1, 0)], xe = X[j * nx1 + min(i + 1, nx)];
         const double xn = X[min(j + 1, ny - 1) * nx1 + i], xs = X[max(j - 1, 0) * nx1 + i];
+        const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);   // looked up together with the solid test
+        const bool act = u_act(c, j, i);
         double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * x
                      + ((i == nx) ? 2.0 : 1.0) * idx2 * xw;
         if (i < nx) lap += idx2 * xe;
@@ -547,11 +548,11 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
       } else {
         const int t = idx - c.usize;
         const int j = t / nx, i = t - j * nx;
-        const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
-        const bool act = v_act(c, j, i);
         // rows clamped for the wall faces j = 0 and j = ny (inactive)
         const double x = Xv[j * nx + i], xn = Xv[min(j + 1, ny) * nx + i], xs = Xv[max(j - 1, 0) * nx + i];
         const double xe = Xv[j * nx + min(i + 1, nx - 1)], xw = Xv[j * nx + max(i - 1, 0)];
+        const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
+        const bool act = v_act(c, j, i);
         double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * x
                      + idy2 * (xn + xs);
         if (i < nx - 1) lap += idx2 * xe;
@@ -601,11 +602,11 @@ DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double d = 0.0;
-    const int r = ov_row(c.ov[MCF_OP_DIV], j, i);   // looked up together with the solid test
-    const bool act = p_act(c, j, i);
     // the cell's four faces (and the inlet value) load in one batch ahead of the activity test
     const double inj = inl[j], xl = X[j * nx1 + i], xr = X[j * nx1 + i + 1];
     const double vt = Xv[(j + 1) * nx + i], vb = Xv[j * nx + i];
+    const int r = ov_row(c.ov[MCF_OP_DIV], j, i);   // looked up together with the solid test
+    const bool act = p_act(c, j, i);
     const double ul = (i == 0) ? inj : xl;
     double dc = (xr - ul) / c.dx + (vt - vb) / c.dy;
     if (act && r >= 0) {
@@ -646,18 +647,18 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
     const double xv = X[idx], dv = Dl[idx];
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
+      const double pe = Pp[j * nx + min(i, nx - 1)], pw = Pp[j * nx + max(i - 1, 0)];
       const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
       act = u_act(c, j, i);
-      const double pe = Pp[j * nx + min(i, nx - 1)], pw = Pp[j * nx + max(i - 1, 0)];
       g = ((i < nx) ? (pe - pw) / c.dx : -2.0 * pw / c.dx) / c.rho;
       if (act && r >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp);
     } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
-      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
-      act = v_act(c, j, i);
       // rows clamped for the wall faces j = 0 and j = ny (inactive)
       const double pc = Pp[min(j, c.ny - 1) * nx + i], ps = Pp[max(j - 1, 0) * nx + i];
+      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
+      act = v_act(c, j, i);
       g = (pc - ps) / c.dy / c.rho;
       if (act && r >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp);
     }
