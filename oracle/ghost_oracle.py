@@ -28,6 +28,14 @@ Pins (each test lives in tests/test_oracle_*.py):
   * divergence reduction     -- the reference C++ ``calculate_mac_divergence_cpp`` built
                                 from solver/src/bindings.cpp into oracle/_ref
   * checkpoint               -- restart parity (tests/test_numba_ghost_cell.py:87-106)
+  * index tables and values  -- golden vectors written by the reference's own functions
+                                (tests/golden/gen_reference_tables.py -> reference_tables.npz) on the
+                                88x16, 40x30, 440x82, 1320x246 and 800x820 grids: active and ghost
+                                node sets, probe cells, stencil growth (biquadratic), weights, the
+                                kNN-with-ties surface stencils, Schur ranks, rectangle / Woodbury /
+                                Helmholtz values (tests/test_reference_golden.py)
+  * knn_with_ties            -- scipy cKDTree query + query_ball_point, the reference's calls
+  * advection                -- closed forms for linear u, v: interior faces, outlet row, wall row
 """
 import json
 import os

@@ -209,6 +209,52 @@ def test_advection_of_uniform_flow_vanishes_in_channel_interior():
     assert np.max(abs(adv[:len(g.ui)][far])) < 1e-12
 
 
+def test_advection_of_linear_fields_closed_form():
+    """For u = a + b x + c y and v = d + e x + f y the centred flux form (numba_kernels.py:17-142) is
+    exact: interior u faces give -d(u^2)/dx - d(uv)/dy = -(2 b u + c v + f u), interior v faces
+    -d(uv)/dx - d(v^2)/dy = -(b v + e u + 2 f v).  The outlet u row (numba_kernels.py:65-74, doubled
+    at optimized_ghost_cell.py:197) replaces the missing cell by the outlet face itself, so
+    (u_c^2 - u_nx^2)/dx, doubled, is -d(u^2)/dx a quarter cell upstream, and its corner fluxes carry
+    the last v column: -2 b u(x_nx - dx/4) - f u(x_nx) - c v(x_nx - dx/2).  The wall rows carry no
+    advective flux through the wall (numba_kernels.py:61-63)."""
+    s = channel()
+    g = s.gg
+    a, b, c, d, e, f = 0.7, -0.3, 0.45, -0.2, 0.25, 0.6
+    U = lambda x, y: a + b * x + c * y
+    V = lambda x, y: d + e * x + f * y
+    gc = s.gc
+    xu, yu = np.meshgrid(gc.xu, gc.yu)
+    xv, yv = np.meshgrid(gc.xv, gc.yv)
+    adv_u, adv_v = g.unpack(g.advection(g.pack(U(xu, yu), V(xv, yv))))
+    adv_u, adv_v = adv_u.copy(), adv_v.copy()
+    h, dx, dy, nx, ny = max(s.dx, s.dy), s.dx, s.dy, s.nx, s.ny
+    # faces whose stencils reach neither the inlet column nor a ghost node near the cylinder
+    far_u = (np.hypot(xu - s.xc, yu - s.yc) > s.D / 2 + 4 * h) & (xu > 1.5 * dx)
+    far_v = (np.hypot(xv - s.xc, yv - s.yc) > s.D / 2 + 4 * h) & (xv > 1.5 * dx)
+    uu, vu = U(xu, yu), V(xu, yu)
+    interior_u = far_u & (xu < nx * dx - dx / 2) & (yu > dy) & (yu < (ny - 1) * dy)
+    exact_u = -(2 * b * uu + c * vu + f * uu)
+    assert interior_u.sum() > 100
+    assert np.max(abs(adv_u - exact_u)[interior_u]) < 1e-12
+    outlet = far_u & (xu > nx * dx - dx / 2) & (yu > dy) & (yu < (ny - 1) * dy)
+    xo = nx * dx
+    exact_out = -2 * b * U(xo - dx / 4, yu) - f * U(xo, yu) - c * V(xo - dx / 2, yu)
+    assert outlet.sum() == ny - 2
+    assert np.max(abs(adv_u - exact_out)[outlet]) < 1e-12
+    # bottom wall row: the wall flux is zero, the first interior corner carries u v
+    bottom = far_u & (yu < dy) & (xu < nx * dx - dx / 2)
+    exact_bottom = -2 * b * uu - U(xu, dy) * V(xu, dy) / dy
+    assert np.max(abs(adv_u - exact_bottom)[bottom]) < 1e-11
+    uv, vv = U(xv, yv), V(xv, yv)
+    # v on the wall rows is the no-slip zero, so the faces next to a wall are left out
+    interior_v = far_v & (xv < (nx - 1) * dx) & (yv > 1.5 * dy) & (yv < (ny - 1.5) * dy)
+    exact_v = -(b * vv + e * uv + 2 * f * vv)
+    assert interior_v.sum() > 100
+    assert np.max(abs(adv_v - exact_v)[interior_v]) < 1e-12
+    # wall rows of v are not unknowns: the extension keeps them at zero
+    assert np.all(adv_v[[0, -1]] == 0)
+
+
 # ----------------------------------------------------------------------------- stepper
 def test_noncommuting_pressure_recovery_order():
     """Closed-form constrained ODE (tests/test_cut_cell_time_restart.py:65-86)."""
