@@ -62,6 +62,7 @@ _SIGNATURES = {
     'mcf_extend': (ctypes.c_int, [_H, _D, _D]),
     'mcf_forces': (ctypes.c_int, [_H, _D, _D, ctypes.c_double, _D]),
     'mcf_rect_inverse': (ctypes.c_int, [_H, ctypes.c_int, _D, _D]),
+    'mcf_rect_inverse_noalloc': (ctypes.c_int, [_H, ctypes.c_int, _D, _D]),
     'mcf_schur_solve': (ctypes.c_int, [_H, ctypes.c_int, _D, _D, _D]),
     'mcf_set_pressure_backend': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double]),

@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <chrono>
 #include <initializer_list>
+#include <mutex>
 #include "ghostflow.h"
 
 #define DEV __device__ __forceinline__
@@ -2248,6 +2249,7 @@ struct mcf_handle {
   int parity = 0;      // state in S[parity], history in H[parity]
   double* tmp_rect = nullptr;   // scratch for the rectangle-inverse entry point
   size_t tmp_rect_n = 0;
+  double* na_scratch = nullptr; // this device's copy of `scratch` (allocation-free rectangle inverse)
   double graph_dt = -1.0;
   bool use_graphs = true;
   int* clock_step = nullptr;
@@ -2879,11 +2881,43 @@ void set_prm_once(mcf_handle* h, double dt) {
 // ======================================================================================
 // C ABI
 // ======================================================================================
+// Scratch of mcf_rect_inverse_noalloc: its solve descriptor, its one-entry buffer table and the
+// staged rectangle, carved out of one file-scope device array.  Each GPU holds its own copy until
+// the process ends; the host reaches it only through the address hipGetSymbolAddress returns on
+// that device (the array's name in host code is a host-side stand-in, not a device address).
+constexpr size_t NA_DESC = 0;                      // SolveDesc, from word 0
+constexpr size_t NA_TABLE = 64;                    // the buffer table's one pointer
+constexpr size_t NA_FIELD = 256;                   // the staged rectangle, 2 KiB in
+constexpr size_t NA_MAX_VALUES = (size_t)3 << 18;  // 786,432 values (6 MiB): up to an 800 x 820 rectangle
+static_assert(sizeof(SolveDesc) <= (NA_TABLE - NA_DESC) * sizeof(double), "descriptor region too small");
+static __device__ __align__(256) double scratch[NA_FIELD + NA_MAX_VALUES];
+
 namespace {
 
 struct HostTables {
   std::vector<int> lo[3], hi[3];
 };
+
+std::mutex na_mutex;
+std::map<int, double*> na_address;   // device ordinal -> that device's `scratch`
+
+// The current device's address of `scratch`, looked up once per device.
+hipError_t na_scratch_address(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(na_mutex);
+  auto it = na_address.find(dev);
+  if (it == na_address.end()) {
+    double* p = nullptr;
+    e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    it = na_address.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
+
 std::map<const mcf_handle*, HostTables> g_host;
 
 int field_rows(const mcf_handle* h, int field) { return field == MCF_FIELD_V ? h->ny + 1 : h->ny; }
@@ -2943,6 +2977,30 @@ int status_error(mcf_handle* h, const double* d, int step) {
   return fail(h, 2, buf);
 }
 
+// The rectangle inverse of kind on host rhs -> host out, staged in `field` (yy*xx values) with its
+// one-entry buffer table `tb` and descriptor `td` in device memory the caller provides.  Copies and
+// launches are ordered on the handle's stream, and the call returns after it drains.
+int rect_inverse_on(mcf_handle* h, int kind, const double* rhs, double* out, double* field, double** tb,
+                    SolveDesc* td) {
+  SolveDesc t = h->hs[kind];
+  const size_t n = (size_t)t.yy * t.xx;
+  t.bufs = tb; t.off = 0; t.ld = t.xx; t.lo = nullptr; t.hi = nullptr;
+  HIPCHK(h, hipMemcpyAsync(tb, &field, sizeof(double*), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(td, &t, sizeof(SolveDesc), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(field, rhs, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  const int gv = kind == MCF_PRESSURE ? h->ln.gemm_p : h->ln.gemm_h, tv = kind == MCF_PRESSURE ? h->ln.tri_p : h->ln.tri_h;
+  GemmArgs g{1, 0, 0, 0, -1, 0};
+  launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, g);
+  PcrArgs p0{0, -1, 0};
+  launch_tri(tv, t.yy, 1, h->stream, td, p0);
+  GemmArgs gi{0, 0, 0, 0, -1, 0};
+  launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, gi);
+  int rc = launch_check(h, "rect inverse");
+  HIPCHK(h, hipMemcpyAsync(out, field, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2958,6 +3016,7 @@ int mcf_create(int nx, int ny, double dx, double dy, double nu, double rho, mcf_
   *out = h;
   g_host[h] = HostTables{};
   HIPCHK(h, hipStreamCreate(&h->stream));
+  HIPCHK(h, na_scratch_address(&h->na_scratch));
   for (int b = 0; b < NBUF; ++b) {
     if (b == B_KV0) { h->hc.buf[b] = nullptr; continue; }   // set with the GMRES workspace
     void* p = nullptr;
@@ -3525,22 +3584,21 @@ int mcf_rect_inverse(mcf_handle* h, int kind, const double* rhs, double* out) {
   SolveDesc* td = nullptr;
   HIPCHK(h, hipMalloc(&tb, sizeof(double*)));
   HIPCHK(h, hipMalloc(&td, sizeof(SolveDesc)));
-  HIPCHK(h, hipMemcpy(tb, &h->tmp_rect, sizeof(double*), hipMemcpyHostToDevice));
-  t.bufs = tb; t.off = 0; t.ld = t.xx; t.lo = nullptr; t.hi = nullptr;
-  HIPCHK(h, hipMemcpy(td, &t, sizeof(SolveDesc), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(h->tmp_rect, rhs, n * sizeof(double), hipMemcpyHostToDevice));
-  const int gv = kind == MCF_PRESSURE ? h->ln.gemm_p : h->ln.gemm_h, tv = kind == MCF_PRESSURE ? h->ln.tri_p : h->ln.tri_h;
-  GemmArgs g{1, 0, 0, 0, -1, 0};
-  launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, g);
-  PcrArgs p0{0, -1, 0};
-  launch_tri(tv, t.yy, 1, h->stream, td, p0);
-  GemmArgs gi{0, 0, 0, 0, -1, 0};
-  launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, gi);
-  int rc = launch_check(h, "rect inverse");
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(out, h->tmp_rect, n * sizeof(double), hipMemcpyDeviceToHost));
+  int rc = rect_inverse_on(h, kind, rhs, out, h->tmp_rect, tb, td);
   (void)hipFree(tb); (void)hipFree(td);
   return rc;
+}
+
+int mcf_rect_inverse_noalloc(mcf_handle* h, int kind, const double* rhs, double* out) {
+  if (!h || kind < 0 || kind > 2 || !rhs || !out || !h->has_rect[kind]) return fail(h, 1, "invalid rectangle inverse arguments");
+  if (!h->na_scratch) return fail(h, 4, "allocation-free scratch is not set up on this device");
+  const size_t n = (size_t)h->hs[kind].yy * h->hs[kind].xx;
+  if (n > NA_MAX_VALUES)
+    return fail(h, 1, "rectangle of " + std::to_string(n) + " values exceeds the allocation-free limit of " +
+                          std::to_string(NA_MAX_VALUES) + " values (6 MiB); use mcf_rect_inverse");
+  double* base = h->na_scratch;
+  return rect_inverse_on(h, kind, rhs, out, base + NA_FIELD, reinterpret_cast<double**>(base + NA_TABLE),
+                         reinterpret_cast<SolveDesc*>(base + NA_DESC));
 }
 
 int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, double* info4) {

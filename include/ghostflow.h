@@ -122,6 +122,16 @@ int mcf_extend(mcf_handle* h, const double* q, double* out);
 int mcf_forces(mcf_handle* h, const double* vel, const double* p, double mu, double* out6);
 /* Separable rectangle inverse of kind on a (yy, xx) right-hand side. */
 int mcf_rect_inverse(mcf_handle* h, int kind, const double* rhs, double* out);
+/* Allocation-free mcf_rect_inverse.  No counterpart in the reference: it is for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments and the same results, bit for bit; it calls no hipMalloc or hipFree.
+   The staged rectangle, its buffer table and its descriptor live in one fixed file-scope device
+   array of 6 MiB per GPU (its address is looked up once per device by mcf_create), so a rectangle
+   may hold at most 786,432 values (yy * xx); larger ones return status 1.  Every call on one GPU
+   shares that array and it keeps the last call's contents: calls that overlap on one GPU (from
+   several host threads, or on handles whose streams run concurrently) race, and the caller must
+   serialise them. */
+int mcf_rect_inverse_noalloc(mcf_handle* h, int kind, const double* rhs, double* out);
 /* Boundary-row corrected solve of the embedded operator (values on the active nodes given as
    a q-array in the kind's full array layout); info4 = residual_linf, refinements, roundoff, target. */
 int mcf_schur_solve(mcf_handle* h, int kind, const double* rhs, double* out, double* info4);
