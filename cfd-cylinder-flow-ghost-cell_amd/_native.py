@@ -24,7 +24,7 @@ DIAG_NAMES = ['time', 'dt', 'cd', 'cl', 'cd_pressure', 'cd_viscous', 'cl_pressur
               'pressure_residual_linf', 'pressure_residual_first_linf', 'pressure_residual_relative_l2',
               'unused_rel1', 'pressure_refinement_iterations', 'pressure_residual_target_linf', 'unused_t1',
               'pressure_roundoff_bound_linf', 'unused_r1', 'pressure_backward_error_linf', 'unused_b1',
-              'helmholtz_refinement_iterations', 'status', 'rhs_max', 'pressure_solve_count']
+              'helmholtz_refinement_iterations', 'status', 'rhs_max', 'pressure_solve_count', 'probe0', 'probe1']
 DIAG_SOLVE0 = 32                     # per-solve records, MCF_SOLVE_LEN doubles each, in call order
 SOLVE_NAMES = ['iterations', 'residual_linf', 'residual_relative_l2', 'refinement_iterations',
                'residual_target_linf', 'roundoff_bound_linf', 'backward_error_linf', 'status', 'seconds']
@@ -46,6 +46,7 @@ _SIGNATURES = {
     'mcf_last_error': (ctypes.c_char_p, [_H]),
     'mcf_device_name': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     'mcf_set_inlet': (ctypes.c_int, [_H, _D]),
+    'mcf_set_inlet_perturbation': (ctypes.c_int, [_H, _D, _D, ctypes.c_double]),
     'mcf_set_solid_intervals': (ctypes.c_int, [_H, ctypes.c_int, _I, _I]),
     'mcf_set_ghost': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, _I, _I, _D]),
     'mcf_set_override': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, _I, _I, _D]),
@@ -54,6 +55,7 @@ _SIGNATURES = {
     'mcf_set_schur': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I, _D]),
     'mcf_set_surface': (ctypes.c_int, [_H, ctypes.c_int, _I, _I, _D, _I, _I, _D, _I, _I, _D, _D, _D,
                                        ctypes.c_double, ctypes.c_double]),
+    'mcf_set_probes': (ctypes.c_int, [_H, ctypes.c_int, _I, _I, _D]),
     'mcf_set_state': (ctypes.c_int, [_H, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double]),
     'mcf_get_state': (ctypes.c_int, [_H, _D, _D, _D, _D, _D, _D, _I]),
     'mcf_step': (ctypes.c_int, [_H, ctypes.c_double, _D]),

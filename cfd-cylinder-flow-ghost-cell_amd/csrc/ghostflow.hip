@@ -98,7 +98,10 @@ struct GhostTab {
 struct CaseDev {
   int nx, ny, usize, vsize, psize, velsize;
   double dx, dy, nu, rho;
-  const double* inlet;
+  double* inlet;                // the inlet profile the step uses (u at x = 0)
+  const double* inlet_base;     // inflow perturbation: base profile, amp sin(2 pi y / Ly), duration T
+  const double* inlet_shape;    //   (pert_T <= 0: no perturbation; the inlet is set by mcf_set_inlet)
+  double pert_T;
   const int* lo[3];
   const int* hi[3];
   int sbox[3][4];   // bounding box (j0, j1, i0, i1) of the solid nodes of each field
@@ -111,7 +114,11 @@ struct CaseDev {
   const int *pp, *pc, *up, *uc, *vp, *vc;
   const double *pv, *uv, *vv, *nrm, *nw;
   double diameter, uref;
-  double* fpart; // per-wave partial force sums (4 per wave)
+  double* fpart; // per-wave partial force sums (4 per wave), then the probe values
+  // pressure probes: at most FORCE_PPW CSR rows over p, evaluated on the step's output pressure
+  int nprobe;
+  const int *qp, *qc;
+  const double* qv;
   double* diag;  // diag ring
   int* clock_step;     // row of the next diagnostics entry
   double* clock_time;  // simulated time of the last completed step
@@ -1333,6 +1340,23 @@ __global__ __launch_bounds__(256) void k_fill(const CaseDev* __restrict__ cases,
   fill_body(cases[blockIdx.y], a, blockIdx.x, gridDim.x);
 }
 
+// The inlet of the coming step from its end time t (main.py:280-286): base (1 + shape (1 - t/T)) while
+// t <= T, the base profile afterwards, with shape = amp sin(2 pi y / Ly); products and sums rounded
+// one at a time, as the reference evaluates them.
+__global__ __launch_bounds__(256) void k_inlet(const CaseDev* __restrict__ cases) {
+  const CaseDev& c = cases[blockIdx.y];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= c.ny || !(c.pert_T > 0.0)) return;
+  const double t = *G(c.clock_time) + G(c.prm)[0];
+  const double base = G(c.inlet_base)[j];
+  double v = base;
+  if (t <= c.pert_T) {
+    const double decay = __dsub_rn(1.0, __ddiv_rn(t, c.pert_T));
+    v = __dmul_rn(base, __dadd_rn(1.0, __dmul_rn(G(c.inlet_shape)[j], decay)));
+  }
+  G(c.inlet)[j] = v;
+}
+
 // half_pressure = pressure + increment; output = half + factor * increment
 struct PressArgs { int p_id, inc_id, half_id, out_id; };
 DEV void press_body(const CaseDev& c, const PressArgs& a, int bx, int gx) {
@@ -1421,6 +1445,18 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* __restrict__ cases
   const int sub = threadIdx.x & 15;
   const int k = blockIdx.x * FORCE_PPW + (threadIdx.x >> 4);
   if (blockIdx.x * FORCE_PPW >= c.nsurf) return;
+  if (blockIdx.x == 0 && c.nprobe > 0) {
+    // the probes, one 16-lane group each like a surface point, stored after the force partials
+    const int g = threadIdx.x >> 4;
+    double pr = 0.0;
+    if (g < c.nprobe) {
+      const gptr<const int> qp = G(c.qp), qc = G(c.qc);
+      const gptr<const double> qv = G(c.qv);
+      for (int t = qp[g] + sub; t < qp[g + 1]; t += 16) pr += qv[t] * Pp[qc[t]];
+    }
+    for (int o = 8; o > 0; o >>= 1) pr += __shfl_xor(pr, o, 64);
+    if (g < c.nprobe && sub == 0) G(c.fpart)[4 * force_blocks(c.nsurf) + g] = pr;
+  }
   double ps = 0.0, du = 0.0, dv = 0.0;
   if (k < c.nsurf) {
     const gptr<const int> pp = G(c.pp), pc = G(c.pc), up = G(c.up), uc = G(c.uc), vp = G(c.vp), vc = G(c.vc);
@@ -1577,6 +1613,11 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
     d[MCF_D_CD_V] = f[4] / denom; d[MCF_D_CL_V] = f[5] / denom;
     d[MCF_D_MAXDIV] = maxdiv;
     d[MCF_D_CFL] = cfl;
+    if (c.nprobe > 0) {
+      const gptr<const double> pv = G(c.fpart) + 4 * force_blocks(c.nsurf);
+      d[MCF_D_PROBE0] = pv[0];
+      if (c.nprobe > 1) d[MCF_D_PROBE1] = pv[1];
+    }
     const double rhs_max = fmax(hu_val, hv_val);
     const double mres = nanmax(hu_fin, hv_fin);
     d[MCF_D_MOMENTUM_RES] = mres;
@@ -2359,7 +2400,8 @@ struct Ctx {
   const SolveDesc* dsp;  // n pressure problems
   const SolveDesc* dsh;  // 2n Helmholtz problems (u, v per case)
   int n;
-  int maxvel, maxp, maxfill, maxsurf;
+  int maxvel, maxp, maxfill, maxsurf, maxny;
+  bool pert;             // some case perturbs its inflow: each step sets its inlet first (k_inlet)
   int p_yy, p_xx, p_np, p_r, h_yy, h_xx, h_np, h_r;
 };
 
@@ -2368,6 +2410,8 @@ void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
   x.maxp = std::max(x.maxp, c.psize);
   x.maxfill = std::max(x.maxfill, c.gt[0].n + c.gt[1].n + c.ny);
   x.maxsurf = std::max(x.maxsurf, c.nsurf);
+  x.maxny = std::max(x.maxny, c.ny);
+  x.pert = x.pert || c.pert_T > 0.0;
   x.p_yy = std::max(x.p_yy, hs3[0].yy); x.p_xx = std::max(x.p_xx, hs3[0].xx);
   x.p_np = std::max(x.p_np, hs3[0].npts); x.p_r = std::max(x.p_r, hs3[0].r);
   for (int q = 1; q < 3; ++q) {
@@ -2711,6 +2755,7 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, boo
   const int Hprev = B_H0 + 1 - parity, Hnow = B_H0 + parity;
   const dim3 ga = gridAdv(x.maxvel, x.n);
   int nproj = 2;
+  if (x.pert) hipLaunchKernelGGL(k_inlet, dim3((x.maxny + 255) / 256, x.n), dim3(256), 0, x.stream, x.dc);
   if (first) {
     // projected midpoint startup (time_integration.py:33-47)
     AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1};
@@ -3339,9 +3384,47 @@ int mcf_set_surface(mcf_handle* h, int n, const int* pp, const int* pc, const do
   c.nrm = nr; c.nw = nw; c.diameter = diameter; c.uref = u_ref;
   if (n > FORCE_MAXP) return fail(h, 1, "too many surface points");
   double* fp;
-  if (int rc = upload<double>(h, "sfp", nullptr, (size_t)4 * force_blocks(n), &fp)) return rc;
+  if (int rc = upload<double>(h, "sfp", nullptr, (size_t)4 * force_blocks(n) + FORCE_PPW, &fp)) return rc;
   c.fpart = fp;
   h->has_surface = true;
+  return sync_descs(h);
+}
+
+int mcf_set_inlet_perturbation(mcf_handle* h, const double* base, const double* shape, double duration) {
+  if (!h || !std::isfinite(duration) || (duration > 0.0 && (!base || !shape)))
+    return fail(h, 1, "invalid inflow perturbation");
+  CaseDev& c = h->hc;
+  const bool was = c.pert_T > 0.0;
+  if (duration > 0.0) {
+    for (int j = 0; j < h->ny; ++j)
+      if (!std::isfinite(base[j]) || !std::isfinite(shape[j])) return fail(h, 1, "inflow perturbation must be finite");
+    double *b, *s;
+    if (int rc = upload(h, "ipb", base, h->ny, &b)) return rc;
+    if (int rc = upload(h, "ips", shape, h->ny, &s)) return rc;
+    c.inlet_base = b; c.inlet_shape = s; c.pert_T = duration;
+  } else {
+    c.pert_T = 0.0;
+  }
+  // the step graphs hold k_inlet only while a perturbation is set
+  if (was != (c.pert_T > 0.0)) drop_graphs(h);
+  return sync_descs(h);
+}
+
+int mcf_set_probes(mcf_handle* h, int n, const int* ptr, const int* col, const double* val) {
+  if (!h || n < 0 || n > 2 || (n > 0 && (!ptr || !col || !val))) return fail(h, 1, "invalid pressure probes");
+  if (!h->has_surface) return fail(h, 1, "set the surface quadrature before the pressure probes");
+  CaseDev& c = h->hc;
+  c.nprobe = 0;
+  if (n == 0) return sync_descs(h);
+  if (ptr[0] != 0) return fail(h, 1, "probe rows must start at 0");
+  for (int k = 0; k < n; ++k) if (ptr[k + 1] < ptr[k]) return fail(h, 1, "probe row pointers must not decrease");
+  for (int t = 0; t < ptr[n]; ++t) if (col[t] < 0 || col[t] >= h->psize) return fail(h, 1, "probe column out of range");
+  int *a, *b;
+  double* x;
+  if (int rc = upload(h, "qpp", ptr, n + 1, &a)) return rc;
+  if (int rc = upload(h, "qpc", col, ptr[n], &b)) return rc;
+  if (int rc = upload(h, "qpv", val, ptr[n], &x)) return rc;
+  c.qp = a; c.qc = b; c.qv = x; c.nprobe = n;
   return sync_descs(h);
 }
 

@@ -322,10 +322,38 @@ class GhostDevice:
 
     # ---------------------------------------------------------------- operations
     def run(self, n, dt):
+        """n steps at dt; returns the (n, DIAG_LEN) diagnostics rows.  When a step fails a gate the
+        library stops after the chunk that holds it: the exception carries ``diagnostics`` (the rows
+        computed, zeros after them) and ``failed_step`` (the first row whose status is set)."""
         self.set_dt(dt)
         diag = np.zeros((n, nat.DIAG_LEN))
-        self.h.call('mcf_run', int(n), float(dt), nat.dptr(diag))
+        try:
+            self.h.call('mcf_run', int(n), float(dt), nat.dptr(diag))
+        except (RuntimeError, FloatingPointError) as error:
+            status = diag[:, nat.DIAG_NAMES.index('status')]
+            bad = np.flatnonzero(status != 0)
+            error.diagnostics = diag
+            error.failed_step = int(bad[0]) if len(bad) else None
+            raise
         return diag
+
+    def set_probes(self, matrix):
+        """Pressure probes (at most two CSR rows over p; None removes them), evaluated by the device on
+        every step's output pressure and recorded in the diagnostics columns probe0 / probe1."""
+        if matrix is None:
+            self.h.call('mcf_set_probes', 0, None, None, None)
+            return
+        m = sp.csr_matrix(matrix)
+        m.sort_indices()
+        self.h.call('mcf_set_probes', m.shape[0], nat.i32(m.indptr), nat.i32(m.indices), nat.f64(m.data))
+
+    def set_inlet_perturbation(self, base, shape, duration):
+        """Steps that follow set their own inlet base (1 + shape (1 - t/duration)) from their end time t
+        while t <= duration (main.py:280-286); duration <= 0 removes the perturbation."""
+        if duration > 0:
+            self.h.call('mcf_set_inlet_perturbation', nat.f64(base), nat.f64(shape), float(duration))
+        else:
+            self.h.call('mcf_set_inlet_perturbation', None, None, 0.0)
 
     def project(self, q, dt):
         """q (packed) -> (projected q, phi (packed)); optimized_ghost_cell.py:218-229."""

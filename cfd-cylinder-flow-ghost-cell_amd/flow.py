@@ -291,16 +291,12 @@ class MAC_CylinderFlow:
         g.previous_advection, g.half_pressure, g.previous_dt = st['previous_advection'], st['half_pressure'], st['previous_dt']
         g.helmholtz_inverse_info = g.device.helmholtz_inverse_info
 
-    def _inlet_for(self, current_time, perturb_transient, perturb_duration, perturb_amp):
-        if perturb_transient and self.inflow == 'uniform' and current_time <= perturb_duration:
-            decay = 1.0 - (current_time / perturb_duration)
-            pert = perturb_amp * np.sin(2.0 * np.pi * self.y_centers / self.Ly) * decay
-            return self.u_inlet_base * (1.0 + pert), True
-        return self.u_inlet_base, False
+    def _perturbation_shape(self, perturb_amp):
+        """amp sin(2 pi y / Ly) at the inlet cell centres (main.py:283)."""
+        return perturb_amp * np.sin(2.0 * np.pi * self.y_centers / self.Ly)
 
-    def solve(self, n_steps=25000, dt=0.002, perturb_transient=True, perturb_duration=None, perturb_amp=0.15,
-              save_artifacts=True, verbose=None, step_callback=None):
-        """Step loop of main.py:245-501; returns (t, cd, cl).  No plot or npz artefacts are written."""
+    def _run_options(self, n_steps, dt, perturb_transient, perturb_duration, perturb_amp):
+        """Argument checks of main.py:245-256; returns the perturbation duration."""
         if isinstance(n_steps, bool) or not isinstance(n_steps, (int, np.integer)) or n_steps <= 0:
             raise ValueError('n_steps must be a positive integer')
         if not np.isfinite(dt) or dt <= 0:
@@ -310,43 +306,125 @@ class MAC_CylinderFlow:
         if not np.isfinite(perturb_duration) or perturb_duration <= 0 or not np.isfinite(perturb_amp) or abs(perturb_amp) >= 1:
             raise ValueError('Require perturb_duration > 0 and finite |perturb_amp| < 1')
         self.run_options = dict(perturb_transient=perturb_transient, perturb_duration=perturb_duration, perturb_amp=perturb_amp)
+        return perturb_duration
+
+    def advance(self, n_steps, dt, *, segment=None, perturb_transient=True, perturb_duration=None, perturb_amp=0.15):
+        """Run n_steps on the device and yield each segment's diagnostics rows ((k, DIAG_LEN) arrays).
+
+        One iteration of main.py:271-434 per step.  The device runs the steps in graph-replayed chunks
+        with no host round trip; ``u, v, p``, ``time`` and ``last_step_diagnostics`` are brought up to
+        date after every segment of at most ``segment`` steps (all steps at once by default).  For a
+        uniform inflow with ``perturb_transient`` the device sets each step's inlet from the step's end
+        time (main.py:280-286) and ``u_inlet`` holds the last step's inlet afterwards.  ``segment`` is a
+        step count or a list of segment lengths adding up to n_steps.  A failed gate
+        raises after the segment that holds the failing step: the exception carries ``diagnostics``
+        (the segment's rows) and ``failed_step`` (its index in them), ``last_step_diagnostics`` is the
+        failing step's, and the state is the one at the end of that segment (the device completes the
+        chunk it was replaying).
+        """
+        T = self._run_options(n_steps, dt, perturb_transient, perturb_duration, perturb_amp)
         rate = np.max(np.abs(self.u)) / self.dx + np.max(np.abs(self.v)) / self.dy
         if not np.isfinite(rate) or dt * rate > .5:
             raise RuntimeError(f'Explicit timestep unsafe at t={self.time:g}: dt*rate={dt * rate:g}; reduce dt')
-        self._push()
         dev = self.gg.device
-        dev.set_dt(dt)
-        rows = []
-        s = 0
+        backend = dev.pressure_info['backend']
         start = self.time
+        shape = self._perturbation_shape(perturb_amp)
+        # steps s whose end time start + (s+1) dt lies in the perturbation window come first
+        perturbed = 0
+        if perturb_transient and self.inflow == 'uniform':
+            perturbed = int(np.sum(start + (np.arange(n_steps) + 1) * dt <= T))
+
+        def inlet_after(steps):
+            """The inlet of the last of `steps` steps (main.py:280-286)."""
+            t = start + steps * dt
+            if steps and steps <= perturbed:
+                return self.u_inlet_base * (1.0 + shape * (1.0 - (t / T)))
+            return self.u_inlet_base
+
+        self._push()
+        dev.set_dt(dt)
+        if perturbed:
+            dev.set_inlet_perturbation(self.u_inlet_base, shape, T)
+        elif not np.array_equal(self.u_inlet, self.u_inlet_base):
+            self.u_inlet[:] = self.u_inlet_base
+            dev.set_inlet(self.u_inlet)
+        perturbing = perturbed > 0
+        if segment is None:
+            plan = [n_steps]
+        elif isinstance(segment, (int, np.integer)):
+            k = max(1, int(segment))
+            plan = [k] * (n_steps // k) + ([n_steps % k] if n_steps % k else [])
+        else:
+            plan = [int(k) for k in segment]
+            if sum(plan) != n_steps or min(plan) <= 0:
+                raise ValueError('segment lengths must be positive and add up to n_steps')
+        done = 0
         try:
-            while s < n_steps:
-                t_next = start + (s + 1) * dt
-                inlet, perturbed = self._inlet_for(t_next, perturb_transient, perturb_duration, perturb_amp)
-                if not np.array_equal(inlet, self.u_inlet):
-                    self.u_inlet[:] = inlet
-                    dev.set_inlet(self.u_inlet)
-                if perturbed or step_callback is not None:
-                    chunk = 1
-                else:
-                    chunk = n_steps - s
-                diag = dev.run(chunk, dt)
-                rows.append(diag)
-                s += chunk
-                if step_callback is not None:
-                    self._pull()
-                    self.time = float(diag[-1, 0])
-                    self.last_step_diagnostics = diagnostics_from_row(diag[-1], dev.pressure_info['backend'])
-                    step_callback(self, self.last_step_diagnostics.copy())
+            for length in plan:
+                pieces = []
+                while sum(len(r) for r in pieces) < length:
+                    if perturbing and done >= perturbed:
+                        # the window is over: the remaining steps keep the base profile (no k_inlet launch)
+                        dev.set_inlet_perturbation(None, None, 0.0)
+                        perturbing = False
+                        self.u_inlet[:] = self.u_inlet_base
+                        dev.set_inlet(self.u_inlet)
+                    k = length - sum(len(r) for r in pieces)
+                    if perturbing:
+                        k = min(k, perturbed - done)
+                    try:
+                        pieces.append(dev.run(k, dt))
+                    except (RuntimeError, FloatingPointError) as error:
+                        self._pull()
+                        rows = error.diagnostics
+                        ran = np.flatnonzero(rows[:, 0])
+                        if len(ran):
+                            self.time = float(rows[ran[-1], 0])
+                            self.u_inlet[:] = inlet_after(done + len(ran))
+                        failed = error.failed_step if error.failed_step is not None else (ran[-1] if len(ran) else 0)
+                        self.last_step_diagnostics = diagnostics_from_row(rows[failed], backend)
+                        # the exception reports the whole planned segment's rows
+                        error.diagnostics = np.concatenate(pieces + [rows])
+                        error.failed_step = failed + sum(len(r) for r in pieces)
+                        raise
+                    done += k
+                rows = pieces[0] if len(pieces) == 1 else np.concatenate(pieces)
+                self._pull()
+                self.time = float(rows[-1, 0])
+                self.u_inlet[:] = inlet_after(done)
+                self.last_step_diagnostics = diagnostics_from_row(rows[-1], backend)
+                yield rows
         finally:
-            self._pull()
-        diag = np.concatenate(rows) if rows else np.zeros((0, nat.DIAG_LEN))
-        self.time = float(diag[-1, 0]) if len(diag) else self.time
-        self.last_step_diagnostics = diagnostics_from_row(diag[-1], dev.pressure_info['backend'])
-        self.gg.last_diagnostics = self.last_step_diagnostics
-        self.gg.pressure_inverse_info['last_solve'] = self.last_step_diagnostics['pressure_solves'][-1]
+            if perturbing:
+                dev.set_inlet_perturbation(None, None, 0.0)
+                dev.set_inlet(self.u_inlet)
+            self.gg.last_diagnostics = self.last_step_diagnostics
+            if self.last_step_diagnostics.get('pressure_solves'):
+                self.gg.pressure_inverse_info['last_solve'] = self.last_step_diagnostics['pressure_solves'][-1]
+
+    def solve(self, n_steps=25000, dt=0.002, perturb_transient=True, perturb_duration=None, perturb_amp=0.15,
+              save_artifacts=True, verbose=None, step_callback=None):
+        """Step loop of main.py:245-501; returns (t, cd, cl).  No plot or npz artefacts are written.
+        With a ``step_callback`` the steps run one per segment so that the callback sees each step's
+        state, as in the reference; without one they run in one device segment."""
+        rows = []
+        for seg in self.advance(n_steps, dt, segment=1 if step_callback is not None else None,
+                                perturb_transient=perturb_transient, perturb_duration=perturb_duration,
+                                perturb_amp=perturb_amp):
+            rows.append(seg)
+            if step_callback is not None:
+                step_callback(self, self.last_step_diagnostics.copy())
+        diag = np.concatenate(rows)
         self.diagnostics_history = diag
         return diag[:, 0].copy(), diag[:, 2].copy(), diag[:, 3].copy()
+
+    def step(self, dt, *, perturb_transient=True, perturb_duration=None, perturb_amp=0.15):
+        """One step (one iteration of main.py:271-434); returns the step's diagnostics dict."""
+        for _ in self.advance(1, dt, perturb_transient=perturb_transient, perturb_duration=perturb_duration,
+                              perturb_amp=perturb_amp):
+            pass
+        return dict(self.last_step_diagnostics)
 
     # ------------------------------------------------------------------ checkpoint (main.py:503-585)
     def save_checkpoint(self, path, *, overwrite=False):

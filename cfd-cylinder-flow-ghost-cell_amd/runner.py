@@ -1,132 +1,213 @@
-"""Stage-JSON runner of one ghost-cell case with telemetry (validation/run_ghost_validation.py:19-111).
+"""Stage-JSON runner of one ghost-cell case on the GPU.
 
     python cfd_amd_loader.py run-case --config-path case.json --output out/ [--restart checkpoint.npz]
 
-A case file holds ``dt``, ``end_time``, ``sample_start``, ``kind`` ('steady', 'periodic' or
-'diagnostic'), ``solver`` (the ``MAC_CylinderFlow`` keywords; ghost_cell / ab2_cn only) and
-optionally ``profile_stride``.  The output directory (created, must not exist) receives
-``manifest.json``, ``history.jsonl`` (one row per step: the step's diagnostics plus ``step``,
-``delta_p`` = front minus back stagnation pressure and ``wall_seconds``), ``centerline.bin`` +
-``.json`` (wake samples), ``checkpoint.npz`` (every 45 s and at the end), ``summary.json``
-(status; on failure the error and the failed step's diagnostics) and ``mean_observables.json``.
+The case file and the output directory follow the reference's runner
+(validation/run_ghost_validation.py:19-102).  A case holds ``dt``, ``end_time``, ``sample_start``,
+``kind`` ('steady', 'periodic' or 'diagnostic'), ``solver`` (``MAC_CylinderFlow`` keywords; ghost_cell /
+ab2_cn only) and optionally ``profile_stride`` (default 10).  The output directory must not exist;
+it receives
+
+* ``manifest.json``: the case, the start time, the restart file and its SHA-256, the solver metadata
+  and the SHA-256 of this runner, the observables and the analysis modules;
+* ``history.jsonl``: one row per step, the step's diagnostics dict plus ``step``, ``delta_p`` (front
+  minus back stagnation pressure) and ``wall_seconds``;
+* ``centerline.bin`` + ``centerline.json``: the wake samples (``observables.CylinderSamples``);
+* ``checkpoint.npz``: every 45 s of wall time, at the end, and after a failure;
+* ``summary.json``: progress while running; at the end the force statistics over the sample window
+  (for 'periodic' cases the last complete shedding cycles), or the error and the failing step's
+  diagnostics;
+* ``mean_observables.json``: window means of the wake samples.
+
+The steps run on the device in segments that end where a wake sample is due (every ``profile_stride``
+steps, as the reference's sampler counts them).  History rows come from the device's per-step
+diagnostics rows, with the stagnation pressures evaluated on the device every step; the host reads
+the state back once per segment.  ``wall_seconds`` of the rows inside a segment is interpolated
+linearly over the segment's wall time.  After a failed step the state (and the checkpoint) is the one
+at the end of the segment that holds it.
 """
 import argparse
 import hashlib
 import json
 import os
 import time
+from dataclasses import dataclass
 from pathlib import Path
 
 import numpy as np
 
+from . import _native as nat
 from .analysis import periodic_metrics, window_statistics
-from .flow import MAC_CylinderFlow
+from .flow import MAC_CylinderFlow, diagnostics_from_row
 from .observables import CylinderSamples, mean_observables
 
 HERE = Path(__file__).resolve().parent
+KINDS = ('steady', 'periodic', 'diagnostic')
+MAXIMA = ('max_divergence', 'cfl', 'pressure_residual_linf')
+CHECKPOINT_SECONDS = 45.0
+PROBE0, PROBE1 = nat.DIAG_NAMES.index('probe0'), nat.DIAG_NAMES.index('probe1')
 
 
-def _sha256(path):
+def digest(path):
     return hashlib.sha256(Path(path).read_bytes()).hexdigest()
 
 
-def _finite_or_none(d):
-    return {k: (None if isinstance(v, float) and not np.isfinite(v) else v) for k, v in d.items()}
+@dataclass(frozen=True)
+class Case:
+    """A validated stage-JSON case."""
+    spec: dict
+    dt: float
+    end_time: float
+    sample_start: float
+    kind: str
+    solver: dict
+    profile_stride: int
+
+    @classmethod
+    def load(cls, path):
+        spec = json.loads(Path(path).read_text(encoding='utf-8'))
+        times = np.array([spec['dt'], spec['end_time'], spec['sample_start']], dtype=float)
+        solver = spec['solver']
+        if (not np.isfinite(times).all() or times[0] <= 0 or not 0 <= times[2] < times[1] or spec['kind'] not in KINDS
+                or solver.get('boundary') != 'ghost_cell' or solver.get('time_scheme') != 'ab2_cn'):
+            raise ValueError('Invalid explicit Ghost validation case')
+        return cls(spec, float(spec['dt']), float(spec['end_time']), float(spec['sample_start']), spec['kind'],
+                   solver, spec.get('profile_stride', 10))
+
+    def steps_from(self, t):
+        """Steps from time t to the end time; the sample window must start at or after t."""
+        n = round((self.end_time - t) / self.dt)
+        if n <= 0 or abs(t + n * self.dt - self.end_time) > 1e-9 or self.sample_start < t:
+            raise ValueError('Window/end must be covered by this run at its declared timestep')
+        return n
+
+    def segments(self, n):
+        """Segment lengths whose ends fall on the sampler's due steps: the sampler's first call is the
+        forced initial sample, so its stride-th call is step stride - 1, then every stride steps."""
+        k, out, done = self.profile_stride, [], 0
+        first = k - 1 if k > 1 else k
+        while done < n:
+            out.append(min(first if not out else k, n - done))
+            done += out[-1]
+        return out
+
+
+class Telemetry:
+    """history.jsonl, the running maxima of the gated quantities, and summary.json."""
+
+    def __init__(self, directory):
+        self.dir = directory
+        self.stream = (directory / 'history.jsonl').open('x', encoding='utf-8')
+        self.maxima = dict.fromkeys(MAXIMA, 0.0)
+        self.steps = 0
+        self.started = time.perf_counter()
+
+    def seconds(self):
+        return time.perf_counter() - self.started
+
+    def add(self, rows, backend, wall0, wall1):
+        for i, row in enumerate(rows):
+            d = diagnostics_from_row(row, backend)
+            self.steps += 1
+            d.update(step=self.steps, delta_p=float(row[PROBE0] - row[PROBE1]),
+                     wall_seconds=wall0 + (wall1 - wall0) * (i + 1) / len(rows))
+            self.stream.write(json.dumps(d, allow_nan=False) + '\n')
+            self.fold(d)
+
+    def fold(self, d):
+        for key in MAXIMA:
+            if key in d:
+                self.maxima[key] = max(self.maxima[key], float(d[key]))
+
+    def summary(self, **fields):
+        tmp = self.dir / 'summary.tmp'
+        tmp.write_text(json.dumps(dict(fields, **self.maxima), indent=2, allow_nan=False), encoding='utf-8')
+        os.replace(tmp, self.dir / 'summary.json')
+
+    def close(self):
+        if not self.stream.closed:
+            self.stream.close()
+
+    def history(self):
+        rows = [json.loads(line) for line in (self.dir / 'history.jsonl').read_text(encoding='utf-8').splitlines()]
+        return {key: np.array([r[key] for r in rows]) for key in ('time', 'cd', 'cl', 'delta_p')}
+
+
+def open_flow(case, restart):
+    flow = MAC_CylinderFlow.from_checkpoint(restart) if restart else MAC_CylinderFlow(**case.solver)
+    if flow.configuration != MAC_CylinderFlow.configuration_of(**case.solver):
+        raise ValueError('Restart/configuration mismatch')
+    return flow
+
+
+def finite_or_none(values):
+    return {k: (None if isinstance(v, float) and not np.isfinite(v) else v) for k, v in values.items()}
 
 
 def run(config_path, output, restart=None):
-    case = json.loads(Path(config_path).read_text(encoding='utf-8'))
-    dt, end, left = case['dt'], case['end_time'], case['sample_start']
-    solver_opts = case['solver']
-    if (not np.all(np.isfinite([dt, end, left])) or dt <= 0 or not 0 <= left < end
-            or case['kind'] not in ('steady', 'periodic', 'diagnostic')
-            or solver_opts.get('boundary') != 'ghost_cell' or solver_opts.get('time_scheme') != 'ab2_cn'):
-        raise ValueError('Invalid explicit Ghost validation case')
-    output = Path(output)
-    output.mkdir(parents=True, exist_ok=False)
-    s = MAC_CylinderFlow.from_checkpoint(restart) if restart else MAC_CylinderFlow(**solver_opts)
-    if s.configuration != MAC_CylinderFlow.configuration_of(**solver_opts):
-        raise ValueError('Restart/configuration mismatch')
-    steps = round((end - s.time) / dt)
-    if steps <= 0 or abs(s.time + steps * dt - end) > 1e-9 or left < s.time:
-        raise ValueError('Window/end must be covered by this run at its declared timestep')
-    t_begin = s.time
-    samples = CylinderSamples(s, output / 'centerline.bin', stride=case.get('profile_stride', 10))
+    case = Case.load(config_path)
+    out = Path(output)
+    out.mkdir(parents=True, exist_ok=False)
+    flow = open_flow(case, restart)
+    n = case.steps_from(flow.time)
+    t_begin = flow.time
+    samples = CylinderSamples(flow, out / 'centerline.bin', stride=case.profile_stride)
+    device = flow.gg.device
+    device.set_probes(samples.pressure_operator)
+    backend = device.pressure_info['backend']
     samples.record(force=True)
-    clock0 = time.perf_counter()
-    last_checkpoint = clock0
-    count = 0
-    worst = dict(max_divergence=0.0, cfl=0.0, pressure_residual_linf=0.0)
-    manifest = dict(case=case, initial_time=t_begin, restart=str(restart) if restart else None,
-                    restart_sha256=_sha256(restart) if restart else None,
-                    metadata=s.metadata(dt, steps, perturb_transient=False), runner_sha256=_sha256(__file__),
-                    observables_sha256=_sha256(HERE / 'observables.py'), analysis_sha256=_sha256(HERE / 'analysis.py'),
-                    ram_stop_guard=False, ready_for_paper1=False)
-    (output / 'manifest.json').write_text(json.dumps(manifest, indent=2), encoding='utf-8')
-
-    def status(value):
-        tmp = output / 'summary.tmp'
-        tmp.write_text(json.dumps(value, indent=2, allow_nan=False), encoding='utf-8')
-        os.replace(tmp, output / 'summary.json')
-
-    with (output / 'history.jsonl').open('x', encoding='utf-8') as history:
-        def observe(state, diagnostics):
-            nonlocal count, last_checkpoint
-            count += 1
-            p = samples.pressures()
-            row = dict(diagnostics, step=count, delta_p=float(p[0] - p[1]), wall_seconds=time.perf_counter() - clock0)
-            history.write(json.dumps(row, allow_nan=False) + '\n')
-            for key in worst:
-                worst[key] = max(worst[key], float(row[key]))
-            samples.record(pressures=p)
-            if time.perf_counter() - last_checkpoint > 45:
+    manifest = dict(case=case.spec, initial_time=t_begin, restart=str(restart) if restart else None,
+                    restart_sha256=digest(restart) if restart else None,
+                    metadata=flow.metadata(case.dt, n, perturb_transient=False), runner_sha256=digest(__file__),
+                    observables_sha256=digest(HERE / 'observables.py'), analysis_sha256=digest(HERE / 'analysis.py'))
+    (out / 'manifest.json').write_text(json.dumps(manifest, indent=2), encoding='utf-8')
+    tel = Telemetry(out)
+    checkpoint = out / 'checkpoint.npz'
+    saved = tel.started
+    try:
+        wall = 0.0
+        for rows in flow.advance(n, case.dt, segment=case.segments(n), perturb_transient=False):
+            wall0, wall = wall, tel.seconds()
+            tel.add(rows, backend, wall0, wall)
+            samples.record(force=True, pressures=rows[-1, [PROBE0, PROBE1]])
+            if time.perf_counter() - saved > CHECKPOINT_SECONDS:
                 samples.flush()
-                history.flush()
-                state.save_checkpoint(output / 'checkpoint.npz', overwrite=True)
-                last_checkpoint = time.perf_counter()
-                status(dict(completed=False, steps=count, time=state.time, seconds=time.perf_counter() - clock0, **worst))
-
-        try:
-            s.solve(n_steps=steps, dt=dt, perturb_transient=False, save_artifacts=False, verbose=False,
-                    step_callback=observe)
-            samples.record(force=True)
-            samples.close()
-            history.flush()
-            s.save_checkpoint(output / 'checkpoint.npz', overwrite=True)
-        except BaseException as error:
-            samples.close()
-            history.flush()
-            s.save_checkpoint(output / 'checkpoint.npz', overwrite=True)
-            failed = dict(s.last_step_diagnostics)
-            if failed.get('time') == s.time:
-                for key in worst:
-                    if key in failed:
-                        worst[key] = max(worst[key], float(failed[key]))
-            status(dict(completed=False, steps=count, time=s.time, error=f'{type(error).__name__}: {error}',
-                        last_step_diagnostics=failed, seconds=time.perf_counter() - clock0, **worst))
-            raise
-    rows = [json.loads(line) for line in (output / 'history.jsonl').read_text(encoding='utf-8').splitlines()]
-    t = np.array([r['time'] for r in rows])
-    cd = np.array([r['cd'] for r in rows])
-    cl = np.array([r['cl'] for r in rows])
-    dp = np.array([r['delta_p'] for r in rows])
-    window = [max(left, float(t[0])), float(t[-1])]
+                tel.stream.flush()
+                flow.save_checkpoint(checkpoint, overwrite=True)
+                saved = time.perf_counter()
+                tel.summary(completed=False, steps=tel.steps, time=flow.time, seconds=tel.seconds())
+        samples.record(force=True)
+    except BaseException as error:
+        rows = getattr(error, 'diagnostics', None)
+        failed = getattr(error, 'failed_step', None)
+        if rows is not None and failed is not None:
+            tel.add(rows[:failed], backend, tel.seconds(), tel.seconds())
+            tel.fold(flow.last_step_diagnostics)
+        samples.close()
+        tel.close()
+        flow.save_checkpoint(checkpoint, overwrite=True)
+        tel.summary(completed=False, steps=tel.steps, time=flow.time, error=f'{type(error).__name__}: {error}',
+                    last_step_diagnostics=dict(flow.last_step_diagnostics), seconds=tel.seconds())
+        raise
+    samples.close()
+    tel.close()
+    flow.save_checkpoint(checkpoint, overwrite=True)
+    h = tel.history()
+    window = [max(case.sample_start, float(h['time'][0])), float(h['time'][-1])]
     periodic = None
-    if case['kind'] == 'periodic':
-        periodic = periodic_metrics(t, cd, cl, left, D=s.D, U_ref=s.U_ref)
+    if case.kind == 'periodic':
+        periodic = periodic_metrics(h['time'], h['cd'], h['cl'], case.sample_start, D=flow.D, U_ref=flow.U_ref)
         if periodic['cycles'] < 5:
-            status(dict(completed=True, accepted=False, ready_for_paper1=False, steps=count,
-                        reason='Insufficient complete cycles; no periodic mean-wake claim', **worst))
+            tel.summary(completed=True, accepted=False, steps=tel.steps,
+                        reason='Insufficient complete cycles; no periodic mean-wake claim')
             return
         window = [periodic['sample_start'], periodic['sample_end']]
-    stats = window_statistics(t, np.column_stack((cd, cl, dp)), *window)
-    mean = mean_observables(output / 'centerline.bin', *window)
-    (output / 'mean_observables.json').write_text(json.dumps(mean, indent=2, allow_nan=False), encoding='utf-8')
-    status(dict(completed=True, ready_for_paper1=False, steps=count, time=s.time, seconds=time.perf_counter() - clock0,
-                statistics=stats, periodic=_finite_or_none(periodic) if periodic else None, mean_flow_available=True,
-                benchmark_accepted=False,
-                acceptance_note='Reference-specific force, pressure, Lr, stationarity and domain checks are separate.',
-                sample_window=window, **worst))
+    stats = window_statistics(h['time'], np.column_stack((h['cd'], h['cl'], h['delta_p'])), *window)
+    mean = mean_observables(out / 'centerline.bin', *window)
+    (out / 'mean_observables.json').write_text(json.dumps(mean, indent=2, allow_nan=False), encoding='utf-8')
+    tel.summary(completed=True, steps=tel.steps, time=flow.time, seconds=tel.seconds(), statistics=stats,
+                periodic=finite_or_none(periodic) if periodic else None, mean_flow_available=True,
+                sample_window=window)
 
 
 def main(argv=None):

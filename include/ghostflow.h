@@ -55,7 +55,7 @@ enum {
   MCF_D_FX, MCF_D_FY, MCF_D_MAXDIV, MCF_D_CFL, MCF_D_MOMENTUM_RES, MCF_D_COUPLED_RES,
   MCF_D_P_RES0, MCF_D_P_RES1, MCF_D_P_REL0, MCF_D_P_REL1, MCF_D_P_REFINE, MCF_D_P_TARGET0, MCF_D_P_TARGET1,
   MCF_D_P_ROUND0, MCF_D_P_ROUND1, MCF_D_P_BACKWARD0, MCF_D_P_BACKWARD1, MCF_D_H_REFINE,
-  MCF_D_STATUS, MCF_D_RHS_MAX, MCF_D_P_SOLVES,
+  MCF_D_STATUS, MCF_D_RHS_MAX, MCF_D_P_SOLVES, MCF_D_PROBE0, MCF_D_PROBE1,
   MCF_D_SOLVE0 = 32,   /* MCF_SOLVE_LEN doubles per pressure solve of the step, in call order (up to 3) */
   MCF_DIAG_LEN = 64
 };
@@ -73,6 +73,11 @@ const char* mcf_last_error(const mcf_handle* h);
 int mcf_device_name(char* buf, int len);
 
 int mcf_set_inlet(mcf_handle* h, const double* u_inlet);
+/* Transient inflow perturbation (solver/main.py:280-286): every following step first sets its inlet
+   from its own end time t, u_inlet = base * (1 + shape * (1 - t / duration)) while t <= duration and
+   base afterwards, with shape = perturb_amp * sin(2 pi y / Ly) (both ny values).  duration <= 0
+   removes it (the inlet then stays as the last step or mcf_set_inlet left it). */
+int mcf_set_inlet_perturbation(mcf_handle* h, const double* base, const double* shape, double duration);
 /* Rows of the solid interval per array row: lo[j] <= i <= hi[j] are solid (hi < lo: none).
    u: ny rows over nx+1 columns; v: ny+1 rows over nx columns; p: ny rows over nx columns. */
 int mcf_set_solid_intervals(mcf_handle* h, int field, const int* lo, const int* hi);
@@ -100,6 +105,12 @@ int mcf_set_surface(mcf_handle* h, int n, const int* pp, const int* pc, const do
                     const int* up, const int* uc, const double* uv, const int* vp, const int* vc,
                     const double* vv, const double* normals, const double* normal_weights,
                     double diameter, double u_ref);
+
+/* Pressure probes (postprocessing/cylinder_observables.py:37-38 reconstructs the front and back
+   stagnation pressures from p every step): n <= 2 CSR rows over p, evaluated on each step's output
+   pressure and recorded in the diagnostics columns MCF_D_PROBE0 / MCF_D_PROBE1 (0 without probes).
+   Call after mcf_set_surface; n = 0 removes them. */
+int mcf_set_probes(mcf_handle* h, int n, const int* ptr, const int* col, const double* val);
 
 /* State: u/v in vel layout (extended arrays, ghost values included), p (ny*nx, full array),
    history: prev_adv (vel layout q-array, zeros off the active faces; NULL = none),

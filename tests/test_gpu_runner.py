@@ -50,6 +50,14 @@ def test_runner_end_to_end_with_restart(pkg, mods, tmp_path):
     assert json.loads((tmp_path / 'a' / 'mean_observables.json').read_text())['window'][1] == pytest.approx(0.03)
     manifest = json.loads((tmp_path / 'a' / 'manifest.json').read_text())
     assert manifest['case'] == case and manifest['metadata']['n_steps'] == 30
+    # delta_p rows come from the device probes: equal to the host reconstruction of the final state
+    ob, _ = mods
+    end = pkg.MAC_CylinderFlow.from_checkpoint(tmp_path / 'a' / 'checkpoint.npz')
+    p = ob.stagnation_operator(end) @ end.p.ravel()
+    assert abs(rows[-1]['delta_p'] - (p[0] - p[1])) <= 1e-12 * np.max(abs(p))
+    # samples as the reference's stride counter takes them: the initial one, steps 4, 9, .., 29, the final one
+    meta = json.loads((tmp_path / 'a' / 'centerline.json').read_text())
+    assert (tmp_path / 'a' / 'centerline.bin').stat().st_size == 8 * 8 * meta['columns']
     # restart to 0.05 equals a direct run of 50 steps
     case2 = dict(case, end_time=0.05, sample_start=0.035)   # after the restart time (30 accumulated steps)
     cfg2 = tmp_path / 'case2.json'
