@@ -31,6 +31,39 @@ STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB
 GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6, 4: 9.657e6, 5: 9.754e6, 6: 9.732e6}
 
 
+BOX = dict(nx=800, ny=820, Lx=40., Ly=41., D=1., xc=10., yc=20.5, nu=0.01, rho=1., inflow='uniform', U_inf=1.)
+BOX_DT = 2e-3
+
+
+def box_leg(pkg, steps, warmup, launch=None):
+    """The 40D x 41D box with uniform inflow U = 1 (800 x 820 cells, h = 0.05, Re = 100, dt = 2e-3, the
+    reference's outlet) from the reference's default start, with the transient inflow perturbation on
+    (amplitude 0.15 over 2.5 D/U, set on the device every step): warmup steps, then `steps` timed steps."""
+    t0 = time.perf_counter()
+    flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **BOX)
+    dev = flow.gg.device
+    dev.set_dt(BOX_DT)
+    if launch:
+        dev.set_launch(*launch)
+    setup_s = time.perf_counter() - t0
+    print(f'bench: 800x820 box ready in {setup_s:.1f} s; {warmup} + {steps} steps', file=sys.stderr, flush=True)
+    flow.solve(n_steps=warmup, dt=BOX_DT, perturb_transient=True, save_artifacts=False, verbose=False)
+    t0 = time.perf_counter()
+    flow.solve(n_steps=steps, dt=BOX_DT, perturb_transient=True, save_artifacts=False, verbose=False)
+    elapsed = time.perf_counter() - t0
+    diag = flow.diagnostics_history
+    cells = BOX['nx'] * BOX['ny']
+    value = cells * steps / elapsed
+    return dict(workload='box 40Dx41D, cylinder D=1 at (10D, 20.5D), 800x820 cells (h=0.05), uniform inflow U=1 '
+                         'with the transient inflow perturbation, Re=100, dt=2e-3, FP64, reference outlet',
+                value=value, unit='cell-steps/s', ms_per_step=elapsed / steps * 1e3, steps=steps, warmup=warmup,
+                window=f'steps {warmup} .. {warmup + steps} from the default start', max_divergence=float(np.max(diag[:, 10])),
+                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
+                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
+                launch=dev.get_launch(), speculation=dev.get_speculation(), setup_seconds=setup_s)
+
+
 def cpu_baseline(seconds=20.0, gmres_seconds=0.0):
     """The oracle (NumPy/SciPy restatement) on this host's cores, on a bounded sample of steps; with
     gmres_seconds, then a bounded sample of the same case on the oracle's GMRES backend (SciPy gmres
@@ -198,6 +231,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
     ap.add_argument('--no-ensemble', action='store_true')
+    ap.add_argument('--box-steps', type=int, default=100,
+                    help='timed steps of the 800x820 uniform-inflow box leg (0: skip)')
     ap.add_argument('--ensemble-steps', type=int, default=200)
     ap.add_argument('--no-warm-start', action='store_true')
     ap.add_argument('--no-kernel-timing', action='store_true',
@@ -275,6 +310,8 @@ def main():
                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
                kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms, lean_misses=lean_misses), launch=launch,
                speculation=dev.get_speculation(), setup_seconds=setup_s)
+    if args.box_steps:
+        out['box'] = box_leg(pkg, args.box_steps, 20, args.launch)
     if args.gmres_steps:
         print(f'bench: GMRES pressure backend, {args.gmres_steps} steps', file=sys.stderr, flush=True)
         out['pressure_iterative'] = gmres_leg(dev, args.gmres_steps)

@@ -170,6 +170,24 @@ struct SolveDesc {
   int* dump_count;
   int dump_cap;
   const int* clock;     // the case's device step counter
+  // x-FFT form of the rectangle inverse: the x operator diagonalised by its orthonormal DCT-IV
+  // (xf 1) or DST-IV (xf 2) basis through a complex FFT of length M = xx / 2, and one tridiagonal
+  // solve along y per x mode m; xf 0: not available for this problem
+  int xf, fm, fns, ftn;
+  const int* fst;       // 3 per radix stage: radix r, offsets of its twiddles and of its r roots in ftw
+  const double* ftw;    // ftn complex: the stage twiddles w_{rL}^{q j} (q = 1..r-1, j < L) at a stage's
+                        // offset + (q-1) L + j, then the roots w_r^e (e < r) of every stage
+  const double* fpre;   // M complex: exp(-i pi (4n+1) / (4 xx))
+  const double* fpost;  // M complex: sqrt(2 / xx) exp(-i pi k / xx)
+  const double* yl;     // yy*xx, [j][m]: multiplier of row j of mode m's y system
+  const double* yi;     // yy*xx, [j][m]: inverse pivot
+  double ye;            // the y systems' off-diagonal b / dy^2
+  const double* xwp;    // (point columns) x xx: basis W[x][m] at each distinct x column of the points
+  const int* pt_c;      // npts: each point's row of xwp
+  const double* xwf;    // rg_n x xx: the basis at the x column of each capacitance row group
+  const int* yr_ptr;    // yy+1: capacitance entries of each rectangle row j
+  const int* yr_t;      // r: their y index (group order), by row
+  const int* yr_g;      // r: their column group (row of xwf)
 };
 
 DEV int stripe_id() { return (blockIdx.x + 7 * blockIdx.y) & (NSTRIPE - 1); }
@@ -1268,6 +1286,337 @@ __global__ __launch_bounds__(256) void k_capacitance(const SolveDesc* __restrict
   if (lane == 0) G(d.y)[row] = s;
 }
 
+// ------------------------------------------------------------------------------------
+// x-FFT rectangle inverse: DCT-IV / DST-IV along x through a complex FFT of half length, tridiagonal
+// solves along y per x mode
+// ------------------------------------------------------------------------------------
+constexpr int XF_MAXM = 2048;   // rows up to 4096 points
+constexpr int YS_MAXY = 2048;   // y systems up to 2048 rows
+constexpr int XF_MAXR = 64;     // largest radix of a stage
+typedef double2 cplx;
+DEV cplx cmul(cplx a, cplx b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+DEV cplx cadd(cplx a, cplx b) { return make_double2(a.x + b.x, a.y + b.y); }
+DEV cplx csub(cplx a, cplx b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// One Stockham stage of radix r over the M points in `in`: for j < L, k < m = M / (r L),
+// out[j + L (p + r k)] = sum_q w_{rL}^{q j} in[j + L (k + m q)] w_r^{q p}, twiddles and roots in LDS.
+// Radix 2 and 4 run one butterfly per thread; other radices one output per thread (each output
+// reads its r inputs), which keeps the threads busy when r is large.  In the first stage (L = 1)
+// every twiddle is 1.
+template <int R>
+DEV void fft_butterfly(const cplx* in, cplx* out, int M, int L, const cplx* tw) {
+  const int m = M / (R * L), nb = M / R;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const int j = b % L, k = b / L;
+    cplx a[R];
+    a[0] = in[j + L * k];
+    #pragma unroll
+    for (int q = 1; q < R; ++q) a[q] = L > 1 ? cmul(in[j + L * (k + m * q)], tw[(q - 1) * L + j]) : in[j + L * (k + m * q)];
+    if constexpr (R == 2) {
+      out[j + L * (0 + 2 * k)] = cadd(a[0], a[1]);
+      out[j + L * (1 + 2 * k)] = csub(a[0], a[1]);
+    } else {
+      const cplx s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+      const cplx s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
+      const cplx md13 = make_double2(d13.y, -d13.x);   // -i (a1 - a3)
+      out[j + L * (0 + 4 * k)] = cadd(s02, s13);
+      out[j + L * (1 + 4 * k)] = cadd(d02, md13);
+      out[j + L * (2 + 4 * k)] = csub(s02, s13);
+      out[j + L * (3 + 4 * k)] = csub(d02, md13);
+    }
+  }
+}
+template <int R>
+DEV void fft_outputs_r(const cplx* in, cplx* out, int M, int L, const cplx* tw, const cplx* root) {
+  const int m = M / (R * L);
+  for (int o = threadIdx.x; o < M; o += blockDim.x) {
+    const int j = o % L, pk = o / L, pp = pk % R, k = pk / R;
+    cplx a[R];
+    #pragma unroll
+    for (int q = 0; q < R; ++q) a[q] = in[j + L * (k + m * q)];
+    if (L > 1) {
+      #pragma unroll
+      for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[(q - 1) * L + j]);
+    }
+    cplx acc = a[0];
+    int e = 0;
+    #pragma unroll
+    for (int q = 1; q < R; ++q) {
+      e += pp;
+      if (e >= R) e -= R;
+      acc = cadd(acc, cmul(a[q], root[e]));
+    }
+    out[o] = acc;
+  }
+}
+DEV void fft_outputs(const cplx* in, cplx* out, int M, int L, int r, const cplx* tw, const cplx* root) {
+  const int m = M / (r * L);
+  for (int o = threadIdx.x; o < M; o += blockDim.x) {
+    const int j = o % L, pk = o / L, pp = pk % r, k = pk / r;
+    cplx acc = in[j + L * k];
+    int e = 0;
+    for (int q = 1; q < r; ++q) {
+      e += pp;
+      if (e >= r) e -= r;
+      cplx a = in[j + L * (k + m * q)];
+      if (L > 1) a = cmul(a, tw[(q - 1) * L + j]);
+      acc = cadd(acc, cmul(a, root[e]));
+    }
+    out[o] = acc;
+  }
+}
+
+// Dynamic LDS of k_xdct for a complex length M and t twiddles + roots: two M-point buffers, then
+// the stage twiddles and roots
+inline size_t xdct_lds(int M, int t) { return (size_t)(2 * M + t) * sizeof(cplx); }
+// A bound on the stage tables of length M: the twiddles of the stages number M - 1 (sum of
+// (r - 1) L telescopes) and the roots sum(r) <= prod(r) = M.
+inline int xdct_tables(int M) { return 2 * M; }
+constexpr size_t XF_MAXLDS = 64 * 1024;
+
+// The orthonormal DCT-IV (xf 1) or DST-IV (xf 2) of each row, one block per row: forward from the
+// input array into F, inverse from F into the output array with the solid mask (epilogue as the
+// y transform's).  X = DCT-IV(x) comes from the M-point FFT of z_n = (x_{2n} + i x_{N-1-2n})
+// exp(-i pi (4n+1)/(4N)): with y_k = sqrt(2/N) exp(-i pi k/N) Z_k, X_{2k} = Re y_k and
+// X_{N-1-2k} = -Im y_k; DST-IV(x)_k = (-1)^k DCT-IV(x reversed)_k.  The basis is its own inverse.
+// The row, the pre- and post-twiddles and the stage tables are loaded in one batch at the start.
+constexpr int XF_U = XF_MAXM / 256;   // points per thread at most
+constexpr int XF_MAXS = 16;           // radix stages at most
+template <int U>
+__global__ __launch_bounds__(256) void k_xdct(const SolveDesc* __restrict__ descs, GemmArgs g) {
+  const SolveDesc& d = descs[blockIdx.y];
+  const int j = blockIdx.x;
+  if (j >= d.yy) return;
+  if (g.gate && *g.gate == 0) return;
+  if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
+  extern __shared__ cplx xs[];
+  const int N = d.xx, M = N >> 1, T = d.ftn, tid = threadIdx.x;
+  cplx* b0 = xs;
+  cplx* b1 = xs + M;
+  cplx* tabs = xs + 2 * M;   // twiddles of every stage, then the roots
+  const bool sine = d.xf == 2;
+  const gptr<const double> src = g.forward ? desc_in(d, g.in_id) + (size_t)j * d.ld : G(d.F) + (size_t)j * N;
+  const gptr<const double> pre = G(d.fpre), post = G(d.fpost), tab = G(d.ftw);
+  __shared__ int stp[3 * XF_MAXS];
+  // tables hold fewer than 2M entries: 2U per thread cover them
+  double lo[U], hi[U], pr[U], pi[U], qr[U], qi[U], tr[2 * U], ti[2 * U];
+  #pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int n = min(tid + 256 * u, M - 1);
+    lo[u] = src[2 * n]; hi[u] = src[N - 1 - 2 * n];
+    pr[u] = pre[2 * n]; pi[u] = pre[2 * n + 1];
+    qr[u] = post[2 * n]; qi[u] = post[2 * n + 1];
+  }
+  #pragma unroll
+  for (int u = 0; u < 2 * U; ++u) {
+    const int t = min(tid + 256 * u, T - 1);
+    tr[u] = tab[2 * t]; ti[u] = tab[2 * t + 1];
+  }
+  const int sv = G(d.fst)[min(tid, 3 * d.fns - 1)];
+  #pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int n = tid + 256 * u;
+    if (n < M) {
+      const cplx z = sine ? make_double2(hi[u], lo[u]) : make_double2(lo[u], hi[u]);
+      b0[n] = cmul(z, make_double2(pr[u], pi[u]));
+    }
+  }
+  #pragma unroll
+  for (int u = 0; u < 2 * U; ++u)
+    if (tid + 256 * u < T) tabs[tid + 256 * u] = make_double2(tr[u], ti[u]);
+  if (tid < 3 * d.fns) stp[tid] = sv;
+  __syncthreads();
+  const int* st = stp;
+  cplx* in = b0;
+  cplx* out = b1;
+  int L = 1;
+  for (int sg = 0; sg < d.fns; ++sg) {
+    const int r = st[3 * sg];
+    const cplx* tw = tabs + st[3 * sg + 1];
+    const cplx* root = tabs + st[3 * sg + 2];
+    switch (r) {
+      case 4: fft_butterfly<4>(in, out, M, L, tw); break;
+      case 2: fft_butterfly<2>(in, out, M, L, tw); break;
+      case 3: fft_outputs_r<3>(in, out, M, L, tw, root); break;
+      case 5: fft_outputs_r<5>(in, out, M, L, tw, root); break;
+      case 7: fft_outputs_r<7>(in, out, M, L, tw, root); break;
+      case 11: fft_outputs_r<11>(in, out, M, L, tw, root); break;
+      case 13: fft_outputs_r<13>(in, out, M, L, tw, root); break;
+      default: fft_outputs(in, out, M, L, r, tw, root); break;
+    }
+    __syncthreads();
+    cplx* t = in; in = out; out = t;
+    L *= r;
+  }
+  #pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int k = tid + 256 * u;
+    if (k >= M) continue;
+    const cplx y = cmul(in[k], make_double2(qr[u], qi[u]));
+    // DST-IV: (-1)^i at i = 2k is +1, at i = N-1-2k (odd) -1
+    const double even = y.x, odd = sine ? y.y : -y.y;
+    if (g.forward) {
+      const gptr<double> Fr = G(d.F) + (size_t)j * N;
+      Fr[2 * k] = even;
+      Fr[N - 1 - 2 * k] = odd;
+    } else {
+      const gptr<double> O = desc_out(d, g.out_id) + (size_t)j * d.ld;
+      const int aj = j + d.joff;
+      const int slo = d.lo ? G(d.lo)[aj] : 1, shi = d.lo ? G(d.hi)[aj] : 0;
+      #pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = h ? N - 1 - 2 * k : 2 * k;
+        const double v = h ? odd : even;
+        const int ai = i + d.ioff;
+        if (g.epi == 0) O[i] = v;
+        else if (slo <= ai && ai <= shi) O[i] = 0.0;
+        else O[i] = (g.epi == 2) ? (O[i] + v) : v;
+      }
+    }
+  }
+}
+
+// y tridiagonal solves of the x modes, in place in F (mode 0), or F <- F - T^{-1} c with c the
+// x transform of the capacitance correction (mode 1).  A block holds MB = 256 / NSEG consecutive
+// modes and splits the rows into NSEG segments of at most C rows; thread (s, q) composes its
+// segment's affine maps, the segments' maps are scanned in LDS, and each thread then runs its
+// segment with the carried value.  Loads of one row of a block's modes are contiguous.
+template <int NSEG, int C>
+__global__ __launch_bounds__(256) void k_ysweep(const SolveDesc* __restrict__ descs, PcrArgs p) {
+  constexpr int MB = 256 / NSEG;
+  const SolveDesc& d = descs[blockIdx.y];
+  const int m0 = blockIdx.x * MB;
+  if (m0 >= d.xx) return;
+  if (p.gate && *p.gate == 0) return;
+  if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
+  const int xx = d.xx, yy = d.yy, tid = threadIdx.x, q = tid % MB, s = tid / MB;
+  const bool live = m0 + q < xx;
+  const int m = min(m0 + q, xx - 1);
+  const int S = (yy + NSEG - 1) / NSEG;
+  const int j0 = min(s * S, yy), n = min(j0 + S, yy) - j0;
+  const gptr<double> F = G(d.F);
+  const gptr<const double> Lm = G(d.yl), Im = G(d.yi);
+  const double e = d.ye;
+  // every load at a clamped, in-bounds row, all in one batch; rows past the segment are not used
+  double r[C], l[C], id[C];
+  #pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const size_t at = (size_t)min(j0 + c, yy - 1) * xx + m;
+    l[c] = Lm[at];
+    id[c] = Im[at];
+    r[c] = F[at];
+  }
+  if (p.mode == 1) {
+    #pragma unroll
+    for (int c = 0; c < C; ++c) r[c] = 0.0;
+  }
+  __shared__ double sa[NSEG][MB], sb[NSEG][MB], carry[NSEG][MB];
+  if (p.mode == 1) {
+    // the correction's right-hand side c[j][m] = sum over row j's capacitance entries of
+    // W[x_e][m] y_e: the row pointers and the entries (column group, y value) go to LDS once
+    __shared__ int rps[YS_MAXY + 1];
+    __shared__ int eg[2048];
+    __shared__ double ey[2048];
+    const gptr<const int> rp = G(d.yr_ptr), rt = G(d.yr_t), rg = G(d.yr_g);
+    const gptr<const double> wf = G(d.xwf), yv = G(d.y);
+    const int ne = d.r;
+    for (int t = tid; t <= yy; t += 256) rps[t] = rp[t];
+    for (int t = tid; t < ne; t += 256) { eg[t] = rg[t]; ey[t] = yv[rt[t]]; }
+    __syncthreads();
+    #pragma unroll
+    for (int c = 0; c < C; ++c)
+      if (c < n) {
+        double acc = 0.0;
+        const int e1 = rps[j0 + c + 1];
+        for (int t = rps[j0 + c]; t < e1; ++t) acc += wf[(size_t)eg[t] * xx + m] * ey[t];
+        r[c] = acc;
+      }
+  }
+  // forward: z_j = r_j - l_j z_{j-1}
+  double A = 1.0, B = 0.0;
+  #pragma unroll
+  for (int c = 0; c < C; ++c)
+    if (c < n) { A *= -l[c]; B = -l[c] * B + r[c]; }
+  sa[s][q] = A; sb[s][q] = B;
+  __syncthreads();
+  if (s == 0) {
+    double z = 0.0;
+    for (int t = 0; t < NSEG; ++t) { carry[t][q] = z; z = sa[t][q] * z + sb[t][q]; }
+  }
+  __syncthreads();
+  double z = carry[s][q];
+  #pragma unroll
+  for (int c = 0; c < C; ++c)
+    if (c < n) { z = r[c] - l[c] * z; r[c] = z; }
+  __syncthreads();
+  // backward: x_j = (z_j - e x_{j+1}) / d_j
+  A = 1.0; B = 0.0;
+  #pragma unroll
+  for (int c = C - 1; c >= 0; --c)
+    if (c < n) {
+      const double al = (j0 + c < yy - 1) ? -e * id[c] : 0.0;
+      A *= al; B = al * B + r[c] * id[c];
+    }
+  sa[s][q] = A; sb[s][q] = B;
+  __syncthreads();
+  if (s == 0) {
+    double x = 0.0;
+    for (int t = NSEG - 1; t >= 0; --t) { carry[t][q] = x; x = sa[t][q] * x + sb[t][q]; }
+  }
+  __syncthreads();
+  double x = carry[s][q];
+  #pragma unroll
+  for (int c = C - 1; c >= 0; --c)
+    if (c < n) {
+      const double al = (j0 + c < yy - 1) ? -e * id[c] : 0.0;
+      x = al * x + r[c] * id[c];
+      r[c] = x;
+    }
+  if (!live) return;
+  if (p.mode == 0) {
+    #pragma unroll
+    for (int c = 0; c < C; ++c)
+      if (c < n) F[(size_t)(j0 + c) * xx + m] = r[c];
+  } else {
+    double f[C];
+    #pragma unroll
+    for (int c = 0; c < C; ++c) f[c] = F[(size_t)min(j0 + c, yy - 1) * xx + m];
+    __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+    for (int c = 0; c < C; ++c)
+      if (c < n) F[(size_t)(j0 + c) * xx + m] = f[c] - r[c];
+  }
+}
+
+// first = R^{-1} b at the capacitance points in the x-FFT form: sum_m W[x_t][m] F[j_t][m], one
+// wave per point, lanes over the x modes
+__global__ __launch_bounds__(256) void k_xpoints(const SolveDesc* __restrict__ descs, PcrArgs p) {
+  const SolveDesc& d = descs[blockIdx.y];
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blockIdx.x * 4 >= d.npts) return;
+  if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
+  if (q >= d.npts) return;
+  const int xx = d.xx, lane = threadIdx.x & 63;
+  const gptr<const double> w = G(d.xwp) + (size_t)G(d.pt_c)[q] * xx;
+  const gptr<const double> f = G(d.F) + (size_t)G(d.pt_j)[q] * xx;
+  constexpr int PT_U = 8;
+  double s = 0.0;
+  for (int m0 = lane; m0 < xx; m0 += 64 * PT_U) {
+    double a[PT_U], b[PT_U];
+    #pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const int mm = min(m0 + 64 * u, xx - 1);
+      a[u] = w[mm];
+      b[u] = f[mm];
+    }
+    #pragma unroll
+    for (int u = 0; u < PT_U; ++u)
+      if (m0 + 64 * u < xx) s += a[u] * b[u];
+  }
+  s = wave_sum(s);
+  if (lane == 0) G(d.firstK)[q] = s;
+}
+
 // Records the pair of a first pressure Schur solve: rhs = C first (first = R^{-1} b at the
 // capacitance points) and the multiplier y = S^{-1} rhs, both in boundary-angle order, tagged with
 // the step and the projection.  One block per problem; a full ring stops recording.
@@ -2212,14 +2561,37 @@ __global__ __launch_bounds__(KR_CT) void k_kry_ctl(KryArgs a, int phase) {
 }  // namespace
 
 constexpr int N_GEMM_V = 7, N_TRI_V = 3;
+// the x-FFT form of the rectangle inverse: transform k_xdct and sweep k_ysweep, a class's pair
+constexpr int GV_XFFT = N_GEMM_V, TV_YSWEEP = N_TRI_V;
 constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64, 32, 16, 32, 16}, GEMM_TN[N_GEMM_V] = {64, 32, 64, 32, 32, 32, 32};
 constexpr int TRI_TH[N_TRI_V] = {128, 256, 512};
 struct Launch { int gemm_p = 0, gemm_h = 0, tri_p = 1, tri_h = 1; };
+
+// A caller's shape for both classes: transform gemm with sweep width tri; gemm GV_XFFT selects the
+// x-FFT pair for a class whose problems all have it, and the 16-row split-K transform with width
+// tri for the other class.
+void pin_launch(Launch& ln, int gemm, int tri, bool xf_p, bool xf_h) {
+  const int fallback = 4;
+  ln.gemm_p = gemm == GV_XFFT ? (xf_p ? GV_XFFT : fallback) : gemm;
+  ln.gemm_h = gemm == GV_XFFT ? (xf_h ? GV_XFFT : fallback) : gemm;
+  ln.tri_p = ln.gemm_p == GV_XFFT ? TV_YSWEEP : tri;
+  ln.tri_h = ln.gemm_h == GV_XFFT ? TV_YSWEEP : tri;
+}
 
 inline dim3 gemm_grid(int v, int yy, int xx, int count) {
   return dim3((xx + GEMM_TN[v] - 1) / GEMM_TN[v], (yy + GEMM_TM[v] - 1) / GEMM_TM[v], count);
 }
 void launch_gemm(int v, int yy, int xx, int count, hipStream_t st, const SolveDesc* D, GemmArgs g) {
+  if (v == GV_XFFT) {
+    const int M = xx / 2;
+    const size_t lds = xdct_lds(M, xdct_tables(M));
+    const dim3 gx(yy, count);
+    if (M <= 256) hipLaunchKernelGGL(k_xdct<1>, gx, dim3(256), lds, st, D, g);
+    else if (M <= 512) hipLaunchKernelGGL(k_xdct<2>, gx, dim3(256), lds, st, D, g);
+    else if (M <= 768) hipLaunchKernelGGL(k_xdct<3>, gx, dim3(256), lds, st, D, g);
+    else hipLaunchKernelGGL(k_xdct<4>, gx, dim3(256), lds, st, D, g);
+    return;
+  }
   const dim3 gg = gemm_grid(v, yy, xx, count);
   switch (v) {
     case 1: hipLaunchKernelGGL((k_gemm<32, 32, 64>), gg, dim3(256), 0, st, D, g); break;
@@ -2231,7 +2603,16 @@ void launch_gemm(int v, int yy, int xx, int count, hipStream_t st, const SolveDe
     default: hipLaunchKernelGGL((k_gemm<32, 64, 64>), gg, dim3(256), 0, st, D, g); break;
   }
 }
-void launch_tri(int v, int yy, int count, hipStream_t st, const SolveDesc* D, PcrArgs p) {
+// y sweep shape for the largest rectangle height: segments per mode x rows per segment
+void launch_ysweep(int yy, int xx, int count, hipStream_t st, const SolveDesc* D, PcrArgs p) {
+  if (yy <= 128) hipLaunchKernelGGL((k_ysweep<16, 8>), dim3((xx + 15) / 16, count), dim3(256), 0, st, D, p);
+  else if (yy <= 256) hipLaunchKernelGGL((k_ysweep<16, 16>), dim3((xx + 15) / 16, count), dim3(256), 0, st, D, p);
+  else if (yy <= 512) hipLaunchKernelGGL((k_ysweep<16, 32>), dim3((xx + 15) / 16, count), dim3(256), 0, st, D, p);
+  else if (yy <= 1024) hipLaunchKernelGGL((k_ysweep<32, 32>), dim3((xx + 7) / 8, count), dim3(256), 0, st, D, p);
+  else hipLaunchKernelGGL((k_ysweep<64, 32>), dim3((xx + 3) / 4, count), dim3(256), 0, st, D, p);
+}
+void launch_tri(int v, int yy, int xx, int count, hipStream_t st, const SolveDesc* D, PcrArgs p) {
+  if (v == TV_YSWEEP) { launch_ysweep(yy, xx, count, st, D, p); return; }
   switch (v) {
     case 0: hipLaunchKernelGGL(k_tri<128>, dim3(yy, count), dim3(128), 0, st, D, p); break;
     case 2: hipLaunchKernelGGL(k_tri<512>, dim3(yy, count), dim3(512), 0, st, D, p); break;
@@ -2402,6 +2783,7 @@ struct Ctx {
   int n;
   int maxvel, maxp, maxfill, maxsurf, maxny;
   bool pert;             // some case perturbs its inflow: each step sets its inlet first (k_inlet)
+  bool noxf_p, noxf_h;   // some pressure / velocity problem has no x-FFT form
   int p_yy, p_xx, p_np, p_r, h_yy, h_xx, h_np, h_r;
 };
 
@@ -2412,6 +2794,8 @@ void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
   x.maxsurf = std::max(x.maxsurf, c.nsurf);
   x.maxny = std::max(x.maxny, c.ny);
   x.pert = x.pert || c.pert_T > 0.0;
+  x.noxf_p = x.noxf_p || !hs3[0].xf;
+  x.noxf_h = x.noxf_h || !hs3[1].xf || !hs3[2].xf;
   x.p_yy = std::max(x.p_yy, hs3[0].yy); x.p_xx = std::max(x.p_xx, hs3[0].xx);
   x.p_np = std::max(x.p_np, hs3[0].npts); x.p_r = std::max(x.p_r, hs3[0].r);
   for (int q = 1; q < 3; ++q) {
@@ -2465,14 +2849,15 @@ int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int p
   GemmArgs g{1, in_id, out_id, 0, pred_slot, slot_shift};
   launch_gemm(gv, yy, xx, count, x.stream, D, g);
   PcrArgs p0{0, pred_slot, slot_shift};
-  launch_tri(tv, yy, count, x.stream, D, p0);
+  launch_tri(tv, yy, xx, count, x.stream, D, p0);
   if (r > 0) {
-    hipLaunchKernelGGL(k_points, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+    if (gv == GV_XFFT) hipLaunchKernelGGL(k_xpoints, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+    else hipLaunchKernelGGL(k_points, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
     hipLaunchKernelGGL(k_capacitance, dim3((r + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
     if (x.dump && pres && pred_slot < 0)
       hipLaunchKernelGGL(k_dump, dim3(1, count), dim3(256), 0, x.stream, D, slot_shift / 8);
     PcrArgs p1{1, pred_slot, slot_shift};
-    launch_tri(tv, yy, count, x.stream, D, p1);
+    launch_tri(tv, yy, xx, count, x.stream, D, p1);
   }
   GemmArgs gi{0, in_id, out_id, epi, pred_slot, slot_shift};
   launch_gemm(gv, yy, xx, count, x.stream, D, gi);
@@ -2550,7 +2935,7 @@ void kry_precondition(const Ctx& x, int in_id, int out_id, const int* gate) {
   GemmArgs g{1, in_id, out_id, 0, -1, 0, gate};
   launch_gemm(x.ln.gemm_p, x.p_yy, x.p_xx, 1, x.stream, x.dsp, g);
   PcrArgs p{0, -1, 0, gate};
-  launch_tri(x.ln.tri_p, x.p_yy, 1, x.stream, x.dsp, p);
+  launch_tri(x.ln.tri_p, x.p_yy, x.p_xx, 1, x.stream, x.dsp, p);
   GemmArgs gi{0, in_id, out_id, 1, -1, 0, gate};
   launch_gemm(x.ln.gemm_p, x.p_yy, x.p_xx, 1, x.stream, x.dsp, gi);
 }
@@ -2855,15 +3240,30 @@ int autotune(const Ctx& x, Launch& out) {
     const int yy = pres ? x.p_yy : x.h_yy, xx = pres ? x.p_xx : x.h_xx;
     const GemmArgs g{1, pres ? B_DIVP : B_RHS, 0, 0, -1, 0};
     const PcrArgs p0{0, -1, 0};
-    float best = 1e30f;
+    float best = 1e30f, best_gemm = 1e30f;
     int bg = 0, bt = 1;
-    for (int v = 0; v < N_GEMM_V + N_TRI_V && !rc; ++v) {
+    // the dense y transforms and the x sweeps, then (where every problem of the class has it) the
+    // x-FFT transform with its y sweep as one pair against the best dense pair
+    const bool xf = pres ? !x.noxf_p : !x.noxf_h;
+    const int nv = N_GEMM_V + N_TRI_V + (xf ? 1 : 0);
+    for (int v = 0; v < nv && !rc; ++v) {
+      const bool pair = v == N_GEMM_V + N_TRI_V;
       const bool gemm = v < N_GEMM_V;
       const int k = gemm ? v : v - N_GEMM_V;
-      if (v == N_GEMM_V) best = 1e30f;
-      for (int r = 0; r < 2; ++r) gemm ? launch_gemm(k, yy, xx, count, x.stream, D, g) : launch_tri(k, yy, count, x.stream, D, p0);
+      if (v == N_GEMM_V) { best_gemm = best; best = 1e30f; }
+      auto launch = [&]() {
+        if (pair) {
+          launch_gemm(GV_XFFT, yy, xx, count, x.stream, D, g);
+          launch_tri(TV_YSWEEP, yy, xx, count, x.stream, D, p0);
+        } else if (gemm) {
+          launch_gemm(k, yy, xx, count, x.stream, D, g);
+        } else {
+          launch_tri(k, yy, xx, count, x.stream, D, p0);
+        }
+      };
+      for (int r = 0; r < 2; ++r) launch();
       (void)hipEventRecord(e0, x.stream);
-      for (int r = 0; r < reps; ++r) gemm ? launch_gemm(k, yy, xx, count, x.stream, D, g) : launch_tri(k, yy, count, x.stream, D, p0);
+      for (int r = 0; r < reps; ++r) launch();
       (void)hipEventRecord(e1, x.stream);
       if ((rc = launch_check(x.err, "autotune"))) break;
       float ms = 0.f;
@@ -2871,7 +3271,12 @@ int autotune(const Ctx& x, Launch& out) {
         rc = fail(x.err, 4, "autotune timing failed");
         break;
       }
-      if (ms < best) { best = ms; if (gemm) bg = k; else bt = k; }
+      if (pair) {
+        if (ms < best_gemm + best) { bg = GV_XFFT; bt = TV_YSWEEP; }
+      } else if (ms < best) {
+        best = ms;
+        if (gemm) bg = k; else bt = k;
+      }
     }
     if (pres) { out.gemm_p = bg; out.tri_p = bt; } else { out.gemm_h = bg; out.tri_h = bt; }
   }
@@ -2902,7 +3307,7 @@ int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
     }
     // GMRES steps read the solver state back between launches: they run eagerly
     if (!steady || !h->use_graphs || x.gm) {
-      if (int rc = enqueue_step_kernels(x, h->parity, !h->has_history)) return rc;
+      if (int rc = enqueue_step_kernels(x, h->parity, !h->has_history, x.gm ? 0 : lean, h->spec == 2)) return rc;
     } else {
       if (h->graph_dt != dt) drop_graphs(h);
       if (!gs->valid)
@@ -2931,7 +3336,7 @@ void set_prm_once(mcf_handle* h, double dt) {
 // the process ends; the host reaches it only through the address hipGetSymbolAddress returns on
 // that device (the array's name in host code is a host-side stand-in, not a device address).
 constexpr size_t NA_DESC = 0;                      // SolveDesc, from word 0
-constexpr size_t NA_TABLE = 64;                    // the buffer table's one pointer
+constexpr size_t NA_TABLE = 128;                   // the buffer table's one pointer
 constexpr size_t NA_FIELD = 256;                   // the staged rectangle, 2 KiB in
 constexpr size_t NA_MAX_VALUES = (size_t)3 << 18;  // 786,432 values (6 MiB): up to an 800 x 820 rectangle
 static_assert(sizeof(SolveDesc) <= (NA_TABLE - NA_DESC) * sizeof(double), "descriptor region too small");
@@ -3022,6 +3427,141 @@ int status_error(mcf_handle* h, const double* d, int step) {
   return fail(h, 2, buf);
 }
 
+// The x-FFT form of a rectangle (kinds pressure: DCT-IV, v: DST-IV; xx even): the radix plan of the
+// complex FFT of length M = xx / 2 (radix 4 first, then the primes up to XF_MAXR), its twiddles, the
+// pre- and post-twiddles of the DCT-IV, and per x mode m the LU factors of the y system
+// T_m = (a + b mu_m) I + b ly, mu_m = -4 sin^2((m + 1/2) pi / (2 xx)) / dx^2.  T_m or -T_m is an
+// M-matrix whose row sums are sums of terms of one sign, so its pivots are formed from the row
+// sums (g_0 = s_0, g_j = s_j + c g_{j-1} / d_{j-1}, d_j = g_j + c_j) without the cancellation the
+// plain recurrence suffers when both ends are Neumann (ly singular).  Leaves xf = 0 where the form
+// does not apply.
+int build_xfft(mcf_handle* h, int kind) {
+  SolveDesc& s = h->hs[kind];
+  s.xf = 0;
+  const int yy = s.yy, xx = s.xx;
+  if (kind == MCF_U || (xx & 1) || xx / 2 > XF_MAXM || yy > YS_MAXY || yy < 2) return 0;
+  const int M = xx / 2;
+  std::vector<int> rad;
+  int m = M;
+  for (int p = 3; m > 1 && p <= XF_MAXR; p += 2)
+    while (m % p == 0) { rad.push_back(p); m /= p; }
+  if (m != 1 && (m & (m - 1))) return 0;   // a prime factor above XF_MAXR
+  // the largest odd radix first (its twiddles are all 1), then the powers of two as radix 4 (and 2)
+  std::sort(rad.begin(), rad.end(), std::greater<int>());
+  while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+  if (m == 2) rad.push_back(2);
+  std::vector<int> st;
+  std::vector<double> tw, root;
+  auto unit = [](std::vector<double>& v, double turns) {   // exp(-2 pi i turns)
+    v.push_back(std::cos(2.0 * M_PI * turns));
+    v.push_back(-std::sin(2.0 * M_PI * turns));
+  };
+  long L = 1;
+  for (int r : rad) {
+    st.push_back(r);
+    st.push_back((int)(tw.size() / 2));
+    st.push_back((int)(root.size() / 2));   // shifted past the twiddles below
+    for (int q = 1; q < r; ++q)
+      for (long j = 0; j < L; ++j) unit(tw, (double)(q * j) / (double)(r * L));
+    for (int e = 0; e < r; ++e) unit(root, (double)e / r);
+    L *= r;
+  }
+  const int ntw = (int)(tw.size() / 2);
+  for (size_t q = 2; q < st.size(); q += 3) st[q] += ntw;
+  tw.insert(tw.end(), root.begin(), root.end());
+  const int ftn = (int)(tw.size() / 2);
+  if ((int)rad.size() > XF_MAXS || ftn > xdct_tables(M) || xdct_lds(M, xdct_tables(M)) > XF_MAXLDS) return 0;
+  std::vector<double> pre(2 * M), post(2 * M);
+  const double scale = std::sqrt(2.0 / xx);
+  for (int n = 0; n < M; ++n) {
+    const double a = M_PI * (4.0 * n + 1.0) / (4.0 * xx), b = M_PI * n / xx;
+    pre[2 * n] = std::cos(a); pre[2 * n + 1] = -std::sin(a);
+    post[2 * n] = scale * std::cos(b); post[2 * n + 1] = -scale * std::sin(b);
+  }
+  // y systems: ly = tridiag(1, diagy, 1) / dy^2 with the kind's wall ends (lower, upper)
+  const double ends[3][2] = {{1.0, 1.0}, {3.0, 3.0}, {2.0, 2.0}};
+  const double dy2 = h->dy * h->dy, dx2 = h->dx * h->dx;
+  const double a = s.a, b = s.b, sgn = b > 0 ? -1.0 : 1.0, c = std::fabs(b) / dy2;
+  std::vector<double> yl((size_t)yy * xx), yi((size_t)yy * xx);
+  for (int mm = 0; mm < xx; ++mm) {
+    const double sx = std::sin((mm + 0.5) * M_PI / (2.0 * xx));
+    const double mu = -4.0 * sx * sx / dx2;
+    double g = 0.0, dprev = 0.0;
+    for (int j = 0; j < yy; ++j) {
+      double end = 0.0;
+      if (j == 0) end += (1.0 - ends[kind][0]) / dy2;
+      if (j == yy - 1) end += (1.0 - ends[kind][1]) / dy2;
+      const double sj = sgn * a + sgn * b * mu + sgn * b * end;   // each term >= 0
+      g = j == 0 ? sj : sj + c * g / dprev;
+      const double dA = g + (j < yy - 1 ? c : 0.0);
+      const double dT = sgn * dA;
+      yl[(size_t)j * xx + mm] = j == 0 ? 0.0 : (b / dy2) / (sgn * dprev);
+      yi[(size_t)j * xx + mm] = 1.0 / dT;
+      dprev = dA;
+    }
+  }
+  const std::string key = "xf" + std::to_string(kind);
+  int* dst;
+  double *dtw, *dpre, *dpost, *dyl, *dyi;
+  if (int rc = upload(h, key + "st", st.data(), st.size(), &dst)) return rc;
+  if (int rc = upload(h, key + "tw", tw.data(), tw.size(), &dtw)) return rc;
+  if (int rc = upload(h, key + "pre", pre.data(), pre.size(), &dpre)) return rc;
+  if (int rc = upload(h, key + "post", post.data(), post.size(), &dpost)) return rc;
+  if (int rc = upload(h, key + "yl", yl.data(), yl.size(), &dyl)) return rc;
+  if (int rc = upload(h, key + "yi", yi.data(), yi.size(), &dyi)) return rc;
+  s.fm = M; s.fns = (int)rad.size(); s.ftn = ftn; s.fst = dst; s.ftw = dtw; s.fpre = dpre; s.fpost = dpost;
+  s.yl = dyl; s.yi = dyi; s.ye = b / dy2;
+  s.xf = kind == MCF_PRESSURE ? 1 : 2;
+  return 0;
+}
+
+// The orthonormal x basis of an x-FFT problem at column x and mode m (its own inverse).
+double xbasis(int xf, int xx, int x, int m) {
+  const long long P = ((long long)(2 * x + 1) * (2 * m + 1)) % (8LL * xx);
+  const double t = M_PI * (double)P / (4.0 * xx);
+  return std::sqrt(2.0 / xx) * (xf == 1 ? std::cos(t) : std::sin(t));
+}
+
+// The capacitance tables of the x-FFT form: the basis at the points' distinct x columns and at
+// the capacitance row groups' columns, and the entries (y index, column group) of each row.
+int build_xfft_schur(mcf_handle* h, int kind, const std::vector<int>& pt_x, const std::vector<int>& rg_x,
+                     const std::vector<int>& rg_ptr, const std::vector<int>& rc_j) {
+  SolveDesc& s = h->hs[kind];
+  if (!s.xf) return 0;
+  const int xx = s.xx, yy = s.yy, npts = (int)pt_x.size(), r = (int)rc_j.size();
+  std::vector<int> pc, pt_c(std::max(npts, 1), 0);
+  for (int t = 0; t < npts; ++t) {
+    if (pc.empty() || pc.back() != pt_x[t]) pc.push_back(pt_x[t]);
+    pt_c[t] = (int)pc.size() - 1;
+  }
+  std::vector<double> wp(std::max<size_t>(pc.size() * xx, 1)), wf(std::max<size_t>(rg_x.size() * xx, 1));
+  for (size_t c = 0; c < pc.size(); ++c)
+    for (int m = 0; m < xx; ++m) wp[c * xx + m] = xbasis(s.xf, xx, pc[c], m);
+  for (size_t g = 0; g < rg_x.size(); ++g)
+    for (int m = 0; m < xx; ++m) wf[g * xx + m] = xbasis(s.xf, xx, rg_x[g], m);
+  std::vector<int> cnt(yy + 1, 0), grp(std::max(r, 1), 0);
+  for (size_t g = 0; g + 1 < rg_ptr.size(); ++g)
+    for (int t = rg_ptr[g]; t < rg_ptr[g + 1]; ++t) grp[t] = (int)g;
+  for (int t = 0; t < r; ++t) cnt[rc_j[t] + 1]++;
+  for (int j = 0; j < yy; ++j) cnt[j + 1] += cnt[j];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1), et(std::max(r, 1), 0), eg(std::max(r, 1), 0);
+  for (int t = 0; t < r; ++t) {
+    const int e = fill[rc_j[t]]++;
+    et[e] = t; eg[e] = grp[t];
+  }
+  const std::string key = "xs" + std::to_string(kind);
+  int *dpc, *drp, *det, *deg;
+  double *dwp, *dwf;
+  if (int rc = upload(h, key + "pc", pt_c.data(), pt_c.size(), &dpc)) return rc;
+  if (int rc = upload(h, key + "wp", wp.data(), wp.size(), &dwp)) return rc;
+  if (int rc = upload(h, key + "wf", wf.data(), wf.size(), &dwf)) return rc;
+  if (int rc = upload(h, key + "rp", cnt.data(), cnt.size(), &drp)) return rc;
+  if (int rc = upload(h, key + "et", et.data(), et.size(), &det)) return rc;
+  if (int rc = upload(h, key + "eg", eg.data(), eg.size(), &deg)) return rc;
+  s.pt_c = dpc; s.xwp = dwp; s.xwf = dwf; s.yr_ptr = drp; s.yr_t = det; s.yr_g = deg;
+  return 0;
+}
+
 // The rectangle inverse of kind on host rhs -> host out, staged in `field` (yy*xx values) with its
 // one-entry buffer table `tb` and descriptor `td` in device memory the caller provides.  Copies and
 // launches are ordered on the handle's stream, and the call returns after it drains.
@@ -3037,7 +3577,7 @@ int rect_inverse_on(mcf_handle* h, int kind, const double* rhs, double* out, dou
   GemmArgs g{1, 0, 0, 0, -1, 0};
   launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, g);
   PcrArgs p0{0, -1, 0};
-  launch_tri(tv, t.yy, 1, h->stream, td, p0);
+  launch_tri(tv, t.yy, t.xx, 1, h->stream, td, p0);
   GemmArgs gi{0, 0, 0, 0, -1, 0};
   launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, gi);
   int rc = launch_check(h, "rect inverse");
@@ -3287,6 +3827,7 @@ int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double 
   else if (kind == MCF_U) { s.off = 1; s.ld = h->nx + 1; s.joff = 0; s.ioff = 1; s.field = MCF_FIELD_U; }
   else { s.off = h->usize + h->nx; s.ld = h->nx; s.joff = 1; s.ioff = 0; s.field = MCF_FIELD_V; }
   s.lo = h->hc.lo[s.field]; s.hi = h->hc.hi[s.field];
+  if (int rc = build_xfft(h, kind)) return rc;
   h->has_rect[kind] = true;
   return sync_descs(h);
 }
@@ -3350,6 +3891,7 @@ int mcf_set_schur(mcf_handle* h, int kind, int r, const int* rows, int npts, con
   s.rg_n = (int)rg_x.size(); s.rg_ptr = drp; s.rg_x = drx; s.gpos = dgp;
   h->rc_j[kind] = drj;
   if (int rc = build_vsel(h, kind)) return rc;
+  if (int rc = build_xfft_schur(h, kind, pt_x, rg_x, rg_ptr, rc_j)) return rc;
   // a new capacitance set invalidates the Schur system and any pair dump built on the old one
   s.Sth = nullptr; s.Cptr = nullptr; s.Ccol = nullptr; s.Cval = nullptr; s.apos = nullptr;
   s.dump_c = s.dump_y = nullptr; s.dump_tag = s.dump_count = nullptr; s.dump_cap = 0;
@@ -3578,7 +4120,7 @@ int mcf_run(mcf_handle* h, int n, double dt, double* diag) {
   h->owned.erase("p_raw_valid");
   // no speculation while pairs are dumped (a replayed chunk would record its pairs twice) or on the
   // GMRES backend (its steps have no lean form)
-  const bool spec = h->spec > 0 && h->use_graphs && h->hs[0].dump_cap == 0 && h->pbackend == MCF_PB_DIRECT;
+  const bool spec = h->spec > 0 && h->hs[0].dump_cap == 0 && h->pbackend == MCF_PB_DIRECT;
   std::vector<double> d((size_t)n * MCF_DIAG_LEN);
   for (int done = 0; done < n;) {
     const int lean = spec_mask(spec, h->spec_full, h->spec_vfull, h->spec_p2);
@@ -3887,7 +4429,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   // one untimed step settles the history so that the timed steps replay the captured graph; with
   // speculation on, the timed steps are the lean ones (a step that would have refined is counted
   // in out5[4], not replayed: the state is not used after profiling)
-  const int lean = h->spec > 0 && h->use_graphs && h->pbackend == MCF_PB_DIRECT ? LEAN_ALL : 0;
+  const int lean = h->spec > 0 && h->pbackend == MCF_PB_DIRECT ? LEAN_ALL : 0;
   if (int rc = enqueue_steps(h, 1, dt, lean)) return rc;
   HIPCHK(h, hipEventRecord(e0, h->stream));
   if (int rc = enqueue_steps(h, n, dt, lean)) return rc;
@@ -3909,7 +4451,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   PcrArgs p0{0, -1, 0};
   HIPCHK(h, hipEventRecord(e0, h->stream));
   for (int r = 0; r < reps; ++r)
-    launch_tri(h->ln.tri_p, p.yy, 1, h->stream, h->ds, p0);
+    launch_tri(h->ln.tri_p, p.yy, p.xx, 1, h->stream, h->ds, p0);
   HIPCHK(h, hipEventRecord(e1, h->stream));
   HIPCHK(h, hipEventSynchronize(e1));
   HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
@@ -3927,14 +4469,14 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
 }
 
 int mcf_set_launch(mcf_handle* h, int gemm, int tri) {
-  if (!h || gemm < -1 || gemm >= N_GEMM_V || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
+  if (!h || gemm < -1 || gemm > GV_XFFT || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
     return fail(h, 1, "invalid launch shape");
   if (gemm < 0) {
     h->ln = Launch();
     h->tuned = false;
   } else {
-    h->ln.gemm_p = h->ln.gemm_h = gemm;
-    h->ln.tri_p = h->ln.tri_h = tri;
+    const Ctx x = ctx_of(h);
+    pin_launch(h->ln, gemm, tri, !x.noxf_p, !x.noxf_h);
     h->tuned = true;
   }
   drop_graphs(h);
@@ -4122,7 +4664,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
   const Ctx& x = b->ctx;
   // speculative chunks over the batch, as mcf_run: lean steady steps in chunks of growing size, a
   // chunk replayed with the refinements from the checkpoint if any step of any case would refine
-  const bool spec = b->spec > 0 && e->use_graphs;
+  const bool spec = b->spec > 0;
   struct Host { int parity; double prev_dt, time; bool hist; };
   std::vector<Host> saved(n);
   auto enqueue = [&](int k, int lean) -> int {
@@ -4140,7 +4682,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       const bool first = !h0->has_history;
       hipLaunchKernelGGL(k_set_prm, dim3(1, n), dim3(64), 0, b->stream, b->dc, -1.0, first ? 0.0 : 0.5, first ? 1.0 : 0.5);
       if (first || !e->use_graphs) {
-        if (int rc = enqueue_step_kernels(x, h0->parity, first)) return rc;
+        if (int rc = enqueue_step_kernels(x, h0->parity, first, lean, b->spec == 2)) return rc;
       } else {
         if (!gs.valid) if (int rc = gs.build(x, lean, b->spec == 2)) return rc;
         HIPCHK(e, hipGraphLaunch(gs.exec[h0->parity], b->stream));
@@ -4218,14 +4760,13 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
 int mcf_batch_set_launch(mcf_batch* b, int gemm, int tri) {
   if (!b) return 1;
   mcf_handle* e = b->cases[0];
-  if (gemm < -1 || gemm >= N_GEMM_V || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
+  if (gemm < -1 || gemm > GV_XFFT || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
     return fail(e, 1, "invalid launch shape");
   if (gemm < 0) {
     b->ln = Launch();
     b->tuned = false;
   } else {
-    b->ln.gemm_p = b->ln.gemm_h = gemm;
-    b->ln.tri_p = b->ln.tri_h = tri;
+    pin_launch(b->ln, gemm, tri, !b->ctx.noxf_p, !b->ctx.noxf_h);
     b->tuned = true;
   }
   b->ctx.ln = b->ln;
