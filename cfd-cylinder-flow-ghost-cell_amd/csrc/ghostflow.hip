@@ -170,15 +170,15 @@ struct SolveDesc {
   int* dump_count;
   int dump_cap;
   const int* clock;     // the case's device step counter
-  // x-FFT form of the rectangle inverse: the x operator diagonalised by its orthonormal DCT-IV
-  // (xf 1) or DST-IV (xf 2) basis through a complex FFT of length M = xx / 2, and one tridiagonal
-  // solve along y per x mode m; xf 0: not available for this problem
+  // x-FFT form of the rectangle inverse: the x operator diagonalised by its DCT-IV (xf 1, pressure),
+  // DST-IV (xf 2, v) or mirrored-outlet sine basis (xf 3, u) through a complex FFT of length
+  // M = xx / 2, and one tridiagonal solve along y per x mode m; xf 0: not available for this problem
   int xf, fm, fns, ftn;
   const int* fst;       // 3 per radix stage: radix r, offsets of its twiddles and of its r roots in ftw
   const double* ftw;    // ftn complex: the stage twiddles w_{rL}^{q j} (q = 1..r-1, j < L) at a stage's
                         // offset + (q-1) L + j, then the roots w_r^e (e < r) of every stage
-  const double* fpre;   // M complex: exp(-i pi (4n+1) / (4 xx))
-  const double* fpost;  // M complex: sqrt(2 / xx) exp(-i pi k / xx)
+  const double* fpre;   // M complex: exp(-i pi (4n+1) / (4 xx)); u: xx complex exp(-i pi k / (2 xx))
+  const double* fpost;  // M complex: sqrt(2 / xx) exp(-i pi k / xx); u: exp(-2 pi i k / xx)
   const double* yl;     // yy*xx, [j][m]: multiplier of row j of mode m's y system
   const double* yi;     // yy*xx, [j][m]: inverse pivot
   double ye;            // the y systems' off-diagonal b / dy^2
@@ -188,6 +188,8 @@ struct SolveDesc {
   const int* yr_ptr;    // yy+1: capacitance entries of each rectangle row j
   const int* yr_t;      // r: their y index (group order), by row
   const int* yr_g;      // r: their column group (row of xwf)
+  int yr_lo, yr_n;      // the band of rows yr_lo .. yr_lo + yr_n - 1 holding capacitance entries
+  double* xcb;          // yr_n x xx: the correction's right-hand side c on the band rows
 };
 
 DEV int stripe_id() { return (blockIdx.x + 7 * blockIdx.y) & (NSTRIPE - 1); }
@@ -1374,14 +1376,23 @@ inline size_t xdct_lds(int M, int t) { return (size_t)(2 * M + t) * sizeof(cplx)
 inline int xdct_tables(int M) { return 2 * M; }
 constexpr size_t XF_MAXLDS = 64 * 1024;
 
-// The orthonormal DCT-IV (xf 1) or DST-IV (xf 2) of each row, one block per row: forward from the
-// input array into F, inverse from F into the output array with the solid mask (epilogue as the
-// y transform's).  X = DCT-IV(x) comes from the M-point FFT of z_n = (x_{2n} + i x_{N-1-2n})
-// exp(-i pi (4n+1)/(4N)): with y_k = sqrt(2/N) exp(-i pi k/N) Z_k, X_{2k} = Re y_k and
-// X_{N-1-2k} = -Im y_k; DST-IV(x)_k = (-1)^k DCT-IV(x reversed)_k.  The basis is its own inverse.
+// The x transform of each row, one block per row: forward from the input array into F, inverse
+// from F into the output array with the solid mask (epilogue as the y transform's), through an
+// M-point complex FFT (M = N / 2).
+//   xf 1, 2: the orthonormal DCT-IV / DST-IV, its own inverse.  X = DCT-IV(x) from the FFT of
+//     z_n = (x_{2n} + i x_{N-1-2n}) exp(-i pi (4n+1)/(4N)): with y_k = sqrt(2/N) exp(-i pi k/N) Z_k,
+//     X_{2k} = Re y_k, X_{N-1-2k} = -Im y_k; DST-IV(x)_k = (-1)^k DCT-IV(x reversed)_k.
+//   xf 3 (u faces 1..N, mirrored outlet): basis W[i][m] = sin(pi (2m+1)(i+1) / (2N)); forward
+//     H = W^{-1} x = (1/N) (-1)^k DCT-III(x reversed)_k, inverse x_i = (1/2) DCT-II((-1)^m H_m)_{N-1-i}
+//     (unnormalised DCT-II / DCT-III).  DCT-II of a real row: v_t = x_{2t}, v_{N-1-t} = x_{2t+1},
+//     Z = FFT(v_{2n} + i v_{2n+1}), V_k = E_k + e^{-2 pi i k/N} O_k, V_{k+M} = E_k - e^{-2 pi i k/N} O_k
+//     with E, O the even / odd parts of Z, and DCT-II_k = 2 Re(V_k e^{-i pi k/(2N)}); DCT-III runs the
+//     same steps backwards (an inverse FFT as the conjugate of the FFT of the conjugate).
+//     Tables: fpre = e^{-i pi k/(2N)} (k < N), fpost = e^{-2 pi i k/N} (k < M).
 // The row, the pre- and post-twiddles and the stage tables are loaded in one batch at the start.
 constexpr int XF_U = XF_MAXM / 256;   // points per thread at most
 constexpr int XF_MAXS = 16;           // radix stages at most
+DEV int vsrc(int t, int N) { return t < (N >> 1) ? 2 * t : 2 * (N - 1 - t) + 1; }   // v_t = x_{vsrc(t)}
 template <int U>
 __global__ __launch_bounds__(256) void k_xdct(const SolveDesc* __restrict__ descs, GemmArgs g) {
   const SolveDesc& d = descs[blockIdx.y];
@@ -1390,22 +1401,31 @@ __global__ __launch_bounds__(256) void k_xdct(const SolveDesc* __restrict__ desc
   if (g.gate && *g.gate == 0) return;
   if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
   extern __shared__ cplx xs[];
-  const int N = d.xx, M = N >> 1, T = d.ftn, tid = threadIdx.x;
+  __shared__ int stp[3 * XF_MAXS];
+  const int N = d.xx, M = N >> 1, T = d.ftn, tid = threadIdx.x, xf = d.xf;
   cplx* b0 = xs;
   cplx* b1 = xs + M;
   cplx* tabs = xs + 2 * M;   // twiddles of every stage, then the roots
-  const bool sine = d.xf == 2;
   const gptr<const double> src = g.forward ? desc_in(d, g.in_id) + (size_t)j * d.ld : G(d.F) + (size_t)j * N;
   const gptr<const double> pre = G(d.fpre), post = G(d.fpost), tab = G(d.ftw);
-  __shared__ int stp[3 * XF_MAXS];
-  // tables hold fewer than 2M entries: 2U per thread cover them
-  double lo[U], hi[U], pr[U], pi[U], qr[U], qi[U], tr[2 * U], ti[2 * U];
+  // batch: four row values, three complex twiddles per point, the stage tables
+  double a0[U], a1[U], a2[U], a3[U], p0r[U], p0i[U], p1r[U], p1i[U], qr[U], qi[U], tr[2 * U], ti[2 * U];
   #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int n = min(tid + 256 * u, M - 1);
-    lo[u] = src[2 * n]; hi[u] = src[N - 1 - 2 * n];
-    pr[u] = pre[2 * n]; pi[u] = pre[2 * n + 1];
-    qr[u] = post[2 * n]; qi[u] = post[2 * n + 1];
+    const int k = min(tid + 256 * u, M - 1);
+    if (xf != 3) {
+      a0[u] = src[2 * k]; a1[u] = src[N - 1 - 2 * k]; a2[u] = 0.0; a3[u] = 0.0;
+      p0r[u] = pre[2 * k]; p0i[u] = pre[2 * k + 1]; p1r[u] = 0.0; p1i[u] = 0.0;
+    } else if (g.forward) {
+      // X'_k = x_{N-1-k}, X'_{N-k} = x_{k-1}, X'_{k+M} = x_{M-1-k}, X'_{M-k} = x_{M-1+k}
+      a0[u] = src[N - 1 - k]; a1[u] = src[max(k - 1, 0)]; a2[u] = src[M - 1 - k]; a3[u] = src[M - 1 + k];
+      p0r[u] = pre[2 * k]; p0i[u] = pre[2 * k + 1]; p1r[u] = pre[2 * (k + M)]; p1i[u] = pre[2 * (k + M) + 1];
+    } else {
+      // v_{2k} and v_{2k+1} of the sign-flipped coefficients
+      a0[u] = src[vsrc(2 * k, N)]; a1[u] = src[vsrc(2 * k + 1, N)]; a2[u] = 0.0; a3[u] = 0.0;
+      p0r[u] = pre[2 * k]; p0i[u] = pre[2 * k + 1]; p1r[u] = pre[2 * (k + M)]; p1i[u] = pre[2 * (k + M) + 1];
+    }
+    qr[u] = post[2 * k]; qi[u] = post[2 * k + 1];
   }
   #pragma unroll
   for (int u = 0; u < 2 * U; ++u) {
@@ -1415,10 +1435,25 @@ __global__ __launch_bounds__(256) void k_xdct(const SolveDesc* __restrict__ desc
   const int sv = G(d.fst)[min(tid, 3 * d.fns - 1)];
   #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int n = tid + 256 * u;
-    if (n < M) {
-      const cplx z = sine ? make_double2(hi[u], lo[u]) : make_double2(lo[u], hi[u]);
-      b0[n] = cmul(z, make_double2(pr[u], pi[u]));
+    const int k = tid + 256 * u;
+    if (k >= M) continue;
+    if (xf == 1) {
+      b0[k] = cmul(make_double2(a0[u], a1[u]), make_double2(p0r[u], p0i[u]));
+    } else if (xf == 2) {
+      b0[k] = cmul(make_double2(a1[u], a0[u]), make_double2(p0r[u], p0i[u]));
+    } else if (g.forward) {
+      // V_k = (X'_k - i X'_{N-k}) / 2 e^{i pi k/(2N)}, V_{k+M} likewise; Z_k = E + i O with
+      // O = (V_k - V_{k+M}) / 2 e^{2 pi i k/N}; the FFT runs on conj(Z)
+      const double xnk = k == 0 ? 0.0 : a1[u];
+      const cplx vk = cmul(make_double2(0.5 * a0[u], -0.5 * xnk), make_double2(p0r[u], -p0i[u]));
+      const cplx vm = cmul(make_double2(0.5 * a2[u], -0.5 * a3[u]), make_double2(p1r[u], -p1i[u]));
+      const cplx e = make_double2(0.5 * (vk.x + vm.x), 0.5 * (vk.y + vm.y));
+      const cplx o = cmul(make_double2(0.5 * (vk.x - vm.x), 0.5 * (vk.y - vm.y)), make_double2(qr[u], -qi[u]));
+      b0[k] = make_double2(e.x - o.y, -(e.y + o.x));
+    } else {
+      // H~_m = (-1)^m H_m: v_{2k} = H~_{vsrc(2k)}, v_{2k+1} = H~_{vsrc(2k+1)}
+      const int s0 = vsrc(2 * k, N), s1 = vsrc(2 * k + 1, N);
+      b0[k] = make_double2((s0 & 1) ? -a0[u] : a0[u], (s1 & 1) ? -a1[u] : a1[u]);
     }
   }
   #pragma unroll
@@ -1448,30 +1483,44 @@ __global__ __launch_bounds__(256) void k_xdct(const SolveDesc* __restrict__ desc
     cplx* t = in; in = out; out = t;
     L *= r;
   }
+  const gptr<double> Fr = G(d.F) + (size_t)j * N;
+  const gptr<double> O = desc_out(d, g.out_id) + (size_t)j * d.ld;
+  const int aj = j + d.joff;
+  const int slo = (!g.forward && d.lo) ? G(d.lo)[aj] : 1, shi = (!g.forward && d.lo) ? G(d.hi)[aj] : 0;
+  auto put = [&](int i, double v) {
+    if (g.forward) { Fr[i] = v; return; }
+    const int ai = i + d.ioff;
+    if (g.epi == 0) O[i] = v;
+    else if (slo <= ai && ai <= shi) O[i] = 0.0;
+    else O[i] = (g.epi == 2) ? (O[i] + v) : v;
+  };
   #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int k = tid + 256 * u;
     if (k >= M) continue;
-    const cplx y = cmul(in[k], make_double2(qr[u], qi[u]));
-    // DST-IV: (-1)^i at i = 2k is +1, at i = N-1-2k (odd) -1
-    const double even = y.x, odd = sine ? y.y : -y.y;
-    if (g.forward) {
-      const gptr<double> Fr = G(d.F) + (size_t)j * N;
-      Fr[2 * k] = even;
-      Fr[N - 1 - 2 * k] = odd;
+    if (xf != 3) {
+      const cplx y = cmul(in[k], make_double2(qr[u], qi[u]));
+      // DST-IV: (-1)^i at i = 2k is +1, at i = N-1-2k (odd) -1
+      put(2 * k, y.x);
+      put(N - 1 - 2 * k, xf == 2 ? y.y : -y.y);
+    } else if (g.forward) {
+      // z = conj(FFT(conj Z)) / M: v_{2k} = Re / M, v_{2k+1} = -Im / M; H_q = 2 (-1)^q v_t at q = vsrc(t)
+      const cplx rv = in[k];
+      const int q0 = vsrc(2 * k, N), q1 = vsrc(2 * k + 1, N);
+      const double s = 2.0 / M;
+      put(q0, (q0 & 1) ? -s * rv.x : s * rv.x);
+      put(q1, (q1 & 1) ? s * rv.y : -s * rv.y);
     } else {
-      const gptr<double> O = desc_out(d, g.out_id) + (size_t)j * d.ld;
-      const int aj = j + d.joff;
-      const int slo = d.lo ? G(d.lo)[aj] : 1, shi = d.lo ? G(d.hi)[aj] : 0;
-      #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int i = h ? N - 1 - 2 * k : 2 * k;
-        const double v = h ? odd : even;
-        const int ai = i + d.ioff;
-        if (g.epi == 0) O[i] = v;
-        else if (slo <= ai && ai <= shi) O[i] = 0.0;
-        else O[i] = (g.epi == 2) ? (O[i] + v) : v;
-      }
+      // V_k = E + e^{-2 pi i k/N} O, V_{k+M} = E - e^{-2 pi i k/N} O; x_{N-1-k} = Re(V_k e^{-i pi k/(2N)})
+      const cplx zk = in[k], zc = in[(M - k) % M];
+      const cplx zcc = make_double2(zc.x, -zc.y);
+      const cplx e = make_double2(0.5 * (zk.x + zcc.x), 0.5 * (zk.y + zcc.y));
+      const cplx dd = make_double2(0.5 * (zk.x - zcc.x), 0.5 * (zk.y - zcc.y));
+      const cplx o = make_double2(dd.y, -dd.x);   // (Z_k - conj Z_{M-k}) / (2i)
+      const cplx wo = cmul(make_double2(qr[u], qi[u]), o);
+      const cplx vk = cadd(e, wo), vm = csub(e, wo);
+      put(N - 1 - k, vk.x * p0r[u] - vk.y * p0i[u]);
+      put(M - 1 - k, vm.x * p1r[u] - vm.y * p1i[u]);
     }
   }
 }
@@ -1512,25 +1561,15 @@ __global__ __launch_bounds__(256) void k_ysweep(const SolveDesc* __restrict__ de
   }
   __shared__ double sa[NSEG][MB], sb[NSEG][MB], carry[NSEG][MB];
   if (p.mode == 1) {
-    // the correction's right-hand side c[j][m] = sum over row j's capacitance entries of
-    // W[x_e][m] y_e: the row pointers and the entries (column group, y value) go to LDS once
-    __shared__ int rps[YS_MAXY + 1];
-    __shared__ int eg[2048];
-    __shared__ double ey[2048];
-    const gptr<const int> rp = G(d.yr_ptr), rt = G(d.yr_t), rg = G(d.yr_g);
-    const gptr<const double> wf = G(d.xwf), yv = G(d.y);
-    const int ne = d.r;
-    for (int t = tid; t <= yy; t += 256) rps[t] = rp[t];
-    for (int t = tid; t < ne; t += 256) { eg[t] = rg[t]; ey[t] = yv[rt[t]]; }
-    __syncthreads();
+    // the correction's right-hand side c (k_xcap), non-zero on the band rows only
+    const gptr<const double> cb = G(d.xcb);
+    const int lo = d.yr_lo, nbnd = d.yr_n;
     #pragma unroll
-    for (int c = 0; c < C; ++c)
-      if (c < n) {
-        double acc = 0.0;
-        const int e1 = rps[j0 + c + 1];
-        for (int t = rps[j0 + c]; t < e1; ++t) acc += wf[(size_t)eg[t] * xx + m] * ey[t];
-        r[c] = acc;
-      }
+    for (int c = 0; c < C; ++c) {
+      const int b = min(max(j0 + c - lo, 0), nbnd - 1);
+      const double v = cb[(size_t)b * xx + m];
+      r[c] = (j0 + c >= lo && j0 + c < lo + nbnd) ? v : 0.0;
+    }
   }
   // forward: z_j = r_j - l_j z_{j-1}
   double A = 1.0, B = 0.0;
@@ -1615,6 +1654,77 @@ __global__ __launch_bounds__(256) void k_xpoints(const SolveDesc* __restrict__ d
   }
   s = wave_sum(s);
   if (lane == 0) G(d.firstK)[q] = s;
+}
+
+// y = (S^{-1} C) first and the x transform of the correction E y on the band rows, one block per
+// band row j: each entry of the row (a capacitance row of that y row) gets its multiplier as
+// k_capacitance forms it (one wave per entry, the same sum order), kept in LDS and stored, and the
+// block then writes c[j][m] = sum over the row's entries of Wf[x_e][m] y_e for every x mode m.
+constexpr int XC_MAXE = 256;   // capacitance entries of one y row at most
+template <int UM>
+__global__ __launch_bounds__(256) void k_xcap(const SolveDesc* __restrict__ descs, PcrArgs p) {
+  const SolveDesc& d = descs[blockIdx.y];
+  const int b = blockIdx.x;
+  if (b >= d.yr_n) return;
+  if (p.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + p.slot_shift, p.pred_slot)) return;
+  __shared__ double ys[XC_MAXE];
+  __shared__ int gs[XC_MAXE];
+  const int j = d.yr_lo + b, lane = threadIdx.x & 63, wv = threadIdx.x >> 6, xx = d.xx, np = d.npts;
+  const gptr<const int> rp = G(d.yr_ptr), rt = G(d.yr_t), rg = G(d.yr_g);
+  const int e0 = rp[j], ne = rp[j + 1] - e0;
+  const gptr<const double> fk = G(d.firstK);
+  constexpr int CAP_U = 16;
+  for (int e = wv; e < ne; e += 4) {
+    const int t = rt[e0 + e];
+    const gptr<const double> Sr = G(d.SC) + (size_t)t * np;
+    double s = 0.0;
+    for (int q0 = lane; q0 < np; q0 += 64 * CAP_U) {
+      double a[CAP_U], c[CAP_U];
+      #pragma unroll
+      for (int u = 0; u < CAP_U; ++u) {
+        const int q = min(q0 + 64 * u, np - 1);
+        a[u] = Sr[q];
+        c[u] = fk[q];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      #pragma unroll
+      for (int u = 0; u < CAP_U; ++u)
+        if (q0 + 64 * u < np) s += a[u] * c[u];
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+      ys[e] = s;
+      gs[e] = rg[e0 + e];
+      G(d.y)[t] = s;
+    }
+  }
+  __syncthreads();
+  // c[j][m] for the UM modes m = tid + 256 u of this thread: the entries' basis values load in
+  // chunks of four entries x UM modes, entries added in row order
+  const gptr<const double> wf = G(d.xwf);
+  const gptr<double> out = G(d.xcb) + (size_t)b * xx;
+  double acc[UM];
+  #pragma unroll
+  for (int u = 0; u < UM; ++u) acc[u] = 0.0;
+  for (int e = 0; e < ne; e += 4) {
+    double w[4][UM];
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t row = (size_t)gs[min(e + k, ne - 1)] * xx;
+      #pragma unroll
+      for (int u = 0; u < UM; ++u) w[k][u] = wf[row + min((int)threadIdx.x + 256 * u, xx - 1)];
+    }
+    #pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (e + k < ne) {
+        const double yk = ys[e + k];
+        #pragma unroll
+        for (int u = 0; u < UM; ++u) acc[u] += w[k][u] * yk;
+      }
+  }
+  #pragma unroll
+  for (int u = 0; u < UM; ++u)
+    if ((int)threadIdx.x + 256 * u < xx) out[threadIdx.x + 256 * u] = acc[u];
 }
 
 // Records the pair of a first pressure Schur solve: rhs = C first (first = R^{-1} b at the
@@ -2784,6 +2894,7 @@ struct Ctx {
   int maxvel, maxp, maxfill, maxsurf, maxny;
   bool pert;             // some case perturbs its inflow: each step sets its inlet first (k_inlet)
   bool noxf_p, noxf_h;   // some pressure / velocity problem has no x-FFT form
+  int p_band, h_band;    // most band rows of a problem's capacitance entries (x-FFT form)
   int p_yy, p_xx, p_np, p_r, h_yy, h_xx, h_np, h_r;
 };
 
@@ -2796,6 +2907,8 @@ void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
   x.pert = x.pert || c.pert_T > 0.0;
   x.noxf_p = x.noxf_p || !hs3[0].xf;
   x.noxf_h = x.noxf_h || !hs3[1].xf || !hs3[2].xf;
+  x.p_band = std::max(x.p_band, hs3[0].yr_n);
+  x.h_band = std::max({x.h_band, hs3[1].yr_n, hs3[2].yr_n});
   x.p_yy = std::max(x.p_yy, hs3[0].yy); x.p_xx = std::max(x.p_xx, hs3[0].xx);
   x.p_np = std::max(x.p_np, hs3[0].npts); x.p_r = std::max(x.p_r, hs3[0].r);
   for (int q = 1; q < 3; ++q) {
@@ -2840,6 +2953,22 @@ inline dim3 gridAdv(int n, int cases) {
 }
 
 // One Woodbury apply (in -> out): epi 1 store masked, 2 accumulate masked.
+// The capacitance points and multipliers of a solve (the x-FFT form's own kernels for GV_XFFT)
+void launch_points_cap(const Ctx& x, bool pres, int gv, const SolveDesc* D, int count, int xx, int np, int r,
+                       PcrArgs p0) {
+  if (gv == GV_XFFT) {
+    hipLaunchKernelGGL(k_xpoints, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+    const dim3 gb(pres ? x.p_band : x.h_band, count);
+    if (xx <= 512) hipLaunchKernelGGL(k_xcap<2>, gb, dim3(256), 0, x.stream, D, p0);
+    else if (xx <= 1024) hipLaunchKernelGGL(k_xcap<4>, gb, dim3(256), 0, x.stream, D, p0);
+    else if (xx <= 2048) hipLaunchKernelGGL(k_xcap<8>, gb, dim3(256), 0, x.stream, D, p0);
+    else hipLaunchKernelGGL(k_xcap<16>, gb, dim3(256), 0, x.stream, D, p0);
+  } else {
+    hipLaunchKernelGGL(k_points, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+    hipLaunchKernelGGL(k_capacitance, dim3((r + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+  }
+}
+
 int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int pred_slot, int slot_shift) {
   const SolveDesc* D = pres ? x.dsp : x.dsh;
   const int count = pres ? x.n : 2 * x.n;
@@ -2851,9 +2980,7 @@ int enqueue_apply(const Ctx& x, bool pres, int in_id, int out_id, int epi, int p
   PcrArgs p0{0, pred_slot, slot_shift};
   launch_tri(tv, yy, xx, count, x.stream, D, p0);
   if (r > 0) {
-    if (gv == GV_XFFT) hipLaunchKernelGGL(k_xpoints, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
-    else hipLaunchKernelGGL(k_points, dim3((np + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
-    hipLaunchKernelGGL(k_capacitance, dim3((r + 3) / 4, count), dim3(256), 0, x.stream, D, p0);
+    launch_points_cap(x, pres, gv, D, count, xx, np, r, p0);
     if (x.dump && pres && pred_slot < 0)
       hipLaunchKernelGGL(k_dump, dim3(1, count), dim3(256), 0, x.stream, D, slot_shift / 8);
     PcrArgs p1{1, pred_slot, slot_shift};
@@ -3240,6 +3367,7 @@ int autotune(const Ctx& x, Launch& out) {
     const int yy = pres ? x.p_yy : x.h_yy, xx = pres ? x.p_xx : x.h_xx;
     const GemmArgs g{1, pres ? B_DIVP : B_RHS, 0, 0, -1, 0};
     const PcrArgs p0{0, -1, 0};
+    const int np = pres ? x.p_np : x.h_np, r = pres ? x.p_r : x.h_r;
     float best = 1e30f, best_gemm = 1e30f;
     int bg = 0, bt = 1;
     // the dense y transforms and the x sweeps, then (where every problem of the class has it) the
@@ -3255,6 +3383,7 @@ int autotune(const Ctx& x, Launch& out) {
         if (pair) {
           launch_gemm(GV_XFFT, yy, xx, count, x.stream, D, g);
           launch_tri(TV_YSWEEP, yy, xx, count, x.stream, D, p0);
+          if (r > 0) launch_points_cap(x, pres, GV_XFFT, D, count, xx, np, r, p0);
         } else if (gemm) {
           launch_gemm(k, yy, xx, count, x.stream, D, g);
         } else {
@@ -3272,7 +3401,21 @@ int autotune(const Ctx& x, Launch& out) {
         break;
       }
       if (pair) {
-        if (ms < best_gemm + best) { bg = GV_XFFT; bt = TV_YSWEEP; }
+        // against the best dense transform and sweep with their capacitance launches
+        float dense = best_gemm + best;
+        if (r > 0) {
+          for (int q = 0; q < 2; ++q) launch_points_cap(x, pres, 0, D, count, xx, np, r, p0);
+          (void)hipEventRecord(e0, x.stream);
+          for (int q = 0; q < reps; ++q) launch_points_cap(x, pres, 0, D, count, xx, np, r, p0);
+          (void)hipEventRecord(e1, x.stream);
+          float mc = 0.f;
+          if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&mc, e0, e1) != hipSuccess) {
+            rc = fail(x.err, 4, "autotune timing failed");
+            break;
+          }
+          dense += mc;
+        }
+        if (ms < dense) { bg = GV_XFFT; bt = TV_YSWEEP; }
       } else if (ms < best) {
         best = ms;
         if (gemm) bg = k; else bt = k;
@@ -3427,7 +3570,8 @@ int status_error(mcf_handle* h, const double* d, int step) {
   return fail(h, 2, buf);
 }
 
-// The x-FFT form of a rectangle (kinds pressure: DCT-IV, v: DST-IV; xx even): the radix plan of the
+// The x-FFT form of a rectangle (pressure: DCT-IV, v: DST-IV, u: mirrored-outlet sine basis; xx
+// even): the radix plan of the
 // complex FFT of length M = xx / 2 (radix 4 first, then the primes up to XF_MAXR), its twiddles, the
 // pre- and post-twiddles of the DCT-IV, and per x mode m the LU factors of the y system
 // T_m = (a + b mu_m) I + b ly, mu_m = -4 sin^2((m + 1/2) pi / (2 xx)) / dx^2.  T_m or -T_m is an
@@ -3439,7 +3583,7 @@ int build_xfft(mcf_handle* h, int kind) {
   SolveDesc& s = h->hs[kind];
   s.xf = 0;
   const int yy = s.yy, xx = s.xx;
-  if (kind == MCF_U || (xx & 1) || xx / 2 > XF_MAXM || yy > YS_MAXY || yy < 2) return 0;
+  if ((xx & 1) || xx / 2 > XF_MAXM || yy > YS_MAXY || yy < 2) return 0;
   const int M = xx / 2;
   std::vector<int> rad;
   int m = M;
@@ -3471,12 +3615,27 @@ int build_xfft(mcf_handle* h, int kind) {
   tw.insert(tw.end(), root.begin(), root.end());
   const int ftn = (int)(tw.size() / 2);
   if ((int)rad.size() > XF_MAXS || ftn > xdct_tables(M) || xdct_lds(M, xdct_tables(M)) > XF_MAXLDS) return 0;
-  std::vector<double> pre(2 * M), post(2 * M);
-  const double scale = std::sqrt(2.0 / xx);
-  for (int n = 0; n < M; ++n) {
-    const double a = M_PI * (4.0 * n + 1.0) / (4.0 * xx), b = M_PI * n / xx;
-    pre[2 * n] = std::cos(a); pre[2 * n + 1] = -std::sin(a);
-    post[2 * n] = scale * std::cos(b); post[2 * n + 1] = -scale * std::sin(b);
+  std::vector<double> pre, post(2 * M);
+  if (kind != MCF_U) {
+    // DCT-IV / DST-IV: exp(-i pi (4n+1) / (4N)) and sqrt(2/N) exp(-i pi k / N), n, k < M
+    pre.resize(2 * M);
+    const double scale = std::sqrt(2.0 / xx);
+    for (int n = 0; n < M; ++n) {
+      const double a = M_PI * (4.0 * n + 1.0) / (4.0 * xx), b = M_PI * n / xx;
+      pre[2 * n] = std::cos(a); pre[2 * n + 1] = -std::sin(a);
+      post[2 * n] = scale * std::cos(b); post[2 * n + 1] = -scale * std::sin(b);
+    }
+  } else {
+    // u: exp(-i pi k / (2N)) for k < N and exp(-2 pi i k / N) for k < M
+    pre.resize(2 * xx);
+    for (int k = 0; k < xx; ++k) {
+      const double a = M_PI * k / (2.0 * xx);
+      pre[2 * k] = std::cos(a); pre[2 * k + 1] = -std::sin(a);
+    }
+    for (int k = 0; k < M; ++k) {
+      const double b = 2.0 * M_PI * k / xx;
+      post[2 * k] = std::cos(b); post[2 * k + 1] = -std::sin(b);
+    }
   }
   // y systems: ly = tridiag(1, diagy, 1) / dy^2 with the kind's wall ends (lower, upper)
   const double ends[3][2] = {{1.0, 1.0}, {3.0, 3.0}, {2.0, 2.0}};
@@ -3511,12 +3670,20 @@ int build_xfft(mcf_handle* h, int kind) {
   if (int rc = upload(h, key + "yi", yi.data(), yi.size(), &dyi)) return rc;
   s.fm = M; s.fns = (int)rad.size(); s.ftn = ftn; s.fst = dst; s.ftw = dtw; s.fpre = dpre; s.fpost = dpost;
   s.yl = dyl; s.yi = dyi; s.ye = b / dy2;
-  s.xf = kind == MCF_PRESSURE ? 1 : 2;
+  s.xf = kind == MCF_PRESSURE ? 1 : kind == MCF_V ? 2 : 3;
   return 0;
 }
 
-// The orthonormal x basis of an x-FFT problem at column x and mode m (its own inverse).
-double xbasis(int xf, int xx, int x, int m) {
+// The x basis of an x-FFT problem at column x and mode m: the inverse transform's coefficient
+// W[x][m] (x_i = sum_m W[i][m] H_m), or with `forward` the forward one (H_m = sum_i Wf[i][m] x_i).
+// DCT-IV / DST-IV are orthonormal and symmetric (Wf = W); for u, W[i][m] = sin(pi (2m+1)(i+1) / (2N))
+// and Wf[i][m] = (2/N) W[i][m], halved at the outlet face i = N-1.
+double xbasis(int xf, int xx, int x, int m, bool forward) {
+  if (xf == 3) {
+    const long long P = ((long long)(2 * m + 1) * (x + 1)) % (4LL * xx);
+    const double w = std::sin(M_PI * (double)P / (2.0 * xx));
+    return forward ? (2.0 / xx) * (x == xx - 1 ? 0.5 : 1.0) * w : w;
+  }
   const long long P = ((long long)(2 * x + 1) * (2 * m + 1)) % (8LL * xx);
   const double t = M_PI * (double)P / (4.0 * xx);
   return std::sqrt(2.0 / xx) * (xf == 1 ? std::cos(t) : std::sin(t));
@@ -3536,9 +3703,9 @@ int build_xfft_schur(mcf_handle* h, int kind, const std::vector<int>& pt_x, cons
   }
   std::vector<double> wp(std::max<size_t>(pc.size() * xx, 1)), wf(std::max<size_t>(rg_x.size() * xx, 1));
   for (size_t c = 0; c < pc.size(); ++c)
-    for (int m = 0; m < xx; ++m) wp[c * xx + m] = xbasis(s.xf, xx, pc[c], m);
+    for (int m = 0; m < xx; ++m) wp[c * xx + m] = xbasis(s.xf, xx, pc[c], m, false);
   for (size_t g = 0; g < rg_x.size(); ++g)
-    for (int m = 0; m < xx; ++m) wf[g * xx + m] = xbasis(s.xf, xx, rg_x[g], m);
+    for (int m = 0; m < xx; ++m) wf[g * xx + m] = xbasis(s.xf, xx, rg_x[g], m, true);
   std::vector<int> cnt(yy + 1, 0), grp(std::max(r, 1), 0);
   for (size_t g = 0; g + 1 < rg_ptr.size(); ++g)
     for (int t = rg_ptr[g]; t < rg_ptr[g + 1]; ++t) grp[t] = (int)g;
@@ -3549,9 +3716,16 @@ int build_xfft_schur(mcf_handle* h, int kind, const std::vector<int>& pt_x, cons
     const int e = fill[rc_j[t]]++;
     et[e] = t; eg[e] = grp[t];
   }
+  int lo = yy, hi = -1, most = 0;
+  for (int j = 0; j < yy; ++j)
+    if (cnt[j + 1] > cnt[j]) { lo = std::min(lo, j); hi = std::max(hi, j); most = std::max(most, cnt[j + 1] - cnt[j]); }
+  if (most > XC_MAXE) { s.xf = 0; return 0; }
+  const int nband = hi >= lo ? hi - lo + 1 : 0;
   const std::string key = "xs" + std::to_string(kind);
   int *dpc, *drp, *det, *deg;
-  double *dwp, *dwf;
+  double *dwp, *dwf, *dcb;
+  if (int rc = upload<double>(h, key + "cb", nullptr, (size_t)std::max(nband, 1) * xx, &dcb)) return rc;
+  s.yr_lo = nband ? lo : 0; s.yr_n = nband; s.xcb = dcb;
   if (int rc = upload(h, key + "pc", pt_c.data(), pt_c.size(), &dpc)) return rc;
   if (int rc = upload(h, key + "wp", wp.data(), wp.size(), &dwp)) return rc;
   if (int rc = upload(h, key + "wf", wf.data(), wf.size(), &dwf)) return rc;

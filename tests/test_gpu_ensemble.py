@@ -25,16 +25,18 @@ def small_cases(ens):
     return [ens.channel_case(m, dh, dl, re, nx_range=(40, 400), ny_range=(12, 80)) for m, dh, dl, re in SMALL]
 
 
-def test_batched_equals_single(ens):
+@pytest.mark.parametrize('launch', [(0, 1), (7, 1)], ids=['dense y transform', 'x-FFT'])
+def test_batched_equals_single(ens, launch):
     cases = small_cases(ens)
-    print('ensemble: shapes', [(c['nx'], c['ny']) for c in cases], flush=True)
+    print('ensemble: shapes', [(c['nx'], c['ny']) for c in cases], launch, flush=True)
     batch = ens.Ensemble(cases, workers=1)
     singles = ens.build_flows(cases, workers=1)
     # same launch shapes on both sides (the autotuner may pick other sweep widths for the batch,
     # which changes the round-off)
-    batch.set_launch(0, 1)
+    batch.set_launch(*launch)
     for f in singles:
-        f.gg.device.set_launch(0, 1)
+        f.gg.device.set_launch(*launch)
+    assert batch.get_launch()['gemm_pressure'] == launch[0]
     nsteps = 7                        # eager startup step, then graph replays of both parities
     d_b = batch.run(3)
     d_b = np.concatenate([d_b, batch.run(nsteps - 3)], axis=1)

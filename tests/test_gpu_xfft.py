@@ -31,21 +31,42 @@ def rel(a, b):
 
 @pytest.mark.parametrize('name', list(GRIDS))
 def test_xfft_rectangle_inverse(pkg, name):
+    """All three kinds: pressure (DCT-IV), u (mirrored-outlet sine basis), v (DST-IV)."""
     print('x-FFT rectangle inverse', name, flush=True)
     f = gpu_flow(pkg, **GRIDS[name])
     dev = f.gg.device
-    dev.set_dt(1e-3)
+    dt = 1e-3
+    dev.set_dt(dt)
     dev.set_launch(XFFT, 1)
     launch = dev.get_launch()
     assert launch['gemm_pressure'] == XFFT and launch['tri_pressure'] == 3
-    assert launch['gemm_velocity'] == 4          # u has no x-FFT form: the velocity class falls back
-    b = np.random.default_rng(3).normal(size=(f.ny, f.nx))
-    x = dev.rect_inverse(0, b)
-    _, inv, _ = O.rectangle(f.nx, f.ny, f.dx, f.dy, 'pressure', f.nu, 1e-3, 1.)
-    err = rel(x, inv(b.copy()))
-    print(name, 'x-FFT pressure inverse error', err, flush=True)
-    assert err < 1e-12
-    assert np.max(abs(dev.pressure_factor.matrix @ x.ravel() - b.ravel())) < 1e-10
+    assert launch['gemm_velocity'] == XFFT and launch['tri_velocity'] == 3
+    factors = [dev.pressure_factor] + [hf[0] for hf in dev.helmholtz_factors]
+    for code, kind in enumerate(('pressure', 'u', 'v')):
+        b = np.random.default_rng(3 + code).normal(size=factors[code].shape)
+        x = dev.rect_inverse(code, b)
+        _, inv, _ = O.rectangle(f.nx, f.ny, f.dx, f.dy, kind, f.nu, dt, 1.)
+        err = rel(x, inv(b.copy()))
+        print(name, kind, 'x-FFT inverse error', err, flush=True)
+        assert err < 1e-12, (kind, err)
+        assert np.max(abs(factors[code].matrix @ x.ravel() - b.ravel())) < 1e-10 * max(1., np.max(abs(x)))
+
+
+def test_xfft_helmholtz_solve(pkg):
+    """The capacitance-corrected u/v Helmholtz solve (optimized_ghost_cell.py:243-255) on the x-FFT path."""
+    kw = GRIDS['440x82']
+    ref = O.CylinderFlow(**kw)
+    gpu = gpu_flow(pkg, **kw)
+    dt = 2e-3
+    g = ref.gg
+    g.build_helmholtz(dt)
+    dev = gpu.gg.device
+    dev.set_dt(dt)
+    dev.set_launch(XFFT, 1)
+    q = np.random.default_rng(8).normal(size=g.qsize)
+    xr = g._helmholtz(q)
+    x, info = dev.schur_solve(1, dev.q_to_vel(q))
+    assert rel(dev.vel_to_q(x), xr) < 1e-11
 
 
 def test_xfft_falls_back_for_odd_widths(pkg):
