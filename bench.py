@@ -64,26 +64,45 @@ def box_leg(pkg, steps, warmup, launch=None):
                 launch=dev.get_launch(), speculation=dev.get_speculation(), setup_seconds=setup_s)
 
 
-def cpu_baseline(seconds=20.0, gmres_seconds=0.0):
-    """The oracle (NumPy/SciPy restatement) on this host's cores, on a bounded sample of steps; with
-    gmres_seconds, then a bounded sample of the same case on the oracle's GMRES backend (SciPy gmres
-    preconditioned by the rectangle inverse).  Returns (baseline, gmres sample or None)."""
+def cpu_baseline(seconds=20.0, gmres_seconds=0.0, single_seconds=10.0):
+    """The oracle on this host's cores, on a bounded sample of steps: its stencil loops and sparse
+    products as C++/OpenMP loops (oracle/cpu_kernels.cpp, the reference's Numba prange loops, with
+    results equal to the NumPy/SciPy ones bit for bit), scipy.fft with workers = the same thread
+    count, the Schur LU through scipy.linalg.  Then the same with one thread (OpenMP and FFT), and,
+    with gmres_seconds, a bounded sample on the oracle's GMRES backend.  Returns (baseline, gmres
+    sample or None)."""
     from oracle.ghost_oracle import CylinderFlow
     cores = len(os.sched_getaffinity(0))
-    s = CylinderFlow(initial='profile', v_perturbation=dict(seed=SEED), fft_workers=cores, **WORKLOAD)
-    s.step(DT)       # startup step builds the per-dt Helmholtz factors (setup, not timed)
-    s.step(DT)
-    times = []
-    start = time.perf_counter()
-    while time.perf_counter() - start < seconds and len(times) < 400:
-        t0 = time.perf_counter()
-        s.step(DT)
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
     cells = WORKLOAD['nx'] * WORKLOAD['ny']
-    base = dict(value=cells / med, unit='cell-steps/s', cores=cores, kind='port',
-                sample=f'{len(times)} steps of the 1320x246 Re=100 case after 2 warm-up steps, median step time '
-                       f'{med * 1e3:.1f} ms (oracle/ghost_oracle.py, scipy.fft workers={cores}, single-threaded SpMV)')
+
+    def sample(threads, budget, cap):
+        s = CylinderFlow(initial='profile', v_perturbation=dict(seed=SEED), fft_workers=threads,
+                         openmp_threads=threads, **WORKLOAD)
+        s.step(DT)       # startup step builds the per-dt Helmholtz factors (setup, not timed)
+        s.step(DT)
+        times = []
+        start = time.perf_counter()
+        while time.perf_counter() - start < budget and len(times) < cap:
+            t0 = time.perf_counter()
+            s.step(DT)
+            times.append(time.perf_counter() - t0)
+        return s, float(np.median(times)), len(times), s.gg.kernels.threads
+
+    # OpenMP threads sleep between the many short parallel regions of a step instead of spinning
+    # against scipy.fft's workers (read when the kernels' OpenMP runtime starts)
+    os.environ.setdefault('OMP_WAIT_POLICY', 'PASSIVE')
+    s, med, n, threads = sample(cores, seconds, 400)
+    runs = [dict(threads=threads, ms_per_step=med * 1e3, steps=n, value=cells / med)]
+    if single_seconds:
+        _, med1, n1, _ = sample(1, single_seconds, 100)
+        runs.append(dict(threads=1, ms_per_step=med1 * 1e3, steps=n1, value=cells / med1))
+    best = max(runs, key=lambda r: r['value'])
+    base = dict(value=best['value'], unit='cell-steps/s', cores=best['threads'], kind='port',
+                sample=f'median step time of the 1320x246 Re=100 case after 2 warm-up steps, the faster of '
+                       f'{" and ".join(str(r["threads"]) for r in runs)} thread(s): {best["steps"]} steps at '
+                       f'{best["ms_per_step"]:.1f} ms (oracle/ghost_oracle.py with the OpenMP stencil and '
+                       f'sparse-product loops of oracle/cpu_kernels.cpp, scipy.fft workers = OpenMP threads)',
+                runs=runs)
     if not gmres_seconds:
         return base, None
     s.gg.set_pressure_solver('iterative')
@@ -96,7 +115,7 @@ def cpu_baseline(seconds=20.0, gmres_seconds=0.0):
         d = s.last_step_diagnostics
         arnoldi.append(d['pressure_iterations'] / d['pressure_solve_count'])
     med = float(np.median(times))
-    return base, dict(ms_per_step=med * 1e3, value=cells / med, steps=len(times), cores=cores,
+    return base, dict(ms_per_step=med * 1e3, value=cells / med, steps=len(times), cores=threads,
                       arnoldi_steps_per_solve=float(np.mean(arnoldi)),
                       sample=f'{len(times)} GMRES-backend steps of the same case after the baseline sample, median')
 
@@ -277,8 +296,12 @@ def main():
     diag = dev.run(args.steps, DT)     # synchronises before returning
     elapsed = time.perf_counter() - t0
     cells = WORKLOAD['nx'] * WORKLOAD['ny']
-    value = cells * args.steps / elapsed
+    value = rate = cells * args.steps / elapsed
     max_div = float(np.max(diag[:, 10]))
+    if not max_div < 1e-12:
+        # the metric is quoted at div < 1e-12: no value is reported for a run that misses it
+        print(f'bench: max divergence {max_div:.3e} >= 1e-12 in the timed steps; value withheld', file=sys.stderr)
+        value = None
     # dominant kernels timed with HIP events on the library's stream
     launch = dev.get_launch()
     tile = launch['gemm_pressure']
@@ -306,8 +329,8 @@ def main():
                    kernel=('k_gemm_sk' if tile >= 3 else 'k_gemm') + ' (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
                    launch_ms=gemm_ms),
                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
-                                  achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                                  frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
+                                  achieved=rate * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                                  frac=rate * STEP_BYTES_PER_CELL / HBM_PEAK),
                kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms, lean_misses=lean_misses), launch=launch,
                speculation=dev.get_speculation(), setup_seconds=setup_s)
     if args.box_steps:

@@ -93,6 +93,7 @@ _SIGNATURES = {
     'mcf_net_profile': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, _F]),
     'mcf_set_launch': (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
     'mcf_get_launch': (ctypes.c_int, [_H, _I]),
+    'mcf_set_launch_classes': (ctypes.c_int, [_H, _I]),
     'mcf_set_speculation': (ctypes.c_int, [_H, ctypes.c_int]),
     'mcf_get_speculation': (ctypes.c_int, [_H, _I]),
     'mcf_batch_set_speculation': (ctypes.c_int, [_H, ctypes.c_int]),

@@ -3353,7 +3353,10 @@ void drop_graphs(mcf_handle* h) {
 // The autotuner: times every launch shape of the transform and of the tridiagonal sweep on the
 // context's own problems (forward transforms into the scratch F and in-place sweeps of F, which no
 // later kernel reads before rewriting; no live buffer is touched) and keeps the fastest per
-// problem class.  Transform shapes give identical results; sweep shapes differ in round-off.
+// problem class.  The staged transform tiles (0-2) give identical results, and so do the two split-K
+// tiles of one block size; the families (staged, 8-wave split-K, 4-wave split-K, x-FFT) and the
+// sweep widths differ in round-off, so the choice is kept with a checkpoint (mcf_get_launch /
+// mcf_set_launch_classes) for a resumed run to round as the uninterrupted one.
 int autotune(const Ctx& x, Launch& out) {
   hipEvent_t e0, e1;
   HIPCHK(x.err, hipEventCreate(&e0));
@@ -4653,6 +4656,21 @@ int mcf_set_launch(mcf_handle* h, int gemm, int tri) {
     pin_launch(h->ln, gemm, tri, !x.noxf_p, !x.noxf_h);
     h->tuned = true;
   }
+  drop_graphs(h);
+  return 0;
+}
+
+int mcf_set_launch_classes(mcf_handle* h, const int* in4) {
+  if (!h || !in4) return fail(h, 1, "launch shapes required");
+  const Ctx x = ctx_of(h);
+  for (int c = 0; c < 2; ++c) {
+    const int g = in4[c], t = in4[2 + c];
+    const bool xf = c == 0 ? !x.noxf_p : !x.noxf_h;
+    const bool ok = (g == GV_XFFT) ? (t == TV_YSWEEP && xf) : (g >= 0 && g < N_GEMM_V && t >= 0 && t < N_TRI_V);
+    if (!ok) return fail(h, 1, "invalid launch shape for a problem class");
+  }
+  h->ln.gemm_p = in4[0]; h->ln.gemm_h = in4[1]; h->ln.tri_p = in4[2]; h->ln.tri_h = in4[3];
+  h->tuned = true;
   drop_graphs(h);
   return 0;
 }

@@ -450,6 +450,11 @@ class GhostDevice:
         """Pin the transform tile / sweep width variants (-1, -1: autotune at the next run)."""
         self.h.call('mcf_set_launch', int(gemm), int(tri))
 
+    def set_launch_classes(self, launch):
+        """Restore the per-class shapes of a ``get_launch`` dict (a run's autotuned choice)."""
+        self.h.call('mcf_set_launch_classes', np.array([launch['gemm_pressure'], launch['gemm_velocity'],
+                                                          launch['tri_pressure'], launch['tri_velocity']], dtype=np.int32))
+
     def get_launch(self):
         out = np.zeros(5, dtype=np.int32)
         self.h.call('mcf_get_launch', out)

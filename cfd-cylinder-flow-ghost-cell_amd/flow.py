@@ -436,9 +436,11 @@ class MAC_CylinderFlow:
         for k in ('previous_advection', 'half_pressure'):
             if getattr(self.gg, k) is not None:
                 arrays[k] = getattr(self.gg, k)
+        launch = self.gg.device.get_launch()
         meta = dict(schema=1, configuration=self.configuration, time=self.time, run_options=self.run_options,
                     prev_dt=self.gg.previous_dt, cn_dt=self.gg.previous_dt, diagnostics=self.last_step_diagnostics,
-                    source_sha256=self.metadata(1., 0)['source_sha256'])
+                    source_sha256=self.metadata(1., 0)['source_sha256'],
+                    launch=launch if launch['chosen'] else None)
         temporary = None
         try:
             with tempfile.NamedTemporaryFile(dir=path.parent, suffix='.npz', delete=False) as stream:
@@ -481,6 +483,9 @@ class MAC_CylinderFlow:
                                     or solver.gg.previous_dt <= 0):
                 raise ValueError('Invalid Ghost timestep history')
             solver.gg.device.set_inlet(solver.u_inlet)
+            if meta.get('launch'):
+                # the launch shapes the run was tuned to, so that the resumed run rounds as it would have
+                solver.gg.device.set_launch_classes(meta['launch'])
             return solver
 
     def metadata(self, dt, n_steps, perturb_transient=True, perturb_duration=None, perturb_amp=.15):

@@ -36,6 +36,15 @@ Pins (each test lives in tests/test_oracle_*.py):
                                 Helmholtz values (tests/test_reference_golden.py)
   * knn_with_ties            -- scipy cKDTree query + query_ball_point, the reference's calls
   * advection                -- closed forms for linear u, v: interior faces, outlet row, wall row
+  * OpenMPKernels            -- the C++/OpenMP flux loops and CSR products of the CPU baseline
+                                (oracle/cpu_kernels.cpp) equal flux_rhs and SciPy's products bit
+                                for bit, and whole steps with them equal the NumPy steps
+  * extra_refinements        -- (a study option, not the reference's rule) residual stays at its
+                                round-off floor; its force moves set the long-run force bars
+
+Options beyond the reference: ``openmp_threads`` (the CPU baseline's OpenMP loops) and
+``extra_refinements`` (refinements past the reference's stopping rule, to measure how far that rule
+determines a step) of ``CylinderFlow``.
 """
 import json
 import os
@@ -347,6 +356,62 @@ def rectangle(nx, ny, dx, dy, kind, nu=0., dt=0., rho=1., workers=None):
     return matrix, solve, (yy, xx)
 
 
+def _scipy_matvec(matrix, x):
+    return matrix @ x
+
+
+class OpenMPKernels:
+    """The reference's Numba prange loops (numba_kernels.py:17-142 flux-form advection/diffusion,
+    :347-354 CSR product) as the OpenMP loops of oracle/cpu_kernels.cpp (``make -C oracle cpu``),
+    for the benchmark's CPU baseline (``CylinderFlow(openmp_threads=n)``).  Products and fluxes equal
+    the SciPy/NumPy ones bit for bit: each CSR row is summed from 0.0 in stored order, as SciPy's
+    csr_matvec does (a CSC matrix is converted to CSR, whose sorted rows sum in csc_matvec's order),
+    and the flux expressions are evaluated in flux_rhs's order without FP contraction."""
+
+    def __init__(self, threads):
+        import ctypes
+        path = Path(__file__).resolve().with_name('libghost_cpu.so')
+        if not path.exists():
+            raise RuntimeError(f'{path} is missing: build it with make -C oracle cpu')
+        lib = ctypes.CDLL(str(path))
+        ptr = np.ctypeslib.ndpointer
+        f8, i4 = ptr(np.float64, flags='C'), ptr(np.int32, flags='C')
+        lib.gc_csr_matvec.argtypes = [ctypes.c_int64, i4, i4, f8, f8, f8]
+        lib.gc_flux_rhs.argtypes = [ctypes.c_int, ctypes.c_int, f8, f8] + [ctypes.c_double] * 5 + [f8, f8, f8]
+        lib.gc_set_threads.argtypes = [ctypes.c_int]
+        lib.gc_get_threads.restype = ctypes.c_int
+        lib.gc_set_threads(int(threads))
+        self.lib = lib
+        self.threads = lib.gc_get_threads()
+        self._csr = {}
+
+    def matvec(self, matrix, x):
+        entry = self._csr.get(id(matrix))
+        if entry is None or entry[0] is not matrix:
+            c = matrix.tocsr()
+            entry = (matrix, np.ascontiguousarray(c.indptr, np.int32), np.ascontiguousarray(c.indices, np.int32),
+                     np.ascontiguousarray(c.data, np.float64), c.shape)
+            self._csr[id(matrix)] = entry
+        _, indptr, indices, data, shape = entry
+        x = np.ascontiguousarray(x, np.float64)
+        if x.shape != (shape[1],):
+            raise ValueError('matvec shape mismatch')
+        y = np.empty(shape[0])
+        self.lib.gc_csr_matvec(shape[0], indptr, indices, data, x, y)
+        return y
+
+    def flux_rhs(self, u, v, dx, dy, nu, include_advection, rho=1.):
+        ny, nxp1 = u.shape
+        nx = nxp1 - 1
+        if v.shape != (ny + 1, nx):
+            raise ValueError('flux_rhs shape mismatch')
+        ru, rv = np.empty_like(u), np.empty_like(v)
+        work = np.empty(2 * (ny + 1) * (nx + 1))
+        self.lib.gc_flux_rhs(ny, nx, np.ascontiguousarray(u, np.float64), np.ascontiguousarray(v, np.float64),
+                             dx, dy, nu, 1. if include_advection else 0., rho, ru, rv, work)
+        return ru, rv
+
+
 # --------------------------------------------------------------------------------------
 # Boundary-row (Woodbury) correction and the monitored pressure solve
 # (solver/pressure_schur.py:16-125)
@@ -369,8 +434,16 @@ def boundary_rows(actual, indices, rect):
     return rows, delta[rows], tolerance
 
 
-def boundary_row_inverse(actual, indices, rect, inverse, shape):
-    """Woodbury correction of a separable inverse with <= 2 refinements (pressure_schur.py:75-125)."""
+def boundary_row_inverse(actual, indices, rect, inverse, shape, extra=0, matvec=None):
+    """Woodbury correction of a separable inverse with <= 2 refinements (pressure_schur.py:75-125).
+
+    ``extra`` (a study option, not the reference's rule) adds that many refinements after the
+    reference's stopping rule.  Each one adds the inverse applied to a residual already at the
+    round-off floor, which moves the solution along the operator's weakly determined directions
+    while the residual stays at the floor: the size of that move is how far the rule determines the
+    solution, and so how closely two correct implementations can agree.  ``matvec(M, x)`` computes
+    the step-time sparse products (default ``M @ x``)."""
+    mv = matvec or _scipy_matvec
     indices = np.asarray(indices)
     rows, correction, tolerance = boundary_rows(actual, indices, rect)
     if len(rows) > 2048:
@@ -392,7 +465,7 @@ def boundary_row_inverse(actual, indices, rect, inverse, shape):
         rhs.ravel()[indices] = values
         first = inverse(rhs)
         rhs.fill(0.)
-        rhs.ravel()[rows] = lu_solve(factor, correction @ first.ravel())
+        rhs.ravel()[rows] = lu_solve(factor, mv(correction, first.ravel()))
         return (first - inverse(rhs)).ravel()[indices]
 
     details = dict(rank=len(rows), dense_bytes=int(schur.nbytes), roundoff_drop_tolerance=tolerance,
@@ -403,18 +476,20 @@ def boundary_row_inverse(actual, indices, rect, inverse, shape):
         target = max(1e-10, 32 * EPS * max(1., float(np.max(abs(values)))))
         count = 0
         for _ in range(2):
-            residual = values - actual @ result
+            residual = values - mv(actual, result)
             if np.max(abs(residual)) <= target:
                 break
             result += apply(residual)
             count += 1
+        for _ in range(extra):
+            result += apply(values - mv(actual, result))
         details['last_refinement_iterations'] = count
         return result
 
     return solve, details
 
 
-def monitored_pressure_solver(actual, indices, rect, inverse, shape, iterative=False):
+def monitored_pressure_solver(actual, indices, rect, inverse, shape, iterative=False, extra=0, matvec=None):
     """Monitored pressure solve (pressure_schur.py:16-72).
 
     Direct: the boundary-row corrected rectangle inverse.  Iterative: SciPy's GMRES on ``actual``,
@@ -424,6 +499,7 @@ def monitored_pressure_solver(actual, indices, rect, inverse, shape, iterative=F
     refinement solves on the true residual while it exceeds the target.  Both end with the same
     true-residual gate.  ``gmres`` is looked up in this module at call time.
     """
+    mv = matvec or _scipy_matvec
     indices = np.asarray(indices)
     absolute = abs(actual).tocsr()
     roundoff_weights = (2 * np.diff(absolute.indptr) + 2) * EPS
@@ -436,7 +512,7 @@ def monitored_pressure_solver(actual, indices, rect, inverse, shape, iterative=F
         info = dict(backend='gmres_rectangle', preconditioner='separable_rectangle', krylov_rtol=1e-12,
                     krylov_atol=1e-12)
     else:
-        direct, info = boundary_row_inverse(actual, indices, rect, inverse, shape)
+        direct, info = boundary_row_inverse(actual, indices, rect, inverse, shape, extra, matvec)
         info['backend'] = 'fft_boundary_schur'
 
     def solve(values):
@@ -456,7 +532,7 @@ def monitored_pressure_solver(actual, indices, rect, inverse, shape, iterative=F
             result, status = krylov(values)
         else:
             result, status = direct(values), 0
-        residual = values - actual @ result
+        residual = values - mv(actual, result)
         target = max(1e-10, 64 * EPS * max(1., float(np.max(abs(values)))))
         refinements = info.get('last_refinement_iterations', 0)
         if iterative:
@@ -464,11 +540,11 @@ def monitored_pressure_solver(actual, indices, rect, inverse, shape, iterative=F
             while refinements < 2 and not status and np.max(abs(residual)) > target:
                 correction, status = krylov(residual)
                 result = result + correction
-                residual = values - actual @ result
+                residual = values - mv(actual, result)
                 refinements += 1
         linf = float(np.max(abs(residual)))
         relative = float(np.linalg.norm(residual) / max(np.linalg.norm(values), np.finfo(float).tiny))
-        action_scale = absolute @ abs(result) + abs(values)
+        action_scale = mv(absolute, abs(result)) + abs(values)
         roundoff_bound = float(np.max(roundoff_weights * action_scale))
         target = max(target, roundoff_bound)
         info['last_solve'] = dict(iterations=steps[0], residual_linf=linf, residual_relative_l2=relative,
@@ -592,11 +668,14 @@ def flux_rhs(u, v, dx, dy, nu, include_advection, rho=1.):
 class GhostIntegrator:
     """Operators, inverses and the AB2/CN driver of optimized_ghost_cell.py:61-286."""
 
-    def __init__(self, solver, fft_workers=None):
+    def __init__(self, solver, fft_workers=None, extra_refinements=0, kernels=None):
         self.solver = s = solver
+        self.kernels = kernels
+        self._mv = kernels.matvec if kernels is not None else _scipy_matvec
         gc = solver.gc
         nx, ny, dx, dy = s.nx, s.ny, s.dx, s.dy
         self.fft_workers = fft_workers
+        self.extra_refinements = extra_refinements
         active_u = gc.dist_u > 1e-13
         active_v = gc.dist_v > 1e-13
         active_u[:, 0] = False
@@ -684,14 +763,15 @@ class GhostIntegrator:
         """(Re)build the monitored pressure solve for 'direct' or 'iterative' (optimized_ghost_cell.py:109-111)."""
         rect, inverse, shape = self._pressure_rectangle
         self.pressure_solve, self.pressure_inverse_info = monitored_pressure_solver(
-            self.A, self.pi, rect, inverse, shape, iterative=pressure_solver == 'iterative')
+            self.A, self.pi, rect, inverse, shape, iterative=pressure_solver == 'iterative',
+            extra=self.extra_refinements, matvec=self._mv)
 
     # --- packing (optimized_ghost_cell.py:174-184)
     def pack(self, u, v):
         return np.concatenate((u.ravel()[self.ui], v.ravel()[self.vi]))
 
     def unpack(self, q):
-        field = self.E @ q + self.offset
+        field = self._mv(self.E, q) + self.offset
         self.u[:] = field[:self.usize].reshape(self.u.shape)
         self.v[:] = field[self.usize:].reshape(self.v.shape)
         return self.u, self.v
@@ -700,32 +780,35 @@ class GhostIntegrator:
     def advection(self, q):
         u, v = self.unpack(q)
         s = self.solver
-        ru, rv = flux_rhs(u, v, s.dx, s.dy, 0., True)
+        if self.kernels is not None:
+            ru, rv = self.kernels.flux_rhs(u, v, s.dx, s.dy, 0., True)
+        else:
+            ru, rv = flux_rhs(u, v, s.dx, s.dy, 0., True)
         result = self.pack(ru, rv) / (s.dx * s.dy)
         result[self.outlet_u] *= 2
         return result
 
     def diffusion(self, q):
-        return self.L @ q + self.diff_offset
+        return self._mv(self.L, q) + self.diff_offset
 
     def gradient(self, p):
-        return self.G @ (p.ravel() if p.ndim > 1 else p)
+        return self._mv(self.G, p.ravel() if p.ndim > 1 else p)
 
     def divergence(self, q):
-        return self.D @ q + self.div_offset
+        return self._mv(self.D, q) + self.div_offset
 
     def project(self, q, dt):
         rhs = self.divergence(q) / dt
         phi = self.pressure_solve(rhs)
         self._pressure_calls.append(dict(self.pressure_inverse_info['last_solve']))
-        out = q - dt * (self.G @ phi)
+        out = q - dt * self._mv(self.G, phi)
         error = float(np.max(np.abs(self.divergence(out))))
         if error > 1e-10 or not np.isfinite(out).all():
             raise RuntimeError(f'Ghost projection failed: divergence={error:g}')
         return out, phi
 
     def H_matvec(self, q, dt):
-        return q - 0.5 * dt * (self.L @ q)
+        return q - 0.5 * dt * self._mv(self.L, q)
 
     def build_helmholtz(self, dt):
         """Per-dt u/v Helmholtz inverses (optimized_ghost_cell.py:243-255)."""
@@ -739,7 +822,8 @@ class GhostIntegrator:
                                       ('v', (jv - 1) * s.nx + iv, self.L[nu:, nu:])):
             rect, inverse, shape = rectangle(s.nx, s.ny, s.dx, s.dy, kind, s.nu, dt, s.rho, workers=self.fft_workers)
             actual = (sp.eye(block.shape[0], format='csc') - .5 * dt * block).tocsc()
-            solve, info = boundary_row_inverse(actual, selected, rect, inverse, shape)
+            solve, info = boundary_row_inverse(actual, selected, rect, inverse, shape, self.extra_refinements,
+                                               self._mv)
             solves.append(solve)
             self.helmholtz_inverse_info.append(info)
         self._helmholtz = lambda rhs: np.concatenate((solves[0](rhs[:nu]), solves[1](rhs[nu:])))
@@ -776,12 +860,12 @@ class GhostIntegrator:
             pressure_seconds=sum(c['seconds'] for c in calls), pressure_solves=list(calls))
         self.last_diagnostics = diagnostics
         uu, vv = self.unpack(q)
-        return uu, vv, (self.P @ output).reshape(p.shape)
+        return uu, vv, self._mv(self.P, output).reshape(p.shape)
 
     def surface_forces(self, u, v, p, mu):
         """(Fx, Fy, Fx_p, Fy_p, Fx_v, Fy_v) (optimized_ghost_cell.py:280-286)."""
-        pressure = self.pressure_surface @ p.ravel()
-        dn = np.column_stack((self.du_dn @ u.ravel(), self.dv_dn @ v.ravel()))
+        pressure = self._mv(self.pressure_surface, p.ravel())
+        dn = np.column_stack((self._mv(self.du_dn, u.ravel()), self._mv(self.dv_dn, v.ravel())))
         traction = mu * (dn + self.normals * np.sum(dn * self.normals, axis=1)[:, None])
         fp = -np.sum(pressure[:, None] * self.normal_weights, axis=0)
         fv = np.sum(traction, axis=0) * (np.pi * self.solver.D / len(pressure))
@@ -815,7 +899,7 @@ class CylinderFlow:
                  boundary='ghost_cell', transport='centered', pressure_solver='direct',
                  marker_spacing_factor=1.5, time_scheme='ab2_cn', ghost_pressure_extension='neumann',
                  ghost_velocity_interpolation='bilinear', initial='profile', v_perturbation=None,
-                 fft_workers=None):
+                 fft_workers=None, extra_refinements=0, openmp_threads=None):
         if boundary != 'ghost_cell' or time_scheme != 'ab2_cn' or transport != 'centered':
             raise ValueError('The oracle covers boundary=ghost_cell, time_scheme=ab2_cn, centered transport')
         if pressure_solver not in ('direct', 'iterative') or slip_walls:
@@ -883,7 +967,8 @@ class CylinderFlow:
         self.gc = GhostGeometry(nx, ny, self.dx, self.dy, CircleSDF(xc, yc, D / 2.0))
         if v_perturbation is not None:
             self.v += gaussian_v_perturbation(self.gc.xv, self.gc.yv, xc, yc, D / 2, D, **v_perturbation)
-        self.gg = GhostIntegrator(self, fft_workers=fft_workers)
+        self.gg = GhostIntegrator(self, fft_workers=fft_workers, extra_refinements=extra_refinements,
+                                  kernels=None if openmp_threads is None else OpenMPKernels(openmp_threads))
         self.A = self.gg.A
         initial_q, _ = self.gg.project(self.gg.pack(self.u, self.v), 1.)
         self.u[:], self.v[:] = self.gg.unpack(initial_q)

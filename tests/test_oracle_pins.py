@@ -363,3 +363,69 @@ def test_complete_restart():
         np.savez(path, **arrays)
         with pytest.raises(ValueError):
             O.CylinderFlow.from_checkpoint(path)
+
+
+def test_committed_long_run_history_is_the_oracles():
+    """The first 15 steps of the committed 440x82 trajectory (tests/golden/oracle_histories.npz, the
+    reference for the whole-run GPU tests) are what the oracle computes now, to the last bit."""
+    hist = np.load(ROOT / 'tests' / 'golden' / 'oracle_histories.npz')['channel440x82/history']
+    s = O.CylinderFlow(nx=440, ny=82, Lx=22., Ly=4.1, xc=2., yc=2., nu=0.05, inflow='parabolic', U_m=1.5,
+                       initial='zero')
+    for k in range(15):
+        s.step(2e-3)
+        d = s.last_step_diagnostics
+        assert np.array_equal([d['time'], d['cd'], d['cl'], d['max_divergence']], hist[k]), k
+
+
+@pytest.fixture(scope='module')
+def openmp_kernels():
+    """The OpenMP loops of the CPU baseline (oracle/cpu_kernels.cpp), built if missing."""
+    if not (ROOT / 'oracle' / 'libghost_cpu.so').exists():
+        import subprocess
+        subprocess.run(['make', '-C', str(ROOT / 'oracle'), 'cpu'], check=True, capture_output=True)
+    return O.OpenMPKernels(3)
+
+
+def test_openmp_fluxes_and_products_equal_numpy(openmp_kernels):
+    """The C++/OpenMP flux loops and CSR products equal the NumPy/SciPy ones bit for bit, also on odd
+    sizes and for CSC input (summed in csc_matvec's column order)."""
+    rng = np.random.default_rng(11)
+    for ny, nx in ((16, 88), (5, 7), (4, 4)):
+        u, v = rng.normal(size=(ny, nx + 1)), rng.normal(size=(ny + 1, nx))
+        for adv in (True, False):
+            a = O.flux_rhs(u, v, 0.1, 0.2, 0.03, adv, 1.3)
+            b = openmp_kernels.flux_rhs(u, v, 0.1, 0.2, 0.03, adv, 1.3)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (ny, nx, adv)
+    m = sp.random(300, 200, density=0.05, random_state=3, format='csr')
+    x = rng.normal(size=200)
+    assert np.array_equal(openmp_kernels.matvec(m, x), m @ x)
+    c = m.tocsc()
+    assert np.array_equal(openmp_kernels.matvec(c, x), c @ x)
+
+
+def test_openmp_steps_equal_numpy_steps(openmp_kernels):
+    """Whole AB2/CN steps with the OpenMP loops (the CPU baseline) are the oracle's steps bit for bit."""
+    a = channel(initial='zero', nu=0.05, inflow='parabolic', U_m=1.5)
+    b = channel(initial='zero', nu=0.05, inflow='parabolic', U_m=1.5, openmp_threads=3)
+    for k in range(5):
+        a.step(2e-3)
+        b.step(2e-3)
+        for key in ('u', 'v', 'p'):
+            assert np.array_equal(getattr(a, key), getattr(b, key)), (k, key)
+        assert a.last_step_diagnostics['cd'] == b.last_step_diagnostics['cd']
+        assert a.last_step_diagnostics['cl'] == b.last_step_diagnostics['cl']
+
+
+def test_extra_refinements_keep_the_residual_at_its_floor():
+    """The study option behind the long-run force spread: refinements past the reference's rule keep
+    the pressure residual at its round-off floor (no growth) while the forces move by at most the
+    impulsive-start spread."""
+    kw = dict(nx=88, ny=16, Lx=22., Ly=4.1, xc=2., yc=2., initial='zero')
+    a, b = O.CylinderFlow(**kw), O.CylinderFlow(extra_refinements=6, **kw)
+    for _ in range(3):
+        a.step(1e-3)
+        b.step(1e-3)
+        ra = a.last_step_diagnostics['pressure_residual_linf']
+        rb = b.last_step_diagnostics['pressure_residual_linf']
+        assert rb <= 4 * max(ra, 1e-14), (ra, rb)
+        assert abs(a.last_step_diagnostics['cd'] - b.last_step_diagnostics['cd']) < 1e-9
