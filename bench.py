@@ -224,7 +224,15 @@ def ensemble_long(pkg, total, chunk=500):
 COPY_BYTES = 256 << 20
 
 
-def traffic_probe(pkg, steps, launch=None):
+def pin(dev, launch=None, classes=None):
+    if launch:
+        dev.set_launch(*launch)
+    if classes:
+        dev.set_launch_classes(dict(gemm_pressure=classes[0], gemm_velocity=classes[1], tri_pressure=classes[2],
+                                    tri_velocity=classes[3]))
+
+
+def traffic_probe(pkg, steps, launch=None, classes=None):
     """Counter calibration and the measured kernels in one process: a plain device copy of
     COPY_BYTES (reads and writes COPY_BYTES each), then `steps` steps of the 1320x246 workload
     (with the transform tile / sweep width pinned to `launch` when given)."""
@@ -235,11 +243,106 @@ def traffic_probe(pkg, steps, launch=None):
     torch.cuda.synchronize()
     flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='profile',
                                 v_perturbation=dict(seed=SEED), **WORKLOAD)
-    if launch is not None:
-        flow.gg.device.set_launch(*launch)
+    pin(flow.gg.device, launch, classes)
     flow._push()
     flow.gg.device.run(steps, DT)
     print(f'bench: traffic probe ran a {COPY_BYTES} B copy and {steps} steps', file=sys.stderr, flush=True)
+
+
+# Algorithmic HBM bytes of each kernel of the lean 1320x246 step, in launch order: every input read
+# once and every output written once (N = 1320 x 246 cells; "vel" = the u and v arrays, 651,006
+# doubles; "p" = one pressure-sized array, 324,720 doubles; V = the y eigenbasis, yy^2 doubles per
+# kind; LU = the x tridiagonal factors, two yy x xx arrays per kind; r = the capacitance rank, 340
+# for the pressure and 336 for each velocity problem; npts = r).
+_VEL, _P = 651006 * 8, 324720 * 8
+_VV = (246 ** 2 + 245 ** 2) * 8                      # V of the u and v problems
+_LUV = 2 * (246 + 245) * 1320 * 8                    # LU factors of the u and v problems
+_LUP = 2 * 246 * 1320 * 8
+_PTS_P, _PTS_V = 2 * 246 * 340 * 8, 2 * (246 + 245) * 336 * 8     # V rows at the points + F gathered there
+_CAP_P, _CAP_V = 340 * 340 * 8, 2 * 336 * 336 * 8                 # S^-1 C
+STEP_KERNEL_BYTES = [
+    ('k_adv_rhs', 'advection + AB2 + CN right-hand side: reads u, v, the half pressure and the previous '
+                  'advection, writes the advection and the rhs', 4 * _VEL + _P),
+    ('velocity transform, forward', 'reads rhs, V_u, V_v; writes F', 2 * _VEL + _VV),
+    ('velocity sweep, mode 0', 'reads F and LU, writes F', 2 * _VEL + _LUV),
+    ('velocity points', 'V rows and F at the capacitance points', _PTS_V),
+    ('velocity capacitance', 'S^-1 C product', _CAP_V),
+    ('velocity sweep, mode 1', 'reads F and LU, writes F', 2 * _VEL + _LUV),
+    ('velocity transform, inverse', 'reads F, V^T; writes u*, v*', 2 * _VEL + _VV),
+    ('Helmholtz statistics + divergence', 'reads u*, v*, rhs; writes div', 2 * _VEL + _P),
+    ('pressure transform, forward', 'reads div, V; writes F', 2 * _P + 246 ** 2 * 8),
+    ('pressure sweep, mode 0', 'reads F and LU, writes F', 2 * _P + _LUP),
+    ('pressure points', '', _PTS_P),
+    ('pressure capacitance', '', _CAP_P),
+    ('pressure sweep, mode 1', 'reads F and LU, writes F', 2 * _P + _LUP),
+    ('pressure transform, inverse', 'reads F, V^T; writes phi', 2 * _P + 246 ** 2 * 8),
+    ('solve statistics + projection update', 'reads phi, div, u*, v*; writes u, v', 2 * _P + 2 * _VEL),
+    ('gate 1 + momentum defect', 'reads u, v, rhs; writes defect, projected field', 4 * _VEL),
+    ('divergence', 'reads the projected field, writes div', _VEL + _P),
+    ('pressure transform, forward', '', 2 * _P + 246 ** 2 * 8),
+    ('pressure sweep, mode 0', '', 2 * _P + _LUP),
+    ('pressure points', '', _PTS_P),
+    ('pressure capacitance', '', _CAP_P),
+    ('pressure sweep, mode 1', '', 2 * _P + _LUP),
+    ('pressure transform, inverse', '', 2 * _P + 246 ** 2 * 8),
+    ('solve statistics + projection update', '', 2 * _P + 2 * _VEL),
+    ('gate 2 + pressures + ghost fill', 'reads u, v, half pressure, phi; writes half and output pressure',
+     _VEL + 4 * _P),
+    ('k_forces', 'surface stencils (756 points)', 756 * 40 * 12),
+    ('k_finalize', 'reduction slots', 64 * 64 * 8),
+]
+
+
+def step_table(trace_dir, fetch_dir, write_dir, last=40):
+    """Markdown table of the lean step's kernels from three rocprofv3 CSV runs of the same
+    `--traffic-probe` command (kernel trace, --pmc FETCH_SIZE, --pmc WRITE_SIZE): per launch position
+    the mean duration and scaled traffic over the last `last` steps, against STEP_KERNEL_BYTES."""
+    import csv
+    import re
+
+    def rows(path, value):
+        out = list(csv.DictReader(open(path)))
+        out.sort(key=lambda r: int(r['Dispatch_Id']))
+        name = lambda r: re.sub(r'\(anonymous namespace\)::', '', r['Kernel_Name']).split('(')[0].replace('void ', '')
+        return [(name(r), value(r)) for r in out]
+
+    trace = rows(Path(trace_dir) / 'run_kernel_trace.csv',
+                 lambda r: (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) / 1e3)
+    fetch = rows(Path(fetch_dir) / 'run_counter_collection.csv', lambda r: float(r['Counter_Value']))
+    write = rows(Path(write_dir) / 'run_counter_collection.csv', lambda r: float(r['Counter_Value']))
+    copy_kib = COPY_BYTES / 1024
+    sf = copy_kib / next(v for n, v in fetch if 'copyBuffer' in n)
+    sw = copy_kib / next(v for n, v in write if 'copyBuffer' in n)
+
+    def per_position(seq):
+        starts = [i for i, (n, _) in enumerate(seq) if n == 'k_adv_rhs']
+        spans = list(zip(starts, starts[1:]))[-last:]
+        width = spans[0][1] - spans[0][0]
+        if any(b - a != width for a, b in spans) or width != len(STEP_KERNEL_BYTES):
+            raise RuntimeError('the timed steps are not all lean steps of the expected shape')
+        names = [n for n, _ in seq[spans[0][0]:spans[0][1]]]
+        vals = np.array([[v for _, v in seq[a:b]] for a, b in spans])
+        return names, vals.mean(axis=0)
+
+    names, dur = per_position(trace)
+    names_f, fet = per_position(fetch)
+    names_w, wri = per_position(write)
+    if not names == names_f == names_w:
+        raise RuntimeError('the three runs launched different kernels (pin --launch-classes)')
+    lines = [f'Counter scale factors from the {COPY_BYTES >> 20} MiB copy: FETCH_SIZE x {sf:.3f}, WRITE_SIZE x {sw:.3f}.',
+             '', '| # | kernel | stage | us | fetched MB | written MB | traffic GB/s | algorithmic MB | algorithmic GB/s | % of 8 TB/s |',
+             '|---|---|---|---|---|---|---|---|---|---|']
+    tot = [0.0] * 4
+    for k, (n, (stage, _, alg)) in enumerate(zip(names, STEP_KERNEL_BYTES)):
+        f, w = fet[k] * sf * 1024 / 1e6, wri[k] * sw * 1024 / 1e6
+        t = dur[k]
+        lines.append(f'| {k} | `{n}` | {stage} | {t:.2f} | {f:.2f} | {w:.2f} | {(f + w) / t * 1e3:.0f} | '
+                     f'{alg / 1e6:.2f} | {alg / t / 1e3:.0f} | {alg / t / 1e3 / 80:.1f} |')
+        tot = [tot[0] + t, tot[1] + f, tot[2] + w, tot[3] + alg / 1e6]
+    lines.append(f'| | **step** | sum of the launches | {tot[0]:.1f} | {tot[1]:.1f} | {tot[2]:.1f} | '
+                 f'{(tot[1] + tot[2]) / tot[0] * 1e3:.0f} | {tot[3]:.1f} | {tot[3] / tot[0] * 1e3:.0f} | '
+                 f'{tot[3] / tot[0] * 1e3 / 80:.1f} |')
+    return '\n'.join(lines)
 
 
 def main():
@@ -261,6 +364,11 @@ def main():
                          '--steps steps of the workload; prints nothing else')
     ap.add_argument('--launch', type=int, nargs=2, default=None, metavar=('TILE', 'WIDTH'),
                     help='pin the transform tile and sweep width variants (default: autotuned)')
+    ap.add_argument('--step-table', nargs=3, default=None, metavar=('TRACE_DIR', 'FETCH_DIR', 'WRITE_DIR'),
+                    help='print the per-kernel table of the lean step from three rocprofv3 CSV runs of '
+                         '--traffic-probe (kernel trace, --pmc FETCH_SIZE, --pmc WRITE_SIZE) and exit')
+    ap.add_argument('--launch-classes', type=int, nargs=4, default=None, metavar=('GP', 'GV', 'TP', 'TV'),
+                    help='pin the transform tile and sweep width per problem class (pressure, velocity)')
     ap.add_argument('--ensemble-launch', type=int, nargs=2, default=None, metavar=('TILE', 'WIDTH'),
                     help="pin the ensemble batch's transform tile and sweep width (default: autotuned)")
     ap.add_argument('--gmres-steps', type=int, default=20,
@@ -271,13 +379,16 @@ def main():
     if args.gpus != 1:
         print(f'bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported', file=sys.stderr)
         sys.exit(2)
+    if args.step_table:
+        print(step_table(*args.step_table))
+        return
     import cfd_amd_loader
     pkg = cfd_amd_loader.load()
     if args.ensemble_long:
         ensemble_long(pkg, args.ensemble_long)
         return
     if args.traffic_probe:
-        traffic_probe(pkg, args.steps, args.launch)
+        traffic_probe(pkg, args.steps, args.launch, args.launch_classes)
         return
     t0 = time.perf_counter()
     flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='profile',
@@ -285,8 +396,7 @@ def main():
     dev = flow.gg.device
     flow._push()
     dev.set_dt(DT)
-    if args.launch:
-        dev.set_launch(*args.launch)
+    pin(dev, args.launch, args.launch_classes)
     setup_s = time.perf_counter() - t0
     print(f'bench: setup {setup_s:.2f} s; warmup {args.warmup} steps', file=sys.stderr, flush=True)
     if args.warmup:
@@ -307,10 +417,11 @@ def main():
     tile = launch['gemm_pressure']
     lean_misses = None
     if args.no_kernel_timing:
-        gemm_ms = pcr_ms = step_ms = None
+        gemm_ms = pcr_ms = step_ms = gemm_isolated_ms = None
     else:
         prof = dev.profile(20, DT)
         gemm_ms, pcr_ms, step_ms = prof[0], prof[1], prof[2]
+        gemm_isolated_ms = prof[3]
         lean_misses = int(prof[4])
     yy, xx = WORKLOAD['ny'], WORKLOAD['nx']
     gemm_flops = 2.0 * yy * yy * xx
@@ -331,7 +442,8 @@ def main():
                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
                                   achieved=rate * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                                   frac=rate * STEP_BYTES_PER_CELL / HBM_PEAK),
-               kernel_ms=dict(gemm=gemm_ms, pcr=pcr_ms, step=step_ms, lean_misses=lean_misses), launch=launch,
+               kernel_ms=dict(gemm=gemm_ms, gemm_isolated=gemm_isolated_ms, pcr=pcr_ms, step=step_ms,
+                              lean_misses=lean_misses), launch=launch,
                speculation=dev.get_speculation(), setup_seconds=setup_s)
     if args.box_steps:
         out['box'] = box_leg(pkg, args.box_steps, 20, args.launch)

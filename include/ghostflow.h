@@ -172,9 +172,11 @@ int mcf_apply_operator(mcf_handle* h, int op, const double* in, double* out, dou
 /* max|div|, sum div^2, count, core sum div^2, core count over cells with mask==0. */
 int mcf_divergence_stats(mcf_handle* h, const double* vel, const unsigned char* mask, double* out5);
 
-/* Time the dominant kernels: average ms per launch of the dense y transform, the x tridiagonal
-   sweep and the whole step over n steps at dt (state advances). out: [gemm_ms, pcr_ms, step_ms,
-   gemm_launches_per_step, pcr_launches_per_step]. */
+/* Time the dominant kernels with HIP events on the library's stream (state advances): out =
+   [ms per launch of the pressure problem's forward transform over 50 back-to-back launches,
+    ms per launch of its tridiagonal sweep (50 back to back), ms per step over n lean steps at dt,
+    ms per launch of the forward transform over 50 launches each bracketed by its own events and
+    completed before the next, count of the n steps that would have refined]. */
 int mcf_profile(mcf_handle* h, int n, double dt, double* out5);
 
 /* The capacitance (Schur) system S y = c of the boundary-row correction (pressure_schur.py:92-111:
