@@ -2001,6 +2001,8 @@ __host__ __device__ inline bool solve_ok(const double* rec) {
 // without any, LEAN_PRES: the pressure solves without any, ONCE_PRES: the pressure solves without
 // the second)
 enum { LEAN_VEL = 1, LEAN_PRES = 2, LEAN_ALL = 3, ONCE_PRES = 4 };
+// The lean masks a speculating run switches between (spec_mask)
+constexpr int SPEC_MASKS[4] = {LEAN_ALL, 0, LEAN_VEL | ONCE_PRES, LEAN_VEL};
 struct FinArgs { int nproj; int lean; int force_miss; int gmres; };
 static_assert(MCF_DIAG_LEN == 64, "one diagnostics entry per lane of k_finalize");
 // Slots gathered by k_finalize: FIN_MAX maxima over the stripes, then FIN_SUM sums
@@ -3324,6 +3326,8 @@ struct StepGraphs {
       if (e != hipSuccess) return fail(x.err, 4, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
       graph[p] = g;
       HIPCHK(x.err, hipGraphInstantiate(&exec[p], g, nullptr, nullptr, 0));
+      // the executable's one-time upload here rather than at its first launch
+      HIPCHK(x.err, hipGraphUpload(exec[p], x.stream));
     }
     valid = true;
     return 0;
@@ -3437,11 +3441,19 @@ int ensure_tuned(mcf_handle* h) {
   return 0;
 }
 
+StepGraphs* graphs_for(mcf_handle* h, int lean) {
+  StepGraphs*& g = lean == LEAN_ALL ? h->lean_graphs : lean == 0 ? h->graphs : h->mixed[lean];
+  if (!g) g = new StepGraphs();
+  return g;
+}
+
 int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
   if (int rc = ensure_tuned(h)) return rc;
   const Ctx x = ctx_of(h);
-  StepGraphs* gs = lean == LEAN_ALL ? h->lean_graphs : lean == 0 ? h->graphs : h->mixed[lean];
-  if (!gs) gs = h->mixed[lean] = new StepGraphs();
+  StepGraphs* gs = graphs_for(h, lean);
+  // a speculating run switches between the SPEC_MASKS pairs as the transient comes and goes: all of
+  // them are captured when the first is needed, so that no later chunk pays a capture
+  const bool spec = h->spec > 0 && h->hs[0].dump_cap == 0 && h->pbackend == MCF_PB_DIRECT;
   hipLaunchKernelGGL(k_set_clock, dim3(1, 1), dim3(64), 0, h->stream, h->dc, h->time);
   for (int s = 0; s < n; ++s) {
     const bool steady = h->has_history && h->prev_dt == dt;
@@ -3456,8 +3468,14 @@ int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
       if (int rc = enqueue_step_kernels(x, h->parity, !h->has_history, x.gm ? 0 : lean, h->spec == 2)) return rc;
     } else {
       if (h->graph_dt != dt) drop_graphs(h);
-      if (!gs->valid)
+      if (!gs->valid) {
         if (int rc = gs->build(x, lean, h->spec == 2)) return rc;
+        for (int m : SPEC_MASKS) {
+          StepGraphs* g = graphs_for(h, m);
+          if (spec && !g->valid)
+            if (int rc = g->build(x, m, h->spec == 2)) return rc;
+        }
+      }
       h->graph_dt = dt;
       HIPCHK(h, hipGraphLaunch(gs->exec[h->parity], h->stream));
     }
@@ -4215,9 +4233,12 @@ void spec_update(int& chunk, int& full, bool lean, bool miss, bool quiet) {
   chunk = SPEC_MIN;
   full = quiet ? 0 : SPEC_FULL;
 }
-// In k diagnostics rows: any velocity-solve refinement (vel), any pressure solve refined twice (p2)
+// In the last SPEC_QUIET of k diagnostics rows: any velocity-solve refinement (vel), any pressure
+// solve refined twice (p2).  The tail predicts the next chunk: refinement needs fade through a
+// transient, and a start-up step (which runs in full anyway) should not make the chunk after it
+// launch every pass.  A step that needs a pass the mask leaves out is flagged and replayed.
 void refine_kinds(const double* rows, int k, bool& vel, bool& p2) {
-  for (int s = 0; s < k; ++s) {
+  for (int s = std::max(0, k - SPEC_QUIET); s < k; ++s) {
     const double* r = rows + (size_t)s * MCF_DIAG_LEN;
     vel = vel || r[MCF_D_H_REFINE] > 0;
     for (int q = 0; q < (int)r[MCF_D_P_SOLVES] && q < 3; ++q)
@@ -4891,7 +4912,15 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
       if (first || !e->use_graphs) {
         if (int rc = enqueue_step_kernels(x, h0->parity, first, lean, b->spec == 2)) return rc;
       } else {
-        if (!gs.valid) if (int rc = gs.build(x, lean, b->spec == 2)) return rc;
+        if (!gs.valid) {
+          if (int rc = gs.build(x, lean, b->spec == 2)) return rc;
+          // every pair the speculation can switch to, captured now (as the handle's)
+          for (int m : SPEC_MASKS) {
+            StepGraphs& g = m == LEAN_ALL ? b->lean_graphs : m == 0 ? b->graphs : b->mixed[m];
+            if (spec && !g.valid)
+              if (int rc = g.build(x, m, b->spec == 2)) return rc;
+          }
+        }
         HIPCHK(e, hipGraphLaunch(gs.exec[h0->parity], b->stream));
       }
       for (int c = 0; c < n; ++c) advance_host_state(b->cases[c], b->dt[c]);
