@@ -375,6 +375,10 @@ def main():
                     help='steps of the GMRES pressure backend leg (0: skip)')
     ap.add_argument('--ensemble-long', type=int, default=0,
                     help='only run the 64-case ensemble for this many steps and print its summary')
+    ap.add_argument('--warm-start-workload', action='store_true',
+                    help='only run the learned warm start on its benchmark workload (about 200k training pairs '
+                         'from the three channel / box cases, the 64 ensemble geometries at Re ~ U[200, 250]) '
+                         'and print its summary')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported', file=sys.stderr)
@@ -386,6 +390,12 @@ def main():
     pkg = cfd_amd_loader.load()
     if args.ensemble_long:
         ensemble_long(pkg, args.ensemble_long)
+        return
+    if args.warm_start_workload:
+        import importlib
+        ws = importlib.import_module(pkg.__name__ + '.warmstart')
+        res = ws.config5_study(log=lambda m: print('bench: ' + m, file=sys.stderr, flush=True))
+        print(json.dumps(res))
         return
     if args.traffic_probe:
         traffic_probe(pkg, args.steps, args.launch, args.launch_classes)

@@ -127,7 +127,7 @@ class GhostDevice:
         Cp.sort_indices()
         self.h.call('mcf_set_schur_system', nat.PRESSURE, nat.f64(ps.S), nat.i32(Cp.indptr), nat.i32(Cp.indices),
                     nat.f64(Cp.data), order)
-        self.schur_geometry = dict(order=order, features=feats)
+        self.schur_geometry = dict(order=order, features=feats, S=np.ascontiguousarray(ps.S[np.ix_(order, order)]))
         self.pressure_info = dict(backend='fft_boundary_schur', rank=ps.rank, dense_bytes=int(ps.S.nbytes),
                                   roundoff_drop_tolerance=ps.tolerance)
         self._upload_surface()

@@ -158,18 +158,31 @@ def min_residual_guess(S, c, basis):
     return x0
 
 
-def gmres_study(dev, c, y, tags, S, net=None, rtol=1e-12, max_iter=400):
+def bf16_round(x):
+    """x rounded to the nearest BF16 value (ties to even), returned as float64."""
+    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7fff + ((b >> 16) & 1)) >> 16) << 16
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def gmres_study(dev, c, y, tags, S, net=None, rtol=1e-12, max_iter=400, ceilings=False):
     """Arnoldi steps of device GMRES (to |c - S y| <= rtol |c|) and the error against the direct
     FP64 multiplier y, per initial guess: zero; the previous step's multiplier; the network's guess
     (raw, and scaled to minimum residual); and the previous multiplier combined with the network's
-    correction of its residual (raw sum, and the minimum-residual combination).  S is the Schur
-    matrix in angle order.  Pairs without a previous step are skipped for every guess, so all
+    correction of its residual (raw sum, and the minimum-residual combination).  With ``ceilings``
+    also the direct multiplier itself rounded to BF16 (at most 2^-8 relative per entry, the resolution of the
+    network's BF16 operands) and to FP32 (its output format): they bound what a guess at that
+    accuracy, however it is obtained, can save.  S is the
+    Schur matrix in angle order.  Pairs without a previous step are skipped for every guess, so all
     guesses see the same systems."""
     prev = previous_index(tags)
     keep = np.flatnonzero(prev >= 0)
     c, y, yp = c[keep], y[keep], y[prev[keep]]
     feats = dev.schur_geometry['features']
     guesses = {'zero': None, 'previous': yp}
+    if ceilings:
+        guesses['direct multiplier in BF16 (ceiling)'] = bf16_round(y)
+        guesses['direct multiplier in FP32'] = y.astype(np.float32).astype(np.float64)
     if net is not None:
         g = net.predict(c, feats)
         guesses['network'] = g
@@ -202,9 +215,8 @@ TRAINING_CASES = (
 
 
 def angle_schur(dev):
-    """The pressure Schur matrix in angle order (host copy)."""
-    o = dev.schur_geometry['order']
-    return dev.pressure_schur.S[np.ix_(o, o)]
+    """The pressure Schur matrix in angle order (host copy, kept with the boundary geometry)."""
+    return dev.schur_geometry['S']
 
 
 def step_differences(c, y, tags):
@@ -278,7 +290,7 @@ def study(steps=(2000, 4000, 2000), train_steps=3000, transient=200, dump_steps=
             continue
         dev = f.gg.device
         S = angle_schur(dev)
-        st = gmres_study(dev, c, y, tags, S, net)
+        st = gmres_study(dev, c, y, tags, S, net, ceilings=True)
         ctl = gmres_study(dev, c, y, tags, S, control)
         for g in ('network', 'network, min-residual scaled', 'previous+network, min-residual'):
             st[g.replace('network', 'untrained network', 1)] = ctl[g]
@@ -300,6 +312,117 @@ def study(steps=(2000, 4000, 2000), train_steps=3000, transient=200, dump_steps=
                loss_first=float(losses[:5].mean()), loss_last=float(losses[-50:].mean()),
                conv_ms_per_launch=net.profile(64, 340), seconds=time.perf_counter() - t0, heldout=cases,
                failed=failed, summary=summary)
+    net.close()
+    control.close()
+    return out
+
+
+def _accumulate(totals, group, st):
+    """Fold one case's per-guess GMRES figures into the group totals."""
+    for g, v in st.items():
+        tot = totals.setdefault(group, {}).setdefault(
+            g, dict(systems=0, iterations=0., max_iterations=0, initial=[], max_relres=0., max_error=0.))
+        tot['systems'] += v['systems']
+        tot['iterations'] += v['mean_iterations'] * v['systems']
+        tot['max_iterations'] = max(tot['max_iterations'], v['max_iterations'])
+        tot['initial'].append(v['median_initial_residual'])
+        tot['max_relres'] = max(tot['max_relres'], v['max_relres'])
+        tot['max_error'] = max(tot['max_error'], v['max_error_vs_direct'])
+
+
+def _summary(totals):
+    return {grp: {g: dict(systems=v['systems'], mean_iterations=v['iterations'] / v['systems'],
+                          max_iterations=v['max_iterations'], median_initial_residual=float(np.median(v['initial'])),
+                          max_relres=v['max_relres'], max_error_vs_direct=v['max_error'])
+                  for g, v in gs.items()} for grp, gs in totals.items()}
+
+
+def config5_study(c1_steps=2000, c2_seeds=(5000, 5001, 5002, 5003), c2_steps=22000, c3_steps=10000, ncases=64,
+                  re_range=(200., 250.), transient=20, dump_steps=30, train_steps=4000, seed=5000, log=None):
+    """The learned warm start on its benchmark workload.  Training pairs (about 200k with the default
+    counts) are dumped from the 440x82 Re 20 channel (zero start, dt 2e-3, c1_steps), the 1320x246
+    Re 100 channel started from the inlet profile with one seeded v-perturbation per seed in c2_seeds
+    (dt 5e-4, c2_steps each) and the 800x820 uniform-inflow box (its 1250 perturbed start-up steps
+    run first, then c3_steps are dumped, dt 2e-3); every group also trains on its step differences.
+    The evaluation systems are those of the 64 seeded ensemble geometries (``ensemble_case(k,
+    re_range)``: the ensemble's geometries, none trained on, with Reynolds numbers drawn from
+    U[200, 250]) from the zero start: `transient` steps run, then `dump_steps` steps are recorded,
+    and every system is solved by device GMRES to 1e-12 from each initial guess, the trained network
+    and an untrained one of the same initialisation (the control), with the BF16 / FP32 roundings
+    of the direct multiplier as ceilings.  Members that fail a gate of the reference step before
+    their systems are recorded are listed in `failed`."""
+    import time
+    from .ensemble import build_flows, ensemble_case
+    from .flow import MAC_CylinderFlow
+
+    def say(msg):
+        if log:
+            log(msg)
+
+    def flow(**kw):
+        return MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **kw)
+
+    t0 = time.perf_counter()
+    net = WarmStartNet(seed=seed)
+    control = WarmStartNet(seed=seed)
+    c1, c2, c3 = (kw for _, kw, _ in TRAINING_CASES)
+    runs = [('440x82 Re 20', c1, TRAINING_CASES[0][2], c1_steps, 0)]
+    runs += [(f'1320x246 Re 100 seed {s}', dict(c2, v_perturbation=dict(seed=s)), TRAINING_CASES[1][2], c2_steps, 0)
+             for s in c2_seeds]
+    runs.append(('800x820 box Re 100', c3, TRAINING_CASES[2][2], c3_steps, 1250))
+    pairs, groups = 0, []
+    for name, kw, dt, n, lead in (run for run in runs if run[3] > 0):
+        f = flow(**kw)
+        if lead:
+            f.solve(n_steps=lead, dt=dt, perturb_transient=True, save_artifacts=False, verbose=False)
+        c, y, tags = dump_pairs(f, n, dt)
+        fe = f.gg.device.schur_geometry['features']
+        net.add_pairs(c, y, fe)
+        net.add_pairs(*step_differences(c, y, tags), fe)
+        pairs += len(c)
+        groups.append(dict(case=name, pairs=int(len(c)), rank=int(c.shape[1]), steps=n))
+        say(f'{name}: {len(c)} pairs, r = {c.shape[1]} ({time.perf_counter() - t0:.0f} s)')
+        del c, y, tags
+        f.gg.device.h.close()
+    t1 = time.perf_counter()
+    losses = net.train(train_steps, positions=16384, lr=1e-3, seed=seed)
+    t2 = time.perf_counter()
+    say(f'trained {train_steps} steps on {pairs} pairs in {t2 - t1:.2f} s; '
+        f'loss {losses[:5].mean():.4g} -> {losses[-50:].mean():.4g}')
+    cases = [ensemble_case(k, re_range=re_range) for k in range(ncases)]
+    starts = {}
+    flows = build_flows(cases, failures=starts)
+    failed = [dict(case=k, error=v['message']) for k, v in starts.items()]
+    group = f'ensemble geometries, Re U[{re_range[0]:g},{re_range[1]:g}]'
+    totals, per_case = {}, []
+    for k, f in enumerate(flows):
+        if f is None:
+            continue
+        case = cases[k]
+        dev = f.gg.device
+        try:
+            if transient:
+                f.solve(n_steps=transient, dt=case['dt'], perturb_transient=False, save_artifacts=False, verbose=False)
+            c, y, tags = dump_pairs(f, dump_steps, case['dt'])
+        except RuntimeError as e:
+            failed.append(dict(case=k, error=str(e)))
+            dev.h.close()
+            continue
+        S = angle_schur(dev)
+        st = gmres_study(dev, c, y, tags, S, net, ceilings=True)
+        ctl = gmres_study(dev, c, y, tags, S, control)
+        for g in ('network', 'network, min-residual scaled', 'previous+network, min-residual'):
+            st[g.replace('network', 'untrained network', 1)] = ctl[g]
+        _accumulate(totals, group, st)
+        per_case.append(dict(case=k, grid=f"{case['nx']}x{case['ny']}", Re=round(case['Re'], 2), rank=int(c.shape[1]),
+                             steps={g: round(v['mean_iterations'], 2) for g, v in st.items()}))
+        say(f"ensemble case {k} ({case['nx']}x{case['ny']}, Re {case['Re']:.0f}, r {c.shape[1]}): "
+            + ', '.join(f"{g} {v['mean_iterations']:.1f}" for g, v in st.items()))
+        dev.h.close()
+        flows[k] = None
+    out = dict(training_pairs=pairs, training_groups=groups, training_steps=train_steps, training_seconds=t2 - t1,
+               loss_first=float(losses[:5].mean()), loss_last=float(losses[-50:].mean()), cases=per_case,
+               failed=failed, summary=_summary(totals), seconds=time.perf_counter() - t0)
     net.close()
     control.close()
     return out

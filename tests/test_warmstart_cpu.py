@@ -115,3 +115,70 @@ def test_previous_index(pkg, nsteps):
     ws = importlib.import_module(pkg.__name__ + '.warmstart')
     tags = np.array([2, 0, 1, 4, 5, 8, 9])
     assert list(ws.previous_index(tags)) == [-1, -1, -1, 1, 2, 3, 4]
+
+
+def _gmres_steps(S, c, x0=None, rtol=1e-12, maxit=400):
+    """Arnoldi steps of full (unrestarted) GMRES with CGS2 until |c - S x| <= rtol |c|."""
+    x = np.zeros_like(c) if x0 is None else x0
+    r0 = c - S @ x
+    beta, target = np.linalg.norm(r0), rtol * np.linalg.norm(c)
+    if beta <= target:
+        return 0
+    V = [r0 / beta]
+    H = np.zeros((maxit + 1, maxit))
+    g = np.zeros(maxit + 1)
+    g[0] = beta
+    cs, sn = np.zeros(maxit), np.zeros(maxit)
+    for j in range(maxit):
+        w = S @ V[j]
+        for _ in range(2):
+            d = np.array([q @ w for q in V])
+            H[:j + 1, j] += d
+            w = w - np.array(V).T @ d
+        H[j + 1, j] = np.linalg.norm(w)
+        V.append(w / H[j + 1, j])
+        for i in range(j):
+            t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+            H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+            H[i, j] = t
+        rho = np.hypot(H[j, j], H[j + 1, j])
+        cs[j], sn[j] = H[j, j] / rho, H[j + 1, j] / rho
+        g[j + 1], g[j] = -sn[j] * g[j], cs[j] * g[j]
+        if abs(g[j + 1]) <= target:
+            return j + 1
+    return maxit
+
+
+def test_guess_accuracy_bounds_the_warm_start_saving(pkg):
+    """Why a learned warm start cannot pay off on the capacitance system: on the 440x82 channel's S
+    (rank 116, cond ~3e5; pressure_schur.py:92-111) GMRES to 1e-12 needs about as many Arnoldi steps
+    from the exact multiplier rounded to BF16 (the resolution of the network's operands) as from
+    zero, while the FP32 rounding saves far more: the steps go into resolving the bulk of the
+    spectrum to 1e-12, and a guess accurate to ~1e-3 removes only the first few digits."""
+    import importlib
+    tables = importlib.import_module(pkg.__name__ + '.tables')
+    device = importlib.import_module(pkg.__name__ + '.device')
+    ws = importlib.import_module(pkg.__name__ + '.warmstart')
+    nx, ny, Ly = 440, 82, 4.1
+    yc = (np.arange(ny) + 0.5) * (Ly / ny)
+    inlet = 4 * 1.5 * yc * (Ly - yc) / Ly ** 2
+    ops = tables.GhostOperators(nx, ny, 22., Ly, 1., 2., 2., 0.05, 1., inlet)
+    S = device.pressure_pieces(ops)[1].S
+    r = S.shape[0]
+    # the network-side rounding is the oracle's BF16 rounding
+    probe = np.random.default_rng(3).normal(size=257)
+    assert np.array_equal(ws.bf16_round(probe), N.bf16(probe.astype(np.float32)).astype(np.float64))
+    rng = np.random.default_rng(4)
+    t = 2 * np.pi * np.arange(r) / r
+    steps = {'zero': [], 'bf16': [], 'fp32': []}
+    for _ in range(3):
+        y = np.cos(t + rng.uniform(0, 2 * np.pi)) + 0.5 * np.sin(2 * t) + 0.1 * rng.normal(size=r)
+        c = S @ y
+        steps['zero'].append(_gmres_steps(S, c))
+        steps['bf16'].append(_gmres_steps(S, c, ws.bf16_round(y)))
+        steps['fp32'].append(_gmres_steps(S, c, y.astype(np.float32).astype(np.float64)))
+    mean = {k: float(np.mean(v)) for k, v in steps.items()}
+    print('rank', r, 'mean Arnoldi steps', mean)
+    assert 40 <= mean['zero'] <= 80
+    assert mean['bf16'] >= 0.8 * mean['zero']          # a BF16-accurate guess saves under 20%
+    assert mean['fp32'] <= mean['bf16'] - 5            # accuracy, not the guess as such, is what saves
