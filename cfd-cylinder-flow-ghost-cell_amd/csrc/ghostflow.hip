@@ -80,9 +80,16 @@ constexpr int NSLOTW = NSTRIPE * SLOT_STRIDE;
 static_assert(NSLOT == 64 && NSLOT <= SLOT_STRIDE, "one slot per lane of a wave, all within a stripe");
 __host__ __device__ constexpr size_t slot_word(int k, int s) { return (size_t)s * SLOT_STRIDE + k; }
 
+// Two ints read as one 8-byte load (a CSR range, a solid interval)
+struct alignas(8) IPair { int x, y; };
+DEV IPair load_pair(const IPair* p, int k) {
+  const long long v = G(reinterpret_cast<const long long*>(p))[k];
+  return IPair{(int)(unsigned)(v & 0xffffffffll), (int)(v >> 32)};
+}
+
 struct Override {
   int j0, i0, bh, bw;
-  const int* map;   // bh*bw, row id or -1
+  const IPair* map;  // bh*bw: the CSR range (k0, k1) of the cell's override row, or (-1, -1)
   const int* ptr;
   const int* col;
   const double* val;
@@ -104,6 +111,7 @@ struct CaseDev {
   double pert_T;
   const int* lo[3];
   const int* hi[3];
+  const IPair* lohi[3];   // the same intervals as (lo, hi) pairs: one load per row
   int sbox[3][4];   // bounding box (j0, j1, i0, i1) of the solid nodes of each field
   Override ov[MCF_N_OPS];
   GhostTab gt[3];
@@ -313,18 +321,21 @@ DEV void block_sum_atomic(double v, double* dst) {
 DEV bool solid(const CaseDev& c, int field, int j, int i) {
   const int* b = c.sbox[field];
   if (j < b[0] || j > b[1] || i < b[2] || i > b[3]) return false;
-  const int lo = G(c.lo[field])[j], hi = G(c.hi[field])[j];   // both loads in flight together
-  return (lo <= i) & (i <= hi);
+  const IPair lh = load_pair(c.lohi[field], j);   // one 8-byte load: the interval's two ends arrive together
+  return (lh.x <= i) & (i <= lh.y);
 }
 DEV bool u_act(const CaseDev& c, int j, int i) { return i >= 1 && !solid(c, 0, j, i); }
 DEV bool v_act(const CaseDev& c, int j, int i) { return j >= 1 && j <= c.ny - 1 && !solid(c, 1, j, i); }
 DEV bool p_act(const CaseDev& c, int j, int i) { return !solid(c, 2, j, i); }
 
-DEV int ov_row(const Override& o, int j, int i) {
-  if (!o.map) return -1;
+// The CSR range of the override row of cell (j, i), or (-1, -1): the map holds the range itself, so
+// a row's entries are one dependent load away from the lookup
+DEV IPair ov_row(const Override& o, int j, int i) {
+  const IPair none{-1, -1};
+  if (!o.map) return none;
   int a = j - o.j0, b = i - o.i0;
-  if (a < 0 || b < 0 || a >= o.bh || b >= o.bw) return -1;
-  return G(o.map)[a * o.bw + b];
+  if (a < 0 || b < 0 || a >= o.bh || b >= o.bw) return none;
+  return load_pair(o.map, a * o.bw + b);
 }
 // Override rows hold up to ~16 nonzeros.  A row is read in batches of OV_B entries: the batch's
 // column and value loads go out together, then its x loads, so a batch costs two memory round trips
@@ -333,10 +344,10 @@ DEV int ov_row(const Override& o, int j, int i) {
 // in nonzero order, as a plain loop over the row adds them.
 constexpr int OV_B = 8;
 template <class P, class F>
-DEV void ov_batches(const Override& o, int r, P x, F use) {
-  const auto ptr = G(o.ptr), col = G(o.col);
+DEV void ov_batches(const Override& o, IPair r, P x, F use) {
+  const auto col = G(o.col);
   const auto val = G(o.val);
-  const int k0 = ptr[r], k1 = ptr[r + 1];
+  const int k0 = r.x, k1 = r.y;
   for (int k = k0; k < k1; k += OV_B) {
     int cc[OV_B];
     double vv[OV_B], xv[OV_B];
@@ -354,13 +365,13 @@ DEV void ov_batches(const Override& o, int r, P x, F use) {
   }
 }
 template <class P>
-DEV double ov_dot(const Override& o, int r, P x) {
+DEV double ov_dot(const Override& o, IPair r, P x) {
   double acc = 0.0;
   ov_batches(o, r, x, [&](double v, double xv) { acc += v * xv; });
   return acc;
 }
 template <class P>
-DEV void ov_dot_abs(const Override& o, int r, P x, double& acc, double& aacc, int& nnz) {
+DEV void ov_dot_abs(const Override& o, IPair r, P x, double& acc, double& aacc, int& nnz) {
   acc = 0.0; aacc = 0.0; nnz = 0;
   ov_batches(o, r, x, [&](double v, double xv) {
     double t = v * xv;
@@ -421,7 +432,7 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
       const double q_s = Q[js * nx1 + i], q_n = Q[jn * nx1 + i];
       const double p_e = Pp[j * nx + ic], p_w = Pp[j * nx + iw];
       const double pv = a.mode == 0 ? prev[idx] : 0.0;
-      const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
+      const IPair r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
       const bool act = u_act(c, j, i);
       // u at column 0 is the inlet profile
       const double a_w = (i == 1) ? inj : a_w0, q_w = (i == 1) ? inj : q_w0;
@@ -445,7 +456,7 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
       if (j > 0) lap += idy2 * q_s;
       const double diff = c.nu * lap;
       double grad = ((i < nx) ? (p_e - p_w) / dx : -2.0 * p_w / dx) / c.rho;
-      if (r >= 0) grad = ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp);
+      if (r.x >= 0) grad = ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp);
       const double doff = (i == 1) ? c.nu * inj * idx2 : 0.0;
       if (a.mode == 0) {
         nw = adv;
@@ -478,7 +489,7 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
       const double q_e = Qv[j * nx + ie], q_w = Qv[j * nx + iw];
       const double p_c = Pp[ju * nx + i], p_s = Pp[js * nx + i];
       const double pv = a.mode == 0 ? prev[idx] : 0.0;
-      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
+      const IPair r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
       const bool act = v_act(c, j, i);
       auto fyy = [&](double lo, double hi) { double vf = 0.5 * (lo + hi); return -(vf * vf * dx); };
       double fe;
@@ -494,7 +505,7 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
       if (i > 0) lap += idx2 * q_w;
       const double diff = c.nu * lap;
       double grad = (p_c - p_s) / dy / c.rho;
-      if (r >= 0) grad = ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp);
+      if (r.x >= 0) grad = ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp);
       if (a.mode == 0) {
         nw = adv;
         const double ex = (1.0 + alpha) * adv - alpha * pv;
@@ -559,7 +570,7 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
         const int j = idx / nx1, i = idx - j * nx1;
         const double x = X[j * nx1 + i], xw = X[j * nx1 + max(i - 1, 0)], xe = X[j * nx1 + min(i + 1, nx)];
         const double xn = X[min(j + 1, ny - 1) * nx1 + i], xs = X[max(j - 1, 0) * nx1 + i];
-        const int r = ov_row(c.ov[MCF_OP_HELM_U], j, i);   // looked up together with the solid test
+        const IPair r = ov_row(c.ov[MCF_OP_HELM_U], j, i);   // looked up together with the solid test
         const bool act = u_act(c, j, i);
         double lap = (-2.0 * idx2 - ((j == 0 || j == ny - 1) ? 3.0 : 2.0) * idy2) * x
                      + ((i == nx) ? 2.0 : 1.0) * idx2 * xw;
@@ -567,7 +578,7 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
         if (j < ny - 1) lap += idy2 * xn;
         if (j > 0) lap += idy2 * xs;
         double hx = x + h * (c.nu * lap);
-        if (act && r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_U], r, X);
+        if (act && r.x >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_U], r, X);
         if (act) {
           res = bv - hx;
           if (a.mode == 2) { r2 = x + res; xu = nanmax(xu, fabs(x)); }
@@ -579,14 +590,14 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
         // rows clamped for the wall faces j = 0 and j = ny (inactive)
         const double x = Xv[j * nx + i], xn = Xv[min(j + 1, ny) * nx + i], xs = Xv[max(j - 1, 0) * nx + i];
         const double xe = Xv[j * nx + min(i + 1, nx - 1)], xw = Xv[j * nx + max(i - 1, 0)];
-        const int r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
+        const IPair r = ov_row(c.ov[MCF_OP_HELM_V], j, i);
         const bool act = v_act(c, j, i);
         double lap = (-((i == 0) ? 3.0 : ((i == nx - 1) ? 1.0 : 2.0)) * idx2 - 2.0 * idy2) * x
                      + idy2 * (xn + xs);
         if (i < nx - 1) lap += idx2 * xe;
         if (i > 0) lap += idx2 * xw;
         double hx = x + h * (c.nu * lap);
-        if (act && r >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_V], r, X);
+        if (act && r.x >= 0) hx = ov_dot(c.ov[MCF_OP_HELM_V], r, X);
         if (act) {
           res = bv - hx;
           if (a.mode == 2) { r2 = x + res; xv = nanmax(xv, fabs(x)); }
@@ -633,11 +644,11 @@ DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
     // the cell's four faces (and the inlet value) load in one batch ahead of the activity test
     const double inj = inl[j], xl = X[j * nx1 + i], xr = X[j * nx1 + i + 1];
     const double vt = Xv[(j + 1) * nx + i], vb = Xv[j * nx + i];
-    const int r = ov_row(c.ov[MCF_OP_DIV], j, i);   // looked up together with the solid test
+    const IPair r = ov_row(c.ov[MCF_OP_DIV], j, i);   // looked up together with the solid test
     const bool act = p_act(c, j, i);
     const double ul = (i == 0) ? inj : xl;
     double dc = (xr - ul) / c.dx + (vt - vb) / c.dy;
-    if (act && r >= 0) {
+    if (act && r.x >= 0) {
       const double off = (i == 0) ? -inj / c.dx : 0.0;
       dc = ov_dot(c.ov[MCF_OP_DIV], r, X) + off;
     }
@@ -676,19 +687,19 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
       const double pe = Pp[j * nx + min(i, nx - 1)], pw = Pp[j * nx + max(i - 1, 0)];
-      const int r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
+      const IPair r = ov_row(c.ov[MCF_OP_GRAD_U], j, i);   // looked up together with the solid test
       act = u_act(c, j, i);
       g = ((i < nx) ? (pe - pw) / c.dx : -2.0 * pw / c.dx) / c.rho;
-      if (act && r >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp);
+      if (act && r.x >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_U], r, Pp);
     } else {
       const int t = idx - c.usize;
       const int j = t / nx, i = t - j * nx;
       // rows clamped for the wall faces j = 0 and j = ny (inactive)
       const double pc = Pp[min(j, c.ny - 1) * nx + i], ps = Pp[max(j - 1, 0) * nx + i];
-      const int r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
+      const IPair r = ov_row(c.ov[MCF_OP_GRAD_V], j, i);
       act = v_act(c, j, i);
       g = (pc - ps) / c.dy / c.rho;
-      if (act && r >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp);
+      if (act && r.x >= 0) g = ov_dot(c.ov[MCF_OP_GRAD_V], r, Pp);
     }
     if (act) out = xv - dt * g;
     O[idx] = out;
@@ -713,7 +724,7 @@ struct PresArgs { int b_id, x_id, out_id; int mode; int slot0; int res_slot; int
 // The stencil's five values load in one batch (a missing neighbour reads the centre, unused) and
 // the stencil row is formed for every row, so that the loads go out ahead of the override test.
 template <class P>
-DEV void p_row(const CaseDev& c, P X, int idx, int j, int i, int r, double& ax, double& aax, int& nnz) {
+DEV void p_row(const CaseDev& c, P X, int idx, int j, int i, IPair r, double& ax, double& aax, int& nnz) {
   const int nx = c.nx, ny = c.ny;
   const double x = X[idx];
   const double tw = X[i > 0 ? idx - 1 : idx], te = X[i < nx - 1 ? idx + 1 : idx];
@@ -728,7 +739,7 @@ DEV void p_row(const CaseDev& c, P X, int idx, int j, int i, int r, double& ax, 
   if (i < nx - 1) { ax += ox * te; aax += ox * fabs(te); ++nnz; }
   if (j > 0) { ax += oy * ts; aax += oy * fabs(ts); ++nnz; }
   if (j < ny - 1) { ax += oy * tn; aax += oy * fabs(tn); ++nnz; }
-  if (r >= 0) ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz);
+  if (r.x >= 0) ov_dot_abs(c.ov[MCF_OP_PRESSURE], r, X, ax, aax, nnz);
 }
 
 DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
@@ -742,7 +753,7 @@ DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
     double res = 0.0;
-    const int orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);   // looked up together with the solid test
+    const IPair orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);   // looked up together with the solid test
     const bool act = p_act(c, j, i);
     const double b = B[idx];
     double ax, aax; int nnz;
@@ -2350,7 +2361,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_matvec(const CaseDev* __restrict__
     if (idx >= c.psize) continue;
     const int j = idx / c.nx, i = idx - j * c.nx;
     double y = 0.0;
-    const int orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
+    const IPair orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
     if (p_act(c, j, i)) {
       double ax, aax; int nnz;
       p_row(c, X, idx, j, i, orow, ax, aax, nnz);
@@ -2506,7 +2517,7 @@ __global__ __launch_bounds__(KR_T) void k_kry_resid(const CaseDev* __restrict__ 
     if (idx >= c.psize) continue;
     const int j = idx / c.nx, i = idx - j * c.nx;
     double r = 0.0;
-    const int orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
+    const IPair orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);
     if (p_act(c, j, i)) {
       double ax, aax; int nnz;
       p_row(c, X, idx, j, i, orow, ax, aax, nnz);
@@ -3887,11 +3898,15 @@ int mcf_set_solid_intervals(mcf_handle* h, int field, const int* lo, const int* 
   const int rows = field_rows(h, field), cols = field_cols(h, field);
   for (int j = 0; j < rows; ++j)
     if (lo[j] < 0 || hi[j] >= cols) return fail(h, 1, "solid interval outside the array");
-  int* dlo; int* dhi;
+  int *dlo, *dhi, *dlh;
   std::string k = "solid" + std::to_string(field);
+  std::vector<int> lh((size_t)2 * rows);
+  for (int j = 0; j < rows; ++j) { lh[2 * j] = lo[j]; lh[2 * j + 1] = hi[j]; }
   if (int rc = upload(h, k + "lo", lo, rows, &dlo)) return rc;
   if (int rc = upload(h, k + "hi", hi, rows, &dhi)) return rc;
+  if (int rc = upload(h, k + "lh", lh.data(), lh.size(), &dlh)) return rc;
   h->hc.lo[field] = dlo; h->hc.hi[field] = dhi;
+  h->hc.lohi[field] = reinterpret_cast<const IPair*>(dlh);
   int j0 = rows, j1 = -1, i0 = cols, i1 = -1;
   for (int j = 0; j < rows; ++j)
     if (lo[j] <= hi[j]) { j0 = std::min(j0, j); j1 = std::max(j1, j); i0 = std::min(i0, lo[j]); i1 = std::max(i1, hi[j]); }
@@ -3959,10 +3974,13 @@ int mcf_set_override(mcf_handle* h, int op, int n, const int* row, const int* pt
   const int nnz = ptr[n];
   for (int k = 0; k < nnz; ++k) if (col[k] < 0 || col[k] >= in_size) return fail(h, 1, "override column out of range");
   const int bh = j1 - j0 + 1, bw = i1 - i0 + 1;
-  std::vector<int> map((size_t)bh * bw, -1);
+  // the map holds each override row's CSR range (k0, k1); (-1, -1) where the stencil row stands
+  std::vector<int> map((size_t)2 * bh * bw, -1);
   for (int k = 0; k < n; ++k) {
     const int j = row[k] / cols, i = row[k] % cols;
-    map[(size_t)(j - j0) * bw + (i - i0)] = k;
+    const size_t e = (size_t)(j - j0) * bw + (i - i0);
+    map[2 * e] = ptr[k];
+    map[2 * e + 1] = ptr[k + 1];
   }
   std::string key = "ov" + std::to_string(op);
   int *dm, *dp, *dcol; double* dv;
@@ -3970,7 +3988,7 @@ int mcf_set_override(mcf_handle* h, int op, int n, const int* row, const int* pt
   if (int rc = upload(h, key + "p", ptr, n + 1, &dp)) return rc;
   if (int rc = upload(h, key + "c", col, nnz, &dcol)) return rc;
   if (int rc = upload(h, key + "v", val, nnz, &dv)) return rc;
-  o = Override{j0, i0, bh, bw, dm, dp, dcol, dv};
+  o = Override{j0, i0, bh, bw, reinterpret_cast<const IPair*>(dm), dp, dcol, dv};
   return sync_descs(h);
 }
 
