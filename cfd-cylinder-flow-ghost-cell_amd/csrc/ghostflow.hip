@@ -387,11 +387,21 @@ DEV void ov_dot_abs(const Override& o, IPair r, P x, double& acc, double& aacc, 
 // ------------------------------------------------------------------------------------
 constexpr int GSB = 4096;
 
+// Step parameters as kernel arguments.  A single case's captured step fixes dt (its graphs are
+// rebuilt when dt changes), so its launches carry dt, the AB2 alpha and the pressure factor and no
+// kernel waits on a parameter load before its first operand load; dt = 0 (batches, whose cases
+// differ, and one-off launches) reads them from the case's parameter array.
+struct StepPrm { double dt, alpha, factor; };
+DEV double prm_dt(const CaseDev& c, const StepPrm& s) { return s.dt > 0.0 ? s.dt : G(c.prm)[0]; }
+DEV double prm_alpha(const CaseDev& c, const StepPrm& s) { return s.dt > 0.0 ? s.alpha : G(c.prm)[1]; }
+DEV double prm_factor(const CaseDev& c, const StepPrm& s) { return s.dt > 0.0 ? s.factor : G(c.prm)[2]; }
+
 // Stage 1: advection (flux form), AB2, diffusion and the CN right-hand side
 // (numba_kernels.py:17-142, 596-616; time_integration.py:30-93)
 struct AdvArgs {
   int q_id, adv_id, p_id, prev_id, now_id, out_id;
   int mode;  // 0: AB2 rhs, 1: projected-midpoint, 2: rhs with explicit = adv(adv_id), 3: out = L q + offset
+  StepPrm sp;
 };
 
 // SAME: the advected and the advecting fields are one buffer (the AB2 and midpoint steps), so their
@@ -409,7 +419,7 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
   const gptr<const double> in = G(c.inlet);
   const gptr<const double> Av = A + c.usize;
   const gptr<const double> Qv = Q + c.usize;
-  const double dt = c.prm[0], alpha = c.prm[1];
+  const double dt = prm_dt(c, a.sp), alpha = prm_alpha(c, a.sp);
   double mu = 0.0, mv = 0.0;
   // Each face loads all of its operands in one batch, at indices clamped into the arrays, and forms
   // its value without branches around the loads (inactive faces compute a value that is then
@@ -544,6 +554,7 @@ struct HelmArgs {
   int mode;   // 0: residual (out = b - Hx, slot RES_k), 1: final (max only), 2: defect
   int res_slot;  // SV_RES1/SV_RES2/SV_FINAL
   int pred_slot; // -1 always, else run only where that refinement happened
+  StepPrm sp;
 };
 
 DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
@@ -560,7 +571,7 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
     const gptr<double> O = a.out_id >= 0 ? G(c.buf[a.out_id]) : nullptr;
     const gptr<double> O2 = a.out2_id >= 0 ? G(c.buf[a.out2_id]) : nullptr;
     const double idx2 = 1.0 / (c.dx * c.dx), idy2 = 1.0 / (c.dy * c.dy);
-    const double h = -0.5 * c.prm[0];
+    const double h = -0.5 * prm_dt(c, a.sp);
     // a face's stencil operands and right-hand side load in one batch at clamped indices, ahead of
     // the activity test (as in the advection stage); override rows replace the stencil value
     for (int idx = lo + bx * blockDim.x + threadIdx.x; idx < hi; idx += gx * blockDim.x) {
@@ -627,7 +638,7 @@ __global__ __launch_bounds__(256) void k_helm(const CaseDev* __restrict__ cases,
 }
 
 // Divergence of a vel q-array: (D q + div_offset) * scale  (optimized_ghost_cell.py:212-216)
-struct DivArgs { int x_id, out_id; double dtmul; int slot; };   // dtmul 0: no scaling, else / (dtmul dt)
+struct DivArgs { int x_id, out_id; double dtmul; int slot; StepPrm sp; };   // dtmul 0: no scaling, else / (dtmul dt)
 
 DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
   if (bx * (int)blockDim.x >= c.psize) return;   // no cell of this case for this block
@@ -636,7 +647,7 @@ DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
   const gptr<const double> Xv = X + c.usize;
   const gptr<double> O = a.out_id >= 0 ? G(c.buf[a.out_id]) : nullptr;
   const gptr<const double> inl = G(c.inlet);
-  const double scale = a.dtmul == 0.0 ? 1.0 : 1.0 / (a.dtmul * c.prm[0]);
+  const double scale = a.dtmul == 0.0 ? 1.0 : 1.0 / (a.dtmul * prm_dt(c, a.sp));
   double m = 0.0;
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const int j = idx / nx, i = idx - j * nx;
@@ -666,7 +677,7 @@ __global__ __launch_bounds__(256) void k_div(const CaseDev* __restrict__ cases, 
 
 // Projection update out = q - dt G phi (numba_kernels.py:619-623), with the optional
 // coupled momentum residual -defect + dt G phi (time_integration.py:137-142)
-struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dtmul; };   // dt = dtmul * case dt
+struct UpdArgs { int x_id, phi_id, out_id, defect_id; double dtmul; StepPrm sp; };   // dt = dtmul * case dt
 
 DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
   if (bx * (int)blockDim.x >= c.velsize) return;   // no face of this case for this block
@@ -675,7 +686,7 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
   const gptr<const double> Pp = G(c.buf[a.phi_id]);
   const gptr<double> O = G(c.buf[a.out_id]);
   const gptr<const double> Df = a.defect_id >= 0 ? G(c.buf[a.defect_id]) : nullptr;
-  const double dt = a.dtmul * c.prm[0];
+  const double dt = a.dtmul * prm_dt(c, a.sp);
   double m = 0.0;
   const gptr<const double> Dl = Df ? Df : X;   // read unconditionally; used only with a defect
   // a face's pressure pair, old value and defect load in one batch at clamped indices, ahead of
@@ -1828,13 +1839,13 @@ __global__ __launch_bounds__(256) void k_inlet(const CaseDev* __restrict__ cases
 }
 
 // half_pressure = pressure + increment; output = half + factor * increment
-struct PressArgs { int p_id, inc_id, half_id, out_id; };
+struct PressArgs { int p_id, inc_id, half_id, out_id; StepPrm sp; };
 DEV void press_body(const CaseDev& c, const PressArgs& a, int bx, int gx) {
   const gptr<const double> P = G(c.buf[a.p_id]);
   const gptr<const double> I = G(c.buf[a.inc_id]);
   const gptr<double> H = G(c.buf[a.half_id]);
   const gptr<double> O = G(c.buf[a.out_id]);
-  const double factor = c.prm[2];
+  const double factor = prm_factor(c, a.sp);
   for (int idx = bx * blockDim.x + threadIdx.x; idx < c.psize; idx += gx * blockDim.x) {
     const double inc = I[idx];
     const double half = P[idx] + inc;
@@ -2909,6 +2920,7 @@ struct Ctx {
   bool noxf_p, noxf_h;   // some pressure / velocity problem has no x-FFT form
   int p_band, h_band;    // most band rows of a problem's capacitance entries (x-FFT form)
   int p_yy, p_xx, p_np, p_r, h_yy, h_xx, h_np, h_r;
+  StepPrm sp{};          // a single case's step parameters (dt = 0: read from the case)
 };
 
 void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
@@ -3008,11 +3020,11 @@ int read_slots(mcf_handle* h, std::vector<double>& out);
 
 // ---- stage bodies and merged launches ----
 Body body_of(int kind, int nblk) { Body b; std::memset(&b, 0, sizeof b); b.kind = kind; b.nblk = nblk; return b; }
-Body body_div(const Ctx& x, DivArgs a) { Body b = body_of(BK_DIV, gridSn(x.maxp, x.n).x); b.d = a; return b; }
-Body body_helm(const Ctx& x, HelmArgs a) { Body b = body_of(BK_HELM, gridSn(x.maxvel, x.n).x); b.h = a; return b; }
+Body body_div(const Ctx& x, DivArgs a) { Body b = body_of(BK_DIV, gridSn(x.maxp, x.n).x); a.sp = x.sp; b.d = a; return b; }
+Body body_helm(const Ctx& x, HelmArgs a) { Body b = body_of(BK_HELM, gridSn(x.maxvel, x.n).x); a.sp = x.sp; b.h = a; return b; }
 Body body_pres(const Ctx& x, PresArgs a) { Body b = body_of(BK_PRES, gridSn(x.maxp, x.n).x); b.p = a; return b; }
-Body body_upd(const Ctx& x, UpdArgs a) { Body b = body_of(BK_UPD, gridSn(x.maxvel, x.n).x); b.u = a; return b; }
-Body body_press(const Ctx& x, PressArgs a) { Body b = body_of(BK_PRESS, gridSn(x.maxp, x.n).x); b.q = a; return b; }
+Body body_upd(const Ctx& x, UpdArgs a) { Body b = body_of(BK_UPD, gridSn(x.maxvel, x.n).x); a.sp = x.sp; b.u = a; return b; }
+Body body_press(const Ctx& x, PressArgs a) { Body b = body_of(BK_PRESS, gridSn(x.maxp, x.n).x); a.sp = x.sp; b.q = a; return b; }
 Body body_fill(const Ctx& x, FillArgs a) { Body b = body_of(BK_FILL, grid1n(x.maxfill, 1).x); b.f = a; return b; }
 Body no_body() { return body_of(BK_NONE, 0); }
 
@@ -3226,6 +3238,7 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, in
       if ((rc = enqueue_apply(x, pres, B_RESP, out_id, 2, res_slot, shift))) return rc;
     } else {
       HelmArgs a{in_id, out_id, B_RES, -1, 0, res_slot, pred};
+      a.sp = x.sp;
       hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
       if ((rc = enqueue_apply(x, pres, B_RES, out_id, 2, res_slot, shift))) return rc;
     }
@@ -3283,16 +3296,16 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, boo
   if (x.pert) hipLaunchKernelGGL(k_inlet, dim3((x.maxny + 255) / 256, x.n), dim3(256), 0, x.stream, x.dc);
   if (first) {
     // projected midpoint startup (time_integration.py:33-47)
-    AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1};
+    AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1, x.sp};
     hipLaunchKernelGGL(k_adv_rhs, ga, dim3(256), 0, x.stream, x.dc, m);
     if ((rc = enqueue_project(x, B_TV0, B_HALF, B_TP0, 0.5, 2, -1))) return rc;
     FillArgs f{B_HALF, 1, 1, 0};
     hipLaunchKernelGGL(k_fill, grid1n(x.maxfill, x.n), dim3(256), 0, x.stream, x.dc, f);
-    AdvArgs r{S, B_HALF, B_PH, -1, -1, B_RHS, 2};
+    AdvArgs r{S, B_HALF, B_PH, -1, -1, B_RHS, 2, x.sp};
     hipLaunchKernelGGL(k_adv_rhs, ga, dim3(256), 0, x.stream, x.dc, r);
     nproj = 3;
   } else {
-    AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0};
+    AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0, x.sp};
     hipLaunchKernelGGL(k_adv_rhs, ga, dim3(256), 0, x.stream, x.dc, r);
   }
   // Stages that do not read each other's output share a launch: the Helmholtz statistics with the
@@ -3460,7 +3473,7 @@ StepGraphs* graphs_for(mcf_handle* h, int lean) {
 
 int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
   if (int rc = ensure_tuned(h)) return rc;
-  const Ctx x = ctx_of(h);
+  Ctx x = ctx_of(h);
   StepGraphs* gs = graphs_for(h, lean);
   // a speculating run switches between the SPEC_MASKS pairs as the transient comes and goes: all of
   // them are captured when the first is needed, so that no later chunk pays a capture
@@ -3470,6 +3483,8 @@ int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
     const bool steady = h->has_history && h->prev_dt == dt;
     const double alpha = !h->has_history ? 0.0 : dt / (2.0 * h->prev_dt);
     const double factor = !h->has_history ? 1.0 : dt / (dt + h->prev_dt);
+    // the launches carry the parameters k_set_prm stores (a captured steady step: dt, 0.5, 0.5)
+    x.sp = StepPrm{dt, alpha, factor};
     if (!(h->prm_valid && h->prm_dt == dt && h->prm_alpha == alpha && h->prm_factor == factor)) {
       hipLaunchKernelGGL(k_set_prm, dim3(1, 1), dim3(64), 0, h->stream, h->dc, dt, alpha, factor);
       h->prm_valid = true; h->prm_dt = dt; h->prm_alpha = alpha; h->prm_factor = factor;
