@@ -26,6 +26,7 @@ SEED = 20261203
 HBM_PEAK = 8.0e12          # B/s
 FP64_PEAK = 78.6e12        # FLOP/s
 STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB2/CN step
+REFINE_BYTES_PER_CELL = 96  # one executed refinement pass of a Woodbury solve: 12 field sweeps
 # HBM bytes per forward launch of the pressure transform by tile variant (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 from rocprofv3 --pmc, scaled by a 256 MiB copy; profiles/k_gemm_1320x246_traffic.md)
 GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6, 4: 9.657e6, 5: 9.754e6, 6: 9.732e6}
@@ -58,9 +59,7 @@ def box_leg(pkg, steps, warmup, launch=None):
                          'with the transient inflow perturbation, Re=100, dt=2e-3, FP64, reference outlet',
                 value=value, unit='cell-steps/s', ms_per_step=elapsed / steps * 1e3, steps=steps, warmup=warmup,
                 window=f'steps {warmup} .. {warmup + steps} from the default start', max_divergence=float(np.max(diag[:, 10])),
-                step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
-                                   achieved=value * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                                   frac=value * STEP_BYTES_PER_CELL / HBM_PEAK),
+                step_roofline=step_roofline(dev, diag[-steps:], value),
                 launch=dev.get_launch(), speculation=dev.get_speculation(), setup_seconds=setup_s)
 
 
@@ -345,6 +344,22 @@ def step_table(trace_dir, fetch_dir, write_dir, last=40):
     return '\n'.join(lines)
 
 
+def step_roofline(dev, diag, rate):
+    """Whole-step roofline of a timed window: the 928 B/cell-step of a steady step, and beside it the
+    bytes the window executed, each refinement pass of a Woodbury solve adding its two rectangle
+    inverses (12 field sweeps, 96 B per cell; a velocity component counts as one solve)."""
+    import importlib
+    nat = importlib.import_module(type(dev).__module__.rsplit('.', 1)[0] + '._native')
+    passes = (diag[:, nat.DIAG_NAMES.index('pressure_refinement_iterations')]
+              + diag[:, nat.DIAG_NAMES.index('helmholtz_refinement_iterations')])
+    per_step = float(np.mean(passes)) if len(passes) else 0.0
+    executed = STEP_BYTES_PER_CELL + REFINE_BYTES_PER_CELL * per_step
+    return dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL, achieved=rate * STEP_BYTES_PER_CELL / 1e9,
+                peak=HBM_PEAK / 1e9, unit='GB/s', frac=rate * STEP_BYTES_PER_CELL / HBM_PEAK,
+                refinement_passes_per_step=per_step, executed_bytes_per_cell_step=executed,
+                executed_frac=rate * executed / HBM_PEAK)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -449,9 +464,7 @@ def main():
                    traffic_unit='bytes per launch', algorithmic_bytes=8.0 * (yy * yy + 2 * yy * xx), tile=tile,
                    kernel=('k_gemm_sk' if tile >= 3 else 'k_gemm') + ' (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
                    launch_ms=gemm_ms),
-               step_roofline=dict(bound='hbm', bytes_per_cell_step=STEP_BYTES_PER_CELL,
-                                  achieved=rate * STEP_BYTES_PER_CELL / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                                  frac=rate * STEP_BYTES_PER_CELL / HBM_PEAK),
+               step_roofline=step_roofline(dev, diag, rate),
                kernel_ms=dict(gemm=gemm_ms, gemm_isolated=gemm_isolated_ms, pcr=pcr_ms, step=step_ms,
                               lean_misses=lean_misses), launch=launch,
                speculation=dev.get_speculation(), setup_seconds=setup_s)

@@ -402,15 +402,18 @@ DEV double prm_factor(const CaseDev& c, const StepPrm& s) { return s.dt > 0.0 ? 
 // id up to a constant shift, so the blocks with equal bx mod 8 share an XCD: its chunks form one
 // band of the field, and the neighbour rows a chunk reads are mostly the band's own, already in
 // that XCD's L2.  (A grid-stride order spread each block over the whole field: a 5-point stencil
-// over 1320 x 246 took 4.2 us per graph node that way and 3.0 us in bands.)
-struct Chunk { int beg, end; };
+// over 1320 x 246 took 4.2 us per graph node that way and 3.0 us in bands.)  Batched launches
+// (one grid row per case, a few hundred blocks each) keep the grid-stride order, which measured
+// faster there (64-case ensemble 2.36e9 against 2.27e9 cell-steps/s).
+struct Chunk { int beg, end, step; };
 DEV Chunk xcd_chunk(int lo, int hi, int bx, int gx) {
+  if (gridDim.y > 1) return Chunk{lo + bx * (int)blockDim.x, hi, gx * (int)blockDim.x};
   const int span = gx * (int)blockDim.x;
   const int per = (hi - lo + span - 1) / span * (int)blockDim.x;
   int lb = bx >> 3;
   for (int q = 0; q < (bx & 7); ++q) lb += (gx - q + 7) >> 3;   // blocks with index = q (mod 8)
   const int beg = lo + lb * per;
-  return Chunk{beg, min(hi, beg + per)};
+  return Chunk{beg, min(hi, beg + per), (int)blockDim.x};
 }
 
 // Stage 1: advection (flux form), AB2, diffusion and the CN right-hand side
@@ -444,7 +447,7 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
   // flight while they resolve.  Per-term loads under the boundary tests took one round trip per
   // test.  The arithmetic is the stencil's, term for term.
   const Chunk ch = xcd_chunk(0, c.velsize, blockIdx.x, gridDim.x);
-  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += blockDim.x) {
+  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += ch.step) {
     double res = 0.0, nw = 0.0;
     if (idx < c.usize) {
       const int j = idx / nx1, i = idx - j * nx1;
@@ -593,7 +596,7 @@ DEV void helm_body(const CaseDev& c, const HelmArgs& a, int bx, int gx) {
     const double h = -0.5 * prm_dt(c, a.sp);
     // a face's stencil operands and right-hand side load in one batch at clamped indices, ahead of
     // the activity test (as in the advection stage); override rows replace the stencil value
-    for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += blockDim.x) {
+    for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += ch.step) {
       double res = 0.0, r2 = 0.0;
       const double bv = B[idx];
       if (idx < c.usize) {
@@ -669,7 +672,7 @@ DEV void div_body(const CaseDev& c, const DivArgs& a, int bx, int gx) {
   const gptr<const double> inl = G(c.inlet);
   const double scale = a.dtmul == 0.0 ? 1.0 : 1.0 / (a.dtmul * prm_dt(c, a.sp));
   double m = 0.0;
-  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += blockDim.x) {
+  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += ch.step) {
     const int j = idx / nx, i = idx - j * nx;
     double d = 0.0;
     // the cell's four faces (and the inlet value) load in one batch ahead of the activity test
@@ -712,7 +715,7 @@ DEV void upd_body(const CaseDev& c, const UpdArgs& a, int bx, int gx) {
   const gptr<const double> Dl = Df ? Df : X;   // read unconditionally; used only with a defect
   // a face's pressure pair, old value and defect load in one batch at clamped indices, ahead of
   // the activity test; override rows replace the stencil gradient
-  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += blockDim.x) {
+  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += ch.step) {
     double out = 0.0, g;
     bool act;
     const double xv = X[idx], dv = Dl[idx];
@@ -783,7 +786,7 @@ DEV void pres_body(const CaseDev& c, const PresArgs& a, int bx, int gx) {
   const gptr<const double> X = G(c.buf[a.x_id]);
   const gptr<double> O = a.mode == 0 ? G(c.buf[a.out_id]) : nullptr;
   double mres = 0.0, mrnd = 0.0, msc = 0.0, ssr = 0.0, ssv = 0.0;
-  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += blockDim.x) {
+  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += ch.step) {
     const int j = idx / nx, i = idx - j * nx;
     double res = 0.0;
     const IPair orow = ov_row(c.ov[MCF_OP_PRESSURE], j, i);   // looked up together with the solid test
@@ -1869,7 +1872,7 @@ DEV void press_body(const CaseDev& c, const PressArgs& a, int bx, int gx) {
   const gptr<double> O = G(c.buf[a.out_id]);
   const double factor = prm_factor(c, a.sp);
   const Chunk ch = xcd_chunk(0, c.psize, bx, gx);
-  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += blockDim.x) {
+  for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += ch.step) {
     const double inc = I[idx];
     const double half = P[idx] + inc;
     H[idx] = half;
