@@ -71,3 +71,40 @@ def test_runner_end_to_end_with_restart(pkg, mods, tmp_path):
     assert abs(cd[-1] - rows2[-1]['cd']) <= 1e-12 * abs(cd[-1])
     with pytest.raises(FileExistsError):
         rn.run(cfg2, tmp_path / 'b', restart=tmp_path / 'a' / 'checkpoint.npz')
+
+
+def test_runner_history_matches_oracle(pkg, mods, tmp_path):
+    """The runner's rows and end state against the oracle's uninterrupted steps of the same case
+    (validation/run_ghost_validation.py:19-102 drives main.py's step loop): C_D and C_L of every
+    history row within 1e-10 of the step's force magnitude, the divergence under 1e-12, the final
+    checkpointed fields within 1e-10 of the oracle's, and delta_p equal to the oracle state's."""
+    from oracle.ghost_oracle import CylinderFlow
+    ob, rn = mods
+    kw = dict(nx=88, ny=16, Lx=22., Ly=4.1, xc=2., yc=2.)
+    case = dict(dt=1e-3, end_time=0.03, sample_start=0.01, kind='steady', profile_stride=5,
+                solver=dict(kw, boundary='ghost_cell', time_scheme='ab2_cn'))
+    cfg = tmp_path / 'case.json'
+    cfg.write_text(json.dumps(case))
+    print('runner: 88x16 channel, 30 steps at dt=1e-3, against the oracle', flush=True)
+    rn.run(cfg, tmp_path / 'out')
+    rows = [json.loads(x) for x in (tmp_path / 'out' / 'history.jsonl').read_text().splitlines()]
+    ref = CylinderFlow(**kw)
+    worst = 0.0
+    for row in rows:
+        ref.step(1e-3)
+        d = ref.last_step_diagnostics
+        scale = max(abs(d['cd']), abs(d['cl']))
+        err = max(abs(row['cd'] - d['cd']), abs(row['cl'] - d['cl'])) / scale
+        worst = max(worst, err)
+        assert err <= 1e-10, (row['step'], row['cd'], d['cd'], row['cl'], d['cl'])
+        assert row['max_divergence'] < 1e-12
+    print(f'worst force difference {worst:.2e} of the force magnitude over {len(rows)} rows', flush=True)
+    assert len(rows) == 30
+    end = pkg.MAC_CylinderFlow.from_checkpoint(tmp_path / 'out' / 'checkpoint.npz')
+    for key in ('u', 'v', 'p'):
+        a, b = getattr(end, key), getattr(ref, key)
+        err = float(np.max(abs(a - b)) / np.max(abs(b)))
+        print(f'final {key}: relative difference {err:.2e}', flush=True)
+        assert err < 1e-10, (key, err)
+    p = ob.stagnation_operator(end) @ ref.p.ravel()
+    assert abs(rows[-1]['delta_p'] - (p[0] - p[1])) <= 1e-10 * np.max(abs(p))
