@@ -390,6 +390,8 @@ def main():
                     help='steps of the GMRES pressure backend leg (0: skip)')
     ap.add_argument('--ensemble-long', type=int, default=0,
                     help='only run the 64-case ensemble for this many steps and print its summary')
+    ap.add_argument('--box-probe', type=int, default=0,
+                    help='for rocprofv3 runs: only this many steps of the 800x820 box (after 20 warm-up steps)')
     ap.add_argument('--warm-start-workload', action='store_true',
                     help='only run the learned warm start on its benchmark workload (about 200k training pairs '
                          'from the three channel / box cases, the 64 ensemble geometries at Re ~ U[200, 250]) '
@@ -414,6 +416,9 @@ def main():
         return
     if args.traffic_probe:
         traffic_probe(pkg, args.steps, args.launch, args.launch_classes)
+        return
+    if args.box_probe:
+        print(json.dumps(box_leg(pkg, args.box_probe, 20, args.launch)))
         return
     t0 = time.perf_counter()
     flow = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, initial='profile',
