@@ -90,7 +90,6 @@ DEV IPair load_pair(const IPair* p, int k) {
 struct Override {
   int j0, i0, bh, bw;
   const IPair* map;  // bh*bw: the CSR range (k0, k1) of the cell's override row, or (-1, -1)
-  const int* ptr;
   const int* col;
   const double* val;
 };
@@ -337,7 +336,7 @@ DEV IPair ov_row(const Override& o, int j, int i) {
   if (a < 0 || b < 0 || a >= o.bh || b >= o.bw) return none;
   return load_pair(o.map, a * o.bw + b);
 }
-// Override rows hold up to ~16 nonzeros.  A row is read in batches of OV_B entries: the batch's
+// Override rows hold up to 17 nonzeros.  A row is read in batches of OV_B entries: the batch's
 // column and value loads go out together, then its x loads, so a batch costs two memory round trips
 // instead of two per nonzero.  Indices past the row's end are clamped to its last entry (the loads
 // stay in the row and need no predicate) and their products are not added; the products are added
@@ -4003,7 +4002,7 @@ int mcf_set_override(mcf_handle* h, int op, int n, const int* row, const int* pt
     default: rows_field = MCF_FIELD_P; in_size = h->psize; break;
   }
   Override& o = h->hc.ov[op];
-  o = Override{0, 0, 0, 0, nullptr, nullptr, nullptr, nullptr};
+  o = Override{0, 0, 0, 0, nullptr, nullptr, nullptr};
   if (n == 0) return sync_descs(h);
   const int cols = field_cols(h, rows_field), rows = field_rows(h, rows_field);
   int j0 = 1 << 30, j1 = -1, i0 = 1 << 30, i1 = -1;
@@ -4024,12 +4023,11 @@ int mcf_set_override(mcf_handle* h, int op, int n, const int* row, const int* pt
     map[2 * e + 1] = ptr[k + 1];
   }
   std::string key = "ov" + std::to_string(op);
-  int *dm, *dp, *dcol; double* dv;
+  int *dm, *dcol; double* dv;
   if (int rc = upload(h, key + "m", map.data(), map.size(), &dm)) return rc;
-  if (int rc = upload(h, key + "p", ptr, n + 1, &dp)) return rc;
   if (int rc = upload(h, key + "c", col, nnz, &dcol)) return rc;
   if (int rc = upload(h, key + "v", val, nnz, &dv)) return rc;
-  o = Override{j0, i0, bh, bw, reinterpret_cast<const IPair*>(dm), dp, dcol, dv};
+  o = Override{j0, i0, bh, bw, reinterpret_cast<const IPair*>(dm), dcol, dv};
   return sync_descs(h);
 }
 

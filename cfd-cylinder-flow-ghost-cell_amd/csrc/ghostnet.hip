@@ -55,11 +55,29 @@ struct ConvArgs {
   const float* zprev; bf16_t* dz;      // mode 1
 };
 
+// The weights [mrows][kp] (at most 64 x 320 BF16, 40 KB) are staged in LDS once per block with 16-byte
+// loads, rows padded by 8 elements so that the 64 lanes' 8-byte fragment reads fall in distinct bank
+// pairs; every wave then reads its A fragments there instead of from L2 (which cost 40 KB per wave).
+constexpr int CONV_PAD = 8;
+inline size_t conv_lds_bytes(int mrows, int kp) { return (size_t)mrows * (kp + CONV_PAD) * sizeof(bf16_t); }
+
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+  extern __shared__ bf16_t wl[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int p = blockIdx.x * PB + w * 16 + l16;
   const int mt_n = a.mrows / 16;
+  const int ldw = a.kp + CONV_PAD;
+  {
+    // kp is a multiple of 16, so every row is a whole number of 8-element (16-byte) chunks
+    const int cpr = a.kp / 8, nchunk = a.mrows * cpr;
+    for (int q = threadIdx.x; q < nchunk; q += 256) {
+      const int row = q / cpr, c8 = q - row * cpr;
+      *reinterpret_cast<uint4*>(wl + (size_t)row * ldw + 8 * c8) =
+          *reinterpret_cast<const uint4*>(a.w + (size_t)row * a.kp + 8 * c8);
+    }
+  }
+  __syncthreads();
   f4 acc[4];
   for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
   int n = 0, t0 = 0;
@@ -74,7 +92,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
       b = *reinterpret_cast<const s4*>(a.in + ((size_t)n * a.L + tt) * a.cin + ci);
     }
     for (int mt = 0; mt < mt_n; ++mt) {
-      const s4 av = *reinterpret_cast<const s4*>(a.w + (size_t)(mt * 16 + l16) * a.kp + kk);
+      const s4 av = *reinterpret_cast<const s4*>(wl + (size_t)(mt * 16 + l16) * ldw + kk);
       acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b, acc[mt], 0, 0, 0);
     }
   }
@@ -365,7 +383,7 @@ void enqueue_forward(mcf_net* n, int L, int P) {
     ConvArgs a{n->wf[l], n->kp[l], n->mrows[l], n->act[l], n->cip[l], n->dil[l], L, P, 0,
                n->p + n->woff[l] + n->co[l] * n->ci[l] * KS, n->co[l], n->cop[l], n->z[l],
                last ? nullptr : n->act[l + 1], nullptr, nullptr};
-    hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, n->stream, a);
+    hipLaunchKernelGGL(k_conv, grid, dim3(256), conv_lds_bytes(a.mrows, a.kp), n->stream, a);
   }
 }
 
@@ -399,7 +417,7 @@ void enqueue_backward(mcf_net* n, int L, int P, const float* S = nullptr, const 
     if (l > 0) {
       ConvArgs a{n->wb[l], n->kpb[l], n->mrows_b[l], n->dz[l], n->cop[l], n->dil[l], L, P, 1,
                  nullptr, n->ci[l], n->cip[l], nullptr, nullptr, n->z[l - 1], n->dz[l - 1]};
-      hipLaunchKernelGGL(k_conv, dim3(nblk), dim3(256), 0, n->stream, a);
+      hipLaunchKernelGGL(k_conv, dim3(nblk), dim3(256), conv_lds_bytes(a.mrows, a.kp), n->stream, a);
     }
   }
 }
@@ -640,9 +658,10 @@ int mcf_net_profile(mcf_net* n, int nsamp, int L, int reps, float* ms_out) {
              n->p + n->woff[l] + n->co[l] * n->ci[l] * KS, n->co[l], n->cop[l], n->z[l], n->act[l + 1], nullptr, nullptr};
   hipEvent_t e0, e1;
   NCHK(n, hipEventCreate(&e0)); NCHK(n, hipEventCreate(&e1));
-  hipLaunchKernelGGL(k_conv, dim3((P + PB - 1) / PB), dim3(256), 0, n->stream, a);
+  const size_t lds = conv_lds_bytes(a.mrows, a.kp);
+  hipLaunchKernelGGL(k_conv, dim3((P + PB - 1) / PB), dim3(256), lds, n->stream, a);
   NCHK(n, hipEventRecord(e0, n->stream));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_conv, dim3((P + PB - 1) / PB), dim3(256), 0, n->stream, a);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_conv, dim3((P + PB - 1) / PB), dim3(256), lds, n->stream, a);
   NCHK(n, hipEventRecord(e1, n->stream));
   NCHK(n, hipEventSynchronize(e1));
   float ms = 0.f;
