@@ -49,7 +49,6 @@ struct ConvArgs {
   const bf16_t* w; int kp, mrows;      // weights [mrows][kp], mrows in {16, 32, 48, 64}
   const bf16_t* in; int cin;           // input [P][cin], cin a multiple of 4
   int dil, L, P;
-  int mode;
   const float* bias; int cout, cstride;
   float* z; bf16_t* act;               // mode 0 outputs ([P][cstride]); act may be null
   const float* zprev; bf16_t* dz;      // mode 1
@@ -61,56 +60,127 @@ struct ConvArgs {
 constexpr int CONV_PAD = 8;
 inline size_t conv_lds_bytes(int mrows, int kp) { return (size_t)mrows * (kp + CONV_PAD) * sizeof(bf16_t); }
 
+template <int MODE, int MT>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   extern __shared__ bf16_t wl[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int p = blockIdx.x * PB + w * 16 + l16;
-  const int mt_n = a.mrows / 16;
   const int ldw = a.kp + CONV_PAD;
-  {
-    // kp is a multiple of 16, so every row is a whole number of 8-element (16-byte) chunks
-    const int cpr = a.kp / 8, nchunk = a.mrows * cpr;
-    for (int q = threadIdx.x; q < nchunk; q += 256) {
-      const int row = q / cpr, c8 = q - row * cpr;
-      *reinterpret_cast<uint4*>(wl + (size_t)row * ldw + 8 * c8) =
-          *reinterpret_cast<const uint4*>(a.w + (size_t)row * a.kp + 8 * c8);
-    }
-  }
-  __syncthreads();
-  f4 acc[4];
-  for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
   int n = 0, t0 = 0;
   if (p < a.P) { n = p / a.L; t0 = p - n * a.L; }
-  for (int kk0 = 0; kk0 < a.kp; kk0 += 16) {
-    const int kk = kk0 + 4 * kq;
+  // Every global read of the block goes out before the first wait: the B fragments of the whole K
+  // loop (kp <= 320 = 5 taps x 64 channels, so at most 20), the epilogue's operands (the bias, or
+  // in the adjoint pass the previous layer's pre-activation, 4 consecutive rows per lane), and the
+  // block's share of the weight staging (at most 64 x 320 / 8 / 256 = 10 16-byte chunks per thread).
+  // Issued one after another as in a plain loop, each would cost its own memory round trip.
+  constexpr int KMAX = 20, STAGE_MAX = 10;
+  s4 bv[KMAX];
+  #pragma unroll
+  for (int t = 0; t < KMAX; ++t) {
+    const int kk = 16 * t + 4 * kq;
     const int tap = kk / a.cin, ci = kk - tap * a.cin;
     s4 b = {0, 0, 0, 0};
-    if (p < a.P && tap < KS) {
+    if (16 * t < a.kp && p < a.P && tap < KS) {
       int tt = (t0 + (tap - 2) * a.dil) % a.L;
       if (tt < 0) tt += a.L;
       b = *reinterpret_cast<const s4*>(a.in + ((size_t)n * a.L + tt) * a.cin + ci);
     }
-    for (int mt = 0; mt < mt_n; ++mt) {
-      const s4 av = *reinterpret_cast<const s4*>(wl + (size_t)(mt * 16 + l16) * ldw + kk);
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b, acc[mt], 0, 0, 0);
+    bv[t] = b;
+  }
+  // output rows o = mt*16 + 4 kq + e (e < 4); the row stride is a multiple of 4, so a lane's 4 rows
+  // are all inside it or all outside, and they are 16-byte (FP32) / 8-byte (BF16) aligned
+  f4 ep[4];
+  #pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const int o = mt * 16 + 4 * kq;
+    f4 v = {0.f, 0.f, 0.f, 0.f};
+    if (mt < MT && p < a.P && o < a.cstride) {
+      if (MODE == 0) {
+        #pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = o + e < a.cout ? a.bias[o + e] : 0.f;
+      } else {
+        v = *reinterpret_cast<const f4*>(a.zprev + (size_t)p * a.cstride + o);
+      }
+    }
+    ep[mt] = v;
+  }
+  {
+    // kp is a multiple of 16, so every row is a whole number of 8-element (16-byte) chunks
+    const int cpr = a.kp / 8, nchunk = a.mrows * cpr;
+    // Indexes beyond the last chunk are clamped to it: those threads read and write the last
+    // chunk again (the same bytes to the same place), which keeps both loops free of branches.
+    uint4 st[STAGE_MAX];
+    int off[STAGE_MAX];
+    #pragma unroll
+    for (int r = 0; r < STAGE_MAX; ++r) {
+      const int q = min((int)threadIdx.x + 256 * r, nchunk - 1);
+      const int row = q / cpr, c8 = q - row * cpr;
+      st[r] = *reinterpret_cast<const uint4*>(a.w + (size_t)row * a.kp + 8 * c8);
+      off[r] = row * ldw + 8 * c8;
+    }
+    #pragma unroll
+    for (int r = 0; r < STAGE_MAX; ++r) *reinterpret_cast<uint4*>(wl + off[r]) = st[r];
+  }
+  __syncthreads();
+  f4 acc[MT];
+  #pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  // MT row tiles (mrows / 16, a template argument) on independent accumulators; the next K step's
+  // A fragments are read from LDS before this step's MFMAs issue.  (A loop over a run-time tile
+  // count kept one accumulator and waited for each fragment read before its MFMA.)
+  s4 av[MT];
+  #pragma unroll
+  for (int mt = 0; mt < MT; ++mt) av[mt] = *reinterpret_cast<const s4*>(wl + (size_t)(mt * 16 + l16) * ldw + 4 * kq);
+  #pragma clang loop unroll(full)
+  for (int t = 0; t < KMAX; ++t) {
+    if (16 * t < a.kp) {
+      s4 an[MT];
+      // the step after the last reads row 0's first fragment again (in bounds, unused)
+      const int kn = 16 * (t + 1) < a.kp ? 16 * (t + 1) : 0;
+      #pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        an[mt] = *reinterpret_cast<const s4*>(wl + (size_t)(mt * 16 + l16) * ldw + kn + 4 * kq);
+      #pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[mt], bv[t], acc[mt], 0, 0, 0);
+      #pragma unroll
+      for (int mt = 0; mt < MT; ++mt) av[mt] = an[mt];
     }
   }
   if (p >= a.P) return;
-  for (int mt = 0; mt < mt_n; ++mt)
-    for (int e = 0; e < 4; ++e) {
-      const int o = mt * 16 + 4 * kq + e;
-      if (o >= a.cstride) continue;
-      const size_t q = (size_t)p * a.cstride + o;
-      if (a.mode == 0) {
-        const float zv = o < a.cout ? acc[mt][e] + a.bias[o] : 0.f;
-        a.z[q] = zv;
-        if (a.act) a.act[q] = f2bf(zv > 0.f ? zv : 0.f);
-      } else {
-        const float g = o < a.cout && a.zprev[q] > 0.f ? acc[mt][e] : 0.f;
-        a.dz[q] = f2bf(g);
+  #pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int o = mt * 16 + 4 * kq;
+    if (o >= a.cstride) continue;
+    const size_t q = (size_t)p * a.cstride + o;
+    unsigned short h[4];
+    if (MODE == 0) {
+      f4 zv;
+      #pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        zv[e] = o + e < a.cout ? acc[mt][e] + ep[mt][e] : 0.f;
+        h[e] = f2bf(zv[e] > 0.f ? zv[e] : 0.f);
       }
+      *reinterpret_cast<f4*>(a.z + q) = zv;
+      if (a.act) *reinterpret_cast<s4*>(a.act + q) = s4{(short)h[0], (short)h[1], (short)h[2], (short)h[3]};
+    } else {
+      #pragma unroll
+      for (int e = 0; e < 4; ++e) h[e] = f2bf(o + e < a.cout && ep[mt][e] > 0.f ? acc[mt][e] : 0.f);
+      *reinterpret_cast<s4*>(a.dz + q) = s4{(short)h[0], (short)h[1], (short)h[2], (short)h[3]};
     }
+  }
+}
+
+template <int MODE>
+void launch_conv(const ConvArgs& a, int nblk, hipStream_t st) {
+  const size_t lds = conv_lds_bytes(a.mrows, a.kp);
+  switch (a.mrows / 16) {
+    case 1: hipLaunchKernelGGL((k_conv<MODE, 1>), dim3(nblk), dim3(256), lds, st, a); break;
+    case 2: hipLaunchKernelGGL((k_conv<MODE, 2>), dim3(nblk), dim3(256), lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_conv<MODE, 3>), dim3(nblk), dim3(256), lds, st, a); break;
+    default: hipLaunchKernelGGL((k_conv<MODE, 4>), dim3(nblk), dim3(256), lds, st, a); break;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -126,55 +196,138 @@ struct ConvWArgs {
   float* part; int stride;             // [nblk][stride], stride = cout*cinr*KS + cout
 };
 
+// Both operands of this GEMM have the positions as their K dimension, which the [P][channels]
+// layouts hold down a column: read straight from global memory every fragment was four 2-byte
+// gathers.  The block stages its 64 positions in LDS transposed instead, dz as dzT[o][p] and the
+// shifted input as inT[k*cin+i][p]: a thread takes one 4-channel chunk c (lanes along the channels,
+// so each row is read by 16 consecutive lanes) of 4 consecutive positions, issues all of its loads
+// before the first write, and writes each channel's 4 positions as one 8-byte store.  With rows
+// CW_PITCH = 68 elements (34 dwords) apart, the 16 lanes of a sweep (4 chunks x 4 position quads)
+// cover the 32 banks once, and an MFMA fragment (4 consecutive positions of one row) is one 8-byte
+// read.  Requires cin <= 64 and mrows <= 64 (16 chunks), as the network's shapes are.
+constexpr int CW_PITCH = PB + 4;
+inline size_t convw_lds_bytes(int mrows, int kp) { return (size_t)(mrows + kp) * CW_PITCH * sizeof(bf16_t); }
+
 __global__ __launch_bounds__(256) void k_convw(ConvWArgs a) {
+  extern __shared__ bf16_t cw[];
+  bf16_t* dzT = cw;                                   // [mrows][CW_PITCH]
+  bf16_t* inT = cw + (size_t)a.mrows * CW_PITCH;      // [kp][CW_PITCH]
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int pbase = blockIdx.x * PB;
+  const int c = threadIdx.x & 15, quad = threadIdx.x >> 4;     // chunk, positions 4 quad .. 4 quad + 3
+  const int c4i = a.cin / 4;
+  int nL[4], t[4];
+  bool ok[4];
+  {
+    const int p0 = pbase + 4 * quad;
+    int n = p0 / a.L, tt = p0 - n * a.L;
+    #pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ok[i] = p0 + i < a.P;
+      nL[i] = n * a.L; t[i] = tt;
+      if (++tt == a.L) { tt = 0; ++n; }
+    }
+  }
+  s4 vin[KS][4], vz[4];
+  #pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int off = (k - 2) * a.dil;
+    #pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      s4 x = {0, 0, 0, 0};
+      if (ok[i] && c < c4i) {
+        int tt = t[i] + off;
+        while (tt < 0) tt += a.L;
+        while (tt >= a.L) tt -= a.L;
+        x = *reinterpret_cast<const s4*>(a.in + (size_t)(nL[i] + tt) * a.cin + 4 * c);
+      }
+      vin[k][i] = x;
+    }
+  }
+  #pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    s4 x = {0, 0, 0, 0};
+    if (ok[i] && 4 * c < a.cop) x = *reinterpret_cast<const s4*>(a.dz + (size_t)(pbase + 4 * quad + i) * a.cop + 4 * c);
+    vz[i] = x;
+  }
+  #pragma unroll
+  for (int k = 0; k < KS; ++k)
+    if (c < c4i) {
+      #pragma unroll
+      for (int j = 0; j < 4; ++j)
+        *reinterpret_cast<s4*>(inT + (size_t)(k * a.cin + 4 * c + j) * CW_PITCH + 4 * quad) =
+            s4{vin[k][0][j], vin[k][1][j], vin[k][2][j], vin[k][3][j]};
+    }
+  if (4 * c < a.mrows) {
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool live = 4 * c + j < a.cout;
+      *reinterpret_cast<s4*>(dzT + (size_t)(4 * c + j) * CW_PITCH + 4 * quad) =
+          live ? s4{vz[0][j], vz[1][j], vz[2][j], vz[3][j]} : s4{0, 0, 0, 0};
+    }
+  }
+  // the K padding rows k*cin+i >= KS*cin (fewer than 16) are zero
+  for (int q = threadIdx.x; q < (a.kp - KS * a.cin) * (PB / 4); q += 256) {
+    const int r = KS * a.cin + q / (PB / 4), p4 = q % (PB / 4);
+    *reinterpret_cast<s4*>(inT + (size_t)r * CW_PITCH + 4 * p4) = s4{0, 0, 0, 0};
+  }
+  __syncthreads();
   const int mt_n = a.mrows / 16, nt_n = a.kp / 16;
   float* part = a.part + (size_t)blockIdx.x * a.stride;
   for (int tile = w; tile < mt_n * nt_n; tile += 4) {
     const int mt = tile / nt_n, nt = tile - mt * nt_n;
-    const int o = mt * 16 + l16;
-    const int kk = nt * 16 + l16, tap = kk / a.cin, ci = kk - tap * a.cin;
     f4 acc = {0.f, 0.f, 0.f, 0.f};
+    #pragma unroll
     for (int s = 0; s < PB; s += 16) {
-      s4 av, bv;
-      for (int e = 0; e < 4; ++e) {
-        const int p = pbase + s + 4 * kq + e;
-        av[e] = (short)((p < a.P && o < a.cout) ? a.dz[(size_t)p * a.cop + o] : 0);
-        short bb = 0;
-        if (p < a.P && tap < KS) {
-          const int n = p / a.L, t = p - n * a.L;
-          int tt = (t + (tap - 2) * a.dil) % a.L;
-          if (tt < 0) tt += a.L;
-          bb = (short)a.in[((size_t)n * a.L + tt) * a.cin + ci];
-        }
-        bv[e] = bb;
-      }
+      const s4 av = *reinterpret_cast<const s4*>(dzT + (size_t)(mt * 16 + l16) * CW_PITCH + s + 4 * kq);
+      const s4 bv = *reinterpret_cast<const s4*>(inT + (size_t)(nt * 16 + l16) * CW_PITCH + s + 4 * kq);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
-    // D[4*kq+e][l16]: row o' = mt*16 + 4*kq + e, column kk = nt*16 + l16
+    // D[4*kq+e][l16]: row o = mt*16 + 4*kq + e, column kk = nt*16 + l16
+    const int kk = nt * 16 + l16, tap = kk / a.cin, ci = kk - tap * a.cin;
+    #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int oo = mt * 16 + 4 * kq + e;
       if (oo < a.cout && tap < KS && ci < a.cinr) part[((size_t)oo * a.cinr + ci) * KS + tap] = acc[e];
     }
   }
   if ((int)threadIdx.x < a.cout) {
-    float s = 0.f;
-    for (int q = 0; q < PB; ++q) {
-      const int p = pbase + q;
-      if (p < a.P) s += bf2f(a.dz[(size_t)p * a.cop + threadIdx.x]);
+    float sum = 0.f;
+    #pragma unroll
+    for (int p4 = 0; p4 < PB / 4; ++p4) {
+      const s4 v = *reinterpret_cast<const s4*>(dzT + (size_t)threadIdx.x * CW_PITCH + 4 * p4);
+      sum += (bf2f(v[0]) + bf2f(v[1])) + (bf2f(v[2]) + bf2f(v[3]));
     }
-    part[(size_t)a.cout * a.cinr * KS + threadIdx.x] = s;
+    part[(size_t)a.cout * a.cinr * KS + threadIdx.x] = sum;
   }
 }
 
-__global__ void k_reduce(const float* part, int nblk, int stride, float* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= stride) return;
+// Column sums of part[nblk][stride]: a block takes 64 columns, its 4 waves a quarter of the rows
+// each (rows y, y + 4, ...), 16 loads in flight per thread; the quarters then add in a fixed order.
+// One thread per column reading its rows in turn waited a memory round trip per row (60 us for 256
+// rows).
+constexpr int RED_COLS = 64, RED_SLICES = 4, RED_BATCH = 16;
+__global__ __launch_bounds__(RED_COLS * RED_SLICES) void k_reduce(const float* part, int nblk, int stride, float* out) {
+  __shared__ float red[RED_SLICES][RED_COLS];
+  const int x = threadIdx.x % RED_COLS, y = threadIdx.x / RED_COLS;
+  const int i = blockIdx.x * RED_COLS + x;
   float s = 0.f;
-  for (int b = 0; b < nblk; ++b) s += part[(size_t)b * stride + i];
-  out[i] = s;
+  if (i < stride) {
+    for (int b0 = y; b0 < nblk; b0 += RED_SLICES * RED_BATCH) {
+      float v[RED_BATCH];
+      #pragma unroll
+      for (int r = 0; r < RED_BATCH; ++r) {
+        const int b = b0 + RED_SLICES * r;
+        v[r] = b < nblk ? part[(size_t)b * stride + i] : 0.f;
+      }
+      #pragma unroll
+      for (int r = 0; r < RED_BATCH; ++r) s += v[r];
+    }
+  }
+  red[y][x] = s;
+  __syncthreads();
+  if (y == 0 && i < stride) out[i] = (red[0][x] + red[1][x]) + (red[2][x] + red[3][x]);
 }
 
 // loss = mean_p (z[p][0] - y[p])^2 ; dz[p][0] = bf16(2 (z - y) / P), other channels 0
@@ -380,10 +533,10 @@ void enqueue_forward(mcf_net* n, int L, int P) {
   const dim3 grid((P + PB - 1) / PB);
   for (int l = 0; l < n->layers; ++l) {
     const bool last = l == n->layers - 1;
-    ConvArgs a{n->wf[l], n->kp[l], n->mrows[l], n->act[l], n->cip[l], n->dil[l], L, P, 0,
+    ConvArgs a{n->wf[l], n->kp[l], n->mrows[l], n->act[l], n->cip[l], n->dil[l], L, P,
                n->p + n->woff[l] + n->co[l] * n->ci[l] * KS, n->co[l], n->cop[l], n->z[l],
                last ? nullptr : n->act[l + 1], nullptr, nullptr};
-    hipLaunchKernelGGL(k_conv, grid, dim3(256), conv_lds_bytes(a.mrows, a.kp), n->stream, a);
+    launch_conv<0>(a, (int)grid.x, n->stream);
   }
 }
 
@@ -411,13 +564,13 @@ void enqueue_backward(mcf_net* n, int L, int P, const float* S = nullptr, const 
     const int stride = layer_params(n, l);
     ConvWArgs w{n->dz[l], n->cop[l], n->co[l], n->act[l], n->cip[l], n->ci[l], n->dil[l], L, P,
                 up16(n->co[l]), up16(KS * n->cip[l]), n->part, stride};
-    hipLaunchKernelGGL(k_convw, dim3(nblk), dim3(256), 0, n->stream, w);
-    hipLaunchKernelGGL(k_reduce, dim3((stride + 255) / 256), dim3(256), 0, n->stream, n->part, nblk, stride,
-                       n->g + n->woff[l]);
+    hipLaunchKernelGGL(k_convw, dim3(nblk), dim3(256), convw_lds_bytes(w.mrows, w.kp), n->stream, w);
+    hipLaunchKernelGGL(k_reduce, dim3((stride + RED_COLS - 1) / RED_COLS), dim3(RED_COLS * RED_SLICES), 0, n->stream,
+                       n->part, nblk, stride, n->g + n->woff[l]);
     if (l > 0) {
-      ConvArgs a{n->wb[l], n->kpb[l], n->mrows_b[l], n->dz[l], n->cop[l], n->dil[l], L, P, 1,
+      ConvArgs a{n->wb[l], n->kpb[l], n->mrows_b[l], n->dz[l], n->cop[l], n->dil[l], L, P,
                  nullptr, n->ci[l], n->cip[l], nullptr, nullptr, n->z[l - 1], n->dz[l - 1]};
-      hipLaunchKernelGGL(k_conv, dim3(nblk), dim3(256), conv_lds_bytes(a.mrows, a.kp), n->stream, a);
+      launch_conv<1>(a, nblk, n->stream);
     }
   }
 }
@@ -654,14 +807,13 @@ int mcf_net_profile(mcf_net* n, int nsamp, int L, int reps, float* ms_out) {
   if (int rc = ensure_cap(n, P)) return rc;
   NCHK(n, hipMemset(n->act[1], 0, (size_t)P * n->cip[1] * sizeof(bf16_t)));
   const int l = 1;
-  ConvArgs a{n->wf[l], n->kp[l], n->mrows[l], n->act[l], n->cip[l], n->dil[l], L, P, 0,
+  ConvArgs a{n->wf[l], n->kp[l], n->mrows[l], n->act[l], n->cip[l], n->dil[l], L, P,
              n->p + n->woff[l] + n->co[l] * n->ci[l] * KS, n->co[l], n->cop[l], n->z[l], n->act[l + 1], nullptr, nullptr};
   hipEvent_t e0, e1;
   NCHK(n, hipEventCreate(&e0)); NCHK(n, hipEventCreate(&e1));
-  const size_t lds = conv_lds_bytes(a.mrows, a.kp);
-  hipLaunchKernelGGL(k_conv, dim3((P + PB - 1) / PB), dim3(256), lds, n->stream, a);
+  launch_conv<0>(a, (P + PB - 1) / PB, n->stream);
   NCHK(n, hipEventRecord(e0, n->stream));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_conv, dim3((P + PB - 1) / PB), dim3(256), lds, n->stream, a);
+  for (int r = 0; r < reps; ++r) launch_conv<0>(a, (P + PB - 1) / PB, n->stream);
   NCHK(n, hipEventRecord(e1, n->stream));
   NCHK(n, hipEventSynchronize(e1));
   float ms = 0.f;
