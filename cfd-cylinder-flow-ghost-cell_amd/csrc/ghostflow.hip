@@ -1936,10 +1936,12 @@ __global__ __launch_bounds__(256) void k_fused_of(const CaseDev* __restrict__ ca
 }
 
 // Surface forces: 16 lanes per quadrature point, four points per wave.  Each lane takes every 16th
-// nonzero of the point's three CSR rows (surface pressure reconstruction, du/dn, dv/dn), so a
-// row's loads are in flight together instead of one dependent index-then-value load per nonzero;
-// the lane groups add their products by a butterfly, and each wave writes one partial sum of its
-// four points, which force_totals adds in a fixed order (bit-reproducible).
+// nonzero of the point's three CSR rows (surface pressure reconstruction, du/dn, dv/dn), and the
+// three rows are walked together: their range loads, then their index and value loads, then their
+// gathers go out as one batch each (three memory round trips for rows of up to 16 nonzeros, where
+// one row after the other took nine).  A lane adds its own nonzeros of each row in row order; the
+// lane groups add their products by a butterfly, and each wave writes one partial sum of its four
+// points, which force_totals adds in a fixed order (bit-reproducible).
 constexpr int FORCE_PPW = 4;      // points per wave
 constexpr int FORCE_MAXP = 4096;  // surface points
 __host__ __device__ inline int force_blocks(int nsurf) { return (nsurf + FORCE_PPW - 1) / FORCE_PPW; }
@@ -1964,12 +1966,26 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* __restrict__ cases
     if (g < c.nprobe && sub == 0) G(c.fpart)[4 * force_blocks(c.nsurf) + g] = pr;
   }
   double ps = 0.0, du = 0.0, dv = 0.0;
+  double nxk = 0.0, nyk = 0.0, nwx = 0.0, nwy = 0.0;   // loaded with the row ranges
   if (k < c.nsurf) {
+    const gptr<const double> nrm = G(c.nrm), nw = G(c.nw);
+    nxk = nrm[2 * k]; nyk = nrm[2 * k + 1]; nwx = nw[2 * k]; nwy = nw[2 * k + 1];
     const gptr<const int> pp = G(c.pp), pc = G(c.pc), up = G(c.up), uc = G(c.uc), vp = G(c.vp), vc = G(c.vc);
     const gptr<const double> pv = G(c.pv), uv = G(c.uv), vv = G(c.vv);
-    for (int t = pp[k] + sub; t < pp[k + 1]; t += 16) ps += pv[t] * Pp[pc[t]];
-    for (int t = up[k] + sub; t < up[k + 1]; t += 16) du += uv[t] * Vel[uc[t]];
-    for (int t = vp[k] + sub; t < vp[k + 1]; t += 16) dv += vv[t] * Vel[c.usize + vc[t]];
+    const int p0 = pp[k], p1 = pp[k + 1], u0 = up[k], u1 = up[k + 1], v0 = vp[k], v1 = vp[k + 1];
+    const int len = max(p1 - p0, max(u1 - u0, v1 - v0));
+    for (int j = sub; j < len; j += 16) {
+      // a nonzero the row does not have reads entry 0 (always present) and is not added: the loads
+      // stay unconditional, so none waits for the one before it
+      const bool hp = p0 + j < p1, hu = u0 + j < u1, hv = v0 + j < v1;
+      const int tp = hp ? p0 + j : 0, tu = hu ? u0 + j : 0, tv = hv ? v0 + j : 0;
+      const int ip = pc[tp], iu = uc[tu], iv = vc[tv];
+      const double wp = pv[tp], wu = uv[tu], wv = vv[tv];
+      const double xp = Pp[ip], xu = Vel[iu], xv = Vel[c.usize + iv];
+      if (hp) ps += wp * xp;
+      if (hu) du += wu * xu;
+      if (hv) dv += wv * xv;
+    }
   }
   for (int o = 8; o > 0; o >>= 1) {
     ps += __shfl_xor(ps, o, 64);
@@ -1978,11 +1994,9 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* __restrict__ cases
   }
   double s[4] = {0.0, 0.0, 0.0, 0.0};  // pressure x, y; traction x, y
   if (k < c.nsurf && sub == 0) {
-    const gptr<const double> nrm = G(c.nrm), nw = G(c.nw);
-    const double nxk = nrm[2 * k], nyk = nrm[2 * k + 1];
     const double dot = du * nxk + dv * nyk;
-    s[0] = ps * nw[2 * k];
-    s[1] = ps * nw[2 * k + 1];
+    s[0] = ps * nwx;
+    s[1] = ps * nwy;
     const double mu = a.mu > 0.0 ? a.mu : c.nu * c.rho;
     s[2] = mu * (du + nxk * dot);
     s[3] = mu * (dv + nyk * dot);
@@ -2002,9 +2016,19 @@ DEV void force_totals(const CaseDev& c, double (&vals)[6]) {
   const int nb = force_blocks(c.nsurf);
   const gptr<const double> fp = G(c.fpart);
   double p[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = lane; b < nb; b += 64)
+  // four partials per lane in flight at a time, added in the same order as one at a time
+  for (int b0 = lane; b0 < nb; b0 += 256) {
+    double x[4][4];
     #pragma unroll
-    for (int q = 0; q < 4; ++q) p[q] += fp[b * 4 + q];
+    for (int i = 0; i < 4; ++i)
+      #pragma unroll
+      for (int q = 0; q < 4; ++q) x[i][q] = b0 + 64 * i < nb ? fp[(b0 + 64 * i) * 4 + q] : 0.0;
+    #pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (b0 + 64 * i < nb)
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) p[q] += x[i][q];
+  }
   double t[4];
   #pragma unroll
   for (int q = 0; q < 4; ++q) t[q] = wave_sum(p[q]);
@@ -4163,15 +4187,20 @@ int mcf_set_surface(mcf_handle* h, int n, const int* pp, const int* pc, const do
   for (int k = 0; k < nv; ++k) if (vc[k] < 0 || vc[k] >= h->vsize) return fail(h, 1, "surface v column out of range");
   CaseDev& c = h->hc;
   int *a, *b, *d, *e, *f, *g; double *x, *y, *z, *nr, *nw;
+  // k_forces reads entry 0 of every column and value array for the nonzeros a row does not have,
+  // so an empty array is stored as one zero entry
+  auto upload_nz = [h](const std::string& key, auto* src, int cnt, auto** dst) {
+    return cnt > 0 ? upload(h, key, src, (size_t)cnt, dst) : upload(h, key, decltype(src)(nullptr), 1, dst);
+  };
   if (int rc = upload(h, "spp", pp, n + 1, &a)) return rc;
-  if (int rc = upload(h, "spc", pc, np, &b)) return rc;
-  if (int rc = upload(h, "spv", pv, np, &x)) return rc;
+  if (int rc = upload_nz("spc", pc, np, &b)) return rc;
+  if (int rc = upload_nz("spv", pv, np, &x)) return rc;
   if (int rc = upload(h, "sup", up, n + 1, &d)) return rc;
-  if (int rc = upload(h, "suc", uc, nu, &e)) return rc;
-  if (int rc = upload(h, "suv", uv, nu, &y)) return rc;
+  if (int rc = upload_nz("suc", uc, nu, &e)) return rc;
+  if (int rc = upload_nz("suv", uv, nu, &y)) return rc;
   if (int rc = upload(h, "svp", vp, n + 1, &f)) return rc;
-  if (int rc = upload(h, "svc", vc, nv, &g)) return rc;
-  if (int rc = upload(h, "svv", vv, nv, &z)) return rc;
+  if (int rc = upload_nz("svc", vc, nv, &g)) return rc;
+  if (int rc = upload_nz("svv", vv, nv, &z)) return rc;
   if (int rc = upload(h, "snr", normals, 2 * n, &nr)) return rc;
   if (int rc = upload(h, "snw", normal_weights, 2 * n, &nw)) return rc;
   c.nsurf = n; c.pp = a; c.pc = b; c.pv = x; c.up = d; c.uc = e; c.uv = y; c.vp = f; c.vc = g; c.vv = z;
