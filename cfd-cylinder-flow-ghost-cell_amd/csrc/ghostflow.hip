@@ -135,6 +135,7 @@ struct CaseDev {
 // One separable solve problem (one rectangle of one case)
 struct SolveDesc {
   int kind, yy, xx;
+  int par;              // 1: V[yy-1-j][m] = (-1)^m V[j][m] (the parity-split transform applies)
   int off, ld;          // region of the rectangle inside its array: base + off, row stride ld
   int joff, ioff;       // rectangle (j,i) -> array (j+joff, i+ioff) for the solid test
   int field;            // solid-interval field of the array
@@ -1055,6 +1056,116 @@ __global__ __launch_bounds__(64 * S) void k_gemm_sk(const SolveDesc* __restrict_
     #pragma unroll
     for (int q = 1; q < S; ++q) v += red[q][st][r][lane];
     if (gr < M) gemm_put(d, g, gr, gc, v);
+  }
+}
+
+// Parity-split transform (launch variants 8 and 9: S = 8 and 4 waves).  The y eigenbases are DCT-II,
+// DST-II and DST-I columns, for which V[yy-1-j][m] = (-1)^m V[j][m] (the library checks this when a
+// rectangle is set, and offers these variants only where it holds).  With h = yy/2 mirrored row
+// pairs (j, yy-1-j) and, for odd yy, the middle row h:
+//   forward  G[m] = sum_{j<h} V[j][m] (F[j] + (-1)^m F[yy-1-j])  (+ V[h][m] F[h] for even m),
+//   inverse  P[j] = sum_{m even} V[j][m] G[m],  Q[j] = sum_{m odd} V[j][m] G[m],
+//            out[j] = P + Q,  out[yy-1-j] = P - Q  (out[h] = P),
+// i.e. two half-size products instead of one full one: half the MFMAs and half the operand loads of
+// the dense transform.  Forward: a block computes 16 modes of one parity (m = 2i + p) x 32 columns,
+// its S waves splitting the h (+1) row pairs, the B operand formed from a row and its mirror as it
+// is loaded.  Inverse: a block computes 16 top rows x 32 columns; waves 0 .. S/2-1 accumulate P over
+// the even modes and waves S/2 .. S-1 Q over the odd ones, and the epilogue writes both rows of each
+// pair.  Partial tiles are added in LDS in slice order; the results agree with the dense variants to
+// round-off.  Mode order in F is the natural one, so the sweeps and capacitance kernels are unchanged.
+template <int S, bool FWD>
+__global__ __launch_bounds__(64 * S) void k_gemm_par(const SolveDesc* __restrict__ descs, GemmArgs g) {
+  int z, cx, ry;
+  xcd_tile(z, cx, ry);
+  const SolveDesc& d = descs[z];
+  const int yy = d.yy, N = d.xx, h = yy >> 1, odd = yy & 1;
+  const int par = FWD ? (ry & 1) : 0;              // forward: the block's mode parity
+  const int i0 = 16 * (FWD ? (ry >> 1) : ry), col0 = cx * 32;
+  const int nrow = (FWD && par) ? h : h + odd;     // modes of this parity / top rows with the middle one
+  if (i0 >= nrow || col0 >= N) return;
+  if (g.gate && *g.gate == 0) return;
+  if (g.pred_slot >= 0 && !pred_refine(d.sl, d.slot0 + g.slot_shift, g.pred_slot)) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, l16 = lane & 15;
+  // this wave's parity half (inverse) and K slice: forward K = row pairs (and the middle row) of the
+  // block's parity; inverse K = the modes of the wave's parity
+  constexpr int SH = FWD ? S : S / 2;
+  const int q = FWD ? par : (w >= SH ? 1 : 0);
+  const int ws = FWD ? w : w - q * SH;
+  const int K = FWD ? nrow : (q ? h : h + odd);
+  const int NG = (K + 3) >> 2, per = (NG + SH - 1) / SH;
+  const int g0 = min(NG, ws * per), g1 = min(NG, g0 + per);
+  const gptr<const double> AT = G(FWD ? d.V : d.VT);
+  gptr<const double> Bm;
+  int ldb;
+  if (FWD) { Bm = desc_in(d, g.in_id); ldb = d.ld; }
+  else { Bm = G(d.F); ldb = d.xx; }
+  // rows / columns past the problem are read at the last one (in bounds): they feed only outputs
+  // that are not stored
+  const int ir = min(i0 + l16, nrow - 1);
+  const int am = FWD ? 2 * ir + par : ir;           // A column index: mode (forward) or top row (inverse)
+  const int cb[2] = {min(col0 + l16, N - 1), min(col0 + 16 + l16, N - 1)};
+  const double sgn = par ? -1.0 : 1.0;
+  d4 acc[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+  for (int gb = g0; gb < g1; gb += SK_KB) {
+    double a[SK_KB], b[SK_KB][2], bm[SK_KB][2];
+    #pragma unroll
+    for (int t = 0; t < SK_KB; ++t) {
+      const int kc = min(4 * (gb + t) + kq, K - 1);   // a valid k; past the slice or K it is zeroed below
+      // forward: k is row j (mirror yy-1-j); inverse: k is the q-parity mode 2k + q
+      const int ka = FWD ? kc : 2 * kc + q;
+      a[t] = AT[(size_t)ka * yy + am];
+      const gptr<const double> Br = Bm + (size_t)ka * ldb;
+      #pragma unroll
+      for (int c = 0; c < 2; ++c) b[t][c] = Br[cb[c]];
+      if (FWD) {
+        const gptr<const double> Bn = Bm + (size_t)(yy - 1 - kc) * ldb;
+        #pragma unroll
+        for (int c = 0; c < 2; ++c) bm[t][c] = Bn[cb[c]];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+    for (int t = 0; t < SK_KB; ++t) {
+      const int kk = 4 * (gb + t) + kq;
+      const bool ok = gb + t < g1 && kk < K;
+      #pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        // forward: a row and its mirror (the middle row k = h, odd yy, stands alone)
+        const double bv = FWD ? (kk < h ? fma(sgn, bm[t][c], b[t][c]) : b[t][c]) : b[t][c];
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? a[t] : 0.0, bv, acc[c], 0, 0, 0);
+      }
+    }
+  }
+  __shared__ double red[S][2][4][64];
+  #pragma unroll
+  for (int c = 0; c < 2; ++c)
+    #pragma unroll
+    for (int r = 0; r < 4; ++r) red[w][c][r][lane] = acc[c][r];
+  __syncthreads();
+  // the 8 (column half, element) pairs of the tile, E per wave
+  constexpr int E = 8 / S;
+  #pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int pr = E * w + e, c = pr >> 2, r = pr & 3;
+    const int gc = col0 + 16 * c + l16;
+    const int i = i0 + kq + 4 * r;
+    if (gc >= N || i >= nrow) continue;
+    if (FWD) {
+      double v = red[0][c][r][lane];
+      #pragma unroll
+      for (int u = 1; u < S; ++u) v += red[u][c][r][lane];
+      gemm_put(d, g, 2 * i + par, gc, v);
+    } else {
+      double P = red[0][c][r][lane], Q = red[SH][c][r][lane];
+      #pragma unroll
+      for (int u = 1; u < SH; ++u) { P += red[u][c][r][lane]; Q += red[SH + u][c][r][lane]; }
+      if (i < h) {
+        gemm_put(d, g, i, gc, P + Q);
+        gemm_put(d, g, yy - 1 - i, gc, P - Q);
+      } else {
+        gemm_put(d, g, i, gc, P);   // the middle row of odd yy
+      }
+    }
   }
 }
 
@@ -2746,6 +2857,9 @@ __global__ __launch_bounds__(KR_CT) void k_kry_ctl(KryArgs a, int phase) {
 constexpr int N_GEMM_V = 7, N_TRI_V = 3;
 // the x-FFT form of the rectangle inverse: transform k_xdct and sweep k_ysweep, a class's pair
 constexpr int GV_XFFT = N_GEMM_V, TV_YSWEEP = N_TRI_V;
+// the parity-split transform with 8- and 4-wave blocks (numbered after the x-FFT form)
+constexpr int GV_PAR8 = GV_XFFT + 1, GV_PAR4 = GV_XFFT + 2, GV_LAST = GV_PAR4;
+inline bool gv_par(int v) { return v == GV_PAR8 || v == GV_PAR4; }
 constexpr int GEMM_TM[N_GEMM_V] = {32, 32, 64, 32, 16, 32, 16}, GEMM_TN[N_GEMM_V] = {64, 32, 64, 32, 32, 32, 32};
 constexpr int TRI_TH[N_TRI_V] = {128, 256, 512};
 struct Launch { int gemm_p = 0, gemm_h = 0, tri_p = 1, tri_h = 1; };
@@ -2760,6 +2874,14 @@ void pin_launch(Launch& ln, int gemm, int tri, bool xf_p, bool xf_h) {
   ln.tri_p = ln.gemm_p == GV_XFFT ? TV_YSWEEP : tri;
   ln.tri_h = ln.gemm_h == GV_XFFT ? TV_YSWEEP : tri;
 }
+// The parity-split transform runs where every problem of its class has the reflection parity (known
+// once the class's rectangles are set, which for the velocity problems is at the first step), and
+// the 16-row split-K transform elsewhere; the request itself is kept.
+Launch effective_launch(Launch ln, bool par_p, bool par_h) {
+  if (gv_par(ln.gemm_p) && !par_p) ln.gemm_p = 4;
+  if (gv_par(ln.gemm_h) && !par_h) ln.gemm_h = 4;
+  return ln;
+}
 
 inline dim3 gemm_grid(int v, int yy, int xx, int count) {
   return dim3((xx + GEMM_TN[v] - 1) / GEMM_TN[v], (yy + GEMM_TM[v] - 1) / GEMM_TM[v], count);
@@ -2773,6 +2895,18 @@ void launch_gemm(int v, int yy, int xx, int count, hipStream_t st, const SolveDe
     else if (M <= 512) hipLaunchKernelGGL(k_xdct<2>, gx, dim3(256), lds, st, D, g);
     else if (M <= 768) hipLaunchKernelGGL(k_xdct<3>, gx, dim3(256), lds, st, D, g);
     else hipLaunchKernelGGL(k_xdct<4>, gx, dim3(256), lds, st, D, g);
+    return;
+  }
+  if (gv_par(v)) {
+    const int hb = ((yy + 1) / 2 + 15) / 16;   // 16-row blocks of the larger parity / of the top rows
+    const dim3 gp((xx + 31) / 32, g.forward ? 2 * hb : hb, count);
+    if (v == GV_PAR8) {
+      if (g.forward) hipLaunchKernelGGL((k_gemm_par<SK_S, true>), gp, dim3(64 * SK_S), 0, st, D, g);
+      else hipLaunchKernelGGL((k_gemm_par<SK_S, false>), gp, dim3(64 * SK_S), 0, st, D, g);
+    } else {
+      if (g.forward) hipLaunchKernelGGL((k_gemm_par<SK_S / 2, true>), gp, dim3(32 * SK_S), 0, st, D, g);
+      else hipLaunchKernelGGL((k_gemm_par<SK_S / 2, false>), gp, dim3(32 * SK_S), 0, st, D, g);
+    }
     return;
   }
   const dim3 gg = gemm_grid(v, yy, xx, count);
@@ -2967,6 +3101,7 @@ struct Ctx {
   int maxvel, maxp, maxfill, maxsurf, maxny;
   bool pert;             // some case perturbs its inflow: each step sets its inlet first (k_inlet)
   bool noxf_p, noxf_h;   // some pressure / velocity problem has no x-FFT form
+  bool nopar_p, nopar_h; // some pressure / velocity problem's basis lacks the reflection parity
   int p_band, h_band;    // most band rows of a problem's capacitance entries (x-FFT form)
   int p_yy, p_xx, p_np, p_r, h_yy, h_xx, h_np, h_r;
   StepPrm sp{};          // a single case's step parameters (dt = 0: read from the case)
@@ -2981,6 +3116,8 @@ void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
   x.pert = x.pert || c.pert_T > 0.0;
   x.noxf_p = x.noxf_p || !hs3[0].xf;
   x.noxf_h = x.noxf_h || !hs3[1].xf || !hs3[2].xf;
+  x.nopar_p = x.nopar_p || !hs3[0].par;
+  x.nopar_h = x.nopar_h || !hs3[1].par || !hs3[2].par;
   x.p_band = std::max(x.p_band, hs3[0].yr_n);
   x.h_band = std::max({x.h_band, hs3[1].yr_n, hs3[2].yr_n});
   x.p_yy = std::max(x.p_yy, hs3[0].yy); x.p_xx = std::max(x.p_xx, hs3[0].xx);
@@ -2998,6 +3135,7 @@ Ctx ctx_of(mcf_handle* h) {
   x.dump = h->hs[0].dump_cap > 0;
   x.gm = h->pbackend == MCF_PB_GMRES ? h : nullptr;
   ctx_shapes(x, h->hc, h->hs);
+  x.ln = effective_launch(x.ln, !x.nopar_p, !x.nopar_h);
   return x;
 }
 
@@ -3452,13 +3590,16 @@ int autotune(const Ctx& x, Launch& out) {
     int bg = 0, bt = 1;
     // the dense y transforms and the x sweeps, then (where every problem of the class has it) the
     // x-FFT transform with its y sweep as one pair against the best dense pair
-    const bool xf = pres ? !x.noxf_p : !x.noxf_h;
-    const int nv = N_GEMM_V + N_TRI_V + (xf ? 1 : 0);
+    const bool xf = pres ? !x.noxf_p : !x.noxf_h, par = pres ? !x.nopar_p : !x.nopar_h;
+    int gv[N_GEMM_V + 2], ngv = 0;
+    for (int k = 0; k < N_GEMM_V; ++k) gv[ngv++] = k;
+    if (par) { gv[ngv++] = GV_PAR8; gv[ngv++] = GV_PAR4; }
+    const int nv = ngv + N_TRI_V + (xf ? 1 : 0);
     for (int v = 0; v < nv && !rc; ++v) {
-      const bool pair = v == N_GEMM_V + N_TRI_V;
-      const bool gemm = v < N_GEMM_V;
-      const int k = gemm ? v : v - N_GEMM_V;
-      if (v == N_GEMM_V) { best_gemm = best; best = 1e30f; }
+      const bool pair = v == ngv + N_TRI_V;
+      const bool gemm = v < ngv;
+      const int k = gemm ? gv[v] : v - ngv;
+      if (v == ngv) { best_gemm = best; best = 1e30f; }
       auto launch = [&]() {
         if (pair) {
           launch_gemm(GV_XFFT, yy, xx, count, x.stream, D, g);
@@ -3843,7 +3984,9 @@ int rect_inverse_on(mcf_handle* h, int kind, const double* rhs, double* out, dou
   HIPCHK(h, hipMemcpyAsync(tb, &field, sizeof(double*), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(td, &t, sizeof(SolveDesc), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(field, rhs, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  const int gv = kind == MCF_PRESSURE ? h->ln.gemm_p : h->ln.gemm_h, tv = kind == MCF_PRESSURE ? h->ln.tri_p : h->ln.tri_h;
+  int gv = kind == MCF_PRESSURE ? h->ln.gemm_p : h->ln.gemm_h;
+  const int tv = kind == MCF_PRESSURE ? h->ln.tri_p : h->ln.tri_h;
+  if (gv_par(gv) && !t.par) gv = 4;
   GemmArgs g{1, 0, 0, 0, -1, 0};
   launch_gemm(gv, t.yy, t.xx, 1, h->stream, td, g);
   PcrArgs p0{0, -1, 0};
@@ -4095,6 +4238,16 @@ int mcf_set_rectangle(mcf_handle* h, int kind, int yy, int xx, double a, double 
   if (int rc = upload(h, key + "luL", Lm.data(), Lm.size(), &dL)) return rc;
   if (int rc = upload(h, key + "luI", Im.data(), Im.size(), &dI)) return rc;
   s.luL = dL; s.luI = dI;
+  // the reflection parity of the basis, V[yy-1-j][m] = (-1)^m V[j][m] (DCT-II, DST-II and DST-I
+  // columns have it), to round-off; the parity-split transform is offered only where it holds
+  double vmax = 0.0, dev = 0.0;
+  for (int j = 0; j < yy; ++j)
+    for (int m = 0; m < yy; ++m) {
+      const double v = V[(size_t)j * yy + m], r = V[(size_t)(yy - 1 - j) * yy + m];
+      vmax = std::max(vmax, std::fabs(v));
+      dev = std::max(dev, std::fabs(r - ((m & 1) ? -v : v)));
+    }
+  s.par = dev <= 1e-12 * vmax ? 1 : 0;
   s.yy = yy; s.xx = xx; s.a = a; s.b = b;
   s.V = dV; s.VT = dVT; s.lam = dl; s.lower = dlo; s.diag = ddg; s.upper = dup; s.F = dF;
   if (h->has_schur[kind])
@@ -4725,9 +4878,10 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
   // dominant kernels, timed back to back on the same stream over the pressure problem
   const SolveDesc& p = h->hs[0];
   const int reps = 50;
+  const int gvp = ctx_of(h).ln.gemm_p;
   GemmArgs g{1, B_DIVP, B_DIVP, 0, -1, 0};
   HIPCHK(h, hipEventRecord(e0, h->stream));
-  for (int r = 0; r < reps; ++r) launch_gemm(h->ln.gemm_p, p.yy, p.xx, 1, h->stream, h->ds, g);
+  for (int r = 0; r < reps; ++r) launch_gemm(gvp, p.yy, p.xx, 1, h->stream, h->ds, g);
   HIPCHK(h, hipEventRecord(e1, h->stream));
   HIPCHK(h, hipEventSynchronize(e1));
   HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
@@ -4739,7 +4893,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
     double sum = 0.0;
     for (int r = 0; r < reps; ++r) {
       HIPCHK(h, hipEventRecord(e0, h->stream));
-      launch_gemm(h->ln.gemm_p, p.yy, p.xx, 1, h->stream, h->ds, g);
+      launch_gemm(gvp, p.yy, p.xx, 1, h->stream, h->ds, g);
       HIPCHK(h, hipEventRecord(e1, h->stream));
       HIPCHK(h, hipEventSynchronize(e1));
       HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
@@ -4768,7 +4922,7 @@ int mcf_profile(mcf_handle* h, int n, double dt, double* out5) {
 }
 
 int mcf_set_launch(mcf_handle* h, int gemm, int tri) {
-  if (!h || gemm < -1 || gemm > GV_XFFT || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
+  if (!h || gemm < -1 || gemm > GV_LAST || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
     return fail(h, 1, "invalid launch shape");
   if (gemm < 0) {
     h->ln = Launch();
@@ -4788,7 +4942,8 @@ int mcf_set_launch_classes(mcf_handle* h, const int* in4) {
   for (int c = 0; c < 2; ++c) {
     const int g = in4[c], t = in4[2 + c];
     const bool xf = c == 0 ? !x.noxf_p : !x.noxf_h;
-    const bool ok = (g == GV_XFFT) ? (t == TV_YSWEEP && xf) : (g >= 0 && g < N_GEMM_V && t >= 0 && t < N_TRI_V);
+    const bool tri_ok = t >= 0 && t < N_TRI_V;
+    const bool ok = (g == GV_XFFT) ? (t == TV_YSWEEP && xf) : ((gv_par(g) || (g >= 0 && g < N_GEMM_V)) && tri_ok);
     if (!ok) return fail(h, 1, "invalid launch shape for a problem class");
   }
   h->ln.gemm_p = in4[0]; h->ln.gemm_h = in4[1]; h->ln.tri_p = in4[2]; h->ln.tri_h = in4[3];
@@ -4799,7 +4954,8 @@ int mcf_set_launch_classes(mcf_handle* h, const int* in4) {
 
 int mcf_get_launch(mcf_handle* h, int* out5) {
   if (!h || !out5) return 1;
-  out5[0] = h->ln.gemm_p; out5[1] = h->ln.gemm_h; out5[2] = h->ln.tri_p; out5[3] = h->ln.tri_h;
+  const Launch ln = ctx_of(h).ln;   // the shapes a run takes now
+  out5[0] = ln.gemm_p; out5[1] = ln.gemm_h; out5[2] = ln.tri_p; out5[3] = ln.tri_h;
   out5[4] = h->tuned ? 1 : 0;
   return 0;
 }
@@ -4901,7 +5057,7 @@ int batch_build(mcf_batch* b) {
   HIPCHK(e, hipMemcpy(b->dc, hc.data(), n * sizeof(CaseDev), hipMemcpyHostToDevice));
   HIPCHK(e, hipMemcpy(b->ds, hs.data(), 3 * n * sizeof(SolveDesc), hipMemcpyHostToDevice));
   x.dc = b->dc; x.dsp = b->ds; x.dsh = b->ds + n;
-  x.ln = b->ln;
+  x.ln = effective_launch(b->ln, !x.nopar_p, !x.nopar_h);
   b->ctx = x;
   b->drop_graphs();
   b->built = true;
@@ -4971,7 +5127,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
   }
   if (!b->tuned) {
     if (int rc = autotune(b->ctx, b->ln)) return rc;
-    b->ctx.ln = b->ln;
+    b->ctx.ln = effective_launch(b->ln, !b->ctx.nopar_p, !b->ctx.nopar_h);
     b->tuned = true;
     b->drop_graphs();
   }
@@ -5082,7 +5238,7 @@ int mcf_batch_run(mcf_batch* b, int nsteps, double* diag) {
 int mcf_batch_set_launch(mcf_batch* b, int gemm, int tri) {
   if (!b) return 1;
   mcf_handle* e = b->cases[0];
-  if (gemm < -1 || gemm > GV_XFFT || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
+  if (gemm < -1 || gemm > GV_LAST || tri < -1 || tri >= N_TRI_V || ((gemm < 0) != (tri < 0)))
     return fail(e, 1, "invalid launch shape");
   if (gemm < 0) {
     b->ln = Launch();
@@ -5091,14 +5247,15 @@ int mcf_batch_set_launch(mcf_batch* b, int gemm, int tri) {
     pin_launch(b->ln, gemm, tri, !b->ctx.noxf_p, !b->ctx.noxf_h);
     b->tuned = true;
   }
-  b->ctx.ln = b->ln;
+  b->ctx.ln = effective_launch(b->ln, !b->ctx.nopar_p, !b->ctx.nopar_h);
   b->drop_graphs();
   return 0;
 }
 
 int mcf_batch_get_launch(mcf_batch* b, int* out5) {
   if (!b || !out5) return 1;
-  out5[0] = b->ln.gemm_p; out5[1] = b->ln.gemm_h; out5[2] = b->ln.tri_p; out5[3] = b->ln.tri_h;
+  const Launch ln = b->built ? effective_launch(b->ln, !b->ctx.nopar_p, !b->ctx.nopar_h) : b->ln;
+  out5[0] = ln.gemm_p; out5[1] = ln.gemm_h; out5[2] = ln.tri_p; out5[3] = ln.tri_h;
   out5[4] = b->tuned ? 1 : 0;
   return 0;
 }

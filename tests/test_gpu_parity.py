@@ -149,10 +149,11 @@ def test_graph_replays_match_eager(pkg, monkeypatch):
 
 def test_launch_shapes_agree(pkg):
     """Every staged transform tile gives bit-identical steps, and so do the two split-K tile heights
-    of one block size (8 or 4 waves); the split-K transforms and every sweep width agree with the
-    staged tiles to round-off."""
+    of one block size (8 or 4 waves); the split-K transforms, the parity-split transforms (8 and 4
+    waves; the 81-row v rectangle has a middle row) and every sweep width agree with the staged tiles
+    to round-off."""
     runs = {}
-    for g, t in [(0, 1), (1, 1), (2, 1), (0, 0), (0, 2), (3, 1), (4, 1), (5, 1), (6, 1)]:
+    for g, t in [(0, 1), (1, 1), (2, 1), (0, 0), (0, 2), (3, 1), (4, 1), (5, 1), (6, 1), (8, 1), (9, 1)]:
         f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CFG1)
         f.gg.device.set_launch(g, t)
         f.solve(n_steps=6, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
@@ -162,7 +163,7 @@ def test_launch_shapes_agree(pkg):
     for key in [(1, 1), (2, 1)]:
         for a, b in zip(runs[key], ref):
             assert np.array_equal(a, b), key
-    for key in [(0, 0), (0, 2), (3, 1), (5, 1)]:
+    for key in [(0, 0), (0, 2), (3, 1), (5, 1), (8, 1), (9, 1)]:
         for a, b in zip(runs[key], ref):
             assert np.max(abs(a - b)) <= 1e-12 * np.max(abs(b)), key
     for a, b in zip(runs[(4, 1)], runs[(3, 1)]):
@@ -173,14 +174,16 @@ def test_launch_shapes_agree(pkg):
     f.solve(n_steps=2, dt=2e-3, perturb_transient=False, save_artifacts=False, verbose=False)
     ln = f.gg.device.get_launch()
     print('autotuned launch shapes (440x82):', ln)
-    assert ln['chosen'] and all(0 <= ln[k] < 7 for k in ('gemm_pressure', 'gemm_velocity'))
+    assert ln['chosen'] and all(ln[k] in (0, 1, 2, 3, 4, 5, 6, 8, 9) for k in ('gemm_pressure', 'gemm_velocity'))
     assert all(0 <= ln[k] < 3 for k in ('tri_pressure', 'tri_velocity'))
 
 
-@pytest.mark.parametrize('tile', [3, 4, 5, 6], ids=['32-row', '16-row', '32-row-4-waves', '16-row-4-waves'])
+@pytest.mark.parametrize('tile', [3, 4, 5, 6, 8, 9], ids=['32-row', '16-row', '32-row-4-waves', '16-row-4-waves',
+                                                   'parity-split', 'parity-split-4-waves'])
 @pytest.mark.parametrize('kind', ['pressure', 'u', 'v'])
 def test_split_k_transform_inverse(pair88, kind, tile):
-    """The split-K transforms inside the rectangle inverse against the oracle's DCT/DST inverse."""
+    """The split-K and parity-split transforms inside the rectangle inverse against the oracle's
+    DCT/DST inverse (the 88x16 grid's v rectangle has 15 rows: a middle row of its own)."""
     ref, gpu = pair88
     dev = gpu.gg.device
     dev.set_dt(1e-3)
@@ -193,6 +196,34 @@ def test_split_k_transform_inverse(pair88, kind, tile):
     _, inv, _ = O.rectangle(gpu.nx, gpu.ny, gpu.dx, gpu.dy, kind, gpu.nu, 1e-3, gpu.rho)
     assert rel(x, inv(b)) < 1e-12
     dev.set_launch(before['gemm_pressure'], before['tri_pressure'])
+
+
+def test_parity_split_needs_the_reflection_parity(pkg):
+    """The parity-split transform runs only for a class whose bases have V[yy-1-j][m] = (-1)^m V[j][m]:
+    with the pressure basis's first two columns rotated into each other (still orthonormal, no longer
+    of one parity each), a pinned parity-split transform runs as the 16-row split-K transform for
+    pressure only."""
+    import importlib
+    nat = importlib.import_module(pkg.__name__ + '._native')
+    f = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **CH)
+    dev = f.gg.device
+    dev.set_launch(8, 1)
+    assert dev.get_launch()['gemm_velocity'] == 4     # the velocity rectangles come with the first step
+    f.solve(n_steps=1, dt=1e-3, perturb_transient=False, save_artifacts=False, verbose=False)
+    ln = dev.get_launch()
+    assert ln['gemm_pressure'] == 8 and ln['gemm_velocity'] == 8
+    fac = dev.pressure_factor
+    V = np.array(fac.V, dtype=float)
+    c0, c1 = V[:, 0].copy(), V[:, 1].copy()
+    V[:, 0], V[:, 1] = (c0 + c1) / np.sqrt(2), (c0 - c1) / np.sqrt(2)
+    yy, xx = fac.shape
+    dev.h.call('mcf_set_rectangle', 0, yy, xx, float(fac.a), float(fac.b), nat.f64(V), nat.f64(fac.lam),
+               nat.f64(fac.lower), nat.f64(fac.diag), nat.f64(fac.upper))
+    ln = dev.get_launch()
+    assert ln['gemm_pressure'] == 4 and ln['gemm_velocity'] == 8, ln
+    dev.set_launch_classes(dict(gemm_pressure=9, gemm_velocity=9, tri_pressure=1, tri_velocity=1))
+    ln = dev.get_launch()
+    assert ln['gemm_pressure'] == 4 and ln['gemm_velocity'] == 9, ln
 
 
 def test_speculative_steps_match_full(pkg):
