@@ -29,7 +29,8 @@ STEP_BYTES_PER_CELL = 928  # algorithmic HBM bytes per cell-step of the whole AB
 REFINE_BYTES_PER_CELL = 96  # one executed refinement pass of a Woodbury solve: 12 field sweeps
 # HBM bytes per forward launch of the pressure transform by tile variant (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 from rocprofv3 --pmc, scaled by a 256 MiB copy; profiles/k_gemm_1320x246_traffic.md)
-GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6, 4: 9.657e6, 5: 9.754e6, 6: 9.732e6}
+GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6, 4: 9.657e6, 5: 9.754e6, 6: 9.732e6,
+                8: 7.714e6, 9: 7.742e6}
 
 
 BOX = dict(nx=800, ny=820, Lx=40., Ly=41., D=1., xc=10., yc=20.5, nu=0.01, rho=1., inflow='uniform', U_inf=1.)
@@ -333,6 +334,8 @@ def step_table(trace_dir, fetch_dir, write_dir, last=40):
              '|---|---|---|---|---|---|---|---|---|---|']
     tot = [0.0] * 4
     for k, (n, (stage, _, alg)) in enumerate(zip(names, STEP_KERNEL_BYTES)):
+        if n.startswith('k_gemm_par'):   # the parity-split transform reads half of V
+            alg -= (_VV if 'velocity' in stage else 246 ** 2 * 8) / 2
         f, w = fet[k] * sf * 1024 / 1e6, wri[k] * sw * 1024 / 1e6
         t = dur[k]
         lines.append(f'| {k} | `{n}` | {stage} | {t:.2f} | {f:.2f} | {w:.2f} | {(f + w) / t * 1e3:.0f} | '
@@ -454,7 +457,10 @@ def main():
         gemm_isolated_ms = prof[3]
         lean_misses = int(prof[4])
     yy, xx = WORKLOAD['ny'], WORKLOAD['nx']
-    gemm_flops = 2.0 * yy * yy * xx
+    # the parity-split transform (tiles 8, 9) computes two half-size products: yy^2 xx flops and half of
+    # V read, against 2 yy^2 xx and all of V for the dense tiles
+    par = tile in (8, 9)
+    gemm_flops = (1.0 if par else 2.0) * yy * yy * xx
     gemm_achieved = gemm_flops / (gemm_ms * 1e-3) if gemm_ms else None
     out = dict(metric='cell-steps/s on 1 MI355X (plain vs batched+NN warm-start) at div<1e-12', value=value,
                unit='cell-steps/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -466,8 +472,10 @@ def main():
                roofline=None if gemm_achieved is None else dict(
                    bound='mfma', achieved=gemm_achieved / 1e12, peak=FP64_PEAK / 1e12, unit='TFLOP/s',
                    frac=gemm_achieved / FP64_PEAK, traffic=GEMM_TRAFFIC.get(tile),
-                   traffic_unit='bytes per launch', algorithmic_bytes=8.0 * (yy * yy + 2 * yy * xx), tile=tile,
-                   kernel=('k_gemm_sk' if tile >= 3 else 'k_gemm') + ' (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)',
+                   traffic_unit='bytes per launch', algorithmic_bytes=8.0 * ((0.5 if par else 1.0) * yy * yy + 2 * yy * xx),
+                   tile=tile, kernel=('k_gemm_par (parity-split FP64 y-eigenbasis transform, two 123x123 by 123x1320 products)'
+                                      if par else ('k_gemm_sk' if tile >= 3 else 'k_gemm')
+                                      + ' (dense FP64 y-eigenbasis transform, 246x246 by 246x1320)'),
                    launch_ms=gemm_ms),
                step_roofline=step_roofline(dev, diag, rate),
                kernel_ms=dict(gemm=gemm_ms, gemm_isolated=gemm_isolated_ms, pcr=pcr_ms, step=step_ms,

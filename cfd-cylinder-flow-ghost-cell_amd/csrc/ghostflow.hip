@@ -1073,7 +1073,10 @@ __global__ __launch_bounds__(64 * S) void k_gemm_sk(const SolveDesc* __restrict_
 // the even modes and waves S/2 .. S-1 Q over the odd ones, and the epilogue writes both rows of each
 // pair.  Partial tiles are added in LDS in slice order; the results agree with the dense variants to
 // round-off.  Mode order in F is the natural one, so the sweeps and capacitance kernels are unchanged.
-template <int S, bool FWD>
+// KB k-groups per load batch: 4 keeps the forward kernel at 80-96 VGPRs (5-6 waves per SIMD, so a
+// velocity launch's 1344 blocks take two rounds instead of three); 8 (140 VGPRs) measured 7.1 against
+// 6.45 us per pressure transform and 0.197 against 0.189 ms per step.
+template <int S, bool FWD, int KB = 4>
 __global__ __launch_bounds__(64 * S) void k_gemm_par(const SolveDesc* __restrict__ descs, GemmArgs g) {
   int z, cx, ry;
   xcd_tile(z, cx, ry);
@@ -1106,10 +1109,10 @@ __global__ __launch_bounds__(64 * S) void k_gemm_par(const SolveDesc* __restrict
   const int cb[2] = {min(col0 + l16, N - 1), min(col0 + 16 + l16, N - 1)};
   const double sgn = par ? -1.0 : 1.0;
   d4 acc[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
-  for (int gb = g0; gb < g1; gb += SK_KB) {
-    double a[SK_KB], b[SK_KB][2], bm[SK_KB][2];
+  for (int gb = g0; gb < g1; gb += KB) {
+    double a[KB], b[KB][2], bm[KB][2];
     #pragma unroll
-    for (int t = 0; t < SK_KB; ++t) {
+    for (int t = 0; t < KB; ++t) {
       const int kc = min(4 * (gb + t) + kq, K - 1);   // a valid k; past the slice or K it is zeroed below
       // forward: k is row j (mirror yy-1-j); inverse: k is the q-parity mode 2k + q
       const int ka = FWD ? kc : 2 * kc + q;
@@ -1125,7 +1128,7 @@ __global__ __launch_bounds__(64 * S) void k_gemm_par(const SolveDesc* __restrict
     }
     __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
-    for (int t = 0; t < SK_KB; ++t) {
+    for (int t = 0; t < KB; ++t) {
       const int kk = 4 * (gb + t) + kq;
       const bool ok = gb + t < g1 && kk < K;
       #pragma unroll
