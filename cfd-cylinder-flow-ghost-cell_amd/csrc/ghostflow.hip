@@ -2878,8 +2878,9 @@ void pin_launch(Launch& ln, int gemm, int tri, bool xf_p, bool xf_h) {
   ln.tri_h = ln.gemm_h == GV_XFFT ? TV_YSWEEP : tri;
 }
 // The parity-split transform runs where every problem of its class has the reflection parity (known
-// once the class's rectangles are set, which for the velocity problems is at the first step), and
-// the 16-row split-K transform elsewhere; the request itself is kept.
+// once the class's rectangles are set, which for the velocity problems is at the first step; until
+// then the request is reported as it stands), and the 16-row split-K transform elsewhere; the
+// request itself is kept.
 Launch effective_launch(Launch ln, bool par_p, bool par_h) {
   if (gv_par(ln.gemm_p) && !par_p) ln.gemm_p = 4;
   if (gv_par(ln.gemm_h) && !par_h) ln.gemm_h = 4;
@@ -3119,8 +3120,10 @@ void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
   x.pert = x.pert || c.pert_T > 0.0;
   x.noxf_p = x.noxf_p || !hs3[0].xf;
   x.noxf_h = x.noxf_h || !hs3[1].xf || !hs3[2].xf;
-  x.nopar_p = x.nopar_p || !hs3[0].par;
-  x.nopar_h = x.nopar_h || !hs3[1].par || !hs3[2].par;
+  // a rectangle not set yet (yy 0; the velocity ones come with the first step) does not count
+  auto nopar = [](const SolveDesc& d) { return d.yy > 0 && !d.par; };
+  x.nopar_p = x.nopar_p || nopar(hs3[0]);
+  x.nopar_h = x.nopar_h || nopar(hs3[1]) || nopar(hs3[2]);
   x.p_band = std::max(x.p_band, hs3[0].yr_n);
   x.h_band = std::max({x.h_band, hs3[1].yr_n, hs3[2].yr_n});
   x.p_yy = std::max(x.p_yy, hs3[0].yy); x.p_xx = std::max(x.p_xx, hs3[0].xx);
