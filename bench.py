@@ -30,7 +30,7 @@ REFINE_BYTES_PER_CELL = 96  # one executed refinement pass of a Woodbury solve: 
 # HBM bytes per forward launch of the pressure transform by tile variant (FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 from rocprofv3 --pmc, scaled by a 256 MiB copy; profiles/k_gemm_1320x246_traffic.md)
 GEMM_TRAFFIC = {0: 10.262e6, 1: 9.697e6, 2: 10.009e6, 3: 9.688e6, 4: 9.657e6, 5: 9.754e6, 6: 9.732e6,
-                8: 7.714e6, 9: 7.742e6}
+                8: 7.710e6, 9: 7.714e6}
 
 
 BOX = dict(nx=800, ny=820, Lx=40., Ly=41., D=1., xc=10., yc=20.5, nu=0.01, rho=1., inflow='uniform', U_inf=1.)

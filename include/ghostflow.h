@@ -230,12 +230,15 @@ int mcf_net_profile(mcf_net* n, int nsamp, int L, int reps, float* ms_out);
    (0: 32x64, 1: 32x32, 2: 64x64 staged in LDS; 3: 32x32, 4: 16x32 split over K in 8-wave blocks;
    5: 32x32, 6: 16x32 split over K in 4-wave blocks; 7: the x-FFT form, a DCT-IV / DST-IV /
    mirrored-outlet sine transform along x through a half-length FFT with tridiagonal sweeps along y,
-   which brings its own sweep, reported as 3) and the tridiagonal
+   which brings its own sweep, reported as 3; 8, 9: the parity-split transform in 8- and 4-wave
+   blocks, two half-size products using V[yy-1-j][m] = (-1)^m V[j][m], which mcf_set_rectangle checks:
+   a class whose bases lack it runs 4 instead) and the tridiagonal
    sweep width (0: 128, 1: 256, 2: 512 threads per mode).  By default the first mcf_run / mcf_step of a handle times every shape (and the
    x-FFT form's whole solve chain against the best dense chain, where every problem of a class has it) on the
    handle's own problems and keeps the fastest per problem class (pressure, velocity); the staged
    transform shapes give identical results, and so do the two split-K shapes of one block size; the
-   families (staged, 8-wave and 4-wave split-K, x-FFT) and the sweep shapes differ in round-off.  mcf_set_launch(h, g, t) pins
+   families (staged, 8-wave and 4-wave split-K, parity-split, x-FFT) and the sweep shapes differ in
+   round-off.  mcf_set_launch(h, g, t) pins
    both classes to (g, t) (g = 7: the x-FFT pair for each class that has it, the 16-row split-K transform with width t for
    the other); (-1, -1) returns to autotuning.  mcf_get_launch: [transform pressure,
    transform velocity, sweep pressure, sweep velocity, chosen (1) or default (0)]. */
