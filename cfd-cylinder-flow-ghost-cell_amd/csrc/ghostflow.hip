@@ -871,6 +871,15 @@ DEV void gemm_put(const SolveDesc& d, const GemmArgs& g, int gr, int gc, double 
   else if (slo <= ai && ai <= shi) *pp = 0.0;
   else *pp = (g.epi == 2) ? (*pp + v) : v;
 }
+// The inverse's put with its operands loaded ahead: the output row's solid interval (slo, shi) and,
+// when accumulating (epi 2), the old value
+DEV void gemm_put_pre(const SolveDesc& d, const GemmArgs& g, int gr, int gc, double v, int slo, int shi, double old) {
+  const gptr<double> pp = desc_out(d, g.out_id) + (size_t)gr * d.ld + gc;
+  const int ai = gc + d.ioff;
+  if (g.epi == 0) *pp = v;
+  else if (slo <= ai && ai <= shi) *pp = 0.0;
+  else *pp = (g.epi == 2) ? (old + v) : v;
+}
 
 // Dense FP64 y transform on the matrix cores: C (M x N) = A (M x K) * B (K x N), M = K = yy, N = xx.
 // A^T is read in row-major (k, m) order: forward (A = V^T) reads V, inverse (A = V) reads V^T.
@@ -1108,6 +1117,31 @@ __global__ __launch_bounds__(64 * S) void k_gemm_par(const SolveDesc* __restrict
   const int am = FWD ? 2 * ir + par : ir;           // A column index: mode (forward) or top row (inverse)
   const int cb[2] = {min(col0 + l16, N - 1), min(col0 + 16 + l16, N - 1)};
   const double sgn = par ? -1.0 : 1.0;
+  // the 8 (column half, element) pairs of the tile, E per wave (the epilogue below)
+  constexpr int E = 8 / S;
+  // inverse: the epilogue's operands -- the solid intervals of the two output rows of each pair and,
+  // when accumulating, their old values -- load before the K loop (read in the epilogue, they were
+  // one more memory round trip after the reduction); indexes past the problem are clamped, the
+  // epilogue stores only inside it
+  int elo[E][2], ehi[E][2];
+  double eold[E][2];
+  if (!FWD) {
+    const gptr<const double> O = desc_in(d, g.out_id);
+    #pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int pr = E * w + e, c = pr >> 2, r = pr & 3;
+      const int gc = min(col0 + 16 * c + l16, N - 1);
+      const int i = min(i0 + kq + 4 * r, nrow - 1);
+      #pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int jr = t == 0 ? i : (i < h ? yy - 1 - i : i);
+        const int aj = jr + d.joff;
+        elo[e][t] = (g.epi != 0 && d.lo) ? G(d.lo)[aj] : 1;
+        ehi[e][t] = (g.epi != 0 && d.lo) ? G(d.hi)[aj] : 0;
+        eold[e][t] = g.epi == 2 ? O[(size_t)jr * d.ld + gc] : 0.0;
+      }
+    }
+  }
   d4 acc[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
   for (int gb = g0; gb < g1; gb += KB) {
     double a[KB], b[KB][2], bm[KB][2];
@@ -1145,8 +1179,6 @@ __global__ __launch_bounds__(64 * S) void k_gemm_par(const SolveDesc* __restrict
     #pragma unroll
     for (int r = 0; r < 4; ++r) red[w][c][r][lane] = acc[c][r];
   __syncthreads();
-  // the 8 (column half, element) pairs of the tile, E per wave
-  constexpr int E = 8 / S;
   #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int pr = E * w + e, c = pr >> 2, r = pr & 3;
@@ -1163,10 +1195,10 @@ __global__ __launch_bounds__(64 * S) void k_gemm_par(const SolveDesc* __restrict
       #pragma unroll
       for (int u = 1; u < SH; ++u) { P += red[u][c][r][lane]; Q += red[SH + u][c][r][lane]; }
       if (i < h) {
-        gemm_put(d, g, i, gc, P + Q);
-        gemm_put(d, g, yy - 1 - i, gc, P - Q);
+        gemm_put_pre(d, g, i, gc, P + Q, elo[e][0], ehi[e][0], eold[e][0]);
+        gemm_put_pre(d, g, yy - 1 - i, gc, P - Q, elo[e][1], ehi[e][1], eold[e][1]);
       } else {
-        gemm_put(d, g, i, gc, P);   // the middle row of odd yy
+        gemm_put_pre(d, g, i, gc, P, elo[e][0], ehi[e][0], eold[e][0]);   // the middle row of odd yy
       }
     }
   }
