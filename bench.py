@@ -288,8 +288,9 @@ STEP_KERNEL_BYTES = [
     ('solve statistics + projection update', '', 2 * _P + 2 * _VEL),
     ('gate 2 + pressures + ghost fill', 'reads u, v, half pressure, phi; writes half and output pressure',
      _VEL + 4 * _P),
-    ('k_forces', 'surface stencils (756 points)', 756 * 40 * 12),
-    ('k_finalize', 'reduction slots', 64 * 64 * 8),
+    ('k_forces', 'surface stencils (756 points); one block gathers and clears the reduction slots',
+     756 * 40 * 12 + 2 * 64 * 64 * 8),
+    ('k_finalize', 'force partial sums into the diagnostics row', 189 * 4 * 8),
 ]
 
 
