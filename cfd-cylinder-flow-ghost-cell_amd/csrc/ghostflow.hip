@@ -250,10 +250,29 @@ DEV double slot_sum(const u64* sl, int k) {
   const int lane = threadIdx.x & 63;
   return wave_sum(__longlong_as_double((long long)G(sl)[slot_word(k, lane)]));
 }
-DEV bool pred_refine(const u64* sl, int slot0, int res_slot) {
+// Whether a refinement pass runs: its residual misses the Woodbury target (or is NaN), from the
+// slot values over all stripes
+DEV bool pred_eval(const u64* sl, int slot0, int res_slot) {
   const double res = slot_max(sl, slot0 + res_slot);
   const double val = slot_max(sl, slot0 + SV_VAL);
   return res > woodbury_target(val) || res != res;
+}
+// The pass's decision as k_decide stored it, one word for slot k at DEC_BASE + k in stripe 0 (past
+// the slots, which the step's slot clear leaves alone): a predicated kernel's waves read one word
+// instead of each reducing the two slots' 64 stripes, 64 cache lines per load (0.7 us per
+// predicated launch; 8.5 us of a transient step's two refinement passes).
+constexpr int DEC_BASE = NSLOT;
+static_assert(DEC_BASE + NSLOT <= SLOT_STRIDE, "the decision words fit in stripe 0");
+DEV bool pred_refine(const u64* sl, int slot0, int res_slot) { return G(sl)[DEC_BASE + slot0 + res_slot] != 0ull; }
+// Decide a refinement pass once its residual is in the slots: slot block k0 (and k1 when >= 0; the
+// velocity solve's u and v), residual slot res_slot; one wave per case
+__global__ __launch_bounds__(64) void k_decide(const CaseDev* __restrict__ cases, int k0, int k1, int res_slot) {
+  const CaseDev& c = cases[blockIdx.y];
+  const bool r0 = pred_eval(c.sl, k0, res_slot);
+  const bool r1 = k1 >= 0 && pred_eval(c.sl, k1, res_slot);
+  if (threadIdx.x != 0) return;
+  G(c.sl)[DEC_BASE + k0 + res_slot] = r0 ? 1ull : 0ull;
+  if (k1 >= 0) G(c.sl)[DEC_BASE + k1 + res_slot] = r1 ? 1ull : 0ull;
 }
 // Block statistics of a 256-thread block (every thread calls): the maxima of |v[k]| and the sums of
 // sv[q] over the block, one atomic per block and statistic, into dst[k] (and dst2[k] when set) and
@@ -3480,14 +3499,17 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, in
   for (int it = 0; it < refs; ++it) {
     const int res_slot = it == 0 ? SV_RES1 : SV_RES2;
     const int pred = it == 0 ? -1 : SV_RES1;
+    // the residual, then the pass's decision (k_decide), which every launch of the pass reads
     if (pres) {
       PresArgs a{in_id, out_id, B_RESP, 0, slot0, res_slot, pred};
       hipLaunchKernelGGL(k_pres, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, a);
+      hipLaunchKernelGGL(k_decide, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, slot0, -1, res_slot);
       if ((rc = enqueue_apply(x, pres, B_RESP, out_id, 2, res_slot, shift))) return rc;
     } else {
       HelmArgs a{in_id, out_id, B_RES, -1, 0, res_slot, pred};
       a.sp = x.sp;
       hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
+      hipLaunchKernelGGL(k_decide, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, (int)SL_HU, (int)SL_HV, res_slot);
       if ((rc = enqueue_apply(x, pres, B_RES, out_id, 2, res_slot, shift))) return rc;
     }
   }
