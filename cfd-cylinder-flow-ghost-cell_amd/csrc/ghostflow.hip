@@ -3478,7 +3478,7 @@ int kry_solve(const Ctx& x, int b_id, int x_id, int psolve) {
 // `refs` refinement passes launched (each runs only when the residual before it calls for it).  The
 // lean form (refs = 0) computes the first and the final residual in one pass; refs = 1 leaves the
 // second pass out.  A step whose solve would have needed a pass that was left out is flagged by
-// k_finalize and replayed in full.
+// the step's slot gather (finalize_slots) and replayed in full.
 // With `tail`, the final-statistics stage is returned there instead of launched, for the caller to
 // merge into its next launch.
 int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, int refs = 2,
