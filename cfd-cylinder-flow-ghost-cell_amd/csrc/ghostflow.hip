@@ -252,9 +252,9 @@ DEV double slot_sum(const u64* sl, int k) {
 }
 // Whether a refinement pass runs: its residual misses the Woodbury target (or is NaN), from the
 // slot values over all stripes
-DEV bool pred_eval(const u64* sl, int slot0, int res_slot) {
+DEV bool pred_eval(const u64* sl, int slot0, int res_slot, int val_slot) {
   const double res = slot_max(sl, slot0 + res_slot);
-  const double val = slot_max(sl, slot0 + SV_VAL);
+  const double val = slot_max(sl, val_slot);
   return res > woodbury_target(val) || res != res;
 }
 // The pass's decision as k_decide stored it, one word for slot k at DEC_BASE + k in stripe 0 (past
@@ -263,13 +263,21 @@ DEV bool pred_eval(const u64* sl, int slot0, int res_slot) {
 // predicated launch; 8.5 us of a transient step's two refinement passes).
 constexpr int DEC_BASE = NSLOT;
 static_assert(DEC_BASE + NSLOT <= SLOT_STRIDE, "the decision words fit in stripe 0");
+// The velocity right-hand side maxima of a step (u, v), striped like the slots, in a pair per step
+// parity past the decision words: SL_RHS + 2 parity + component.  A step's advection stage writes
+// its pair while the previous step's slot gather may still read and clear the other one (the step
+// graph runs that gather beside the next step's first stage).
+constexpr int SL_RHS = DEC_BASE + NSLOT;
+static_assert(SL_RHS + 4 <= SLOT_STRIDE, "the right-hand side maxima fit in a stripe");
 DEV bool pred_refine(const u64* sl, int slot0, int res_slot) { return G(sl)[DEC_BASE + slot0 + res_slot] != 0ull; }
 // Decide a refinement pass once its residual is in the slots: slot block k0 (and k1 when >= 0; the
-// velocity solve's u and v), residual slot res_slot; one wave per case
-__global__ __launch_bounds__(64) void k_decide(const CaseDev* __restrict__ cases, int k0, int k1, int res_slot) {
+// velocity solve's u and v), residual slot res_slot, right-hand side maxima in slots v0 (and v1);
+// one wave per case
+__global__ __launch_bounds__(64) void k_decide(const CaseDev* __restrict__ cases, int k0, int k1, int res_slot, int v0,
+                                               int v1) {
   const CaseDev& c = cases[blockIdx.y];
-  const bool r0 = pred_eval(c.sl, k0, res_slot);
-  const bool r1 = k1 >= 0 && pred_eval(c.sl, k1, res_slot);
+  const bool r0 = pred_eval(c.sl, k0, res_slot, v0);
+  const bool r1 = k1 >= 0 && pred_eval(c.sl, k1, res_slot, v1);
   if (threadIdx.x != 0) return;
   G(c.sl)[DEC_BASE + k0 + res_slot] = r0 ? 1ull : 0ull;
   if (k1 >= 0) G(c.sl)[DEC_BASE + k1 + res_slot] = r1 ? 1ull : 0ull;
@@ -441,12 +449,13 @@ struct AdvArgs {
   int q_id, adv_id, p_id, prev_id, now_id, out_id;
   int mode;  // 0: AB2 rhs, 1: projected-midpoint, 2: rhs with explicit = adv(adv_id), 3: out = L q + offset
   StepPrm sp;
+  int rhs_slot;  // rhs modes: the slot pair of the step's right-hand side maxima (u, v)
 };
 
 // SAME: the advected and the advecting fields are one buffer (the AB2 and midpoint steps), so their
 // reads at one index are one load
 template <bool SAME>
-DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
+DEV void adv_body(const CaseDev& c, const AdvArgs& a, int bx, int gx) {
   const int nx = c.nx, ny = c.ny, nx1 = nx + 1;
   const gptr<const double> A = G(c.buf[a.adv_id]);
   const gptr<const double> Q = SAME ? A : G(c.buf[a.q_id]);
@@ -465,7 +474,7 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
   // discarded); the override and solid lookups are issued after the batch, so the batch is in
   // flight while they resolve.  Per-term loads under the boundary tests took one round trip per
   // test.  The arithmetic is the stencil's, term for term.
-  const Chunk ch = xcd_chunk(0, c.velsize, blockIdx.x, gridDim.x);
+  const Chunk ch = xcd_chunk(0, c.velsize, bx, gx);
   for (int idx = ch.beg + threadIdx.x; idx < ch.end; idx += ch.step) {
     double res = 0.0, nw = 0.0;
     if (idx < c.usize) {
@@ -578,13 +587,13 @@ DEV void adv_body(const CaseDev& c, const AdvArgs& a) {
   }
   double v2[2] = {mu, mv};
   const bool rhs_mode = a.mode == 0 || a.mode == 2;
-  u64* const d2[2] = {rhs_mode ? c.sl + (SL_HU + SV_VAL) : nullptr, rhs_mode ? c.sl + (SL_HV + SV_VAL) : nullptr};
+  u64* const d2[2] = {rhs_mode ? c.sl + a.rhs_slot : nullptr, rhs_mode ? c.sl + (a.rhs_slot + 1) : nullptr};
   block_max_atomic<2>(v2, d2);
 }
 __global__ __launch_bounds__(256) void k_adv_rhs(const CaseDev* __restrict__ cases, AdvArgs a) {
   const CaseDev& c = cases[blockIdx.y];
-  if (a.q_id == a.adv_id) adv_body<true>(c, a);
-  else adv_body<false>(c, a);
+  if (a.q_id == a.adv_id) adv_body<true>(c, a, blockIdx.x, gridDim.x);
+  else adv_body<false>(c, a, blockIdx.x, gridDim.x);
 }
 
 // Helmholtz residual / defect on vel q-arrays: H x = x - dt/2 L x
@@ -2051,11 +2060,13 @@ __global__ __launch_bounds__(256) void k_pressures(const CaseDev* __restrict__ c
 // then the third's.  Each body keeps its own block count and grid-stride loop, so every element
 // is computed exactly as by its own kernel; a launch costs the same few microseconds whatever its
 // size, so merging stages that do not read each other's output shortens the step.
-enum BodyKind { BK_NONE = 0, BK_DIV, BK_HELM, BK_PRES, BK_UPD, BK_PRESS, BK_FILL };
+// BK_FIN: the previous step's forces into its row and the clock (finalize_forces; one block).
+enum BodyKind { BK_NONE = 0, BK_DIV, BK_HELM, BK_PRES, BK_UPD, BK_PRESS, BK_FILL, BK_FIN };
 struct Body {
   int kind, nblk;
   union { DivArgs d; HelmArgs h; PresArgs p; UpdArgs u; PressArgs q; FillArgs f; };
 };
+DEV void finalize_forces(const CaseDev& c);
 struct Fused { Body b[3]; int n; };
 DEV void run_body(const CaseDev& c, const Body& b, int bx) {
   switch (b.kind) {
@@ -2065,6 +2076,7 @@ DEV void run_body(const CaseDev& c, const Body& b, int bx) {
     case BK_UPD: upd_body(c, b.u, bx, b.nblk); break;
     case BK_PRESS: press_body(c, b.q, bx, b.nblk); break;
     case BK_FILL: fill_body(c, b.f, bx, b.nblk); break;
+    case BK_FIN: if (threadIdx.x < 64) finalize_forces(c); break;
     default: break;
   }
 }
@@ -2088,6 +2100,9 @@ DEV void run_kind(const CaseDev& c, const Body& b, int bx) {
   else if constexpr (K == BK_UPD) upd_body(c, b.u, bx, b.nblk);
   else if constexpr (K == BK_PRESS) press_body(c, b.q, bx, b.nblk);
   else if constexpr (K == BK_FILL) fill_body(c, b.f, bx, b.nblk);
+  else if constexpr (K == BK_FIN) {
+    if (threadIdx.x < 64) finalize_forces(c);
+  }
 }
 template <int K0, int K1, int K2>
 __global__ __launch_bounds__(256) void k_fused_of(const CaseDev* __restrict__ cases, Fused f) {
@@ -2111,15 +2126,15 @@ constexpr int FORCE_PPW = 4;      // points per wave
 constexpr int FORCE_MAXP = 4096;  // surface points
 __host__ __device__ inline int force_blocks(int nsurf) { return (nsurf + FORCE_PPW - 1) / FORCE_PPW; }
 struct ForceArgs { int vel_id, p_id; double mu; };
-DEV void forces_body(const CaseDev& c, const ForceArgs& a) {
+DEV void forces_body(const CaseDev& c, const ForceArgs& a, int unit) {   // one wave, partial sum `unit`
   const gptr<const double> Vel = G(c.buf[a.vel_id]);
   const gptr<const double> Pp = G(c.buf[a.p_id]);
-  const int sub = threadIdx.x & 15;
-  const int k = blockIdx.x * FORCE_PPW + (threadIdx.x >> 4);
-  if (blockIdx.x * FORCE_PPW >= c.nsurf) return;
-  if (blockIdx.x == 0 && c.nprobe > 0) {
+  const int lane = threadIdx.x & 63, sub = lane & 15;
+  const int k = unit * FORCE_PPW + (lane >> 4);
+  if (unit * FORCE_PPW >= c.nsurf) return;
+  if (unit == 0 && c.nprobe > 0) {
     // the probes, one 16-lane group each like a surface point, stored after the force partials
-    const int g = threadIdx.x >> 4;
+    const int g = lane >> 4;
     double pr = 0.0;
     if (g < c.nprobe) {
       const gptr<const int> qp = G(c.qp), qc = G(c.qc);
@@ -2168,7 +2183,7 @@ DEV void forces_body(const CaseDev& c, const ForceArgs& a) {
   #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const double v = wave_sum(s[q]);
-    if (threadIdx.x == 0) G(c.fpart)[blockIdx.x * 4 + q] = v;
+    if (lane == 0) G(c.fpart)[unit * 4 + q] = v;
   }
 }
 
@@ -2238,7 +2253,7 @@ __host__ __device__ inline bool solve_ok(const double* rec) {
 enum { LEAN_VEL = 1, LEAN_PRES = 2, LEAN_ALL = 3, ONCE_PRES = 4 };
 // The lean masks a speculating run switches between (spec_mask)
 constexpr int SPEC_MASKS[4] = {LEAN_ALL, 0, LEAN_VEL | ONCE_PRES, LEAN_VEL};
-struct FinArgs { int nproj; int lean; int force_miss; int gmres; };
+struct FinArgs { int nproj; int lean; int force_miss; int gmres; int hval; };   // hval: the step's SL_RHS pair
 static_assert(MCF_DIAG_LEN == 64, "one diagnostics entry per lane of finalize_slots");
 // Slots gathered by finalize_slots: FIN_MAX maxima over the stripes, then FIN_SUM sums
 constexpr int FIN_MAX = 41, FIN_SUM = 6, FIN_NQ = FIN_MAX + FIN_SUM;
@@ -2254,16 +2269,17 @@ __constant__ const int kFinSlots[FIN_NQ] = {
 
 // Gather the step's scalars into the diagnostics ring and clear the slots (one wave per case), in
 // two parts.  finalize_slots needs only the slots, which are final once the step's last stage has
-// run, so it runs as one more block of the step's k_forces launch, beside the force blocks: every
-// lane stores its stripe of every slot in LDS; lane q then reduces slot q over the stripes (maxima
-// of |.|, which do not depend on the order, or sums in stripe order); lane 0 evaluates the gates and
+// run, so it runs as one more block beside the force waves (k_forces, k_adv_tail): every lane
+// stores its stripe of every slot in LDS; lane q then reduces slot q over the stripes (maxima of
+// |.|, which do not depend on the order, or sums in stripe order); lane 0 evaluates the gates and
 // the per-solve records; the row is stored once, one entry per lane, with the force entries zero,
-// and the slots are cleared.  k_finalize then adds k_forces' partial sums into the forces, C_D and
-// C_L, the probes and the finite-force gate, and advances the clock.  (As one kernel after
-// k_forces, the whole gather took 7.3 us of a 0.191 ms step.)
-DEV void finalize_slots(const CaseDev& c, const FinArgs& a) {
+// and the slots are cleared.  A later launch then adds the force partial sums into the forces, C_D
+// and C_L, the probes and the finite-force gate, and advances the clock (finalize_forces).  (As one
+// kernel after the forces, the whole gather took 7.3 us of a 0.191 ms step.)
+DEV void finalize_slots(const CaseDev& c, const FinArgs& a) {   // every thread of the block calls; wave 0 works
   const gptr<const u64> s = G(c.sl);
   const int lane = threadIdx.x;
+  const bool w0 = lane < 64;
   // step parameters and clock, loaded together with the slots
   const double dt = G(c.prm)[0];
   const int row = *G(c.clock_step);
@@ -2271,9 +2287,14 @@ DEV void finalize_slots(const CaseDev& c, const FinArgs& a) {
   __shared__ double raw[FIN_NQ][65];   // padded: lane q reads row q
   __shared__ double v[FIN_NQ];
   __shared__ double d[MCF_DIAG_LEN];
-  #pragma unroll
-  for (int q = 0; q < FIN_NQ; ++q) raw[q][lane] = __longlong_as_double((long long)s[slot_word(kFinSlots[q], lane)]);
-  d[lane] = 0.0;
+  if (w0) {
+    #pragma unroll
+    for (int q = 0; q < FIN_NQ; ++q) {
+      const int k = q == 2 ? a.hval : q == 3 ? a.hval + 1 : kFinSlots[q];   // the step's right-hand side maxima
+      raw[q][lane] = __longlong_as_double((long long)s[slot_word(k, lane)]);
+    }
+    d[lane] = 0.0;
+  }
   __syncthreads();
   if (lane < FIN_NQ) {
     double r = 0.0;
@@ -2356,12 +2377,15 @@ DEV void finalize_slots(const CaseDev& c, const FinArgs& a) {
     if (((a.lean & LEAN_PRES) && refine > 0) || ((a.lean & LEAN_VEL) && hrefine > 0) || second ||
         (a.lean && a.force_miss))
       status |= 32;
-    d[MCF_D_STATUS] = status;   // the finite-force gate (16) is added by k_finalize
+    d[MCF_D_STATUS] = status;   // the finite-force gate (16) is added by finalize_forces
   }
   __syncthreads();
+  if (!w0) return;
   G(c.diag)[(size_t)row * MCF_DIAG_LEN + lane] = d[lane];
   const gptr<u64> wsl = G(c.sl);
   for (int st = 0; st < NSTRIPE; ++st) wsl[slot_word(lane, st)] = 0ull;   // the used words of every stripe
+  if (lane < 2)
+    for (int st = 0; st < NSTRIPE; ++st) wsl[slot_word(a.hval + lane, st)] = 0ull;   // and the step's pair
 }
 
 // Surface forces (forces_body); with `slots`, the launch's last block instead gathers the step's
@@ -2372,19 +2396,20 @@ __global__ __launch_bounds__(64) void k_forces(const CaseDev* __restrict__ cases
     finalize_slots(c, fin);
     return;
   }
-  forces_body(c, a);
+  forces_body(c, a, blockIdx.x);
 }
 
-// The step's forces into its diagnostics row (written by finalize_slots), the finite-force gate,
-// and the clock (one wave per case)
-__global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cases) {
-  const CaseDev& c = cases[blockIdx.y];
+// The step's forces into its diagnostics row (written by finalize_slots in an earlier launch), the
+// finite-force gate, and the clock (one wave per case): a kernel of its own after k_forces
+// (k_finalize), or one more body of a later fused launch (BK_FIN)
+DEV void finalize_forces(const CaseDev& c) {
+  const int lane = threadIdx.x & 63;
   const double dt = G(c.prm)[0];
   const int row = *G(c.clock_step);
   const double time = *G(c.clock_time) + dt;
   double f[6];
-  force_totals(c, f);   // the step's forces, from k_forces' per-wave partial sums
-  if (threadIdx.x != 0) return;
+  force_totals(c, f);   // the step's forces, from the force waves' partial sums
+  if (lane != 0) return;
   const gptr<double> dr = G(c.diag) + (size_t)row * MCF_DIAG_LEN;
   const double denom = 0.5 * c.rho * c.uref * c.uref * c.diameter;
   const double cd = f[0] / denom, cl = f[1] / denom;
@@ -2400,6 +2425,32 @@ __global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cas
   if (!isfinite(cd) || !isfinite(cl)) dr[MCF_D_STATUS] = (double)((int)dr[MCF_D_STATUS] | 16);
   *G(c.clock_step) = row + 1;
   *G(c.clock_time) = time;
+}
+__global__ __launch_bounds__(64) void k_finalize(const CaseDev* __restrict__ cases) {
+  finalize_forces(cases[blockIdx.y]);
+}
+
+// The first stage of a step with the previous step's slot gather and force waves in its first `tb`
+// blocks (one case): block 0 gathers the slots, blocks 1 .. tb - 1 hold four force waves each.  The
+// previous step's forces are completed by a later launch of the step (BK_FIN), after the kernel
+// boundary has made the partial sums and the row visible: an in-kernel hand-over (count the waves
+// in, the last completes) needs a device-scope release per wave, whose L2 write-back cost the
+// advection blocks more than the launch it saved.  tb is a multiple of 8, so the advection blocks
+// keep their XCD placement.
+constexpr int TAIL_WAVES = 4;
+__host__ __device__ inline int adv_tail_blocks(int nsurf) {
+  return (1 + (force_blocks(nsurf) + TAIL_WAVES - 1) / TAIL_WAVES + 7) / 8 * 8;
+}
+__global__ __launch_bounds__(256) void k_adv_tail(const CaseDev* __restrict__ cases, AdvArgs a, ForceArgs fa,
+                                                  FinArgs fin, int tb) {
+  const CaseDev& c = cases[blockIdx.y];
+  if ((int)blockIdx.x < tb) {
+    if (blockIdx.x == 0) finalize_slots(c, fin);   // the whole block: it synchronizes
+    else forces_body(c, fa, TAIL_WAVES * ((int)blockIdx.x - 1) + (int)(threadIdx.x >> 6));
+    return;
+  }
+  if (a.q_id == a.adv_id) adv_body<true>(c, a, blockIdx.x - tb, gridDim.x - tb);
+  else adv_body<false>(c, a, blockIdx.x - tb, gridDim.x - tb);
 }
 
 // Step parameters of every case of the launch; dt <= 0 keeps each case's own dt
@@ -2439,6 +2490,7 @@ __global__ __launch_bounds__(64) void k_slot_values(const CaseDev* __restrict__ 
 __global__ void k_clear_slots(const CaseDev* __restrict__ cases) {
   const CaseDev& c = cases[blockIdx.y];
   for (int w = threadIdx.x; w < NSTRIPE * NSLOT; w += blockDim.x) c.sl[slot_word(w % NSLOT, w / NSLOT)] = 0ull;
+  for (int w = threadIdx.x; w < NSTRIPE * 4; w += blockDim.x) c.sl[slot_word(SL_RHS + (w & 3), w >> 2)] = 0ull;
 }
 
 // Checkpoint of what a steady step reads from earlier steps: the state S[parity], the advection
@@ -3183,6 +3235,9 @@ struct Ctx {
   int p_band, h_band;    // most band rows of a problem's capacitance entries (x-FFT form)
   int p_yy, p_xx, p_np, p_r, h_yy, h_xx, h_np, h_r;
   StepPrm sp{};          // a single case's step parameters (dt = 0: read from the case)
+  // the velocity solve's right-hand side maxima (a step's SL_RHS pair; an entry point's own slots)
+  int hval_u = SL_HU + SV_VAL, hval_v = SL_HV + SV_VAL;
+  bool defer = false;    // the caller launches a pending step tail (one handle's steps)
 };
 
 void ctx_shapes(Ctx& x, const CaseDev& c, const SolveDesc* hs3) {
@@ -3214,6 +3269,7 @@ Ctx ctx_of(mcf_handle* h) {
   x.ln = h->ln;
   x.dump = h->hs[0].dump_cap > 0;
   x.gm = h->pbackend == MCF_PB_GMRES ? h : nullptr;
+  x.defer = true;
   ctx_shapes(x, h->hc, h->hs);
   x.ln = effective_launch(x.ln, !x.nopar_p, !x.nopar_h);
   return x;
@@ -3294,6 +3350,7 @@ Body body_upd(const Ctx& x, UpdArgs a) { Body b = body_of(BK_UPD, gridSn(x.maxve
 Body body_press(const Ctx& x, PressArgs a) { Body b = body_of(BK_PRESS, gridSn(x.maxp, x.n).x); a.sp = x.sp; b.q = a; return b; }
 Body body_fill(const Ctx& x, FillArgs a) { Body b = body_of(BK_FILL, grid1n(x.maxfill, 1).x); b.f = a; return b; }
 Body no_body() { return body_of(BK_NONE, 0); }
+Body body_fin() { return body_of(BK_FIN, 1); }
 
 // Launch one stage body with its own kernel
 void launch_body(const Ctx& x, const Body& b) {
@@ -3305,6 +3362,7 @@ void launch_body(const Ctx& x, const Body& b) {
     case BK_UPD: hipLaunchKernelGGL(k_update, g, blk, 0, x.stream, x.dc, b.u); break;
     case BK_PRESS: hipLaunchKernelGGL(k_pressures, g, blk, 0, x.stream, x.dc, b.q); break;
     case BK_FILL: hipLaunchKernelGGL(k_fill, g, blk, 0, x.stream, x.dc, b.f); break;
+    case BK_FIN: hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc); break;
     default: break;
   }
 }
@@ -3331,6 +3389,8 @@ void launch_bodies(const Ctx& x, std::initializer_list<Body> bodies) {
     hipLaunchKernelGGL((k_fused_of<BK_PRES, BK_UPD, BK_NONE>), g, blk, 0, x.stream, x.dc, f);
   else if (k0 == BK_DIV && k1 == BK_HELM && k2 == BK_NONE)
     hipLaunchKernelGGL((k_fused_of<BK_DIV, BK_HELM, BK_NONE>), g, blk, 0, x.stream, x.dc, f);
+  else if (k0 == BK_HELM && k1 == BK_DIV && k2 == BK_FIN)
+    hipLaunchKernelGGL((k_fused_of<BK_HELM, BK_DIV, BK_FIN>), g, blk, 0, x.stream, x.dc, f);
   else if (k0 == BK_DIV && k1 == BK_PRESS && k2 == BK_FILL)
     hipLaunchKernelGGL((k_fused_of<BK_DIV, BK_PRESS, BK_FILL>), g, blk, 0, x.stream, x.dc, f);
   else
@@ -3503,13 +3563,14 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, in
     if (pres) {
       PresArgs a{in_id, out_id, B_RESP, 0, slot0, res_slot, pred};
       hipLaunchKernelGGL(k_pres, gridSn(x.maxp, x.n), dim3(256), 0, x.stream, x.dc, a);
-      hipLaunchKernelGGL(k_decide, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, slot0, -1, res_slot);
+      hipLaunchKernelGGL(k_decide, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, slot0, -1, res_slot, slot0 + (int)SV_VAL, -1);
       if ((rc = enqueue_apply(x, pres, B_RESP, out_id, 2, res_slot, shift))) return rc;
     } else {
       HelmArgs a{in_id, out_id, B_RES, -1, 0, res_slot, pred};
       a.sp = x.sp;
       hipLaunchKernelGGL(k_helm, gridSn(x.maxvel, x.n), dim3(256), 0, x.stream, x.dc, a);
-      hipLaunchKernelGGL(k_decide, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, (int)SL_HU, (int)SL_HV, res_slot);
+      hipLaunchKernelGGL(k_decide, dim3(1, x.n), dim3(64), 0, x.stream, x.dc, (int)SL_HU, (int)SL_HV, res_slot,
+                         x.hval_u, x.hval_v);
       if ((rc = enqueue_apply(x, pres, B_RES, out_id, 2, res_slot, shift))) return rc;
     }
   }
@@ -3525,9 +3586,10 @@ int enqueue_solve(const Ctx& x, bool pres, int in_id, int out_id, int psolve, in
 // statistics are launched with the update; with `gate`, the divergence gate is returned there for
 // the caller to merge into its next launch.
 int enqueue_project(const Ctx& x, int x_id, int out_id, int phi_id, double dtmul, int psolve, int defect_id,
-                    int refs = 2, const Body* head = nullptr, Body* gate = nullptr) {
+                    int refs = 2, const Body* head = nullptr, Body* gate = nullptr, bool prev_fin = false) {
   int rc;
-  launch_bodies(x, {head ? *head : no_body(), body_div(x, DivArgs{x_id, B_DIVP, dtmul, SL_P0 + 8 * psolve + SV_VAL})});
+  launch_bodies(x, {head ? *head : no_body(), body_div(x, DivArgs{x_id, B_DIVP, dtmul, SL_P0 + 8 * psolve + SV_VAL}),
+                    prev_fin ? body_fin() : no_body()});
   Body fin;
   if ((rc = enqueue_solve(x, true, B_DIVP, phi_id, psolve, refs, &fin))) return rc;
   launch_bodies(x, {fin, body_upd(x, UpdArgs{x_id, phi_id, out_id, defect_id, dtmul})});
@@ -3551,11 +3613,36 @@ int check_ready(mcf_handle* h, double dt) {
   return 0;
 }
 
+// A step's tail: its surface forces and the slot gather into its diagnostics row (k_forces), then
+// the forces into the row and the clock (k_finalize), for the step of parity `parity`, whose new
+// state is S[1 - parity].
+FinArgs step_fin(const Ctx& x, int parity, int nproj, int lean, bool force_miss) {
+  return FinArgs{nproj, lean, force_miss ? 1 : 0, x.gm ? 1 : 0, SL_RHS + 2 * parity};
+}
+ForceArgs step_forces(int parity) { return ForceArgs{B_S0 + 1 - parity, B_POUT, -1.0}; }
+void launch_tail(const Ctx& x, int parity, const FinArgs& fin) {
+  hipLaunchKernelGGL(k_forces, dim3(force_blocks(x.maxsurf) + 1, x.n), dim3(64), 0, x.stream, x.dc,
+                     step_forces(parity), fin, 1);
+  hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc);
+}
+// A step's tail reads the new state, the output pressure, the slots (which it clears) and the
+// clock; the next step's first stage reads the state and writes only its own SL_RHS pair, so the
+// tail's gather and force waves can run in the first stage's launch (k_adv_tail) and its last part
+// in the launch before the first pressure solve (BK_FIN).  TAIL_OWN: the step launches its own tail last; TAIL_DEFER: it
+// leaves its tail to the caller; TAIL_PREV: it leaves its own and runs the previous step's (parity
+// 1 - parity, same lean form) inside its own launches.  Only one case's steady
+// steps defer: a step that sets the inlet first (k_inlet reads the clock the tail advances) and a
+// GMRES step (it reads the slots back) run their own.
+enum { TAIL_OWN = 0, TAIL_DEFER = 1, TAIL_PREV = 2 };
+bool tail_can_defer(const Ctx& x) { return x.defer && x.n == 1 && !x.pert && !x.gm; }
+
 // One AB2/CN step of every case of the context (time_integration.py:7-145,
 // optimized_ghost_cell.py:236-286, main.py:383-415).  Launches only: dt, alpha and the pressure
 // factor are read from each case's parameters (k_set_prm); the caller advances the host state.
-int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, bool force_miss = false) {
+int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, bool force_miss = false,
+                         int tail = TAIL_OWN) {
   if (first) lean = 0;
+  if (first || !tail_can_defer(x)) tail = TAIL_OWN;
   const int vrefs = (lean & LEAN_VEL) ? 0 : 2;
   const int prefs = (lean & LEAN_PRES) ? 0 : (lean & ONCE_PRES) ? 1 : 2;
   int rc;
@@ -3563,66 +3650,94 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, boo
   const int Hprev = B_H0 + 1 - parity, Hnow = B_H0 + parity;
   const dim3 ga = gridAdv(x.maxvel, x.n);
   int nproj = 2;
+  const int rhs = SL_RHS + 2 * parity;
   if (x.pert) hipLaunchKernelGGL(k_inlet, dim3((x.maxny + 255) / 256, x.n), dim3(256), 0, x.stream, x.dc);
   if (first) {
     // projected midpoint startup (time_integration.py:33-47)
-    AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1, x.sp};
+    AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1, x.sp, rhs};
     hipLaunchKernelGGL(k_adv_rhs, ga, dim3(256), 0, x.stream, x.dc, m);
     if ((rc = enqueue_project(x, B_TV0, B_HALF, B_TP0, 0.5, 2, -1))) return rc;
     FillArgs f{B_HALF, 1, 1, 0};
     hipLaunchKernelGGL(k_fill, grid1n(x.maxfill, x.n), dim3(256), 0, x.stream, x.dc, f);
-    AdvArgs r{S, B_HALF, B_PH, -1, -1, B_RHS, 2, x.sp};
+    AdvArgs r{S, B_HALF, B_PH, -1, -1, B_RHS, 2, x.sp, rhs};
     hipLaunchKernelGGL(k_adv_rhs, ga, dim3(256), 0, x.stream, x.dc, r);
     nproj = 3;
+  } else if (tail == TAIL_PREV) {
+    AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0, x.sp, rhs};
+    // the advection grid (GSB_ADV blocks: four per CU, one residency) gives up the tail's blocks, so
+    // that no advection block waits for a free slot behind the others
+    const int tb = adv_tail_blocks(x.maxsurf);
+    const unsigned ax = std::max<int>(8, std::min<int>(ga.x, GSB_ADV - tb));
+    hipLaunchKernelGGL(k_adv_tail, dim3(ax + tb, 1), dim3(256), 0, x.stream, x.dc, r, step_forces(1 - parity),
+                       step_fin(x, 1 - parity, 2, lean, force_miss), tb);
   } else {
-    AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0, x.sp};
+    AdvArgs r{S, S, B_PH, Hprev, Hnow, B_RHS, 0, x.sp, rhs};
     hipLaunchKernelGGL(k_adv_rhs, ga, dim3(256), 0, x.stream, x.dc, r);
   }
+  Ctx xv = x;   // the velocity solve reads this step's right-hand side maxima
+  xv.hval_u = rhs; xv.hval_v = rhs + 1;
   // Stages that do not read each other's output share a launch: the Helmholtz statistics with the
   // first projection's divergence, the first gate with the defect, the second gate with the
   // pressures and the ghost fill of the new state.
   Body hfin, gate1, gate2;
-  if ((rc = enqueue_solve(x, false, B_RHS, B_STAR, 0, vrefs, &hfin))) return rc;
-  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, prefs, &hfin, &gate1))) return rc;
+  if ((rc = enqueue_solve(xv, false, B_RHS, B_STAR, 0, vrefs, &hfin))) return rc;
+  // the previous step's forces and clock complete before this step's first pressure solve (a
+  // recording solve tags its pairs with the clock)
+  if ((rc = enqueue_project(x, B_STAR, Snew, B_PHI, 1.0, 0, -1, prefs, &hfin, &gate1, tail == TAIL_PREV))) return rc;
   launch_bodies(x, {gate1, body_helm(x, HelmArgs{B_RHS, Snew, B_DEFECT, B_PROJ, 2, SV_FINAL, -1})});
   if ((rc = enqueue_project(x, B_PROJ, B_PROJ, B_PHI2, 1.0, 1, B_DEFECT, prefs, nullptr, &gate2))) return rc;
   launch_bodies(x, {gate2, body_press(x, PressArgs{B_PH, B_PHI2, B_PH, B_POUT}), body_fill(x, FillArgs{Snew, 1, 1, 1})});
-  ForceArgs fa{Snew, B_POUT, -1.0};
-  FinArgs fin{nproj, lean, force_miss ? 1 : 0, x.gm ? 1 : 0};
-  hipLaunchKernelGGL(k_forces, dim3(force_blocks(x.maxsurf) + 1, x.n), dim3(64), 0, x.stream, x.dc, fa, fin, 1);
-  hipLaunchKernelGGL(k_finalize, dim3(1, x.n), dim3(64), 0, x.stream, x.dc);
+  if (tail == TAIL_OWN) launch_tail(x, parity, step_fin(x, parity, nproj, lean, force_miss));
   return launch_check(x.err, "step");
 }
 
 }  // namespace
 
-// Step graphs: one per parity, for steady AB2 steps (history present, unchanged dt).
+// Step graphs: one per parity, for steady AB2 steps (history present, unchanged dt).  Where the
+// step's tail can be deferred (one case, no inlet perturbation, direct pressure backend), exec[p]
+// leaves its own tail pending and exec_prev[p] also runs the pending tail of the previous step
+// beside its first stage; the caller launches the last pending tail itself (launch_tail).
 struct StepGraphs {
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr}, graph_prev[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr}, exec_prev[2] = {nullptr, nullptr};
   bool valid = false;
+  bool defer = false;   // exec leaves its tail pending
+  int lean = 0;
+  bool force_miss = false;
   void drop() {
     for (int p = 0; p < 2; ++p) {
-      if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
-      if (graph[p]) (void)hipGraphDestroy(graph[p]);
-      exec[p] = nullptr; graph[p] = nullptr;
+      for (hipGraphExec_t* e : {&exec[p], &exec_prev[p]})
+        if (*e) { (void)hipGraphExecDestroy(*e); *e = nullptr; }
+      for (hipGraph_t* g : {&graph[p], &graph_prev[p]})
+        if (*g) { (void)hipGraphDestroy(*g); *g = nullptr; }
     }
     valid = false;
+    defer = false;
   }
-  int build(const Ctx& x, int lean = 0, bool force_miss = false) {
+  static int capture(const Ctx& x, int p, int lean, bool force_miss, int tail, hipGraph_t* graph, hipGraphExec_t* exec) {
+    HIPCHK(x.err, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step_kernels(x, p, false, lean, force_miss, tail);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(x.stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(x.err, 4, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    *graph = g;
+    HIPCHK(x.err, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    // the executable's one-time upload here rather than at its first launch
+    HIPCHK(x.err, hipGraphUpload(*exec, x.stream));
+    return 0;
+  }
+  int build(const Ctx& x, int lean_ = 0, bool force_miss_ = false) {
     drop();
+    lean = lean_;
+    force_miss = force_miss_;
+    const bool d = tail_can_defer(x);
     for (int p = 0; p < 2; ++p) {
-      HIPCHK(x.err, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_step_kernels(x, p, false, lean, force_miss);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(x.stream, &g);
-      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-      if (e != hipSuccess) return fail(x.err, 4, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      graph[p] = g;
-      HIPCHK(x.err, hipGraphInstantiate(&exec[p], g, nullptr, nullptr, 0));
-      // the executable's one-time upload here rather than at its first launch
-      HIPCHK(x.err, hipGraphUpload(exec[p], x.stream));
+      if (int rc = capture(x, p, lean, force_miss, d ? TAIL_DEFER : TAIL_OWN, &graph[p], &exec[p])) return rc;
+      if (d)
+        if (int rc = capture(x, p, lean, force_miss, TAIL_PREV, &graph_prev[p], &exec_prev[p])) return rc;
     }
+    defer = d;
     valid = true;
     return 0;
   }
@@ -3748,6 +3863,13 @@ int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
   if (int rc = ensure_tuned(h)) return rc;
   Ctx x = ctx_of(h);
   StepGraphs* gs = graphs_for(h, lean);
+  // the parity of the step whose tail a deferring graph left pending (-1: none), launched before
+  // an eager step and at the end of the call
+  int pend = -1;
+  auto flush = [&]() {
+    if (pend >= 0) launch_tail(x, pend, step_fin(x, pend, 2, gs->lean, gs->force_miss));
+    pend = -1;
+  };
   // a speculating run switches between the SPEC_MASKS pairs as the transient comes and goes: all of
   // them are captured when the first is needed, so that no later chunk pays a capture
   const bool spec = h->spec > 0 && h->hs[0].dump_cap == 0 && h->pbackend == MCF_PB_DIRECT;
@@ -3764,6 +3886,7 @@ int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
     }
     // GMRES steps read the solver state back between launches: they run eagerly
     if (!steady || !h->use_graphs || x.gm) {
+      flush();
       if (int rc = enqueue_step_kernels(x, h->parity, !h->has_history, x.gm ? 0 : lean, h->spec == 2)) return rc;
     } else {
       if (h->graph_dt != dt) drop_graphs(h);
@@ -3776,11 +3899,13 @@ int enqueue_steps(mcf_handle* h, int n, double dt, int lean = 0) {
         }
       }
       h->graph_dt = dt;
-      HIPCHK(h, hipGraphLaunch(gs->exec[h->parity], h->stream));
+      HIPCHK(h, hipGraphLaunch(pend >= 0 ? gs->exec_prev[h->parity] : gs->exec[h->parity], h->stream));
+      pend = gs->defer ? h->parity : -1;
     }
     advance_host_state(h, dt);
   }
-  return 0;
+  flush();
+  return launch_check(h, "step tail");
 }
 
 // Step parameters for a one-off operation at an explicit dt (projection, operator, solve entry points)

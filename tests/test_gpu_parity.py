@@ -147,6 +147,36 @@ def test_graph_replays_match_eager(pkg, monkeypatch):
         assert np.array_equal(cde, cdg) and np.array_equal(cle, clg) and np.array_equal(te, tg)
 
 
+def test_deferred_step_tails_give_the_eager_rows(pkg, monkeypatch):
+    """A replayed step leaves its tail (surface forces, slot gather into its diagnostics row, clock) to
+    the next step's launches or to the end of the call: the rows of calls of 1, 4, 1 and 3 steps equal
+    those of eagerly launched steps bit for bit, every row holds its forces, and the clock advances
+    by dt per row."""
+    kw = dict(nx=88, ny=16, Lx=22., Ly=4.1, xc=2., yc=2.)
+    monkeypatch.setenv('GHOSTFLOW_NO_GRAPH', '1')
+    eager = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **kw)
+    monkeypatch.setenv('GHOSTFLOW_NO_GRAPH', '0')
+    graph = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **kw)
+    for f in (eager, graph):
+        f.gg.device.set_launch(0, 1)
+    names = list(nat_names(pkg))
+    dt = 1e-3
+    rows_e, rows_g = [], []
+    for chunk in (1, 4, 1, 3):
+        rows_e.append(eager.gg.device.run(chunk, dt))
+        rows_g.append(graph.gg.device.run(chunk, dt))
+    re, rg = np.concatenate(rows_e), np.concatenate(rows_g)
+    assert np.array_equal(re, rg)
+    t = rg[:, names.index('time')]
+    assert np.allclose(np.diff(t), dt, rtol=0, atol=1e-15)
+    assert np.all(rg[:, names.index('cd')] != 0.0) and np.all(rg[:, names.index('status')] == 0)
+
+
+def nat_names(pkg):
+    import importlib
+    return importlib.import_module(pkg.__name__ + '._native').DIAG_NAMES
+
+
 def test_launch_shapes_agree(pkg):
     """Every staged transform tile gives bit-identical steps, and so do the two split-K tile heights
     of one block size (8 or 4 waves); the split-K transforms, the parity-split transforms (8 and 4
