@@ -2021,12 +2021,16 @@ __global__ __launch_bounds__(256) void k_fill(const CaseDev* __restrict__ cases,
 
 // The inlet of the coming step from its end time t (main.py:280-286): base (1 + shape (1 - t/T)) while
 // t <= T, the base profile afterwards, with shape = amp sin(2 pi y / Ly); products and sums rounded
-// one at a time, as the reference evaluates them.
-__global__ __launch_bounds__(256) void k_inlet(const CaseDev* __restrict__ cases) {
+// one at a time, as the reference evaluates them.  `pending`: the previous step's clock advance has not
+// run yet (its tail runs later in this step), so t = (clock + dt) + dt, rounded as the advance and
+// then this sum would round it.
+__global__ __launch_bounds__(256) void k_inlet(const CaseDev* __restrict__ cases, int pending) {
   const CaseDev& c = cases[blockIdx.y];
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= c.ny || !(c.pert_T > 0.0)) return;
-  const double t = *G(c.clock_time) + G(c.prm)[0];
+  const double dt = G(c.prm)[0];
+  double t = *G(c.clock_time) + dt;
+  if (pending) t = t + dt;
   const double base = G(c.inlet_base)[j];
   double v = base;
   if (t <= c.pert_T) {
@@ -3631,10 +3635,10 @@ void launch_tail(const Ctx& x, int parity, const FinArgs& fin) {
 // in the launch before the first pressure solve (BK_FIN).  TAIL_OWN: the step launches its own tail last; TAIL_DEFER: it
 // leaves its tail to the caller; TAIL_PREV: it leaves its own and runs the previous step's (parity
 // 1 - parity, same lean form) inside its own launches.  Only one case's steady
-// steps defer: a step that sets the inlet first (k_inlet reads the clock the tail advances) and a
-// GMRES step (it reads the slots back) run their own.
+// steps defer (a GMRES step, which reads the slots back, runs its own); a step that sets the inlet
+// first tells k_inlet that the clock advance of the step before is still pending.
 enum { TAIL_OWN = 0, TAIL_DEFER = 1, TAIL_PREV = 2 };
-bool tail_can_defer(const Ctx& x) { return x.defer && x.n == 1 && !x.pert && !x.gm; }
+bool tail_can_defer(const Ctx& x) { return x.defer && x.n == 1 && !x.gm; }
 
 // One AB2/CN step of every case of the context (time_integration.py:7-145,
 // optimized_ghost_cell.py:236-286, main.py:383-415).  Launches only: dt, alpha and the pressure
@@ -3651,7 +3655,8 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, boo
   const dim3 ga = gridAdv(x.maxvel, x.n);
   int nproj = 2;
   const int rhs = SL_RHS + 2 * parity;
-  if (x.pert) hipLaunchKernelGGL(k_inlet, dim3((x.maxny + 255) / 256, x.n), dim3(256), 0, x.stream, x.dc);
+  if (x.pert)
+    hipLaunchKernelGGL(k_inlet, dim3((x.maxny + 255) / 256, x.n), dim3(256), 0, x.stream, x.dc, tail == TAIL_PREV ? 1 : 0);
   if (first) {
     // projected midpoint startup (time_integration.py:33-47)
     AdvArgs m{S, S, B_PH, -1, Hnow, B_TV0, 1, x.sp, rhs};
@@ -3694,7 +3699,7 @@ int enqueue_step_kernels(const Ctx& x, int parity, bool first, int lean = 0, boo
 }  // namespace
 
 // Step graphs: one per parity, for steady AB2 steps (history present, unchanged dt).  Where the
-// step's tail can be deferred (one case, no inlet perturbation, direct pressure backend), exec[p]
+// step's tail can be deferred (one case, direct pressure backend), exec[p]
 // leaves its own tail pending and exec_prev[p] also runs the pending tail of the previous step
 // beside its first stage; the caller launches the last pending tail itself (launch_tail).
 struct StepGraphs {

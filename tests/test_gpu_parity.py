@@ -129,10 +129,15 @@ def test_divergence_stats(pkg, pair88):
     assert abs(mine[1] - want[1]) <= 1e-12 * want[1]
 
 
-def test_graph_replays_match_eager(pkg, monkeypatch):
+@pytest.mark.parametrize('perturbed', [False, True], ids=['plain', 'perturbed inflow'])
+def test_graph_replays_match_eager(pkg, monkeypatch, perturbed):
     """Steps replayed from the captured HIP graphs equal eagerly launched steps bit for bit,
-    across several solve() calls with host reads (synchronisations) between the replays."""
+    across several solve() calls with host reads (synchronisations) between the replays; with the
+    inflow perturbation (its window ends in the second call) the replayed steps set each inlet from
+    the clock while the previous step's clock advance is still pending."""
     kw = dict(nx=88, ny=16, Lx=22., Ly=4.1, xc=2., yc=2.)
+    pk = dict(perturb_transient=True, perturb_duration=0.005, perturb_amp=0.15) if perturbed else \
+        dict(perturb_transient=False)
     monkeypatch.setenv('GHOSTFLOW_NO_GRAPH', '1')
     eager = pkg.MAC_CylinderFlow(boundary='ghost_cell', time_scheme='ab2_cn', verbose=False, **kw)
     monkeypatch.setenv('GHOSTFLOW_NO_GRAPH', '0')
@@ -140,8 +145,8 @@ def test_graph_replays_match_eager(pkg, monkeypatch):
     for f in (eager, graph):
         f.gg.device.set_launch(0, 1)   # the same launch shapes for both (autotuned shapes can differ in round-off)
     for chunk in (3, 5, 4):
-        te, cde, cle = eager.solve(n_steps=chunk, dt=1e-3, perturb_transient=False, save_artifacts=False)
-        tg, cdg, clg = graph.solve(n_steps=chunk, dt=1e-3, perturb_transient=False, save_artifacts=False)
+        te, cde, cle = eager.solve(n_steps=chunk, dt=1e-3, save_artifacts=False, **pk)
+        tg, cdg, clg = graph.solve(n_steps=chunk, dt=1e-3, save_artifacts=False, **pk)
         for key in ('u', 'v', 'p'):
             assert np.array_equal(getattr(eager, key), getattr(graph, key)), key
         assert np.array_equal(cde, cdg) and np.array_equal(cle, clg) and np.array_equal(te, tg)
